@@ -169,7 +169,8 @@ def maximum_of_all_pixels(src: np.ndarray) -> np.float32:
 
 
 def histogram_bin_index(values: np.ndarray, mn, mx, num_bins: int) -> np.ndarray:
-    """bin = convert_uint_sat(((v - min) * (bins-1)) / (max - min) + 0.5), float32 math.
+    """bin = convert_uint_sat(((v - min) * (bins-1)) / (max - min) + 0.5): float32
+    quotient, the literal 0.5 added in float64, then truncation with saturation.
 
     Restates the clij histogram kernel launched at
     pyclesperanto_prototype/_tier3/_histogram.py:117-122 (kernel source absent).
@@ -182,7 +183,8 @@ def histogram_bin_index(values: np.ndarray, mn, mx, num_bins: int) -> np.ndarray
     with np.errstate(invalid="ignore", divide="ignore"):
         t = ((v - mn) * F32(num_bins - 1)).astype(np.float32)
         t = (t / rng).astype(np.float32)
-        t = (t + F32(0.5)).astype(np.float32)
+        # the unsuffixed literal 0.5 is a double in OpenCL C on fp64 devices
+        t = t.astype(np.float64) + 0.5
     t = np.nan_to_num(t, nan=0.0, posinf=np.inf, neginf=0.0)
     idx = np.floor(np.clip(t, 0, None)).astype(np.int64)
     return np.minimum(idx, num_bins - 1)

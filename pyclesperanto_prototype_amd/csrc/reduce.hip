@@ -1,0 +1,330 @@
+// Reductions of the Otsu stage: global min/max, sum, the 256-bin histogram and the
+// Otsu threshold search, all on the device (no host round trip between them).
+//
+// Reference semantics (pyclesperanto_prototype/...):
+//   minimum/maximum_of_all_pixels  _tier2/_minimum_of_all_pixels.py:6-59 (exact)
+//   histogram                      _tier3/_histogram.py:25-130: one partial histogram
+//       per row y over all z and x, bin = convert_uint_sat(((v-min)*(bins-1))/(max-min)+0.5)
+//       in float32, partials stored as float32 and summed over y in y order
+//   threshold_otsu                 _tier9/_threshold_otsu.py:38-54 + the vendored
+//       scikit-image routine :64-106 (float64 cumulative sums, first argmax)
+//
+// Histogram layout: counts are exact uint32 per (row y, bin) -- hist2d[H][bins] --
+// built from per-wavefront LDS sub-histograms (block = one row y over a chunk of z
+// slices), then each bin is summed over y in float32 in y order, which reproduces
+// the reference's float32 totals bit for bit (also above 2^24 per bin).
+#include "common.hpp"
+
+namespace cle {
+
+// order-preserving map float -> uint32 (for atomicMin / atomicMax)
+__device__ __forceinline__ uint32_t f2o(float f) {
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(uint32_t o) {
+    uint32_t b = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+    return __uint_as_float(b);
+}
+
+constexpr int RB = 256;
+
+template <typename LD>
+__global__ __launch_bounds__(RB) void k_minmax(LD in, int64_t n, uint32_t* out) {
+    float mn = INFINITY, mx = -INFINITY;
+    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+        const float v = in(i);
+        mn = fminf(mn, v);
+        mx = fmaxf(mx, v);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = fminf(mn, __shfl_xor(mn, off));
+        mx = fmaxf(mx, __shfl_xor(mx, off));
+    }
+    __shared__ float smn[RB / 64], smx[RB / 64];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) { smn[w] = mn; smx[w] = mx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < RB / 64; ++k) { mn = fminf(mn, smn[k]); mx = fmaxf(mx, smx[k]); }
+        atomicMin(out, f2o(mn));
+        atomicMax(out + 1, f2o(mx));
+    }
+}
+
+__global__ void k_minmax_init(uint32_t* o) {
+    o[0] = 0xFFFFFFFFu;
+    o[1] = 0u;
+}
+__global__ void k_minmax_fin(const uint32_t* o, float* out2) {
+    out2[0] = o2f(o[0]);
+    out2[1] = o2f(o[1]);
+}
+
+struct LdF32R {
+    const float* p;
+    __device__ __forceinline__ float operator()(int64_t i) const { return p[i]; }
+};
+struct LdAnyR {
+    DBuf b;
+    __device__ __forceinline__ float operator()(int64_t i) const { return ld(b, i); }
+};
+
+int minmax_impl(DBuf src, int64_t n, float* out2, hipStream_t s) {
+    Scratch tmp;
+    CLE_TRY(tmp.alloc(8, s));
+    k_minmax_init<<<1, 1, 0, s>>>(tmp.get<uint32_t>());
+    unsigned g = grid_for((size_t)n, RB * 8);
+    if (g > 2048) g = 2048;
+    if (n > 0) {
+        if (src.dt == CLE_F32) k_minmax<<<g, RB, 0, s>>>(LdF32R{(const float*)src.p}, n, tmp.get<uint32_t>());
+        else k_minmax<<<g, RB, 0, s>>>(LdAnyR{src}, n, tmp.get<uint32_t>());
+    }
+    k_minmax_fin<<<1, 1, 0, s>>>(tmp.get<uint32_t>(), out2);
+    CLE_LAUNCHED();
+    return CLE_OK;
+}
+
+__global__ __launch_bounds__(RB) void k_sum(DBuf in, int64_t n, double* out) {
+    double acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) acc += ld(in, i);
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    __shared__ double sm[RB / 64];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < RB / 64; ++k) acc += sm[k];
+        atomicAdd(out, acc);
+    }
+}
+
+// ----------------------------------------------------------------- histogram
+constexpr int HB = 256;                // threads per block
+constexpr int HWAVES = HB / 64;
+constexpr int HIST_LDS_MAX_BINS = 2048;
+
+// The float32 quotient gets the literal 0.5 added in float64 (an unsuffixed OpenCL
+// literal is double on fp64 devices), then convert_uint_sat truncates.
+__device__ __forceinline__ uint32_t bin_of(float v, float mn, float range, int bins) {
+#pragma clang fp contract(off)
+    const float q = ((v - mn) * (float)(bins - 1)) / range;
+    const double t = (double)q + 0.5;
+    // convert_uint_sat: NaN -> 0, negative -> 0, truncation toward zero
+    if (!(t > 0.0)) return 0u;
+    if (t >= (double)bins) return (uint32_t)(bins - 1);
+    return (uint32_t)t;
+}
+
+// one block = one row y over slices [z0, z0 + zc); per-wave LDS sub-histograms
+template <typename LD>
+__global__ __launch_bounds__(HB) void k_hist_rows(LD in, Geom g, int bins, int zc, int zchunks, float mn_arg,
+                                                 float mx_arg, const float* mm, uint32_t* hist2d) {
+    extern __shared__ uint32_t sh[];
+    const int64_t y = blockIdx.x / zchunks;
+    const int64_t z0 = (int64_t)(blockIdx.x % zchunks) * zc;
+    const float mn = mm ? mm[0] : mn_arg;
+    const float mx = mm ? mm[1] : mx_arg;
+    const float range = mx - mn;
+    const int wave = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < bins * HWAVES; i += HB) sh[i] = 0;
+    __syncthreads();
+    uint32_t* mine = sh + wave * bins;
+    const int64_t z1 = (z0 + zc < g.D) ? z0 + zc : g.D;
+    for (int64_t z = z0; z < z1; ++z) {
+        const int64_t base = (z * g.H + y) * g.W;
+        for (int64_t x = threadIdx.x; x < g.W; x += HB) atomicAdd(&mine[bin_of(in(base + x), mn, range, bins)], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < bins; b += HB) {
+        uint32_t c = 0;
+        for (int w = 0; w < HWAVES; ++w) c += sh[w * bins + b];
+        if (c) atomicAdd(&hist2d[y * bins + b], c);
+    }
+}
+
+// fallback for very many bins: global atomics
+template <typename LD>
+__global__ void k_hist_global(LD in, Geom g, int bins, float mn_arg, float mx_arg, const float* mm, uint32_t* hist2d) {
+    const float mn = mm ? mm[0] : mn_arg;
+    const float mx = mm ? mm[1] : mx_arg;
+    const float range = mx - mn;
+    const int64_t n = g.n();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t y = (i / g.W) % g.H;
+        atomicAdd(&hist2d[y * bins + bin_of(in(i), mn, range, bins)], 1u);
+    }
+}
+
+// total[b] = sum over y (in y order, float32) of float(hist2d[y][b])
+__global__ void k_hist_rowsum(const uint32_t* hist2d, int64_t H, int bins, DBuf out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= bins) return;
+    float acc = 0.f;
+    for (int64_t y = 0; y < H; ++y) acc += (float)hist2d[y * bins + b];
+    st(out, b, acc);
+}
+
+template <typename LD>
+static int hist_launch(LD in, const Geom& g, int bins, float mn, float mx, const float* mm, uint32_t* h2,
+                       hipStream_t s) {
+    if (bins <= HIST_LDS_MAX_BINS) {
+        const int64_t per_block = 16384;
+        int zc = (int)(per_block / (g.W > 0 ? g.W : 1));
+        if (zc < 1) zc = 1;
+        if (zc > g.D) zc = (int)g.D;
+        const int zchunks = (int)((g.D + zc - 1) / zc);
+        const size_t lds = (size_t)bins * HWAVES * sizeof(uint32_t);
+        k_hist_rows<<<(unsigned)(g.H * zchunks), HB, lds, s>>>(in, g, bins, zc, zchunks, mn, mx, mm, h2);
+    } else {
+        unsigned gr = grid_for((size_t)g.n(), 256);
+        if (gr > 65536) gr = 65536;
+        k_hist_global<<<gr, 256, 0, s>>>(in, g, bins, mn, mx, mm, h2);
+    }
+    CLE_LAUNCHED();
+    return CLE_OK;
+}
+
+static int histogram_impl(DBuf src, const Geom& g, int bins, float mn, float mx, const float* mm, DBuf out,
+                          hipStream_t s) {
+    Scratch h2;
+    const size_t cells = (size_t)g.H * bins;
+    CLE_TRY(h2.alloc(cells * sizeof(uint32_t), s));
+    CLE_TRY(fill_u32(h2.get<uint32_t>(), 0u, cells, s));
+    if (g.n() > 0) {
+        if (src.dt == CLE_F32) CLE_TRY(hist_launch(LdF32R{(const float*)src.p}, g, bins, mn, mx, mm, h2.get<uint32_t>(), s));
+        else CLE_TRY(hist_launch(LdAnyR{src}, g, bins, mn, mx, mm, h2.get<uint32_t>(), s));
+    }
+    k_hist_rowsum<<<grid_for(bins, 256), 256, 0, s>>>(h2.get<uint32_t>(), g.H, bins, out);
+    CLE_LAUNCHED();
+    return CLE_OK;
+}
+
+// ----------------------------------------------------------------- Otsu
+// Single block.  Thread 0 runs the cumulative sums in the exact order numpy's
+// cumsum uses (sequential, float64, no contraction); the 255 between-class variances
+// are then formed in parallel and the first maximum (NaN counting as maximum, as
+// numpy.argmax does) is selected.
+__global__ __launch_bounds__(256) void k_otsu(const float* hist, const float* mm, double* thr) {
+#pragma clang fp contract(off)
+    __shared__ double bc[256], w1[256], w2[256], m1[256], m2[256];
+    __shared__ double var[256];
+    const int t = threadIdx.x;
+    const float mnf = mm[0], mxf = mm[1];
+    const float rng = mxf - mnf;  // float32 subtraction, as in the reference
+    bc[t] = (double)t * (double)rng / 255.0 + (double)mnf;
+    __syncthreads();
+    if (t == 0) {
+        double a = 0, b = 0;
+        for (int i = 0; i < 256; ++i) {
+            const double c = (double)hist[i];
+            a = a + c;
+            b = b + c * bc[i];
+            w1[i] = a;
+            m1[i] = b / a;
+        }
+    } else if (t == 64) {
+        double a = 0, b = 0;
+        for (int i = 255; i >= 0; --i) {
+            const double c = (double)hist[i];
+            a = a + c;
+            b = b + c * bc[i];
+            w2[i] = a;
+            m2[i] = b / a;
+        }
+    }
+    __syncthreads();
+    if (t < 255) {
+        const double d = m1[t] - m2[t + 1];
+        var[t] = (w1[t] * w2[t + 1]) * (d * d);
+    }
+    __syncthreads();
+    if (t == 0) {
+        int best = 0;
+        double bv = var[0];
+        if (bv == bv) {
+            for (int i = 1; i < 255; ++i) {
+                const double v = var[i];
+                if (v != v) { best = i; break; }
+                if (v > bv) { bv = v; best = i; }
+            }
+        }
+        thr[0] = bc[best];
+    }
+}
+
+// dst = src > float32(threshold)
+__global__ void k_threshold(DBuf s, DBuf d, const double* thr, int64_t n) {
+    const float t = (float)thr[0];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        st(d, i, ld(s, i) > t ? 1.f : 0.f);
+}
+
+int otsu_impl(DBuf src, const Geom& g, float* minmax_dev, double* thr_dev, hipStream_t s) {
+    CLE_TRY(minmax_impl(src, g.n(), minmax_dev, s));
+    Scratch hist;
+    CLE_TRY(hist.alloc(256 * sizeof(float), s));
+    CLE_TRY(histogram_impl(src, g, 256, 0.f, 0.f, minmax_dev, DBuf{hist.p, CLE_F32}, s));
+    k_otsu<<<1, 256, 0, s>>>(hist.get<float>(), minmax_dev, thr_dev);
+    CLE_LAUNCHED();
+    return CLE_OK;
+}
+
+}  // namespace cle
+
+using namespace cle;
+
+extern "C" {
+
+int cle_minimum_maximum(const cle_buf* src, float* out_minmax, void* stream) {
+    Geom g;
+    CLE_TRY(check_buf(src, "source", &g));
+    if (!out_minmax) return fail(CLE_E_VALUE, "null output");
+    return minmax_impl(dbuf(src), g.n(), out_minmax, (hipStream_t)stream);
+}
+
+int cle_sum_of_all_pixels(const cle_buf* src, double* out_sum, void* stream) {
+    Geom g;
+    CLE_TRY(check_buf(src, "source", &g));
+    hipStream_t s = (hipStream_t)stream;
+    CLE_TRY(fill_f64(out_sum, 0.0, 1, s));
+    if (!g.n()) return CLE_OK;
+    unsigned gr = grid_for((size_t)g.n(), RB * 8);
+    if (gr > 2048) gr = 2048;
+    k_sum<<<gr, RB, 0, s>>>(dbuf(src), g.n(), out_sum);
+    CLE_LAUNCHED();
+    return CLE_OK;
+}
+
+int cle_histogram(const cle_buf* src, const cle_buf* dst, int num_bins, float mn, float mx,
+                  const float* minmax_dev, void* stream) {
+    Geom g, gd;
+    CLE_TRY(check_buf(src, "source", &g));
+    CLE_TRY(check_buf(dst, "destination", &gd));
+    if (num_bins < 1) return fail(CLE_E_VALUE, "num_bins must be positive");
+    if (gd.n() < num_bins) return fail(CLE_E_SHAPE, "destination holds %lld < %d bins", (long long)gd.n(), num_bins);
+    return histogram_impl(dbuf(src), g, num_bins, mn, mx, minmax_dev, dbuf(dst), (hipStream_t)stream);
+}
+
+int cle_threshold_otsu(const cle_buf* src, const cle_buf* dst, double* out_threshold, void* stream) {
+    Geom g, gd;
+    CLE_TRY(check_buf(src, "source", &g));
+    CLE_TRY(check_buf(dst, "destination", &gd));
+    CLE_TRY(check_same_shape(g, gd, "source", "destination"));
+    hipStream_t s = (hipStream_t)stream;
+    if (!g.n()) return CLE_OK;
+    Scratch mm, thr;
+    CLE_TRY(mm.alloc(2 * sizeof(float), s));
+    double* t = out_threshold;
+    if (!t) {
+        CLE_TRY(thr.alloc(sizeof(double), s));
+        t = thr.get<double>();
+    }
+    CLE_TRY(otsu_impl(dbuf(src), g, mm.get<float>(), t, s));
+    unsigned gr = grid_for((size_t)g.n(), 256);
+    if (gr > 131072) gr = 131072;
+    k_threshold<<<gr, 256, 0, s>>>(dbuf(src), dbuf(dst), t, g.n());
+    CLE_LAUNCHED();
+    return CLE_OK;
+}
+
+}  // extern "C"

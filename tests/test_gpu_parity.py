@@ -1,0 +1,342 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Integer / binary / label outputs must be bit-exact; float32 blur within max-abs
+1e-4 relative to the image maximum; float statistics within 1e-5 relative.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_npz
+from oracle import cle_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BLUR_TOL = 1e-4    # max |gpu - oracle| <= BLUR_TOL * max(1, max |oracle|)
+STATS_RTOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def cle():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import pyclesperanto_prototype_amd as cle
+    cle._load_library()
+    return cle
+
+
+def assert_blur_close(got, want):
+    got, want = np.asarray(got, np.float64), np.asarray(want, np.float64)
+    assert got.shape == want.shape
+    err = np.abs(got - want).max() if got.size else 0.0
+    assert err <= BLUR_TOL * max(1.0, np.abs(want).max()), f"max abs err {err}"
+
+
+def blobs_image(shape, n, seed, sig=(3, 6), amp=(100, 200), noise=5.0, dtype=np.uint8):
+    rng = np.random.default_rng(seed)
+    grids = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+    img = np.full(shape, 10.0)
+    for _ in range(n):
+        c = [rng.uniform(0, s) for s in shape]
+        s = rng.uniform(*sig)
+        r2 = sum((g - ci) ** 2 for g, ci in zip(grids, c))
+        img += rng.uniform(*amp) * np.exp(-r2 / (2 * s * s))
+    img += rng.normal(0, noise, shape)
+    hi = 255 if dtype == np.uint8 else 65535
+    return np.clip(img, 0, hi).astype(dtype)
+
+
+# ----------------------------------------------------------------- reference test vectors
+def test_reference_vectors(cle, ref_tests):
+    A = np.asarray
+    t = ref_tests["detect_maxima_box"]
+    out = cle.create_like(cle.push(A(t["input"])))
+    cle.detect_maxima_box(A(t["input"]), out)
+    assert np.array_equal(cle.pull(out), A(t["expected"]))
+    for k in ("histogram_2d", "histogram_3d", "histogram_3d_2"):
+        t = ref_tests[k]
+        assert np.allclose(cle.pull(cle.histogram(A(t["input"]), num_bins=t["num_bins"])), t["expected"]), k
+    t = ref_tests["greater_constant"]
+    r = cle.create(cle.push(A(t["input"])))
+    cle.greater_constant(A(t["input"]), r, t["constant"])
+    assert np.array_equal(cle.pull(r), A(t["expected"]))
+    t = ref_tests["binary_and"]
+    assert np.array_equal(cle.pull(cle.binary_and(A(t["a"]), A(t["b"]))), A(t["expected"]))
+    t = ref_tests["mask"]
+    r = cle.create(cle.push(A(t["input"])))
+    cle.mask(A(t["input"]), A(t["mask"]), r)
+    assert np.array_equal(cle.pull(r), A(t["expected"]))
+    t = ref_tests["set_nonzero_pixels_to_pixel_index"]
+    assert np.array_equal(cle.pull(cle.set_non_zero_pixels_to_pixel_index(A(t["input"]))), A(t["expected"]))
+    for kind in ("box", "diamond"):
+        t = ref_tests["nonzero_minimum_" + kind]
+        src = cle.push(A(t["input"]))
+        res = cle.create(src)
+        cle.set(res, 0)
+        flag = cle.create((1, 1, 1))
+        cle.set(flag, 0)
+        getattr(cle, "nonzero_minimum_" + kind)(src, flag, res)
+        assert np.allclose(cle.pull(res), A(t["expected"]), atol=1e-5)
+        assert cle.pull(flag).ravel()[0] == 1
+        t = ref_tests["onlyzero_overwrite_maximum_" + kind]
+        flag = cle.create((1, 1, 1))
+        cle.set(flag, 0)
+        res = getattr(cle, "onlyzero_overwrite_maximum_" + kind)(A(t["input"]), flag)
+        assert np.array_equal(cle.pull(res), A(t["expected"]))
+        t = ref_tests["connected_components_labeling_" + kind]
+        out = getattr(cle, "connected_components_labeling_" + kind)(A(t["input"]))
+        assert out.dtype == np.uint32 and np.array_equal(cle.pull(out), A(t["expected"]))
+    t = ref_tests["relabel_sequential"]
+    out = cle.create_like(cle.push(A(t["input"])))
+    cle.relabel_sequential(A(t["input"]), out)
+    assert np.array_equal(cle.pull(out), A(t["expected"]))
+    for k in ("label_spots_2d", "label_spots_3d"):
+        t = ref_tests[k]
+        assert np.array_equal(cle.pull(cle.label_spots(A(t["input"]))), A(t["expected"])), k
+    t = ref_tests["masked_voronoi_labeling"]
+    assert np.array_equal(cle.pull(cle.masked_voronoi_labeling(A(t["input"]), A(t["mask"]))), A(t["expected"]))
+    t = ref_tests["voronoi_labeling"]
+    assert np.array_equal(cle.pull(cle.voronoi_labeling(A(t["input"]))), A(t["expected"]))
+    t = ref_tests["voronoi_otsu_labeling"]
+    out = cle.voronoi_otsu_labeling(A(t["input"]), spot_sigma=t["spot_sigma"], outline_sigma=t["outline_sigma"])
+    assert out.dtype == np.uint32 and np.array_equal(cle.pull(out), A(t["expected"]))
+    t = ref_tests["gauss_otsu_labeling"]
+    out = cle.gauss_otsu_labeling(A(t["input"]), outline_sigma=t["outline_sigma"])
+    assert np.array_equal(cle.pull(out), A(t["expected"]))
+
+
+def test_blur_reference_bounds(cle, ref_tests):
+    t = ref_tests["gaussian_blur_delta"]
+    src = cle.push(np.asarray(t["input"]))
+    res = cle.create(src)
+    cle.gaussian_blur(src, res, *t["sigma"])
+    a = cle.pull(res)
+    assert a.min() > t["min_gt"] and t["max_gt"] < a.max() < t["max_lt"]
+    t = ref_tests["difference_of_gaussian"]
+    d = cle.pull(cle.gaussian_blur(src, None, *t["sigma1"])) - cle.pull(cle.gaussian_blur(src, None, *t["sigma2"]))
+    assert t["min_gt"] < d.min() < t["min_lt"] and t["max_gt"] < d.max() < t["max_lt"]
+
+
+def test_statistics_reference_tables(cle, ref_tests):
+    for key in ("statistics_2d", "statistics_3d"):
+        t = ref_tests[key]
+        r = cle.statistics_of_labelled_pixels(np.asarray(t["intensity"]), np.asarray(t["labels"]))
+        assert list(r.keys()) == O.STAT_KEYS
+        for k, v in t["expected"].items():
+            assert np.allclose(r[k], v, t["rtol"]), (key, k)
+    t = ref_tests["statistics_std"]
+    r = cle.statistics_of_labelled_pixels(np.asarray(t["intensity"]), np.asarray(t["labels"]))
+    np.testing.assert_almost_equal(r["standard_deviation_intensity"][0], t["std"], decimal=t["decimal"])
+    t = ref_tests["statistics_clij_csv"]
+    fx = load_npz("mini_blobs.npz")
+    r = cle.statistics_of_background_and_labelled_pixels(fx["image"], fx["labels"])
+    for i, k in enumerate(t["columns"]):
+        assert np.allclose(fx["clij_table"][:, i], r[k], t["rtol"]), k
+
+
+def test_known_answers(cle, ref_tests):
+    img = load_npz("blobs.npz")["image"]
+    blurred = cle.gaussian_blur(img, sigma_x=1, sigma_y=1)
+    binary = cle.threshold_otsu(blurred)
+    labels = cle.connected_components_labeling_box(binary)
+    assert cle.maximum_of_all_pixels(labels) == 62
+    t = ref_tests["bbbc022_spots"]
+    crop = load_npz("bbbc022_crop.npz")["image"]
+    s = t["spot_sigma"]
+    spots = cle.detect_maxima_box(cle.gaussian_blur(crop, sigma_x=s, sigma_y=s, sigma_z=s))
+    assert cle.sum_of_all_pixels(spots) == t["detected_spots"]
+    o = t["outline_sigma"]
+    binary = cle.threshold_otsu(cle.gaussian_blur(crop, sigma_x=o, sigma_y=o, sigma_z=o))
+    assert cle.sum_of_all_pixels(cle.binary_and(binary, spots)) == t["selected_spots"]
+
+
+# ----------------------------------------------------------------- seeded parity vs the oracle
+@pytest.mark.parametrize("shape", [(1, 1), (1, 7), (5, 1), (13, 17), (64, 200), (3, 4, 5), (9, 33, 70), (40, 70, 90)])
+@pytest.mark.parametrize("sig", [0.0, 0.5, 1.0, 1.5, 2.0, 3.5, 5.0])
+def test_blur_parity(cle, shape, sig):
+    rng = np.random.default_rng(hash((shape, sig)) % 2**32)
+    a = (rng.random(shape) * 1000).astype(np.float32)
+    sz = sig * 0.7
+    got = cle.pull(cle.gaussian_blur(a, None, sig, sig * 1.3, sz))
+    assert_blur_close(got, O.gaussian_blur(a, sig, sig * 1.3, sz))
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32])
+def test_blur_typed_source(cle, dtype):
+    rng = np.random.default_rng(3)
+    a = (rng.random((6, 40, 50)) * 200).astype(dtype)
+    import torch
+    t = torch.from_numpy(a.astype(np.int32)).cuda().to(dtype={np.uint8: torch.uint8, np.uint16: torch.uint16,
+                                                                np.float32: torch.float32}[dtype])
+    src = cle.asarray(t)
+    assert src.dtype == np.dtype(dtype)
+    dst = cle.create(src.shape, np.float32)
+    cle.gaussian_blur(src, dst, 2, 2, 2)
+    assert_blur_close(cle.pull(dst), O.gaussian_blur(a.astype(np.float32), 2, 2, 2))
+
+
+def test_mean_box_parity(cle):
+    rng = np.random.default_rng(4)
+    a = (rng.random((7, 20, 30)) * 100).astype(np.float32)
+    got = cle.pull(cle.mean_box(a, None, 1, 2, 1))
+    assert_blur_close(got, O.mean_box(a, 1, 2, 1))
+
+
+@pytest.mark.parametrize("shape", [(31, 45), (6, 17, 23), (1, 9, 9)])
+def test_maxima_parity(cle, shape):
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 5, shape).astype(np.float32)  # plateaus and ties
+    assert np.array_equal(cle.pull(cle.detect_maxima_box(a)), O.detect_maxima_box(a))
+    b = (rng.random(shape) * 10).astype(np.float32)
+    assert np.array_equal(cle.pull(cle.detect_maxima_box(b, None, 1, 1, 1)), O.detect_maxima_box(b, 1, 1, 1))
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (50, 60), (8, 33, 47)])
+def test_histogram_otsu_parity(cle, shape):
+    img = blobs_image(shape, 6, 7).astype(np.float32)
+    blurred = O.gaussian_blur(img, 1, 1, 1)
+    d = cle.push(blurred)
+    assert cle.minimum_of_all_pixels(d) == blurred.min()
+    assert cle.maximum_of_all_pixels(d) == blurred.max()
+    assert np.array_equal(cle.pull(cle.histogram(d, num_bins=256)), O.histogram(blurred, 256))
+    assert np.array_equal(cle.pull(cle.histogram(d, num_bins=17)), O.histogram(blurred, 17))
+    out, t = O.threshold_otsu(blurred, return_threshold=True)
+    assert cle.otsu_threshold(d) == t
+    assert np.array_equal(cle.pull(cle.threshold_otsu(d)), out)
+
+
+def test_otsu_constant_image(cle):
+    a = np.full((5, 6), 3.0, np.float32)
+    assert cle.otsu_threshold(a) == 3.0
+    assert cle.pull(cle.threshold_otsu(a)).sum() == 0
+
+
+def test_elementwise_dtypes(cle):
+    rng = np.random.default_rng(8)
+    a = (rng.random((9, 10)) * 600 - 100).astype(np.float32)
+    b = (rng.random((9, 10)) > 0.5).astype(np.float32)
+    for dt in (np.uint8, np.uint16, np.int32, np.uint32, np.float32):
+        d = cle.create(a.shape, dt)
+        cle.add_image_and_scalar(a, d, 0.5)
+        assert np.array_equal(cle.pull(d), O.add_image_and_scalar(a, 0.5, dt)), dt
+        cle.mask(a, b, d)
+        assert np.array_equal(cle.pull(d), O.mask(a, b, dt)), dt
+    assert np.array_equal(cle.pull(cle.add_images_weighted(a, b, None, 2, -3)), O.add_images_weighted(a, b, 2, -3))
+    assert np.array_equal(cle.pull(cle.greater_constant(a, None, 42.5)), O.greater_constant(a, 42.5))
+
+
+@pytest.mark.parametrize("shape,kind,p", [((64, 64), "box", 0.45), ((120, 97), "diamond", 0.5),
+                                          ((10, 30, 31), "box", 0.3), ((12, 20, 25), "diamond", 0.45),
+                                          ((1, 40, 40), "box", 0.4)])
+def test_ccl_parity(cle, shape, kind, p):
+    rng = np.random.default_rng(9)
+    b = (rng.random(shape) < p).astype(np.uint8)
+    fn = cle.connected_components_labeling_box if kind == "box" else cle.connected_components_labeling_diamond
+    got = cle.pull(fn(b))
+    assert np.array_equal(got, O.connected_components_labeling(b, kind))
+    # into a float32 destination, as the reference's masked Voronoi uses it
+    d = cle.create(b.shape, np.float32)
+    fn(b, d)
+    assert np.array_equal(cle.pull(d), got.astype(np.float32))
+    # determinism
+    assert np.array_equal(cle.pull(fn(b)), got)
+
+
+def test_ccl_int32_destination(cle):
+    rng = np.random.default_rng(10)
+    b = (rng.random((30, 40)) < 0.4).astype(np.uint8)
+    d = cle.create(b.shape, np.int32)
+    cle.connected_components_labeling_box(b, d)
+    assert np.array_equal(cle.pull(d), O.connected_components_labeling(b).astype(np.int32))
+
+
+def _ccl_expected_fp32_keys(b):
+    """Composite ID rule: scipy components, min x-major key per component, rounded to
+    float32, dense rank of the rounded keys (equal keys share a label)."""
+    import scipy.ndimage as ndi
+    sp, n = ndi.label(b != 0, structure=np.ones((3,) * b.ndim))
+    keys = O.pixel_keys(b)
+    mins = np.asarray(ndi.minimum(keys, sp, index=np.arange(1, n + 1)), np.int64)
+    rounded = mins.astype(np.float32).astype(np.int64)
+    uniq = np.unique(rounded)
+    lut = np.zeros(n + 1, np.int64)
+    lut[1:] = np.searchsorted(uniq, rounded) + 1
+    return lut[sp].astype(np.uint32), n, len(uniq)
+
+
+def test_ccl_fp32_key_merge_above_2_24(cle):
+    # 8192 x 4097 pixels: keys x*H + y + 1 reach 2^25 + 8192 > 2^24.  Two single-pixel
+    # components at x = 4096, y = 1002 and 1004 have keys 2^25 + 1003 and 2^25 + 1005,
+    # which both round to the float32 2^25 + 1004: the reference gives them one label.
+    H, W = 8192, 4097
+    b = np.zeros((H, W), np.uint8)
+    b[1002, W - 1] = 1
+    b[1004, W - 1] = 1
+    b[10:13, 5:8] = 1
+    k1, k2 = (W - 1) * H + 1002 + 1, (W - 1) * H + 1004 + 1
+    assert k1 > 2 ** 24 and np.float32(k1) == np.float32(k2) and k1 != k2
+    want, n, n_labels = _ccl_expected_fp32_keys(b)
+    assert n == 3 and n_labels == 2
+    got = cle.pull(cle.connected_components_labeling_box(b))
+    assert np.array_equal(got, want)
+    cle.set_fp32_key_compat(False)
+    try:
+        exact = cle.pull(cle.connected_components_labeling_box(b))
+    finally:
+        cle.set_fp32_key_compat(True)
+    assert exact.max() == 3
+
+
+def test_relabel_and_spots_parity(cle):
+    rng = np.random.default_rng(11)
+    lab = rng.choice([0, 3, 7, 8, 20, 21, 1000], size=(15, 16, 17)).astype(np.float32)
+    assert np.array_equal(cle.pull(cle.relabel_sequential(lab)), O.relabel_sequential(lab))
+    spots = (rng.random((6, 13, 14)) < 0.05).astype(np.float32)
+    assert np.array_equal(cle.pull(cle.label_spots(spots)), O.label_spots(spots))
+
+
+@pytest.mark.parametrize("shape", [(50, 70), (12, 40, 45)])
+def test_masked_voronoi_parity(cle, shape):
+    rng = np.random.default_rng(12)
+    seeds = (rng.random(shape) < 0.01).astype(np.uint8)
+    m = (rng.random(shape) < 0.8).astype(np.uint8)
+    got = cle.pull(cle.masked_voronoi_labeling(seeds, m))
+    assert np.array_equal(got, O.masked_voronoi_labeling(seeds, m))
+
+
+@pytest.mark.parametrize("shape,ss,so", [((128, 128), 3.5, 1.0), ((200, 180), 2.0, 1.5), ((24, 80, 90), 2.0, 1.0),
+                                         ((9, 64, 64), 1.0, 0.5)])
+def test_voronoi_otsu_parity(cle, shape, ss, so):
+    img = blobs_image(shape, 15 if len(shape) == 2 else 20, 13)
+    got = cle.pull(cle.voronoi_otsu_labeling(img, spot_sigma=ss, outline_sigma=so))
+    assert np.array_equal(got, O.voronoi_otsu_labeling(img, ss, so))
+
+
+def test_voronoi_otsu_fixtures(cle):
+    for name, ss, so in (("blobs.npz", 2.0, 1.0), ("bbbc022_crop.npz", 5.0, 1.0)):
+        img = load_npz(name)["image"]
+        got = cle.pull(cle.voronoi_otsu_labeling(img, spot_sigma=ss, outline_sigma=so))
+        assert np.array_equal(got, O.voronoi_otsu_labeling(img, ss, so)), name
+
+
+def _assert_stats_close(got, want):
+    assert list(got.keys()) == list(want.keys())
+    for k in want:
+        g, w = np.asarray(got[k], np.float64), np.asarray(want[k], np.float64)
+        assert g.shape == w.shape, k
+        np.testing.assert_allclose(g, w, rtol=STATS_RTOL, atol=0, equal_nan=True, err_msg=k)
+
+
+@pytest.mark.parametrize("shape", [(60, 70), (8, 30, 40)])
+def test_statistics_parity(cle, shape):
+    img = blobs_image(shape, 8, 14).astype(np.float32)
+    labels = O.voronoi_otsu_labeling(img, 2.0, 1.0)
+    _assert_stats_close(cle.statistics_of_labelled_pixels(img, labels), O.statistics_of_labelled_pixels(img, labels))
+
+
+def test_statistics_label_gaps(cle):
+    lab = np.zeros((10, 12), np.uint32)
+    lab[2:4, 3:6] = 2
+    lab[7, 1:9] = 5
+    img = np.arange(lab.size, dtype=np.float32).reshape(lab.shape)
+    _assert_stats_close(cle.statistics_of_labelled_pixels(img, lab), O.statistics_of_labelled_pixels(img, lab))
