@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-slices", type=int, default=16, help="depth of the slab timed on the CPU")
+    ap.add_argument("--cpu-slices", type=int, default=256, help="depth of the slab timed on the CPU")
     return ap.parse_args()
 
 
@@ -145,11 +145,10 @@ def cpu_baseline(vol_t, slices):
     row histogram + Otsu, Jacobi CCL, per-row statistics) on the first `slices`
     slices of the volume, on this host's cores."""
     import numpy as np
+    import torch
     from oracle import c_oracle
 
-    sub = vol_t[:slices].contiguous().view(vol_t.dtype if vol_t.dtype != __import__("torch").uint16
-                                           else __import__("torch").int16).cpu().numpy()
-    sub = sub.view(np.uint16).astype(np.float32)
+    sub = vol_t[:slices].contiguous().view(torch.int16).cpu().numpy().view(np.uint16).astype(np.float32)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     c_oracle.lib()
     t0 = time.perf_counter()

@@ -97,7 +97,7 @@ def statistics_table(img, labels):
 def blur_otsu_ccl_stats(a, sigma, with_stats=True):
     a = np.ascontiguousarray(a, np.float32)
     labels = np.empty(a.shape, np.int32)
-    cap = 36 * (a.size // 2 + 2) if with_stats else 0
+    cap = 36 * 200000 if with_stats else 0  # up to 200k labels
     table = np.zeros(max(cap, 1), np.float32)
     L = lib().ora_blur_otsu_ccl_stats(a, *_dims(a), a.ndim, sigma, labels, table if with_stats else table, cap)
     return labels, (table[: 36 * (L - 1)].reshape(36, L - 1) if with_stats else None)
