@@ -155,6 +155,123 @@ __global__ __launch_bounds__(256) void k_sep_direct(LD in, ST out, int64_t W, in
     out(base + a * sa, acc);
 }
 
+// ----------------------------------------------------------------- fused 3D pass
+// One launch for the whole separable blur of a float32 / uint16 / uint8 volume into
+// float32.  A block owns a FB_TY x FB_TX column tile and a range of output slices.
+// It streams the input slices through LDS: each slice tile (with its x and y halo,
+// clamped reads) is blurred along x (LDS -> LDS) and then along y (LDS -> registers);
+// every thread keeps the last 2*CZ+1 xy-blurred values of its two columns in a
+// register ring and emits the z-blurred output slice as soon as the ring is full.
+// The next slice's global loads are issued before the current slice is computed.
+// HBM traffic: each input voxel read once per z-chunk (+ halo), each output written
+// once -- 6 B/voxel for uint16 -> float32.
+constexpr int FB_TX = 64, FB_TY = 32, FB_THREADS = 1024;
+constexpr int FB_MAXC = 24;
+constexpr int FB_MAXPF = ((FB_TY + 2 * FB_MAXC) * (FB_TX + 2 * FB_MAXC) + FB_THREADS - 1) / FB_THREADS;
+
+struct FusedW {
+    float wx[FB_MAXC + 1], wy[FB_MAXC + 1], wz[FB_MAXC + 1];
+};
+
+// consecutive tiles (sharing halos) go to the same XCD: hardware dispatches block b
+// to XCD b % 8, so logical tile = (b % 8) * per_xcd + b / 8
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
+    const int64_t per = nb / 8, rem = nb % 8;
+    const int64_t xcd = b % 8, k = b / 8;
+    // the first `rem` XCDs own one extra tile
+    const int64_t start = xcd * per + (xcd < rem ? xcd : rem);
+    return start + k;
+}
+
+template <int CZ, typename LD>
+__global__ __launch_bounds__(FB_THREADS) void k_blur_fused(LD in, float* __restrict__ out, Geom g, int cx, int cy,
+                                                          int zc, int ntx, int nty, const FusedW w) {
+    extern __shared__ float sm[];
+    const int AW = FB_TX + 2 * cx, AH = FB_TY + 2 * cy;
+    float* A = sm;
+    float* B = sm + AH * AW;
+    const int64_t nb = (int64_t)gridDim.x;
+    const int64_t bid = xcd_tile(blockIdx.x, nb);
+    const int64_t tx = bid % ntx, rest = bid / ntx;
+    const int64_t ty = rest % nty, tzc = rest / nty;
+    const int64_t x0 = tx * FB_TX, y0 = ty * FB_TY;
+    const int64_t zb = tzc * zc;
+    const int64_t ze = zb + zc < g.D ? zb + zc : g.D;
+    const int row = threadIdx.x / (FB_TX / 2);
+    const int col = (threadIdx.x % (FB_TX / 2)) * 2;
+    const int na = AH * AW;
+    float pf[FB_MAXPF];
+    float r0[2 * CZ + 1], r1[2 * CZ + 1];
+#pragma unroll
+    for (int j = 0; j < 2 * CZ + 1; ++j) { r0[j] = 0.f; r1[j] = 0.f; }
+
+    // in-slice offsets of this thread's staged values (a slice holds < 2^31 voxels)
+    const int Hm = (int)g.H - 1, Wm = (int)g.W - 1, Wi = (int)g.W;
+    int off[FB_MAXPF];
+#pragma unroll
+    for (int k = 0; k < FB_MAXPF; ++k) {
+        const int idx = threadIdx.x + k * FB_THREADS;
+        const int ay = idx / AW, ax = idx - ay * AW;
+        int gy = (int)y0 - cy + ay, gx = (int)x0 - cx + ax;
+        gy = gy < 0 ? 0 : (gy > Hm ? Hm : gy);
+        gx = gx < 0 ? 0 : (gx > Wm ? Wm : gx);
+        off[k] = gy * Wi + gx;
+    }
+    auto load_slice = [&](int64_t zi) {
+        const int64_t zz = zi < 0 ? 0 : (zi >= g.D ? g.D - 1 : zi);
+        const int64_t base = zz * g.H * g.W;
+#pragma unroll
+        for (int k = 0; k < FB_MAXPF; ++k)
+            if (threadIdx.x + k * FB_THREADS < na) pf[k] = in(base + off[k]);
+    };
+
+    load_slice(zb - CZ);
+    for (int64_t zi = zb - CZ; zi < ze + CZ; ++zi) {
+#pragma unroll
+        for (int k = 0; k < FB_MAXPF; ++k) {
+            const int idx = threadIdx.x + k * FB_THREADS;
+            if (idx < na) A[idx] = pf[k];
+        }
+        __syncthreads();
+        if (zi + 1 < ze + CZ) load_slice(zi + 1);
+        // x pass: AH rows x FB_TX columns
+        for (int idx = threadIdx.x; idx < AH * FB_TX; idx += FB_THREADS) {
+            const int ay = idx / FB_TX, bx = idx - ay * FB_TX;
+            const float* a = A + ay * AW + bx + cx;
+            float acc = w.wx[0] * a[0];
+            for (int v = 1; v <= cx; ++v) acc = fmaf(w.wx[v], a[-v] + a[v], acc);
+            B[idx] = acc;
+        }
+        __syncthreads();
+        // y pass for this thread's two columns
+        const float* b = B + (row + cy) * FB_TX + col;
+        float v0 = w.wy[0] * b[0], v1 = w.wy[0] * b[1];
+        for (int v = 1; v <= cy; ++v) {
+            v0 = fmaf(w.wy[v], b[-v * FB_TX] + b[v * FB_TX], v0);
+            v1 = fmaf(w.wy[v], b[-v * FB_TX + 1] + b[v * FB_TX + 1], v1);
+        }
+        // z ring
+#pragma unroll
+        for (int j = 0; j < 2 * CZ; ++j) { r0[j] = r0[j + 1]; r1[j] = r1[j + 1]; }
+        r0[2 * CZ] = v0;
+        r1[2 * CZ] = v1;
+        if (zi >= zb + CZ) {
+            float o0 = w.wz[0] * r0[CZ], o1 = w.wz[0] * r1[CZ];
+#pragma unroll
+            for (int k = 1; k <= CZ; ++k) {
+                o0 = fmaf(w.wz[k], r0[CZ - k] + r0[CZ + k], o0);
+                o1 = fmaf(w.wz[k], r1[CZ - k] + r1[CZ + k], o1);
+            }
+            const int64_t zo = zi - CZ, y = y0 + row, x = x0 + col;
+            if (y < g.H) {
+                float* o = out + (zo * g.H + y) * g.W + x;
+                if (x + 1 < g.W) { o[0] = o0; o[1] = o1; }
+                else if (x < g.W) o[0] = o0;
+            }
+        }
+    }
+}
+
 // ----------------------------------------------------------------- host side
 static void gauss_taps(float sigma, Taps* t) {
     int n = (int)(sigma * 8.0f);  // _tier0/_sigma_to_kernel_size.py:1-4
@@ -174,6 +291,63 @@ static void gauss_taps(float sigma, Taps* t) {
 static void box_taps(int r, Taps* t) {
     t->c = r;
     for (int v = 0; v <= r; ++v) t->w[v] = 1.0f / (float)(2 * r + 1);
+}
+
+template <int CZ, typename LD>
+static void launch_fused_cz(LD in, float* out, const Geom& g, int cx, int cy, const FusedW& w, hipStream_t s) {
+    const int ntx = (int)((g.W + FB_TX - 1) / FB_TX), nty = (int)((g.H + FB_TY - 1) / FB_TY);
+    const int64_t nxy = (int64_t)ntx * nty;
+    // enough blocks to fill the chip, but z chunks long enough to amortise the z halo
+    int64_t nzc = (2048 + nxy - 1) / nxy;
+    const int64_t max_nzc = g.D / (4 * CZ + 8) > 1 ? g.D / (4 * CZ + 8) : 1;
+    if (nzc > max_nzc) nzc = max_nzc;
+    if (nzc < 1) nzc = 1;
+    const int zc = (int)((g.D + nzc - 1) / nzc);
+    nzc = (g.D + zc - 1) / zc;
+    const size_t lds = (size_t)((FB_TY + 2 * cy) * (FB_TX + 2 * cx) + (FB_TY + 2 * cy) * FB_TX) * sizeof(float);
+    k_blur_fused<CZ, LD><<<(unsigned)(nxy * nzc), FB_THREADS, lds, s>>>(in, out, g, cx, cy, zc, ntx, nty, w);
+}
+
+template <typename LD>
+static int launch_fused(LD in, float* out, const Geom& g, int cx, int cy, int cz, const FusedW& w, hipStream_t s) {
+    switch (cz) {
+#define CZ_CASE(n) case n: launch_fused_cz<n>(in, out, g, cx, cy, w, s); break;
+        CZ_CASE(0) CZ_CASE(1) CZ_CASE(2) CZ_CASE(3) CZ_CASE(4) CZ_CASE(5) CZ_CASE(6) CZ_CASE(7) CZ_CASE(8)
+        CZ_CASE(9) CZ_CASE(10) CZ_CASE(11) CZ_CASE(12) CZ_CASE(13) CZ_CASE(14) CZ_CASE(15) CZ_CASE(16)
+        CZ_CASE(17) CZ_CASE(18) CZ_CASE(19) CZ_CASE(20) CZ_CASE(21) CZ_CASE(22) CZ_CASE(23) CZ_CASE(24)
+#undef CZ_CASE
+        default: return fail(CLE_E_INTERNAL, "fused blur: z half-width %d out of range", cz);
+    }
+    CLE_LAUNCHED();
+    return CLE_OK;
+}
+
+// Fused path when the destination is float32, the source is f32/u16/u8 and every
+// half-width fits the LDS tile; returns 1 if it ran, 0 if the caller must use the
+// per-axis passes.
+static int try_fused(const cle_buf* src, const cle_buf* dst, const Taps* taps[3], int* ran, hipStream_t s) {
+    *ran = 0;
+    if (dst->dtype != CLE_F32 || dst->data == src->data) return CLE_OK;
+    if (src->dtype != CLE_F32 && src->dtype != CLE_U16 && src->dtype != CLE_U8) return CLE_OK;
+    Geom g;
+    CLE_TRY(check_buf(src, "source", &g));
+    int c[3];
+    FusedW w;
+    float* ws[3] = {w.wx, w.wy, w.wz};
+    for (int a = 0; a < 3; ++a) {
+        c[a] = taps[a] ? taps[a]->c : 0;
+        if (c[a] > FB_MAXC) return CLE_OK;
+        for (int v = 0; v <= FB_MAXC; ++v) ws[a][v] = 0.f;
+        if (taps[a]) for (int v = 0; v <= c[a]; ++v) ws[a][v] = taps[a]->w[v];
+        else ws[a][0] = 1.f;
+    }
+    float* out = (float*)dst->data;
+    int rc;
+    if (src->dtype == CLE_F32) rc = launch_fused(LdF32{(const float*)src->data}, out, g, c[0], c[1], c[2], w, s);
+    else if (src->dtype == CLE_U16) rc = launch_fused(LdU16{(const uint16_t*)src->data}, out, g, c[0], c[1], c[2], w, s);
+    else rc = launch_fused(LdU8{(const uint8_t*)src->data}, out, g, c[0], c[1], c[2], w, s);
+    if (rc == CLE_OK) *ran = 1;
+    return rc;
 }
 
 template <typename LD, typename ST>
@@ -246,6 +420,11 @@ static int separable_impl(const cle_buf* src, const cle_buf* dst, const Taps* ta
     for (int a = 0; a < 3; ++a)
         if (taps[a]) axes[npass++] = a;
     DBuf in = dbuf(src), out = dbuf(dst);
+    if (npass > 0 && gs.ndim >= 2) {
+        int ran = 0;
+        CLE_TRY(try_fused(src, dst, taps, &ran, s));
+        if (ran) return CLE_OK;
+    }
     if (npass == 0) {
         unsigned grid = grid_for(gs.n(), 256);
         if (grid > 65536) grid = 65536;

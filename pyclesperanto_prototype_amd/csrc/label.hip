@@ -9,12 +9,13 @@
 //   relabel_sequential  _tier3/_relabel_sequential.py:12-59 (rank among present values)
 //   label_spots         _tier2/_label_spots.py:9-56 + label_spots_in_x.cl (raster rank)
 //
-// MI355X design: no iteration.  Union-find over linear indices (link to the smaller
-// root with atomicMin, Playne & Hawick 2018), then each root takes the minimum
-// x-major key of its component (atomicMin), the keys (optionally rounded to float32,
-// reproducing the reference's merges above 2^24 voxels) are set in a bitmap over
-// the key space, a popcount prefix scan gives every key its rank, and one gather
-// writes the labels.  Everything stays on the device; no host synchronisation.
+// MI355X design: no iteration.  Union-find (link to the smaller root with atomicMin,
+// Playne & Hawick 2018), first inside 2048-voxel tiles in LDS, then across tile faces
+// in global memory; every component's root carries the minimum x-major key of the
+// component, the keys (optionally rounded to float32, reproducing the reference's
+// merges above 2^24 voxels) are set in a bitmap over the key space, a popcount prefix
+// scan gives every key its rank, and one gather writes the labels.  Everything stays
+// on the device; no host synchronisation.
 #include "common.hpp"
 
 namespace cle {
@@ -53,61 +54,224 @@ __device__ __forceinline__ void unite(uint32_t* L, uint32_t a, uint32_t b) {
     } while (!done);
 }
 
-__global__ void k_ccl_init(DBuf bin, uint32_t* L, uint32_t* K, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        L[i] = nz(bin, i) ? (uint32_t)i : NONE;
-        K[i] = NONE;
-    }
+// ----------------------------------------------------------------- tiled union-find
+// Tiles of CT_Z x CT_Y x CT_X voxels (3D) or 1 x 32 x 64 (2D), 2048 voxels each.
+// Pass 1 (k_ccl_local): per tile, union-find in LDS over the tile's foreground
+//   (links to the smallest local index), then L[i] = global index of i's local root
+//   for every foreground voxel, K[local root] = the component's smallest 0-based
+//   x-major key inside the tile, and the tile's local roots go to its root list.
+// Pass 2 (k_ccl_border): foreground voxels on a tile's low faces unite, in global
+//   memory, with their foreground neighbours in the preceding tiles.
+// Pass 3 (k_ccl_resolve): every local root finds its global root r, sets
+//   L[local root] = r and lowers K[r] to its own key.
+// Pass 4 (k_ccl_mark_roots): global roots set their key bit.
+// Pass 5 (k_ccl_write): label = rank(key bit of K[L[L[i]]]).
+constexpr int CT_N = 2048;
+constexpr int CT_THREADS = 256;
+constexpr int CT_PER = CT_N / CT_THREADS;
+
+struct CclTiles {
+    int tz, ty, tx;             // tile extents
+    int64_t nz, ny, nx;         // tile counts
+    __host__ __device__ int64_t count() const { return nz * ny * nx; }
+};
+
+static CclTiles ccl_tiles(const Geom& g) {
+    CclTiles t;
+    if (g.ndim == 3 && g.D > 1) { t.tz = 4; t.ty = 8; t.tx = 64; }
+    else { t.tz = 1; t.ty = 32; t.tx = 64; }
+    t.nz = (g.D + t.tz - 1) / t.tz;
+    t.ny = (g.H + t.ty - 1) / t.ty;
+    t.nx = (g.W + t.tx - 1) / t.tx;
+    return t;
 }
 
-// unite every foreground voxel with its foreground neighbours that come earlier in
-// linear order (13 of the 26 box neighbours in 3D, 4 of 8 in 2D; diamond: 3 / 2)
-__global__ void k_ccl_merge(uint32_t* L, Geom g, int diamond) {
-    const int64_t n = g.n();
+__device__ __forceinline__ uint32_t lds_load(uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ uint32_t lfind(uint32_t* lab, uint32_t a) {
+    uint32_t p = lds_load(lab + a);
+    while (p != a) {
+        a = p;
+        p = lds_load(lab + a);
+    }
+    return a;
+}
+
+__device__ __forceinline__ void lunite(uint32_t* lab, uint32_t a, uint32_t b) {
+    bool done;
+    do {
+        a = lfind(lab, a);
+        b = lfind(lab, b);
+        if (a < b) {
+            const uint32_t old = atomicMin(lab + b, a);
+            done = (old == b);
+            b = old;
+        } else if (b < a) {
+            const uint32_t old = atomicMin(lab + a, b);
+            done = (old == a);
+            a = old;
+        } else {
+            done = true;
+        }
+    } while (!done);
+}
+
+template <typename FG>
+__global__ __launch_bounds__(CT_THREADS) void k_ccl_local(FG fg, Geom g, CclTiles T, int diamond, uint32_t* L,
+                                                         uint32_t* K, uint32_t* roots, uint32_t* nroots) {
+    __shared__ uint32_t lab[CT_N];
+    __shared__ uint32_t kmin[CT_N];
+    __shared__ uint32_t cnt;
+    const int64_t tile = blockIdx.x;
+    const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
+    const int64_t x0 = tx * T.tx, y0 = ty * T.ty, z0 = tz * T.tz;
     const int zr = g.ndim == 3 ? 1 : 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        if (L[i] == NONE) continue;
-        const int64_t x = i % g.W, y = (i / g.W) % g.H, z = i / (g.W * g.H);
+    if (threadIdx.x == 0) cnt = 0;
+    int64_t gi[CT_PER];
+    bool f[CT_PER];
+#pragma unroll
+    for (int k = 0; k < CT_PER; ++k) {
+        const int li = threadIdx.x + k * CT_THREADS;
+        const int lx = li % T.tx, ly = (li / T.tx) % T.ty, lz = li / (T.tx * T.ty);
+        const int64_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
+        const bool in = lz < T.tz && x < g.W && y < g.H && z < g.D;
+        gi[k] = in ? (z * g.H + y) * g.W + x : -1;
+        f[k] = in && fg(gi[k]);
+        lab[li] = f[k] ? (uint32_t)li : NONE;
+        kmin[li] = NONE;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < CT_PER; ++k) {
+        if (!f[k]) continue;
+        const int li = threadIdx.x + k * CT_THREADS;
+        const int lx = li % T.tx, ly = (li / T.tx) % T.ty, lz = li / (T.tx * T.ty);
         for (int dz = -zr; dz <= 0; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx) {
                     if (dz == 0 && (dy > 0 || (dy == 0 && dx >= 0))) continue;
                     if (diamond && (abs(dx) + abs(dy) + abs(dz)) != 1) continue;
+                    const int qx = lx + dx, qy = ly + dy, qz = lz + dz;
+                    if (qx < 0 || qx >= T.tx || qy < 0 || qy >= T.ty || qz < 0) continue;
+                    const int lq = (qz * T.ty + qy) * T.tx + qx;
+                    if (lds_load(lab + lq) == NONE) continue;
+                    lunite(lab, (uint32_t)li, (uint32_t)lq);
+                }
+    }
+    __syncthreads();
+    uint32_t r[CT_PER];
+#pragma unroll
+    for (int k = 0; k < CT_PER; ++k) {
+        const int li = threadIdx.x + k * CT_THREADS;
+        r[k] = f[k] ? lfind(lab, (uint32_t)li) : NONE;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < CT_PER; ++k) {
+        if (!f[k]) continue;
+        const int li = threadIdx.x + k * CT_THREADS;
+        const int lx = li % T.tx, ly = (li / T.tx) % T.ty, lz = li / (T.tx * T.ty);
+        const int64_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
+        atomicMin(kmin + r[k], (uint32_t)(x * g.H * g.D + y * g.D + z));
+        // global index of the local root
+        const int rx = (int)(r[k] % T.tx), ry = (int)((r[k] / T.tx) % T.ty), rz = (int)(r[k] / (T.tx * T.ty));
+        L[gi[k]] = (uint32_t)(((z0 + rz) * g.H + (y0 + ry)) * g.W + (x0 + rx));
+    }
+    __syncthreads();
+    uint32_t* myroots = roots + tile * (CT_N / 2);
+#pragma unroll
+    for (int k = 0; k < CT_PER; ++k) {
+        const int li = threadIdx.x + k * CT_THREADS;
+        if (f[k] && r[k] == (uint32_t)li) {
+            K[gi[k]] = kmin[li];
+            const uint32_t slot = atomicAdd(&cnt, 1u);
+            myroots[slot] = (uint32_t)gi[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) nroots[tile] = cnt;
+}
+
+// foreground voxels on the faces of each tile unite with their foreground neighbours
+// (in the 13 earlier directions) that lie in another tile
+template <typename FG>
+__global__ __launch_bounds__(CT_THREADS) void k_ccl_border(FG fg, Geom g, CclTiles T, int diamond, uint32_t* L,
+                                                          const uint32_t* nroots) {
+    const int64_t tile = blockIdx.x;
+    if (nroots[tile] == 0) return;  // no foreground in this tile
+    const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
+    const int64_t x0 = tx * T.tx, y0 = ty * T.ty, z0 = tz * T.tz;
+    const int zr = g.ndim == 3 ? 1 : 0;
+    for (int li = threadIdx.x; li < CT_N; li += CT_THREADS) {
+        const int lx = li % T.tx, ly = (li / T.tx) % T.ty, lz = li / (T.tx * T.ty);
+        if (lz >= T.tz) continue;
+        // pairs with a neighbour in another tile: low z/y/x faces, and the high y/x
+        // faces (their dz = -1 / dy = -1 neighbours reach into the next y / x tile)
+        if (!((zr && lz == 0) || ly == 0 || lx == 0 || ly == T.ty - 1 || lx == T.tx - 1)) continue;
+        const int64_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
+        if (x >= g.W || y >= g.H || z >= g.D) continue;
+        const int64_t i = (z * g.H + y) * g.W + x;
+        if (!fg(i)) continue;
+        for (int dz = -zr; dz <= 0; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    if (dz == 0 && (dy > 0 || (dy == 0 && dx >= 0))) continue;
+                    if (diamond && (abs(dx) + abs(dy) + abs(dz)) != 1) continue;
+                    // only neighbours outside this tile
+                    if (lz + dz >= 0 && ly + dy >= 0 && lx + dx >= 0 && lx + dx < T.tx && ly + dy < T.ty) continue;
                     const int64_t qx = x + dx, qy = y + dy, qz = z + dz;
                     if (qx < 0 || qx >= g.W || qy < 0 || qy >= g.H || qz < 0) continue;
                     const int64_t q = (qz * g.H + qy) * g.W + qx;
-                    if (ld_relaxed(L + q) == NONE) continue;
-                    unite(L, (uint32_t)i, (uint32_t)q);
+                    if (!fg(q)) continue;
+                    unite(L, ld_relaxed(L + i), ld_relaxed(L + q));
                 }
     }
 }
 
-// path compression + minimum x-major key per root
-__global__ void k_ccl_roots(uint32_t* L, uint32_t* K, Geom g) {
-    const int64_t n = g.n();
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        if (L[i] == NONE) continue;
-        const uint32_t r = find_root(L, (uint32_t)i);
-        const int64_t x = i % g.W, y = (i / g.W) % g.H, z = i / (g.W * g.H);
-        const uint32_t key = (uint32_t)(x * g.H * g.D + y * g.D + z);  // 0-based
-        L[i] = r;
-        atomicMin(K + r, key);
+__global__ __launch_bounds__(CT_THREADS) void k_ccl_resolve(uint32_t* L, uint32_t* K, const uint32_t* roots,
+                                                            const uint32_t* nroots) {
+    const int64_t tile = blockIdx.x;
+    const uint32_t n = nroots[tile];
+    for (uint32_t k = threadIdx.x; k < n; k += CT_THREADS) {
+        const uint32_t lr = roots[tile * (CT_N / 2) + k];
+        const uint32_t r = find_root(L, lr);
+        if (r != lr) {
+            atomicMin(K + r, K[lr]);
+            L[lr] = r;
+        }
     }
 }
+
+__global__ __launch_bounds__(CT_THREADS) void k_ccl_mark_roots(const uint32_t* L, const uint32_t* K,
+                                                               const uint32_t* roots, const uint32_t* nroots,
+                                                               int compat, uint32_t* bm) {
+    const int64_t tile = blockIdx.x;
+    const uint32_t n = nroots[tile];
+    for (uint32_t k = threadIdx.x; k < n; k += CT_THREADS) {
+        const uint32_t lr = roots[tile * (CT_N / 2) + k];
+        if (L[lr] != lr) continue;  // not a global root
+        const uint64_t b = (uint64_t)K[lr] + 1;
+        const uint64_t bit = compat ? (uint64_t)(float)b : b;
+        atomicOr(bm + (bit >> 5), 1u << (bit & 31));
+    }
+}
+
+struct FgAny {
+    DBuf b;
+    __device__ __forceinline__ bool operator()(int64_t i) const { return nz(b, i); }
+};
+struct FgU8 {
+    const uint8_t* p;
+    __device__ __forceinline__ bool operator()(int64_t i) const { return p[i] != 0; }
+};
 
 // 1-based key -> bit position; with fp32 compatibility the key is rounded to the
 // nearest float32 first (the reference holds keys in float32 images)
 __device__ __forceinline__ uint64_t key_bit(uint32_t key0, int compat) {
     const uint64_t k = (uint64_t)key0 + 1;
     return compat ? (uint64_t)(float)k : k;
-}
-
-__global__ void k_ccl_mark(const uint32_t* L, const uint32_t* K, int64_t n, int compat, uint32_t* bm) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        if (L[i] != (uint32_t)i) continue;  // roots only (background holds NONE)
-        const uint64_t b = key_bit(K[i], compat);
-        atomicOr(bm + (b >> 5), 1u << (b & 31));
-    }
 }
 
 // ----------------------------------------------------------------- bitmap ranks
@@ -212,13 +376,50 @@ struct RankTable {
     }
 };
 
-__global__ void k_ccl_write(const uint32_t* L, const uint32_t* K, int64_t n, int compat, const uint32_t* bm,
-                            const uint32_t* prefix, DBuf dst) {
+struct StLabI32 {
+    int32_t* p;
+    __device__ __forceinline__ void operator()(int64_t i, float v) const {
+        p[i] = v > 2147483647.f ? 2147483647 : (int32_t)v;
+    }
+};
+struct StLabU32 {
+    uint32_t* p;
+    __device__ __forceinline__ void operator()(int64_t i, float v) const {
+        p[i] = v > 4294967295.f ? 4294967295u : (uint32_t)v;
+    }
+};
+struct StLabF32 {
+    float* p;
+    __device__ __forceinline__ void operator()(int64_t i, float v) const { p[i] = v; }
+};
+struct StLabAny {
+    DBuf b;
+    __device__ __forceinline__ void operator()(int64_t i, float v) const { st(b, i, v); }
+};
+
+// labels are ranks (>= 0) held as float, as the reference stores them, then
+// converted with saturation to the destination type
+template <typename FG, typename ST>
+__global__ void k_ccl_write(FG fg, const uint32_t* L, const uint32_t* K, int64_t n, int compat, const uint32_t* bm,
+                            const uint32_t* prefix, ST out) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t r = L[i];
         float lab = 0.f;
-        if (r != NONE) lab = (float)bit_rank(bm, prefix, key_bit(K[r], compat));
-        st(dst, i, lab);
+        if (fg(i)) {
+            const uint32_t r = L[L[i]];
+            lab = (float)bit_rank(bm, prefix, key_bit(K[r], compat));
+        }
+        out(i, lab);
+    }
+}
+
+template <typename FG>
+static void launch_ccl_write(FG fg, const uint32_t* L, const uint32_t* K, int64_t n, int compat, const uint32_t* bm,
+                             const uint32_t* prefix, DBuf dst, unsigned grid, hipStream_t s) {
+    switch (dst.dt) {
+        case CLE_I32: k_ccl_write<<<grid, 256, 0, s>>>(fg, L, K, n, compat, bm, prefix, StLabI32{(int32_t*)dst.p}); break;
+        case CLE_U32: k_ccl_write<<<grid, 256, 0, s>>>(fg, L, K, n, compat, bm, prefix, StLabU32{(uint32_t*)dst.p}); break;
+        case CLE_F32: k_ccl_write<<<grid, 256, 0, s>>>(fg, L, K, n, compat, bm, prefix, StLabF32{(float*)dst.p}); break;
+        default: k_ccl_write<<<grid, 256, 0, s>>>(fg, L, K, n, compat, bm, prefix, StLabAny{dst}); break;
     }
 }
 
@@ -227,30 +428,42 @@ static unsigned lg(int64_t n) {
     return g > 131072 ? 131072 : g;
 }
 
-int ccl_impl(DBuf binary, const Geom& g, DBuf dst, int connectivity, int options, uint32_t* n_labels_dev,
-             hipStream_t s) {
+template <typename FG>
+static int ccl_run(FG fg, const Geom& g, DBuf dst, int connectivity, int options, uint32_t* n_labels_dev,
+                   hipStream_t s) {
     const int64_t n = g.n();
-    if (n == 0) return n_labels_dev ? fill_u32(n_labels_dev, 0u, 1, s) : CLE_OK;
     const int compat = (options & CLE_FP32_KEY_COMPAT) ? 1 : 0;
-    Scratch L, K;
-    CLE_TRY(L.alloc((size_t)n * 4, s));
-    CLE_TRY(K.alloc((size_t)n * 4, s));
-    uint32_t* l = L.get<uint32_t>();
-    uint32_t* k = K.get<uint32_t>();
-    k_ccl_init<<<lg(n), 256, 0, s>>>(binary, l, k, n);
-    k_ccl_merge<<<lg(n), 256, 0, s>>>(l, g, connectivity == CLE_CONN_DIAMOND);
-    k_ccl_roots<<<lg(n), 256, 0, s>>>(l, k, g);
+    const int diamond = connectivity == CLE_CONN_DIAMOND;
+    const CclTiles T = ccl_tiles(g);
+    const int64_t nt = T.count();
+    Scratch Ls, Ks, Rs, Ns;
+    CLE_TRY(Ls.alloc((size_t)n * 4, s));
+    CLE_TRY(Ks.alloc((size_t)n * 4, s));
+    CLE_TRY(Rs.alloc((size_t)nt * (CT_N / 2) * 4, s));
+    CLE_TRY(Ns.alloc((size_t)nt * 4, s));
+    uint32_t *l = Ls.get<uint32_t>(), *k = Ks.get<uint32_t>(), *roots = Rs.get<uint32_t>(), *nr = Ns.get<uint32_t>();
+    k_ccl_local<<<(unsigned)nt, CT_THREADS, 0, s>>>(fg, g, T, diamond, l, k, roots, nr);
+    k_ccl_border<<<(unsigned)nt, CT_THREADS, 0, s>>>(fg, g, T, diamond, l, nr);
+    k_ccl_resolve<<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr);
     CLE_LAUNCHED();
     // bit positions run to float32(n) (key n rounded up at most to the next float)
     const uint64_t maxbit = compat ? (uint64_t)(float)(uint64_t)n : (uint64_t)n;
     RankTable rt;
     CLE_TRY(rt.init(maxbit + 1, s));
-    k_ccl_mark<<<lg(n), 256, 0, s>>>(l, k, n, compat, rt.bm.get<uint32_t>());
+    k_ccl_mark_roots<<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>());
     CLE_LAUNCHED();
     CLE_TRY(rt.scan(n_labels_dev, s));
-    k_ccl_write<<<lg(n), 256, 0, s>>>(l, k, n, compat, rt.bm.get<uint32_t>(), rt.prefix.get<uint32_t>(), dst);
+    launch_ccl_write(fg, l, k, n, compat, rt.bm.get<uint32_t>(), rt.prefix.get<uint32_t>(), dst, lg(n), s);
     CLE_LAUNCHED();
     return CLE_OK;
+}
+
+int ccl_impl(DBuf binary, const Geom& g, DBuf dst, int connectivity, int options, uint32_t* n_labels_dev,
+             hipStream_t s) {
+    if (g.n() == 0) return n_labels_dev ? fill_u32(n_labels_dev, 0u, 1, s) : CLE_OK;
+    if (binary.dt == CLE_U8 || binary.dt == CLE_I8)
+        return ccl_run(FgU8{(const uint8_t*)binary.p}, g, dst, connectivity, options, n_labels_dev, s);
+    return ccl_run(FgAny{binary}, g, dst, connectivity, options, n_labels_dev, s);
 }
 
 // ----------------------------------------------------------------- relabel / spots

@@ -52,6 +52,44 @@ __global__ __launch_bounds__(RB) void k_minmax(LD in, int64_t n, uint32_t* out) 
     }
 }
 
+// 16-byte vector loads: T is the element type, V the 16-byte vector of it
+template <typename T>
+__global__ __launch_bounds__(RB) void k_minmax_vec(const T* __restrict__ p, int64_t n, uint32_t* out) {
+    constexpr int E = 16 / sizeof(T);
+    float mn = INFINITY, mx = -INFINITY;
+    const int64_t nv = n / E;
+    const uint4* pv = (const uint4*)p;
+    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < nv; i += (int64_t)gridDim.x * RB) {
+        const uint4 q = pv[i];
+        const T* e = (const T*)&q;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const float v = (float)e[k];
+            mn = fminf(mn, v);
+            mx = fmaxf(mx, v);
+        }
+    }
+    // tail
+    for (int64_t i = nv * E + (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+        const float v = (float)p[i];
+        mn = fminf(mn, v);
+        mx = fmaxf(mx, v);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = fminf(mn, __shfl_xor(mn, off));
+        mx = fmaxf(mx, __shfl_xor(mx, off));
+    }
+    __shared__ float smn[RB / 64], smx[RB / 64];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) { smn[w] = mn; smx[w] = mx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < RB / 64; ++k) { mn = fminf(mn, smn[k]); mx = fmaxf(mx, smx[k]); }
+        atomicMin(out, f2o(mn));
+        atomicMax(out + 1, f2o(mx));
+    }
+}
+
 __global__ void k_minmax_init(uint32_t* o) {
     o[0] = 0xFFFFFFFFu;
     o[1] = 0u;
@@ -74,11 +112,17 @@ int minmax_impl(DBuf src, int64_t n, float* out2, hipStream_t s) {
     Scratch tmp;
     CLE_TRY(tmp.alloc(8, s));
     k_minmax_init<<<1, 1, 0, s>>>(tmp.get<uint32_t>());
-    unsigned g = grid_for((size_t)n, RB * 8);
-    if (g > 2048) g = 2048;
+    unsigned g = grid_for((size_t)n, RB * 16);
+    if (g > 4096) g = 4096;
+    const bool aligned = ((uintptr_t)src.p & 15) == 0;
     if (n > 0) {
-        if (src.dt == CLE_F32) k_minmax<<<g, RB, 0, s>>>(LdF32R{(const float*)src.p}, n, tmp.get<uint32_t>());
-        else k_minmax<<<g, RB, 0, s>>>(LdAnyR{src}, n, tmp.get<uint32_t>());
+        uint32_t* o = tmp.get<uint32_t>();
+        if (aligned && src.dt == CLE_F32) k_minmax_vec<float><<<g, RB, 0, s>>>((const float*)src.p, n, o);
+        else if (aligned && src.dt == CLE_I32) k_minmax_vec<int32_t><<<g, RB, 0, s>>>((const int32_t*)src.p, n, o);
+        else if (aligned && src.dt == CLE_U32) k_minmax_vec<uint32_t><<<g, RB, 0, s>>>((const uint32_t*)src.p, n, o);
+        else if (aligned && src.dt == CLE_U16) k_minmax_vec<uint16_t><<<g, RB, 0, s>>>((const uint16_t*)src.p, n, o);
+        else if (aligned && src.dt == CLE_U8) k_minmax_vec<uint8_t><<<g, RB, 0, s>>>((const uint8_t*)src.p, n, o);
+        else k_minmax<<<g, RB, 0, s>>>(LdAnyR{src}, n, o);
     }
     k_minmax_fin<<<1, 1, 0, s>>>(tmp.get<uint32_t>(), out2);
     CLE_LAUNCHED();
@@ -101,7 +145,7 @@ __global__ __launch_bounds__(RB) void k_sum(DBuf in, int64_t n, double* out) {
 // ----------------------------------------------------------------- histogram
 constexpr int HB = 256;                // threads per block
 constexpr int HWAVES = HB / 64;
-constexpr int HIST_LDS_MAX_BINS = 2048;
+constexpr int HIST_LDS_MAX_BINS = 480;  // LDS replicas: 32 x (bins | 1) x 4 B <= 64 KB
 
 // The float32 quotient gets the literal 0.5 added in float64 (an unsuffixed OpenCL
 // literal is double on fp64 devices), then convert_uint_sat truncates.
@@ -115,29 +159,53 @@ __device__ __forceinline__ uint32_t bin_of(float v, float mn, float range, int b
     return (uint32_t)t;
 }
 
-// one block = one row y over slices [z0, z0 + zc); per-wave LDS sub-histograms
+// One block = one row y over slices [z0, z0 + zc).  Each wavefront owns HREP
+// replicas of the histogram (lane l counts into replica l % HREP; replicas padded to
+// an odd stride so equal bins of different replicas sit in different LDS banks), and
+// each lane first merges runs of equal bins among its own consecutive voxels.
+constexpr int HREP = 8;
+constexpr int HVPT = 4;  // consecutive voxels per lane
+
 template <typename LD>
 __global__ __launch_bounds__(HB) void k_hist_rows(LD in, Geom g, int bins, int zc, int zchunks, float mn_arg,
                                                  float mx_arg, const float* mm, uint32_t* hist2d) {
     extern __shared__ uint32_t sh[];
+    const int stride = bins | 1;
     const int64_t y = blockIdx.x / zchunks;
     const int64_t z0 = (int64_t)(blockIdx.x % zchunks) * zc;
     const float mn = mm ? mm[0] : mn_arg;
     const float mx = mm ? mm[1] : mx_arg;
     const float range = mx - mn;
-    const int wave = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < bins * HWAVES; i += HB) sh[i] = 0;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nsub = HWAVES * HREP;
+    for (int i = threadIdx.x; i < stride * nsub; i += HB) sh[i] = 0;
     __syncthreads();
-    uint32_t* mine = sh + wave * bins;
+    uint32_t* mine = sh + (wave * HREP + (lane % HREP)) * stride;
     const int64_t z1 = (z0 + zc < g.D) ? z0 + zc : g.D;
     for (int64_t z = z0; z < z1; ++z) {
         const int64_t base = (z * g.H + y) * g.W;
-        for (int64_t x = threadIdx.x; x < g.W; x += HB) atomicAdd(&mine[bin_of(in(base + x), mn, range, bins)], 1u);
+        for (int64_t x = (int64_t)threadIdx.x * HVPT; x < g.W; x += (int64_t)HB * HVPT) {
+            uint32_t cur = 0xFFFFFFFFu, run = 0;
+#pragma unroll
+            for (int k = 0; k < HVPT; ++k) {
+                if (x + k < g.W) {
+                    const uint32_t b = bin_of(in(base + x + k), mn, range, bins);
+                    if (b == cur) {
+                        ++run;
+                    } else {
+                        if (run) atomicAdd(&mine[cur], run);
+                        cur = b;
+                        run = 1;
+                    }
+                }
+            }
+            if (run) atomicAdd(&mine[cur], run);
+        }
     }
     __syncthreads();
     for (int b = threadIdx.x; b < bins; b += HB) {
         uint32_t c = 0;
-        for (int w = 0; w < HWAVES; ++w) c += sh[w * bins + b];
+        for (int w = 0; w < nsub; ++w) c += sh[w * stride + b];
         if (c) atomicAdd(&hist2d[y * bins + b], c);
     }
 }
@@ -155,13 +223,30 @@ __global__ void k_hist_global(LD in, Geom g, int bins, float mn_arg, float mx_ar
     }
 }
 
-// total[b] = sum over y (in y order, float32) of float(hist2d[y][b])
-__global__ void k_hist_rowsum(const uint32_t* hist2d, int64_t H, int bins, DBuf out) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= bins) return;
+// total[b] = sum over y (in y order, float32) of float(hist2d[y][b]).  One wavefront
+// per bin: the lanes stage the bin's column in LDS in chunks, lane 0 adds it in order.
+constexpr int RS_CHUNK = 2048;
+__global__ __launch_bounds__(64) void k_hist_rowsum(const uint32_t* hist2d, int64_t H, int bins, DBuf out) {
+    __shared__ float col[RS_CHUNK];
+    const int b = blockIdx.x;
     float acc = 0.f;
-    for (int64_t y = 0; y < H; ++y) acc += (float)hist2d[y * bins + b];
-    st(out, b, acc);
+    for (int64_t y0 = 0; y0 < H; y0 += RS_CHUNK) {
+        const int n = (int)(H - y0 < RS_CHUNK ? H - y0 : RS_CHUNK);
+        for (int k = threadIdx.x; k < n; k += 64) col[k] = (float)hist2d[(y0 + k) * bins + b];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int k = 0;
+            for (; k + 8 <= n; k += 8) {
+                const float4 a = *(const float4*)(col + k);
+                const float4 c = *(const float4*)(col + k + 4);
+                acc += a.x; acc += a.y; acc += a.z; acc += a.w;
+                acc += c.x; acc += c.y; acc += c.z; acc += c.w;
+            }
+            for (; k < n; ++k) acc += col[k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) st(out, b, acc);
 }
 
 template <typename LD>
@@ -173,7 +258,7 @@ static int hist_launch(LD in, const Geom& g, int bins, float mn, float mx, const
         if (zc < 1) zc = 1;
         if (zc > g.D) zc = (int)g.D;
         const int zchunks = (int)((g.D + zc - 1) / zc);
-        const size_t lds = (size_t)bins * HWAVES * sizeof(uint32_t);
+        const size_t lds = (size_t)(bins | 1) * HWAVES * HREP * sizeof(uint32_t);
         k_hist_rows<<<(unsigned)(g.H * zchunks), HB, lds, s>>>(in, g, bins, zc, zchunks, mn, mx, mm, h2);
     } else {
         unsigned gr = grid_for((size_t)g.n(), 256);
@@ -194,7 +279,7 @@ static int histogram_impl(DBuf src, const Geom& g, int bins, float mn, float mx,
         if (src.dt == CLE_F32) CLE_TRY(hist_launch(LdF32R{(const float*)src.p}, g, bins, mn, mx, mm, h2.get<uint32_t>(), s));
         else CLE_TRY(hist_launch(LdAnyR{src}, g, bins, mn, mx, mm, h2.get<uint32_t>(), s));
     }
-    k_hist_rowsum<<<grid_for(bins, 256), 256, 0, s>>>(h2.get<uint32_t>(), g.H, bins, out);
+    k_hist_rowsum<<<(unsigned)bins, 64, 0, s>>>(h2.get<uint32_t>(), g.H, bins, out);
     CLE_LAUNCHED();
     return CLE_OK;
 }
@@ -252,7 +337,21 @@ __global__ __launch_bounds__(256) void k_otsu(const float* hist, const float* mm
     }
 }
 
-// dst = src > float32(threshold)
+// dst = src > float32(threshold); float32 -> uint8 with 16-byte loads, 4-byte stores
+__global__ __launch_bounds__(256) void k_threshold_f32_u8(const float* __restrict__ s, uint8_t* __restrict__ d,
+                                                          const double* thr, int64_t n) {
+    const float t = (float)thr[0];
+    const int64_t nv = n / 4;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
+        const float4 v = ((const float4*)s)[i];
+        const uint32_t o = (v.x > t ? 1u : 0u) | (v.y > t ? 1u << 8 : 0u) | (v.z > t ? 1u << 16 : 0u) |
+                           (v.w > t ? 1u << 24 : 0u);
+        ((uint32_t*)d)[i] = o;
+    }
+    for (int64_t i = nv * 4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = s[i] > t ? 1 : 0;
+}
+
 __global__ void k_threshold(DBuf s, DBuf d, const double* thr, int64_t n) {
     const float t = (float)thr[0];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -320,9 +419,12 @@ int cle_threshold_otsu(const cle_buf* src, const cle_buf* dst, double* out_thres
         t = thr.get<double>();
     }
     CLE_TRY(otsu_impl(dbuf(src), g, mm.get<float>(), t, s));
-    unsigned gr = grid_for((size_t)g.n(), 256);
+    unsigned gr = grid_for((size_t)g.n(), 256 * 4);
     if (gr > 131072) gr = 131072;
-    k_threshold<<<gr, 256, 0, s>>>(dbuf(src), dbuf(dst), t, g.n());
+    const bool vec = src->dtype == CLE_F32 && dst->dtype == CLE_U8 && ((uintptr_t)src->data & 15) == 0 &&
+                     ((uintptr_t)dst->data & 3) == 0;
+    if (vec) k_threshold_f32_u8<<<gr, 256, 0, s>>>((const float*)src->data, (uint8_t*)dst->data, t, g.n());
+    else k_threshold<<<gr, 256, 0, s>>>(dbuf(src), dbuf(dst), t, g.n());
     CLE_LAUNCHED();
     return CLE_OK;
 }

@@ -43,18 +43,54 @@ __device__ __forceinline__ float o2f(uint32_t o) {
     return __uint_as_float(b);
 }
 
-__device__ __forceinline__ int label_at(DBuf lab, int64_t i) { return (int)ld(lab, i); }
+// Label readers: the reference reads a label as (int) of the float it loads.
+template <typename T>
+struct LabT {
+    const T* p;
+    __device__ __forceinline__ int operator()(int64_t i) const { return (int)(float)p[i]; }
+    __device__ __forceinline__ bool nz(int64_t i) const { return p[i] != (T)0; }
+};
+struct LabAny {
+    DBuf b;
+    __device__ __forceinline__ int operator()(int64_t i) const { return (int)ld(b, i); }
+    __device__ __forceinline__ bool nz(int64_t i) const { return cle::nz(b, i); }
+};
 
-__global__ __launch_bounds__(256) void k_stats_pass1(DBuf inten, DBuf lab, Geom g, int64_t L, Acc a) {
+// Work split shared by both passes: a wavefront takes a span of SPAN voxels of one
+// row; it first checks, with coalesced loads, whether any label in the span is
+// non-zero, and skips the span if not.  Otherwise lane l walks voxels
+// [x0 + l*RUN, x0 + (l+1)*RUN) of the span.
+constexpr int SPAN = 64 * RUN;
+
+template <typename LAB>
+__device__ __forceinline__ bool span_has_labels(LAB lab, int64_t base, int64_t x0, int64_t x1, int lane) {
+    bool any = false;
+#pragma unroll 4
+    for (int k = 0; k < RUN; ++k) {
+        const int64_t x = x0 + lane + 64 * k;
+        if (x < x1) any |= lab.nz(base + x);
+    }
+    return __any(any);
+}
+
+template <typename LAB>
+__global__ __launch_bounds__(256) void k_stats_pass1(DBuf inten, LAB lab, Geom g, int64_t L, Acc a) {
 #pragma clang fp contract(off)
-    const int64_t chunks = (g.W + RUN - 1) / RUN;
-    const int64_t total = g.D * g.H * chunks;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t row = t / chunks;
-        const int64_t x0 = (t - row * chunks) * RUN;
-        const int64_t x1 = x0 + RUN < g.W ? x0 + RUN : g.W;
-        const int64_t y = row % g.H, z = row / g.H;
+    const int64_t spans = (g.W + SPAN - 1) / SPAN;
+    const int64_t total = g.D * g.H * spans;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t sp = wave0; sp < total; sp += nwaves) {
+        const int64_t row = sp / spans;
+        const int64_t sx0 = (sp - row * spans) * SPAN;
+        const int64_t sx1 = sx0 + SPAN < g.W ? sx0 + SPAN : g.W;
         const int64_t base = row * g.W;
+        if (!span_has_labels(lab, base, sx0, sx1, lane)) continue;
+        const int64_t x0 = sx0 + (int64_t)lane * RUN;
+        const int64_t x1 = x0 + RUN < sx1 ? x0 + RUN : sx1;
+        if (x0 >= x1) continue;
+        const int64_t y = row % g.H, z = row / g.H;
         int cur = -1;
         unsigned long long c = 0, sx = 0;
         double sv = 0, sxv = 0, syv = 0, szv = 0;
@@ -62,7 +98,7 @@ __global__ __launch_bounds__(256) void k_stats_pass1(DBuf inten, DBuf lab, Geom 
         int64_t xa = 0, xb = 0;
         const float yf = (float)y, zf = (float)z;
         for (int64_t x = x0; x <= x1; ++x) {
-            const int l = x < x1 ? label_at(lab, base + x) : -1;
+            const int l = x < x1 ? lab(base + x) : -1;
             if (l != cur) {
                 if (cur > 0 && cur < L) {
                     atomicAdd(a.cnt + cur, c);
@@ -85,7 +121,7 @@ __global__ __launch_bounds__(256) void k_stats_pass1(DBuf inten, DBuf lab, Geom 
                 cur = l;
                 c = 0; sx = 0; sv = 0; sxv = 0; syv = 0; szv = 0;
                 xa = x;
-                if (x < x1) { mn = mx = ld(inten, base + x); }
+                if (x < x1 && l > 0) { mn = mx = ld(inten, base + x); }
             }
             if (x == x1) break;
             if (l > 0) {
@@ -123,23 +159,31 @@ __global__ void k_stats_mid(Acc a, int64_t L) {
     a.img[7 * L + l] = area;
 }
 
-__global__ __launch_bounds__(256) void k_stats_pass2(DBuf inten, DBuf lab, Geom g, int64_t L, Acc a) {
+template <typename LAB>
+__global__ __launch_bounds__(256) void k_stats_pass2(DBuf inten, LAB lab, Geom g, int64_t L, Acc a) {
 #pragma clang fp contract(off)
-    const int64_t chunks = (g.W + RUN - 1) / RUN;
-    const int64_t total = g.D * g.H * chunks;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t row = t / chunks;
-        const int64_t x0 = (t - row * chunks) * RUN;
-        const int64_t x1 = x0 + RUN < g.W ? x0 + RUN : g.W;
-        const int64_t y = row % g.H, z = row / g.H;
+    const int64_t spans = (g.W + SPAN - 1) / SPAN;
+    const int64_t total = g.D * g.H * spans;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t sp = wave0; sp < total; sp += nwaves) {
+        const int64_t row = sp / spans;
+        const int64_t sx0 = (sp - row * spans) * SPAN;
+        const int64_t sx1 = sx0 + SPAN < g.W ? sx0 + SPAN : g.W;
         const int64_t base = row * g.W;
+        if (!span_has_labels(lab, base, sx0, sx1, lane)) continue;
+        const int64_t x0 = sx0 + (int64_t)lane * RUN;
+        const int64_t x1 = x0 + RUN < sx1 ? x0 + RUN : sx1;
+        if (x0 >= x1) continue;
+        const int64_t y = row % g.H, z = row / g.H;
         const float yf = (float)y, zf = (float)z;
         int cur = -1;
         double sdc = 0, sdm = 0, ssq = 0;
         float mdc = 0, mdm = 0;
         float cx = 0, cy = 0, cz = 0, mx = 0, my = 0, mz = 0, mean = 0, area = 1;
         for (int64_t x = x0; x <= x1; ++x) {
-            const int l = x < x1 ? label_at(lab, base + x) : -1;
+            const int l = x < x1 ? lab(base + x) : -1;
             if (l != cur) {
                 if (cur > 0 && cur < L) {
                     atomicAdd(a.sdc + cur, sdc);
@@ -175,6 +219,16 @@ __global__ __launch_bounds__(256) void k_stats_pass2(DBuf inten, DBuf lab, Geom 
             }
         }
     }
+}
+
+template <typename LAB>
+static void stats_passes(DBuf inten, LAB lab, const Geom& g, int64_t L, const Acc& a, hipStream_t s) {
+    const int64_t spans = g.D * g.H * ((g.W + SPAN - 1) / SPAN);
+    unsigned gr = grid_for((size_t)spans, 4);  // 4 wavefronts per block
+    if (gr > 65536) gr = 65536;
+    if (g.n() > 0) k_stats_pass1<<<gr, 256, 0, s>>>(inten, lab, g, L, a);
+    k_stats_mid<<<grid_for((size_t)L, 256), 256, 0, s>>>(a, L);
+    if (g.n() > 0) k_stats_pass2<<<gr, 256, 0, s>>>(inten, lab, g, L, a);
 }
 
 // STATISTICS_ENTRY table, one column per measured label 1..L-1
@@ -263,12 +317,14 @@ int cle_statistics_of_labelled_pixels(const cle_buf* intensity, const cle_buf* l
     CLE_TRY(fill_u32(a.bmaxz, 0u, (size_t)L, s));
     CLE_TRY(fill_u32(a.maxdc, 0u, (size_t)L, s));
     CLE_TRY(fill_u32(a.maxdm, 0u, (size_t)L, s));
-    const int64_t runs = gi.D * gi.H * ((gi.W + RUN - 1) / RUN);
-    unsigned gr = grid_for((size_t)runs, 256);
-    if (gr > 65536) gr = 65536;
-    if (gi.n() > 0) k_stats_pass1<<<gr, 256, 0, s>>>(dbuf(intensity), dbuf(labels), gi, L, a);
-    k_stats_mid<<<grid_for((size_t)L, 256), 256, 0, s>>>(a, L);
-    if (gi.n() > 0) k_stats_pass2<<<gr, 256, 0, s>>>(dbuf(intensity), dbuf(labels), gi, L, a);
+    const DBuf in = dbuf(intensity);
+    switch (labels->dtype) {
+        case CLE_I32: stats_passes(in, LabT<int32_t>{(const int32_t*)labels->data}, gi, L, a, s); break;
+        case CLE_U32: stats_passes(in, LabT<uint32_t>{(const uint32_t*)labels->data}, gi, L, a, s); break;
+        case CLE_F32: stats_passes(in, LabT<float>{(const float*)labels->data}, gi, L, a, s); break;
+        case CLE_U16: stats_passes(in, LabT<uint16_t>{(const uint16_t*)labels->data}, gi, L, a, s); break;
+        default: stats_passes(in, LabAny{dbuf(labels)}, gi, L, a, s); break;
+    }
     if (L > 1) k_stats_table<<<grid_for((size_t)(L - 1), 256), 256, 0, s>>>(a, L, (float*)out_table->data);
     CLE_LAUNCHED();
     return CLE_OK;

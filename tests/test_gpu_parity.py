@@ -390,3 +390,70 @@ def test_statistics_label_gaps(cle):
     lab[7, 1:9] = 5
     img = np.arange(lab.size, dtype=np.float32).reshape(lab.shape)
     _assert_stats_close(cle.statistics_of_labelled_pixels(img, lab), O.statistics_of_labelled_pixels(img, lab))
+
+
+# ----------------------------------------------------------------- tiled / vectorised paths
+@pytest.mark.parametrize("shape,kind,p", [((9, 19, 150), "box", 0.3), ((9, 19, 150), "diamond", 0.5),
+                                          ((70, 150), "box", 0.45), ((70, 150), "diamond", 0.55),
+                                          ((13, 33, 65), "box", 0.6), ((5, 17, 129), "box", 0.2)])
+def test_ccl_multi_tile(cle, shape, kind, p):
+    """Shapes spanning several 4x8x64 (3D) / 32x64 (2D) tiles in every direction."""
+    rng = np.random.default_rng(21)
+    b = (rng.random(shape) < p).astype(np.uint8)
+    fn = cle.connected_components_labeling_box if kind == "box" else cle.connected_components_labeling_diamond
+    assert np.array_equal(cle.pull(fn(b)), O.connected_components_labeling(b, kind))
+
+
+def test_ccl_diagonal_tile_crossings(cle):
+    """Diagonal chains crossing tile corners (+x/-y, +x/-z, +y/-z neighbours)."""
+    b = np.zeros((12, 24, 140), np.uint8)
+    for t in range(12):
+        b[t, 20 - t, 60 + t] = 1          # +x, -y per slice step
+        b[t, 3 + t, 130 - t] = 1          # -x, +y
+    for t in range(9):
+        b[11 - t, 8 + t, 63 + t] = 1      # -z, +y, +x
+    for kind in ("box", "diamond"):
+        fn = cle.connected_components_labeling_box if kind == "box" else cle.connected_components_labeling_diamond
+        assert np.array_equal(cle.pull(fn(b)), O.connected_components_labeling(b, kind)), kind
+
+
+@pytest.mark.parametrize("shape,sig", [((96, 40, 70), (2.0, 2.0, 2.0)), ((90, 33, 65), (1.5, 1.0, 3.0)),
+                                       ((50, 66, 130), (3.5, 3.5, 3.5))])
+def test_blur_fused_z_chunks(cle, shape, sig):
+    rng = np.random.default_rng(22)
+    a = rng.integers(0, 4000, shape).astype(np.uint16)
+    got = cle.pull(cle.gaussian_blur(a, None, *sig))
+    assert_blur_close(got, O.gaussian_blur(a.astype(np.float32), *sig))
+
+
+def test_histogram_many_bins(cle):
+    rng = np.random.default_rng(23)
+    a = (rng.random((20, 30)) * 1000).astype(np.float32)
+    assert np.array_equal(cle.pull(cle.histogram(a, num_bins=1000)), O.histogram(a, 1000))
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.int32, np.uint32, np.float32])
+def test_minmax_vector_tails(cle, dtype):
+    rng = np.random.default_rng(24)
+    for shape in ((7, 13, 17), (1, 3), (1030,)):
+        a = rng.integers(1, 250, shape).astype(dtype)
+        a.flat[-1] = 251
+        a.flat[0] = 0
+        d = cle.asarray(a)
+        assert d.dtype == np.dtype(dtype)
+        assert cle.minimum_of_all_pixels(d) == 0 and cle.maximum_of_all_pixels(d) == 251, (dtype, shape)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.uint32, np.float32, np.uint16, np.uint8])
+def test_statistics_label_dtypes_and_spans(cle, dtype):
+    rng = np.random.default_rng(25)
+    lab = np.zeros((3, 5, 1500), np.int64)
+    lab[0, 1, 10:1400] = 1
+    lab[1, 2:4, 1000:1100] = 2
+    lab[2, 4, 1499] = 3
+    lab[:, :, 200:203] = 4
+    img = (rng.random(lab.shape) * 100).astype(np.float32)
+    d = cle.asarray(lab.astype(dtype))
+    assert d.dtype == np.dtype(dtype)
+    got = cle.statistics_of_labelled_pixels(img, d)
+    _assert_stats_close(got, O.statistics_of_labelled_pixels(img, lab))
