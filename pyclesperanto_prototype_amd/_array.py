@@ -67,6 +67,9 @@ class Array:
     __slots__ = ("_t", "__weakref__")
 
     def __init__(self, tensor):
+        # kernels dereference the data pointer on the GPU: host memory must never get here
+        if tensor.device.type != "cuda":
+            raise ValueError(f"cle.Array needs a device tensor, got one on {tensor.device}; use cle.push / cle.asarray")
         if not tensor.is_contiguous():
             tensor = tensor.contiguous()
         self._t = tensor
@@ -187,15 +190,18 @@ def _from_numpy(a: np.ndarray) -> Array:
 
 
 def asarray(x) -> Array:
-    """Wrap a device tensor / DLPack object / Array without conversion."""
+    """Device Array of `x` keeping its element type: an Array or device tensor is
+    wrapped without a copy, other DLPack producers are imported (and uploaded if they
+    live in host memory), numpy / array-likes are uploaded."""
     if isinstance(x, Array):
         return x
+    if isinstance(x, (np.ndarray, list, tuple)) or np.isscalar(x):
+        return _from_numpy(np.asarray(x))
     if torch is not None and isinstance(x, torch.Tensor):
-        if x.device.type != "cuda":
-            x = x.to(device())
-        return Array(x)
+        return Array(x if x.device.type == "cuda" else x.to(device()))
     if hasattr(x, "__dlpack__"):
-        return Array(torch.from_dlpack(x))
+        t = torch.from_dlpack(x)
+        return Array(t if t.device.type == "cuda" else t.to(device()))
     return _from_numpy(np.asarray(x))
 
 

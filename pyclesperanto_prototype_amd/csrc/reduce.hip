@@ -52,7 +52,31 @@ __global__ __launch_bounds__(RB) void k_minmax(LD in, int64_t n, uint32_t* out) 
     }
 }
 
-// 16-byte vector loads: T is the element type, V the 16-byte vector of it
+// Element k of a 16-byte vector holding elements of type T, converted to float by
+// explicit shifts / bit casts (no type punning through pointers).
+template <typename T> __device__ __forceinline__ float vec_elem(const uint4& q, int k);
+template <> __device__ __forceinline__ float vec_elem<uint8_t>(const uint4& q, int k) {
+    const uint32_t w = k < 4 ? q.x : (k < 8 ? q.y : (k < 12 ? q.z : q.w));
+    return (float)((w >> (8 * (k & 3))) & 0xFFu);
+}
+template <> __device__ __forceinline__ float vec_elem<uint16_t>(const uint4& q, int k) {
+    const uint32_t w = k < 2 ? q.x : (k < 4 ? q.y : (k < 6 ? q.z : q.w));
+    return (float)((w >> (16 * (k & 1))) & 0xFFFFu);
+}
+template <> __device__ __forceinline__ float vec_elem<uint32_t>(const uint4& q, int k) {
+    const uint32_t w = k == 0 ? q.x : (k == 1 ? q.y : (k == 2 ? q.z : q.w));
+    return (float)w;
+}
+template <> __device__ __forceinline__ float vec_elem<int32_t>(const uint4& q, int k) {
+    const uint32_t w = k == 0 ? q.x : (k == 1 ? q.y : (k == 2 ? q.z : q.w));
+    return (float)(int32_t)w;
+}
+template <> __device__ __forceinline__ float vec_elem<float>(const uint4& q, int k) {
+    const uint32_t w = k == 0 ? q.x : (k == 1 ? q.y : (k == 2 ? q.z : q.w));
+    return __uint_as_float(w);
+}
+
+// min/max with 16-byte vector loads (T = element type)
 template <typename T>
 __global__ __launch_bounds__(RB) void k_minmax_vec(const T* __restrict__ p, int64_t n, uint32_t* out) {
     constexpr int E = 16 / sizeof(T);
@@ -61,10 +85,9 @@ __global__ __launch_bounds__(RB) void k_minmax_vec(const T* __restrict__ p, int6
     const uint4* pv = (const uint4*)p;
     for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < nv; i += (int64_t)gridDim.x * RB) {
         const uint4 q = pv[i];
-        const T* e = (const T*)&q;
 #pragma unroll
         for (int k = 0; k < E; ++k) {
-            const float v = (float)e[k];
+            const float v = vec_elem<T>(q, k);
             mn = fminf(mn, v);
             mx = fmaxf(mx, v);
         }

@@ -84,3 +84,13 @@ def test_plugin_signature_binding_errors():
         cle.voronoi_otsu_labeling(np.zeros((2, 2)), spot_size=3)
     with pytest.raises(NotImplementedError):
         cle.execute(None, "x.cl", "k", (1,), {})
+
+
+def test_array_refuses_host_memory():
+    """Kernels dereference Array data on the GPU; a host tensor must be rejected."""
+    import torch
+    import pyclesperanto_prototype_amd as cle
+    with pytest.raises(ValueError):
+        cle.Array(torch.zeros(4))
+    with pytest.raises(ValueError):
+        cle.Array(torch.from_dlpack(np.zeros(3, np.float32)))
