@@ -272,6 +272,114 @@ __global__ __launch_bounds__(FB_THREADS) void k_blur_fused(LD in, float* __restr
     }
 }
 
+// ----------------------------------------------------------------- isotropic fused pass
+// The common case (sigma equal on every blurred axis): half-width C is a compile-time
+// constant, so the tap loops unroll, the weights live in scalar registers and every
+// LDS offset is an immediate.  Same tiling as k_blur_fused; the staged slice values
+// are prefetched two slices ahead (double-buffered registers) to cover HBM latency.
+template <int C>
+struct IsoW {
+    float w[C + 1];
+};
+
+template <int C, bool Z3, typename LD>
+__global__ __launch_bounds__(FB_THREADS) void k_blur_iso(LD in, float* __restrict__ out, Geom g, int zc, int ntx,
+                                                        int nty, const IsoW<C> w) {
+    constexpr int AW = FB_TX + 2 * C, AH = FB_TY + 2 * C, NA = AW * AH;
+    constexpr int PF = (NA + FB_THREADS - 1) / FB_THREADS;
+    constexpr int CZ = Z3 ? C : 0;
+    constexpr int XO = (AH * FB_TX + FB_THREADS - 1) / FB_THREADS;  // x-pass outputs per thread
+    __shared__ float A[NA];
+    __shared__ float B[AH * FB_TX];
+    const int64_t nb = (int64_t)gridDim.x;
+    const int64_t bid = xcd_tile(blockIdx.x, nb);
+    const int64_t tx = bid % ntx, rest = bid / ntx;
+    const int64_t ty = rest % nty, tzc = rest / nty;
+    const int64_t x0 = tx * FB_TX, y0 = ty * FB_TY;
+    const int64_t zb = tzc * zc;
+    const int64_t ze = zb + zc < g.D ? zb + zc : g.D;
+    const int row = threadIdx.x / (FB_TX / 2);
+    const int col = (threadIdx.x % (FB_TX / 2)) * 2;
+    const int Hm = (int)g.H - 1, Wm = (int)g.W - 1, Wi = (int)g.W;
+    int off[PF];
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+        const int idx = threadIdx.x + k * FB_THREADS;
+        const int ay = idx / AW, ax = idx - ay * AW;
+        int gy = (int)y0 - C + ay, gx = (int)x0 - C + ax;
+        gy = gy < 0 ? 0 : (gy > Hm ? Hm : gy);
+        gx = gx < 0 ? 0 : (gx > Wm ? Wm : gx);
+        off[k] = gy * Wi + gx;
+    }
+    float pa[PF], pb[PF];
+    float r0[2 * CZ + 1], r1[2 * CZ + 1];
+#pragma unroll
+    for (int j = 0; j < 2 * CZ + 1; ++j) { r0[j] = 0.f; r1[j] = 0.f; }
+    const int64_t zlast = ze + CZ;  // one past the last input slice
+
+    auto load = [&](float (&pf)[PF], int64_t zi) {
+        if (zi >= zlast) return;
+        const int64_t zz = zi < 0 ? 0 : (zi >= g.D ? g.D - 1 : zi);
+        const int64_t base = zz * g.H * g.W;
+#pragma unroll
+        for (int k = 0; k < PF; ++k)
+            if (threadIdx.x + k * FB_THREADS < NA) pf[k] = in(base + off[k]);
+    };
+    auto step = [&](float (&pf)[PF], int64_t zi) {
+#pragma unroll
+        for (int k = 0; k < PF; ++k)
+            if (threadIdx.x + k * FB_THREADS < NA) A[threadIdx.x + k * FB_THREADS] = pf[k];
+        __syncthreads();
+        load(pf, zi + 2);
+#pragma unroll
+        for (int k = 0; k < XO; ++k) {
+            const int idx = threadIdx.x + k * FB_THREADS;
+            if (idx < AH * FB_TX) {
+                const int ay = idx / FB_TX, bx = idx - ay * FB_TX;
+                const float* a = A + ay * AW + bx + C;
+                float acc = w.w[0] * a[0];
+#pragma unroll
+                for (int v = 1; v <= C; ++v) acc = fmaf(w.w[v], a[-v] + a[v], acc);
+                B[idx] = acc;
+            }
+        }
+        __syncthreads();
+        const float* b = B + (row + C) * FB_TX + col;
+        float v0 = w.w[0] * b[0], v1 = w.w[0] * b[1];
+#pragma unroll
+        for (int v = 1; v <= C; ++v) {
+            v0 = fmaf(w.w[v], b[-v * FB_TX] + b[v * FB_TX], v0);
+            v1 = fmaf(w.w[v], b[-v * FB_TX + 1] + b[v * FB_TX + 1], v1);
+        }
+#pragma unroll
+        for (int j = 0; j < 2 * CZ; ++j) { r0[j] = r0[j + 1]; r1[j] = r1[j + 1]; }
+        r0[2 * CZ] = v0;
+        r1[2 * CZ] = v1;
+        if (zi >= zb + CZ) {
+            float o0 = w.w[0] * r0[CZ], o1 = w.w[0] * r1[CZ];
+            if (CZ == 0) { o0 = r0[0]; o1 = r1[0]; }
+#pragma unroll
+            for (int k = 1; k <= CZ; ++k) {
+                o0 = fmaf(w.w[k], r0[CZ - k] + r0[CZ + k], o0);
+                o1 = fmaf(w.w[k], r1[CZ - k] + r1[CZ + k], o1);
+            }
+            const int64_t zo = zi - CZ, y = y0 + row, x = x0 + col;
+            if (y < g.H) {
+                float* o = out + (zo * g.H + y) * g.W + x;
+                if (x + 1 < g.W) { o[0] = o0; o[1] = o1; }
+                else if (x < g.W) o[0] = o0;
+            }
+        }
+    };
+
+    load(pa, zb - CZ);
+    load(pb, zb - CZ + 1);
+    for (int64_t zi = zb - CZ; zi < zlast; zi += 2) {
+        step(pa, zi);
+        if (zi + 1 < zlast) step(pb, zi + 1);
+    }
+}
+
 // ----------------------------------------------------------------- host side
 static void gauss_taps(float sigma, Taps* t) {
     int n = (int)(sigma * 8.0f);  // _tier0/_sigma_to_kernel_size.py:1-4
@@ -350,6 +458,69 @@ static int try_fused(const cle_buf* src, const cle_buf* dst, const Taps* taps[3]
     return rc;
 }
 
+template <int C, bool Z3, typename LD>
+static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hipStream_t s) {
+    const int ntx = (int)((g.W + FB_TX - 1) / FB_TX), nty = (int)((g.H + FB_TY - 1) / FB_TY);
+    const int64_t nxy = (int64_t)ntx * nty;
+    constexpr int CZ = Z3 ? C : 0;
+    int64_t nzc = (2048 + nxy - 1) / nxy;
+    const int64_t max_nzc = g.D / (4 * CZ + 8) > 1 ? g.D / (4 * CZ + 8) : 1;
+    if (nzc > max_nzc) nzc = max_nzc;
+    if (nzc < 1) nzc = 1;
+    const int zc = (int)((g.D + nzc - 1) / nzc);
+    nzc = (g.D + zc - 1) / zc;
+    IsoW<C> w;
+    for (int v = 0; v <= C; ++v) w.w[v] = wts[v];
+    k_blur_iso<C, Z3, LD><<<(unsigned)(nxy * nzc), FB_THREADS, 0, s>>>(in, out, g, zc, ntx, nty, w);
+}
+
+constexpr int ISO_MAXC = 20;
+
+template <bool Z3, typename LD>
+static int launch_iso(LD in, float* out, const Geom& g, int c, const float* wts, hipStream_t s) {
+    switch (c) {
+#define ISO_CASE(n) case n: launch_iso_c<n, Z3>(in, out, g, wts, s); break;
+        ISO_CASE(1) ISO_CASE(2) ISO_CASE(3) ISO_CASE(4) ISO_CASE(5) ISO_CASE(6) ISO_CASE(7) ISO_CASE(8)
+        ISO_CASE(9) ISO_CASE(10) ISO_CASE(11) ISO_CASE(12) ISO_CASE(13) ISO_CASE(14) ISO_CASE(15) ISO_CASE(16)
+        ISO_CASE(17) ISO_CASE(18) ISO_CASE(19) ISO_CASE(20)
+#undef ISO_CASE
+        default: return fail(CLE_E_INTERNAL, "isotropic blur: half-width %d out of range", c);
+    }
+    CLE_LAUNCHED();
+    return CLE_OK;
+}
+
+// Isotropic fused path: destination float32, source f32/u16/u8, equal sigma on every
+// blurred axis (x, y, and z for 3D), half-width 1..ISO_MAXC.  Sets *ran.
+static int try_iso(const cle_buf* src, const cle_buf* dst, const Taps* taps[3], int* ran, hipStream_t s) {
+    *ran = 0;
+    if (dst->dtype != CLE_F32 || dst->data == src->data) return CLE_OK;
+    if (src->dtype != CLE_F32 && src->dtype != CLE_U16 && src->dtype != CLE_U8) return CLE_OK;
+    Geom g;
+    CLE_TRY(check_buf(src, "source", &g));
+    const bool z3 = g.ndim == 3;
+    if (!taps[0] || !taps[1] || (z3 && !taps[2])) return CLE_OK;
+    const int c = taps[0]->c;
+    if (c < 1 || c > ISO_MAXC || taps[1]->c != c || (z3 && taps[2]->c != c)) return CLE_OK;
+    for (int v = 0; v <= c; ++v)
+        if (taps[1]->w[v] != taps[0]->w[v] || (z3 && taps[2]->w[v] != taps[0]->w[v])) return CLE_OK;
+    float* out = (float*)dst->data;
+    const float* wts = taps[0]->w;
+    int rc;
+    if (src->dtype == CLE_F32) {
+        LdF32 in{(const float*)src->data};
+        rc = z3 ? launch_iso<true>(in, out, g, c, wts, s) : launch_iso<false>(in, out, g, c, wts, s);
+    } else if (src->dtype == CLE_U16) {
+        LdU16 in{(const uint16_t*)src->data};
+        rc = z3 ? launch_iso<true>(in, out, g, c, wts, s) : launch_iso<false>(in, out, g, c, wts, s);
+    } else {
+        LdU8 in{(const uint8_t*)src->data};
+        rc = z3 ? launch_iso<true>(in, out, g, c, wts, s) : launch_iso<false>(in, out, g, c, wts, s);
+    }
+    if (rc == CLE_OK) *ran = 1;
+    return rc;
+}
+
 template <typename LD, typename ST>
 static int launch_axis(LD in, ST out, const Geom& g, int axis, const Taps& t, hipStream_t s) {
     if (axis == 0) {
@@ -422,6 +593,8 @@ static int separable_impl(const cle_buf* src, const cle_buf* dst, const Taps* ta
     DBuf in = dbuf(src), out = dbuf(dst);
     if (npass > 0 && gs.ndim >= 2) {
         int ran = 0;
+        CLE_TRY(try_iso(src, dst, taps, &ran, s));
+        if (ran) return CLE_OK;
         CLE_TRY(try_fused(src, dst, taps, &ran, s));
         if (ran) return CLE_OK;
     }

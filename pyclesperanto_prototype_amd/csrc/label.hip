@@ -55,35 +55,38 @@ __device__ __forceinline__ void unite(uint32_t* L, uint32_t a, uint32_t b) {
 }
 
 // ----------------------------------------------------------------- tiled union-find
-// Tiles of CT_Z x CT_Y x CT_X voxels (3D) or 1 x 32 x 64 (2D), 2048 voxels each.
+// Tiles of TZ x TY x TX voxels (compile-time: 8 x 16 x 32 in 3D, 1 x 64 x 64 in 2D,
+// 4096 voxels, 512 threads x 8 voxels).
 // Pass 1 (k_ccl_local): per tile, union-find in LDS over the tile's foreground
 //   (links to the smallest local index), then L[i] = global index of i's local root
 //   for every foreground voxel, K[local root] = the component's smallest 0-based
 //   x-major key inside the tile, and the tile's local roots go to its root list.
-// Pass 2 (k_ccl_border): foreground voxels on a tile's low faces unite, in global
-//   memory, with their foreground neighbours in the preceding tiles.
+// Pass 2 (k_ccl_border): foreground voxels on a tile face unite, in global memory,
+//   with their foreground neighbours (13 earlier directions) in other tiles, skipping
+//   neighbours whose local root repeats the previous one.
 // Pass 3 (k_ccl_resolve): every local root finds its global root r, sets
 //   L[local root] = r and lowers K[r] to its own key.
 // Pass 4 (k_ccl_mark_roots): global roots set their key bit.
 // Pass 5 (k_ccl_write): label = rank(key bit of K[L[L[i]]]).
-constexpr int CT_N = 2048;
-constexpr int CT_THREADS = 256;
-constexpr int CT_PER = CT_N / CT_THREADS;
+constexpr int CT_THREADS = 512;
 
-struct CclTiles {
-    int tz, ty, tx;             // tile extents
-    int64_t nz, ny, nx;         // tile counts
+template <int TZ_, int TY_, int TX_>
+struct TileShape {
+    static constexpr int TZ = TZ_, TY = TY_, TX = TX_, N = TZ_ * TY_ * TX_;
+    static constexpr int PER = N / CT_THREADS;
+    static constexpr int ROOTS = N / 2;  // a tile holds at most N/2 components
+};
+using Tile3D = TileShape<8, 16, 32>;
+using Tile2D = TileShape<1, 64, 64>;
+
+struct TileGrid {
+    int64_t nz, ny, nx;
     __host__ __device__ int64_t count() const { return nz * ny * nx; }
 };
 
-static CclTiles ccl_tiles(const Geom& g) {
-    CclTiles t;
-    if (g.ndim == 3 && g.D > 1) { t.tz = 4; t.ty = 8; t.tx = 64; }
-    else { t.tz = 1; t.ty = 32; t.tx = 64; }
-    t.nz = (g.D + t.tz - 1) / t.tz;
-    t.ny = (g.H + t.ty - 1) / t.ty;
-    t.nx = (g.W + t.tx - 1) / t.tx;
-    return t;
+template <typename TS>
+static TileGrid tile_grid(const Geom& g) {
+    return TileGrid{(g.D + TS::TZ - 1) / TS::TZ, (g.H + TS::TY - 1) / TS::TY, (g.W + TS::TX - 1) / TS::TX};
 }
 
 __device__ __forceinline__ uint32_t lds_load(uint32_t* p) {
@@ -118,71 +121,76 @@ __device__ __forceinline__ void lunite(uint32_t* lab, uint32_t a, uint32_t b) {
     } while (!done);
 }
 
-template <typename FG>
-__global__ __launch_bounds__(CT_THREADS) void k_ccl_local(FG fg, Geom g, CclTiles T, int diamond, uint32_t* L,
+template <typename TS, typename FG>
+__global__ __launch_bounds__(CT_THREADS) void k_ccl_local(FG fg, Geom g, TileGrid T, int diamond, uint32_t* L,
                                                          uint32_t* K, uint32_t* roots, uint32_t* nroots) {
-    __shared__ uint32_t lab[CT_N];
-    __shared__ uint32_t kmin[CT_N];
+    constexpr int N = TS::N, PER = TS::PER, TX = TS::TX, TY = TS::TY, TZ = TS::TZ;
+    __shared__ uint32_t lab[N];
+    __shared__ uint32_t kmin[N];
     __shared__ uint32_t cnt;
     const int64_t tile = blockIdx.x;
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
-    const int64_t x0 = tx * T.tx, y0 = ty * T.ty, z0 = tz * T.tz;
+    const int64_t x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
     const int zr = g.ndim == 3 ? 1 : 0;
     if (threadIdx.x == 0) cnt = 0;
-    int64_t gi[CT_PER];
-    bool f[CT_PER];
+    int64_t gi[PER];
+    bool f[PER];
+    bool anyf = false;
 #pragma unroll
-    for (int k = 0; k < CT_PER; ++k) {
+    for (int k = 0; k < PER; ++k) {
         const int li = threadIdx.x + k * CT_THREADS;
-        const int lx = li % T.tx, ly = (li / T.tx) % T.ty, lz = li / (T.tx * T.ty);
+        const int lx = li % TX, ly = (li / TX) % TY, lz = li / (TX * TY);
         const int64_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
-        const bool in = lz < T.tz && x < g.W && y < g.H && z < g.D;
-        gi[k] = in ? (z * g.H + y) * g.W + x : -1;
+        const bool in = x < g.W && y < g.H && z < g.D;
+        gi[k] = in ? (z * g.H + y) * g.W + x : 0;
         f[k] = in && fg(gi[k]);
+        anyf |= f[k];
         lab[li] = f[k] ? (uint32_t)li : NONE;
         kmin[li] = NONE;
     }
-    __syncthreads();
+    if (!__syncthreads_or(anyf ? 1 : 0)) {
+        if (threadIdx.x == 0) nroots[tile] = 0;
+        return;
+    }
 #pragma unroll
-    for (int k = 0; k < CT_PER; ++k) {
+    for (int k = 0; k < PER; ++k) {
         if (!f[k]) continue;
         const int li = threadIdx.x + k * CT_THREADS;
-        const int lx = li % T.tx, ly = (li / T.tx) % T.ty, lz = li / (T.tx * T.ty);
+        const int lx = li % TX, ly = (li / TX) % TY, lz = li / (TX * TY);
         for (int dz = -zr; dz <= 0; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx) {
                     if (dz == 0 && (dy > 0 || (dy == 0 && dx >= 0))) continue;
                     if (diamond && (abs(dx) + abs(dy) + abs(dz)) != 1) continue;
                     const int qx = lx + dx, qy = ly + dy, qz = lz + dz;
-                    if (qx < 0 || qx >= T.tx || qy < 0 || qy >= T.ty || qz < 0) continue;
-                    const int lq = (qz * T.ty + qy) * T.tx + qx;
+                    if (qx < 0 || qx >= TX || qy < 0 || qy >= TY || qz < 0) continue;
+                    const int lq = (qz * TY + qy) * TX + qx;
                     if (lds_load(lab + lq) == NONE) continue;
                     lunite(lab, (uint32_t)li, (uint32_t)lq);
                 }
     }
     __syncthreads();
-    uint32_t r[CT_PER];
+    uint32_t r[PER];
 #pragma unroll
-    for (int k = 0; k < CT_PER; ++k) {
+    for (int k = 0; k < PER; ++k) {
         const int li = threadIdx.x + k * CT_THREADS;
         r[k] = f[k] ? lfind(lab, (uint32_t)li) : NONE;
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < CT_PER; ++k) {
+    for (int k = 0; k < PER; ++k) {
         if (!f[k]) continue;
         const int li = threadIdx.x + k * CT_THREADS;
-        const int lx = li % T.tx, ly = (li / T.tx) % T.ty, lz = li / (T.tx * T.ty);
+        const int lx = li % TX, ly = (li / TX) % TY, lz = li / (TX * TY);
         const int64_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
         atomicMin(kmin + r[k], (uint32_t)(x * g.H * g.D + y * g.D + z));
-        // global index of the local root
-        const int rx = (int)(r[k] % T.tx), ry = (int)((r[k] / T.tx) % T.ty), rz = (int)(r[k] / (T.tx * T.ty));
+        const int rx = (int)(r[k] % TX), ry = (int)((r[k] / TX) % TY), rz = (int)(r[k] / (TX * TY));
         L[gi[k]] = (uint32_t)(((z0 + rz) * g.H + (y0 + ry)) * g.W + (x0 + rx));
     }
     __syncthreads();
-    uint32_t* myroots = roots + tile * (CT_N / 2);
+    uint32_t* myroots = roots + tile * TS::ROOTS;
 #pragma unroll
-    for (int k = 0; k < CT_PER; ++k) {
+    for (int k = 0; k < PER; ++k) {
         const int li = threadIdx.x + k * CT_THREADS;
         if (f[k] && r[k] == (uint32_t)li) {
             K[gi[k]] = kmin[li];
@@ -196,46 +204,52 @@ __global__ __launch_bounds__(CT_THREADS) void k_ccl_local(FG fg, Geom g, CclTile
 
 // foreground voxels on the faces of each tile unite with their foreground neighbours
 // (in the 13 earlier directions) that lie in another tile
-template <typename FG>
-__global__ __launch_bounds__(CT_THREADS) void k_ccl_border(FG fg, Geom g, CclTiles T, int diamond, uint32_t* L,
+template <typename TS, typename FG>
+__global__ __launch_bounds__(CT_THREADS) void k_ccl_border(FG fg, Geom g, TileGrid T, int diamond, uint32_t* L,
                                                           const uint32_t* nroots) {
+    constexpr int N = TS::N, TX = TS::TX, TY = TS::TY, TZ = TS::TZ;
     const int64_t tile = blockIdx.x;
     if (nroots[tile] == 0) return;  // no foreground in this tile
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
-    const int64_t x0 = tx * T.tx, y0 = ty * T.ty, z0 = tz * T.tz;
+    const int64_t x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
     const int zr = g.ndim == 3 ? 1 : 0;
-    for (int li = threadIdx.x; li < CT_N; li += CT_THREADS) {
-        const int lx = li % T.tx, ly = (li / T.tx) % T.ty, lz = li / (T.tx * T.ty);
-        if (lz >= T.tz) continue;
+    for (int li = threadIdx.x; li < N; li += CT_THREADS) {
+        const int lx = li % TX, ly = (li / TX) % TY, lz = li / (TX * TY);
         // pairs with a neighbour in another tile: low z/y/x faces, and the high y/x
         // faces (their dz = -1 / dy = -1 neighbours reach into the next y / x tile)
-        if (!((zr && lz == 0) || ly == 0 || lx == 0 || ly == T.ty - 1 || lx == T.tx - 1)) continue;
+        if (!((zr && lz == 0) || ly == 0 || lx == 0 || ly == TY - 1 || lx == TX - 1)) continue;
         const int64_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
         if (x >= g.W || y >= g.H || z >= g.D) continue;
         const int64_t i = (z * g.H + y) * g.W + x;
         if (!fg(i)) continue;
+        const uint32_t li_root = ld_relaxed(L + i);
+        uint32_t last = NONE;
         for (int dz = -zr; dz <= 0; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx) {
                     if (dz == 0 && (dy > 0 || (dy == 0 && dx >= 0))) continue;
                     if (diamond && (abs(dx) + abs(dy) + abs(dz)) != 1) continue;
                     // only neighbours outside this tile
-                    if (lz + dz >= 0 && ly + dy >= 0 && lx + dx >= 0 && lx + dx < T.tx && ly + dy < T.ty) continue;
+                    if (lz + dz >= 0 && ly + dy >= 0 && lx + dx >= 0 && lx + dx < TX && ly + dy < TY) continue;
                     const int64_t qx = x + dx, qy = y + dy, qz = z + dz;
                     if (qx < 0 || qx >= g.W || qy < 0 || qy >= g.H || qz < 0) continue;
                     const int64_t q = (qz * g.H + qy) * g.W + qx;
                     if (!fg(q)) continue;
-                    unite(L, ld_relaxed(L + i), ld_relaxed(L + q));
+                    const uint32_t lq = ld_relaxed(L + q);
+                    if (lq == last) continue;  // same local component as the previous neighbour
+                    last = lq;
+                    unite(L, li_root, lq);
                 }
     }
 }
 
+template <typename TS>
 __global__ __launch_bounds__(CT_THREADS) void k_ccl_resolve(uint32_t* L, uint32_t* K, const uint32_t* roots,
                                                             const uint32_t* nroots) {
     const int64_t tile = blockIdx.x;
     const uint32_t n = nroots[tile];
     for (uint32_t k = threadIdx.x; k < n; k += CT_THREADS) {
-        const uint32_t lr = roots[tile * (CT_N / 2) + k];
+        const uint32_t lr = roots[tile * TS::ROOTS + k];
         const uint32_t r = find_root(L, lr);
         if (r != lr) {
             atomicMin(K + r, K[lr]);
@@ -244,13 +258,14 @@ __global__ __launch_bounds__(CT_THREADS) void k_ccl_resolve(uint32_t* L, uint32_
     }
 }
 
+template <typename TS>
 __global__ __launch_bounds__(CT_THREADS) void k_ccl_mark_roots(const uint32_t* L, const uint32_t* K,
                                                                const uint32_t* roots, const uint32_t* nroots,
                                                                int compat, uint32_t* bm) {
     const int64_t tile = blockIdx.x;
     const uint32_t n = nroots[tile];
     for (uint32_t k = threadIdx.x; k < n; k += CT_THREADS) {
-        const uint32_t lr = roots[tile * (CT_N / 2) + k];
+        const uint32_t lr = roots[tile * TS::ROOTS + k];
         if (L[lr] != lr) continue;  // not a global root
         const uint64_t b = (uint64_t)K[lr] + 1;
         const uint64_t bit = compat ? (uint64_t)(float)b : b;
@@ -412,6 +427,40 @@ __global__ void k_ccl_write(FG fg, const uint32_t* L, const uint32_t* K, int64_t
     }
 }
 
+// uint8 binary, 32-bit destination: four voxels per thread (one 4-byte load, four
+// 4-byte stores as one 16-byte store when all four are background)
+template <typename T, typename ST>
+__global__ void k_ccl_write_u8x4(const uint8_t* bin, const uint32_t* L, const uint32_t* K, int64_t n, int compat,
+                                 const uint32_t* bm, const uint32_t* prefix, T* dst, ST out) {
+    const int64_t nq = n / 4;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t b4 = ((const uint32_t*)bin)[q];
+        if (b4 == 0) {
+            ((uint4*)dst)[q] = make_uint4(0u, 0u, 0u, 0u);
+            continue;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int64_t i = q * 4 + k;
+            float lab = 0.f;
+            if ((b4 >> (8 * k)) & 0xFFu) {
+                const uint32_t r = L[L[i]];
+                lab = (float)bit_rank(bm, prefix, key_bit(K[r], compat));
+            }
+            out(i, lab);
+        }
+    }
+    for (int64_t i = nq * 4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float lab = 0.f;
+        if (bin[i]) {
+            const uint32_t r = L[L[i]];
+            lab = (float)bit_rank(bm, prefix, key_bit(K[r], compat));
+        }
+        out(i, lab);
+    }
+}
+
 template <typename FG>
 static void launch_ccl_write(FG fg, const uint32_t* L, const uint32_t* K, int64_t n, int compat, const uint32_t* bm,
                              const uint32_t* prefix, DBuf dst, unsigned grid, hipStream_t s) {
@@ -423,37 +472,50 @@ static void launch_ccl_write(FG fg, const uint32_t* L, const uint32_t* K, int64_
     }
 }
 
+static void launch_ccl_write(FgU8 fg, const uint32_t* L, const uint32_t* K, int64_t n, int compat, const uint32_t* bm,
+                             const uint32_t* prefix, DBuf dst, unsigned grid, hipStream_t s) {
+    const bool aligned = ((uintptr_t)fg.p & 3) == 0 && ((uintptr_t)dst.p & 15) == 0;
+    if (aligned && dst.dt == CLE_I32)
+        k_ccl_write_u8x4<<<grid, 256, 0, s>>>(fg.p, L, K, n, compat, bm, prefix, (int32_t*)dst.p, StLabI32{(int32_t*)dst.p});
+    else if (aligned && dst.dt == CLE_U32)
+        k_ccl_write_u8x4<<<grid, 256, 0, s>>>(fg.p, L, K, n, compat, bm, prefix, (uint32_t*)dst.p, StLabU32{(uint32_t*)dst.p});
+    else if (aligned && dst.dt == CLE_F32)
+        k_ccl_write_u8x4<<<grid, 256, 0, s>>>(fg.p, L, K, n, compat, bm, prefix, (float*)dst.p, StLabF32{(float*)dst.p});
+    else
+        launch_ccl_write<FgU8>(fg, L, K, n, compat, bm, prefix, dst, grid, s);
+}
+
 static unsigned lg(int64_t n) {
     unsigned g = grid_for((size_t)n, 256);
     return g > 131072 ? 131072 : g;
 }
 
-template <typename FG>
+template <typename TS, typename FG>
 static int ccl_run(FG fg, const Geom& g, DBuf dst, int connectivity, int options, uint32_t* n_labels_dev,
                    hipStream_t s) {
     const int64_t n = g.n();
     const int compat = (options & CLE_FP32_KEY_COMPAT) ? 1 : 0;
     const int diamond = connectivity == CLE_CONN_DIAMOND;
-    const CclTiles T = ccl_tiles(g);
+    const TileGrid T = tile_grid<TS>(g);
     const int64_t nt = T.count();
     Scratch Ls, Ks, Rs, Ns;
     CLE_TRY(Ls.alloc((size_t)n * 4, s));
     CLE_TRY(Ks.alloc((size_t)n * 4, s));
-    CLE_TRY(Rs.alloc((size_t)nt * (CT_N / 2) * 4, s));
+    CLE_TRY(Rs.alloc((size_t)nt * TS::ROOTS * 4, s));
     CLE_TRY(Ns.alloc((size_t)nt * 4, s));
     uint32_t *l = Ls.get<uint32_t>(), *k = Ks.get<uint32_t>(), *roots = Rs.get<uint32_t>(), *nr = Ns.get<uint32_t>();
-    k_ccl_local<<<(unsigned)nt, CT_THREADS, 0, s>>>(fg, g, T, diamond, l, k, roots, nr);
-    k_ccl_border<<<(unsigned)nt, CT_THREADS, 0, s>>>(fg, g, T, diamond, l, nr);
-    k_ccl_resolve<<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr);
+    k_ccl_local<TS><<<(unsigned)nt, CT_THREADS, 0, s>>>(fg, g, T, diamond, l, k, roots, nr);
+    k_ccl_border<TS><<<(unsigned)nt, CT_THREADS, 0, s>>>(fg, g, T, diamond, l, nr);
+    k_ccl_resolve<TS><<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr);
     CLE_LAUNCHED();
     // bit positions run to float32(n) (key n rounded up at most to the next float)
     const uint64_t maxbit = compat ? (uint64_t)(float)(uint64_t)n : (uint64_t)n;
     RankTable rt;
     CLE_TRY(rt.init(maxbit + 1, s));
-    k_ccl_mark_roots<<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>());
+    k_ccl_mark_roots<TS><<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>());
     CLE_LAUNCHED();
     CLE_TRY(rt.scan(n_labels_dev, s));
-    launch_ccl_write(fg, l, k, n, compat, rt.bm.get<uint32_t>(), rt.prefix.get<uint32_t>(), dst, lg(n), s);
+    launch_ccl_write(fg, l, k, n, compat, rt.bm.get<uint32_t>(), rt.prefix.get<uint32_t>(), dst, lg(n / 4 + 1), s);
     CLE_LAUNCHED();
     return CLE_OK;
 }
@@ -461,9 +523,15 @@ static int ccl_run(FG fg, const Geom& g, DBuf dst, int connectivity, int options
 int ccl_impl(DBuf binary, const Geom& g, DBuf dst, int connectivity, int options, uint32_t* n_labels_dev,
              hipStream_t s) {
     if (g.n() == 0) return n_labels_dev ? fill_u32(n_labels_dev, 0u, 1, s) : CLE_OK;
-    if (binary.dt == CLE_U8 || binary.dt == CLE_I8)
-        return ccl_run(FgU8{(const uint8_t*)binary.p}, g, dst, connectivity, options, n_labels_dev, s);
-    return ccl_run(FgAny{binary}, g, dst, connectivity, options, n_labels_dev, s);
+    const bool three_d = g.ndim == 3 && g.D > 1;
+    if (binary.dt == CLE_U8 || binary.dt == CLE_I8) {
+        FgU8 fg{(const uint8_t*)binary.p};
+        return three_d ? ccl_run<Tile3D>(fg, g, dst, connectivity, options, n_labels_dev, s)
+                       : ccl_run<Tile2D>(fg, g, dst, connectivity, options, n_labels_dev, s);
+    }
+    FgAny fg{binary};
+    return three_d ? ccl_run<Tile3D>(fg, g, dst, connectivity, options, n_labels_dev, s)
+                   : ccl_run<Tile2D>(fg, g, dst, connectivity, options, n_labels_dev, s);
 }
 
 // ----------------------------------------------------------------- relabel / spots

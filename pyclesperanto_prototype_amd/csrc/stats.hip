@@ -1,52 +1,40 @@
-// statistics_of_labelled_pixels as two segmented reductions.
+// statistics_of_labelled_pixels, reproducing the reference's float32 accumulation.
 //
 // Reference semantics: pyclesperanto_prototype/_tier9/_statistics_of_labelled_pixels.py:43-291,
-// statistics_per_label_x.cl:1-146 (pass 1: count, coordinate sums, intensity sums,
-// intensity-weighted coordinate sums, min/max intensity, bounding box) and
-// standard_deviation_per_label_x.cl:1-96 (pass 2, from the float32 centroid, mass
-// centre, mean and area of pass 1: sums of Euclidean distances, their maxima, and
-// the sum of (v - mean)^2 / area).  Column layout of the output table:
-// _tier9/_statistics_entry.py:3-46 (STATISTICS_ENTRY).
+// statistics_per_label_x.cl:1-146 and standard_deviation_per_label_x.cl:1-96.  The
+// reference keeps one float32 cell per (plane, row y, label); the kernel for slice z
+// walks row y along x and adds every voxel of label l to cell (y, l) in turn, so a
+// cell holds the float32 running sum of its voxels in (z, x) order.  The per-label
+// result is the float32 sum of the cells over y in y order (min / max over y for the
+// extreme planes).  Pass 2 does the same with distances to the float32 centroid and
+// mass centre and with (v - mean)^2 / area.  Column layout of the output table:
+// _tier9/_statistics_entry.py:3-46.
 //
-// MI355X design.  Each thread walks a run of RUN consecutive voxels of one row,
-// accumulating while the label stays the same and flushing a partial (one label,
-// one row segment) with atomics into per-label accumulators: exact integer sums
-// (uint64) for counts and coordinates, float64 for intensity sums, ordered-uint32
-// atomics for min/max.  The per-voxel float32 arithmetic (x*v products, distances,
-// squared deviations) is the reference's; only the summation is carried in higher
-// precision, so results differ from the reference's float32 running sums by that
-// rounding only.
+// MI355X design.  (1) Label runs: each wavefront takes a 1024-voxel row span, tests it
+// for labels with coalesced loads and skips it if empty; lanes emit the runs of equal
+// non-zero labels in their 16 voxels as (label, y, z, x0) keys.  A first pass counts
+// the runs (one synchronisation to size the buffers), a second writes them.  (2) A
+// radix sort orders the runs by key, i.e. by cell (label, y) and, inside a cell, in
+// the reference's (z, x) order.  (3) One thread per cell accumulates its voxels in
+// float32 with the reference's per-voxel operations (products and sums rounded
+// separately).  (4) One thread per label adds its cells in y order.  Only labelled
+// voxels are read after the run pass, so the cost is one read of the label image plus
+// the foreground.
 #include <float.h>
+
+#include <hipcub/hipcub.hpp>
 
 #include "common.hpp"
 
 namespace cle {
 
 constexpr int RUN = 16;
+constexpr int SPAN = 64 * RUN;
 
-struct Acc {
-    unsigned long long *cnt, *sx, *sy, *sz;
-    double *sv, *sxv, *syv, *szv;
-    uint32_t *minv, *maxv;              // ordered-float encoding
-    uint32_t *bminx, *bmaxx, *bminy, *bmaxy, *bminz, *bmaxz;
-    double *sdc, *sdm, *ssq;
-    uint32_t *maxdc, *maxdm;            // float bits of non-negative values
-    float* img;                         // [8][L]: centroid xyz, mass centre xyz, mean, area
-};
-
-__device__ __forceinline__ uint32_t f2o(float f) {
-    uint32_t b = __float_as_uint(f);
-    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-}
-__device__ __forceinline__ float o2f(uint32_t o) {
-    uint32_t b = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
-    return __uint_as_float(b);
-}
-
-// Label readers: the reference reads a label as (int) of the float it loads.
 template <typename T>
 struct LabT {
     const T* p;
+    // the reference reads a label as (int) of the float it loads
     __device__ __forceinline__ int operator()(int64_t i) const { return (int)(float)p[i]; }
     __device__ __forceinline__ bool nz(int64_t i) const { return p[i] != (T)0; }
 };
@@ -56,11 +44,30 @@ struct LabAny {
     __device__ __forceinline__ bool nz(int64_t i) const { return cle::nz(b, i); }
 };
 
-// Work split shared by both passes: a wavefront takes a span of SPAN voxels of one
-// row; it first checks, with coalesced loads, whether any label in the span is
-// non-zero, and skips the span if not.  Otherwise lane l walks voxels
-// [x0 + l*RUN, x0 + (l+1)*RUN) of the span.
-constexpr int SPAN = 64 * RUN;
+struct KeyBits {
+    int xb, zb, yb, lb;
+    __host__ __device__ int total() const { return xb + zb + yb + lb; }
+};
+
+__device__ __forceinline__ uint64_t run_key(const KeyBits& kb, uint64_t l, uint64_t y, uint64_t z, uint64_t x0) {
+    return (((l << kb.yb | y) << kb.zb | z) << kb.xb) | x0;
+}
+
+// Runs of one lane's voxels [x0, x1) of row `base`; calls emit(label, xs, xe).
+template <typename LAB, typename F>
+__device__ __forceinline__ void lane_runs(LAB lab, int64_t base, int64_t x0, int64_t x1, int64_t L, F emit) {
+    int cur = 0;
+    int64_t xs = x0;
+    for (int64_t x = x0; x < x1; ++x) {
+        const int l = lab(base + x);
+        if (l != cur) {
+            if (cur > 0 && cur < L) emit(cur, xs, x);
+            cur = l;
+            xs = x;
+        }
+    }
+    if (cur > 0 && cur < L) emit(cur, xs, x1);
+}
 
 template <typename LAB>
 __device__ __forceinline__ bool span_has_labels(LAB lab, int64_t base, int64_t x0, int64_t x1, int lane) {
@@ -73,9 +80,11 @@ __device__ __forceinline__ bool span_has_labels(LAB lab, int64_t base, int64_t x
     return __any(any);
 }
 
-template <typename LAB>
-__global__ __launch_bounds__(256) void k_stats_pass1(DBuf inten, LAB lab, Geom g, int64_t L, Acc a) {
-#pragma clang fp contract(off)
+// EMIT = false: count runs into *cursor; EMIT = true: write keys / ends at slots
+// reserved from *cursor (one atomic per wavefront).
+template <bool EMIT, typename LAB>
+__global__ __launch_bounds__(256) void k_runs(LAB lab, Geom g, int64_t L, KeyBits kb, unsigned long long* cursor,
+                                              uint64_t* keys, uint32_t* ends) {
     const int64_t spans = (g.W + SPAN - 1) / SPAN;
     const int64_t total = g.D * g.H * spans;
     const int lane = threadIdx.x & 63;
@@ -89,120 +98,179 @@ __global__ __launch_bounds__(256) void k_stats_pass1(DBuf inten, LAB lab, Geom g
         if (!span_has_labels(lab, base, sx0, sx1, lane)) continue;
         const int64_t x0 = sx0 + (int64_t)lane * RUN;
         const int64_t x1 = x0 + RUN < sx1 ? x0 + RUN : sx1;
-        if (x0 >= x1) continue;
-        const int64_t y = row % g.H, z = row / g.H;
-        int cur = -1;
-        unsigned long long c = 0, sx = 0;
-        double sv = 0, sxv = 0, syv = 0, szv = 0;
-        float mn = 0, mx = 0;
-        int64_t xa = 0, xb = 0;
-        const float yf = (float)y, zf = (float)z;
-        for (int64_t x = x0; x <= x1; ++x) {
-            const int l = x < x1 ? lab(base + x) : -1;
-            if (l != cur) {
-                if (cur > 0 && cur < L) {
-                    atomicAdd(a.cnt + cur, c);
-                    atomicAdd(a.sx + cur, sx);
-                    atomicAdd(a.sy + cur, c * (unsigned long long)y);
-                    atomicAdd(a.sz + cur, c * (unsigned long long)z);
-                    atomicAdd(a.sv + cur, sv);
-                    atomicAdd(a.sxv + cur, sxv);
-                    atomicAdd(a.syv + cur, syv);
-                    atomicAdd(a.szv + cur, szv);
-                    atomicMin(a.minv + cur, f2o(mn));
-                    atomicMax(a.maxv + cur, f2o(mx));
-                    atomicMin(a.bminx + cur, (uint32_t)xa);
-                    atomicMax(a.bmaxx + cur, (uint32_t)xb);
-                    atomicMin(a.bminy + cur, (uint32_t)y);
-                    atomicMax(a.bmaxy + cur, (uint32_t)y);
-                    atomicMin(a.bminz + cur, (uint32_t)z);
-                    atomicMax(a.bmaxz + cur, (uint32_t)z);
-                }
-                cur = l;
-                c = 0; sx = 0; sv = 0; sxv = 0; syv = 0; szv = 0;
-                xa = x;
-                if (x < x1 && l > 0) { mn = mx = ld(inten, base + x); }
-            }
-            if (x == x1) break;
-            if (l > 0) {
-                const float v = ld(inten, base + x);
-                const float xf = (float)x;
-                c += 1;
-                sx += (unsigned long long)x;
-                sv += (double)v;
-                sxv += (double)(xf * v);
-                syv += (double)(yf * v);
-                szv += (double)(zf * v);
-                mn = fminf(mn, v);
-                mx = fmaxf(mx, v);
-                xb = x;
-            }
+        uint32_t cnt = 0;
+        if (x0 < x1) lane_runs(lab, base, x0, x1, L, [&](int, int64_t, int64_t) { ++cnt; });
+        // inclusive prefix of the lane counts
+        uint32_t inc = cnt;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t t = __shfl_up(inc, off);
+            if (lane >= off) inc += t;
+        }
+        const uint32_t wave_total = __shfl(inc, 63);
+        if (!EMIT) {
+            if (lane == 0 && wave_total) atomicAdd(cursor, (unsigned long long)wave_total);
+            continue;
+        }
+        unsigned long long wbase = 0;
+        if (lane == 0 && wave_total) wbase = atomicAdd(cursor, (unsigned long long)wave_total);
+        wbase = __shfl(wbase, 0);
+        uint64_t slot = wbase + inc - cnt;
+        if (x0 < x1) {
+            const uint64_t y = (uint64_t)(row % g.H), z = (uint64_t)(row / g.H);
+            lane_runs(lab, base, x0, x1, L, [&](int l, int64_t xs, int64_t xe) {
+                keys[slot] = run_key(kb, (uint64_t)l, y, z, (uint64_t)xs);
+                ends[slot] = (uint32_t)xe;
+                ++slot;
+            });
         }
     }
 }
 
-// float32 centroid, mass centre, mean and area per label, in the reference's order
-// of operations (divide_images of float32 sums)
-__global__ void k_stats_mid(Acc a, int64_t L) {
+// cell (label, y) boundaries in the sorted runs
+__global__ void k_cell_flags(const uint64_t* keys, int64_t n, int cell_shift, uint32_t* flags) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        flags[i] = (i == 0 || (keys[i] >> cell_shift) != (keys[i - 1] >> cell_shift)) ? 1u : 0u;
+}
+
+__global__ void k_cell_scatter(const uint32_t* flags, const uint32_t* cidx, int64_t n, uint32_t* cell_start,
+                               uint32_t* ncells) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (flags[i]) cell_start[cidx[i]] = (uint32_t)i;
+        if (i == n - 1) {
+            const uint32_t total = cidx[i] + flags[i];
+            cell_start[total] = (uint32_t)n;
+            *ncells = total;
+        }
+    }
+}
+
+// [lstart[l], lend[l]) = the cells of label l (cells are sorted by label, then y)
+__global__ void k_label_ranges(const uint64_t* keys, const uint32_t* cell_start, const uint32_t* ncells_p,
+                               int64_t cap, KeyBits kb, uint32_t* lstart, uint32_t* lend) {
+    const uint32_t ncells = *ncells_p;
+    const int lshift = kb.yb + kb.zb + kb.xb;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < cap && c < ncells;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t l = keys[cell_start[c]] >> lshift;
+        if (c == 0 || (keys[cell_start[c - 1]] >> lshift) != l) lstart[l] = (uint32_t)c;
+        if (c == (int64_t)ncells - 1 || (keys[cell_start[c + 1]] >> lshift) != l) lend[l] = (uint32_t)c + 1;
+    }
+}
+
+struct CellGeom {
+    KeyBits kb;
+    int64_t H, W;
+    __device__ __forceinline__ void decode(uint64_t key, int64_t& y, int64_t& z, int64_t& x0) const {
+        x0 = (int64_t)(key & ((1ull << kb.xb) - 1));
+        z = (int64_t)((key >> kb.xb) & ((1ull << kb.zb) - 1));
+        y = (int64_t)((key >> (kb.xb + kb.zb)) & ((1ull << kb.yb) - 1));
+    }
+};
+
+// pass 1, one thread per cell: 16 float32 planes in the reference's order:
+// 0-2 sum x,y,z  3 count  4-6 sum x*v, y*v, z*v  7 sum v  8/9 min/max v  10-15 min/max x,y,z
+__global__ __launch_bounds__(256) void k_cells_pass1(DBuf inten, CellGeom cg, const uint64_t* keys,
+                                                     const uint32_t* ends, const uint32_t* cell_start,
+                                                     const uint32_t* ncells_p, int64_t cap, float* acc) {
+#pragma clang fp contract(off)
+    const uint32_t ncells = *ncells_p;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < cap && c < ncells;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t r0 = cell_start[c], r1 = cell_start[c + 1];
+        float s[16];
+        for (int k = 0; k < 8; ++k) s[k] = 0.f;
+        for (int k = 8; k < 16; k += 2) { s[k] = FLT_MAX; s[k + 1] = -FLT_MAX; }
+        for (uint32_t r = r0; r < r1; ++r) {
+            int64_t y, z, x0;
+            cg.decode(keys[r], y, z, x0);
+            const int64_t x1 = ends[r];
+            const float yf = (float)y, zf = (float)z;
+            const int64_t base = (z * cg.H + y) * cg.W;
+            for (int64_t x = x0; x < x1; ++x) {
+                const float v = ld(inten, base + x);
+                const float xf = (float)x;
+                if (s[3] == 0.f) {
+                    s[8] = v; s[9] = v; s[10] = xf; s[11] = xf; s[12] = yf; s[13] = yf; s[14] = zf; s[15] = zf;
+                } else {
+                    s[8] = s[8] > v ? v : s[8];
+                    s[9] = s[9] < v ? v : s[9];
+                    s[10] = s[10] > xf ? xf : s[10];
+                    s[11] = s[11] < xf ? xf : s[11];
+                    s[12] = s[12] > yf ? yf : s[12];
+                    s[13] = s[13] < yf ? yf : s[13];
+                    s[14] = s[14] > zf ? zf : s[14];
+                    s[15] = s[15] < zf ? zf : s[15];
+                }
+                s[0] = s[0] + xf;
+                s[1] = s[1] + yf;
+                s[2] = s[2] + zf;
+                s[3] = s[3] + 1.f;
+                s[4] = s[4] + xf * v;
+                s[5] = s[5] + yf * v;
+                s[6] = s[6] + zf * v;
+                s[7] = s[7] + v;
+            }
+        }
+        for (int k = 0; k < 16; ++k) acc[(int64_t)k * cap + c] = s[k];
+    }
+}
+
+// Per-label results of pass 1: the y-reduced planes [16][L] (sums in y order,
+// min / max), and the float32 image the second pass needs [8][L]: centroid xyz,
+// mass centre xyz, mean, area.
+__global__ void k_labels_pass1(const float* acc, int64_t cap, const uint32_t* lstart, const uint32_t* lend,
+                               int64_t L, float* lab16, float* img) {
 #pragma clang fp contract(off)
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= L) return;
-    const float area = (float)a.cnt[l];
-    const float sv = (float)a.sv[l];
-    a.img[0 * L + l] = (float)a.sx[l] / area;
-    a.img[1 * L + l] = (float)a.sy[l] / area;
-    a.img[2 * L + l] = (float)a.sz[l] / area;
-    a.img[3 * L + l] = (float)a.sxv[l] / sv;
-    a.img[4 * L + l] = (float)a.syv[l] / sv;
-    a.img[5 * L + l] = (float)a.szv[l] / sv;
-    a.img[6 * L + l] = sv / area;
-    a.img[7 * L + l] = area;
+    float s[16];
+    for (int k = 0; k < 8; ++k) s[k] = 0.f;
+    for (int k = 8; k < 16; k += 2) { s[k] = FLT_MAX; s[k + 1] = -FLT_MAX; }
+    const uint32_t c0 = lstart[l], c1 = lend[l];
+    for (uint32_t c = c0; c < c1; ++c) {
+        for (int k = 0; k < 8; ++k) s[k] = s[k] + acc[(int64_t)k * cap + c];
+        for (int k = 8; k < 16; k += 2) {
+            const float mn = acc[(int64_t)k * cap + c], mx = acc[(int64_t)(k + 1) * cap + c];
+            s[k] = s[k] > mn ? mn : s[k];
+            s[k + 1] = s[k + 1] < mx ? mx : s[k + 1];
+        }
+    }
+    for (int k = 0; k < 16; ++k) lab16[(int64_t)k * L + l] = s[k];
+    const float area = s[3], sv = s[7];
+    img[0 * L + l] = s[0] / area;
+    img[1 * L + l] = s[1] / area;
+    img[2 * L + l] = s[2] / area;
+    img[3 * L + l] = s[4] / sv;
+    img[4 * L + l] = s[5] / sv;
+    img[5 * L + l] = s[6] / sv;
+    img[6 * L + l] = sv / area;
+    img[7 * L + l] = area;
 }
 
-template <typename LAB>
-__global__ __launch_bounds__(256) void k_stats_pass2(DBuf inten, LAB lab, Geom g, int64_t L, Acc a) {
+// pass 2, one thread per cell: 0 sum distance to centroid, 1 to mass centre,
+// 2 sum (v - mean)^2 / area, 3 count, 4/5 max distances
+__global__ __launch_bounds__(256) void k_cells_pass2(DBuf inten, CellGeom cg, const uint64_t* keys,
+                                                     const uint32_t* ends, const uint32_t* cell_start,
+                                                     const uint32_t* ncells_p, int64_t cap, const float* img,
+                                                     int64_t L, float* acc2) {
 #pragma clang fp contract(off)
-    const int64_t spans = (g.W + SPAN - 1) / SPAN;
-    const int64_t total = g.D * g.H * spans;
-    const int lane = threadIdx.x & 63;
-    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t sp = wave0; sp < total; sp += nwaves) {
-        const int64_t row = sp / spans;
-        const int64_t sx0 = (sp - row * spans) * SPAN;
-        const int64_t sx1 = sx0 + SPAN < g.W ? sx0 + SPAN : g.W;
-        const int64_t base = row * g.W;
-        if (!span_has_labels(lab, base, sx0, sx1, lane)) continue;
-        const int64_t x0 = sx0 + (int64_t)lane * RUN;
-        const int64_t x1 = x0 + RUN < sx1 ? x0 + RUN : sx1;
-        if (x0 >= x1) continue;
-        const int64_t y = row % g.H, z = row / g.H;
-        const float yf = (float)y, zf = (float)z;
-        int cur = -1;
-        double sdc = 0, sdm = 0, ssq = 0;
-        float mdc = 0, mdm = 0;
-        float cx = 0, cy = 0, cz = 0, mx = 0, my = 0, mz = 0, mean = 0, area = 1;
-        for (int64_t x = x0; x <= x1; ++x) {
-            const int l = x < x1 ? lab(base + x) : -1;
-            if (l != cur) {
-                if (cur > 0 && cur < L) {
-                    atomicAdd(a.sdc + cur, sdc);
-                    atomicAdd(a.sdm + cur, sdm);
-                    atomicAdd(a.ssq + cur, ssq);
-                    atomicMax(a.maxdc + cur, __float_as_uint(mdc));
-                    atomicMax(a.maxdm + cur, __float_as_uint(mdm));
-                }
-                cur = l;
-                sdc = sdm = ssq = 0;
-                mdc = mdm = 0;
-                if (l > 0 && l < L) {
-                    cx = a.img[0 * L + l]; cy = a.img[1 * L + l]; cz = a.img[2 * L + l];
-                    mx = a.img[3 * L + l]; my = a.img[4 * L + l]; mz = a.img[5 * L + l];
-                    mean = a.img[6 * L + l]; area = a.img[7 * L + l];
-                }
-            }
-            if (x == x1) break;
-            if (l > 0) {
+    const uint32_t ncells = *ncells_p;
+    const int lshift = cg.kb.yb + cg.kb.zb + cg.kb.xb;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < cap && c < ncells;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t r0 = cell_start[c], r1 = cell_start[c + 1];
+        const int64_t l = (int64_t)(keys[r0] >> lshift);
+        const float cx = img[0 * L + l], cy = img[1 * L + l], cz = img[2 * L + l];
+        const float mx = img[3 * L + l], my = img[4 * L + l], mz = img[5 * L + l];
+        const float mean = img[6 * L + l], area = img[7 * L + l];
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, m4 = 0.f, m5 = 0.f;
+        for (uint32_t r = r0; r < r1; ++r) {
+            int64_t y, z, x0;
+            cg.decode(keys[r], y, z, x0);
+            const int64_t x1 = ends[r];
+            const float yf = (float)y, zf = (float)z;
+            const int64_t base = (z * cg.H + y) * cg.W;
+            for (int64_t x = x0; x < x1; ++x) {
                 const float v = ld(inten, base + x);
                 const float xf = (float)x;
                 const float dx = xf - cx, dy = yf - cy, dz = zf - cz;
@@ -211,60 +279,154 @@ __global__ __launch_bounds__(256) void k_stats_pass2(DBuf inten, LAB lab, Geom g
                 const float dm = sqrtf((ex * ex + ey * ey) + ez * ez);
                 const float dv = v - mean;
                 const float sq = (dv * dv) / area;
-                sdc += (double)dc;
-                sdm += (double)dm;
-                ssq += (double)sq;
-                mdc = fmaxf(mdc, dc);
-                mdm = fmaxf(mdm, dm);
+                if (s3 == 0.f) { m4 = dc; m5 = dm; }
+                else { m4 = m4 < dc ? dc : m4; m5 = m5 < dm ? dm : m5; }
+                s0 = s0 + dc;
+                s1 = s1 + dm;
+                s2 = s2 + sq;
+                s3 = s3 + 1.f;
             }
         }
+        acc2[0 * cap + c] = s0;
+        acc2[1 * cap + c] = s1;
+        acc2[2 * cap + c] = s2;
+        acc2[3 * cap + c] = s3;
+        acc2[4 * cap + c] = m4;
+        acc2[5 * cap + c] = m5;
     }
 }
 
-template <typename LAB>
-static void stats_passes(DBuf inten, LAB lab, const Geom& g, int64_t L, const Acc& a, hipStream_t s) {
-    const int64_t spans = g.D * g.H * ((g.W + SPAN - 1) / SPAN);
-    unsigned gr = grid_for((size_t)spans, 4);  // 4 wavefronts per block
-    if (gr > 65536) gr = 65536;
-    if (g.n() > 0) k_stats_pass1<<<gr, 256, 0, s>>>(inten, lab, g, L, a);
-    k_stats_mid<<<grid_for((size_t)L, 256), 256, 0, s>>>(a, L);
-    if (g.n() > 0) k_stats_pass2<<<gr, 256, 0, s>>>(inten, lab, g, L, a);
-}
-
-// STATISTICS_ENTRY table, one column per measured label 1..L-1
-__global__ void k_stats_table(Acc a, int64_t L, float* out) {
+// y-reduction of pass 2 and the STATISTICS_ENTRY table, one thread per label 1..L-1
+__global__ void k_labels_table(const float* acc2, int64_t cap, const uint32_t* lstart, const uint32_t* lend,
+                               const float* lab16, const float* img, int64_t L, float* out) {
 #pragma clang fp contract(off)
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (l >= L) return;
-    const int64_t N = L - 1;
-    const int64_t j = l - 1;
-    const bool present = a.cnt[l] > 0;
-    const float minx = present ? (float)a.bminx[l] : FLT_MAX, maxx = present ? (float)a.bmaxx[l] : -FLT_MAX;
-    const float miny = present ? (float)a.bminy[l] : FLT_MAX, maxy = present ? (float)a.bmaxy[l] : -FLT_MAX;
-    const float minz = present ? (float)a.bminz[l] : FLT_MAX, maxz = present ? (float)a.bmaxz[l] : -FLT_MAX;
-    const float area = a.img[7 * L + l];
-    const float sv = (float)a.sv[l];
-    const float sdm = (float)a.sdm[l], sdc = (float)a.sdc[l];
-    const float meandm = sdm / area, meandc = sdc / area;
-    const float maxdm = __uint_as_float(a.maxdm[l]), maxdc = __uint_as_float(a.maxdc[l]);
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, m4 = 0.f, m5 = 0.f;
+    for (uint32_t c = lstart[l]; c < lend[l]; ++c) {
+        s0 = s0 + acc2[0 * cap + c];
+        s1 = s1 + acc2[1 * cap + c];
+        s2 = s2 + acc2[2 * cap + c];
+        // the reference's max over y also sees the zero cells of rows without the label
+        const float a4 = acc2[4 * cap + c], a5 = acc2[5 * cap + c];
+        m4 = m4 < a4 ? a4 : m4;
+        m5 = m5 < a5 ? a5 : m5;
+    }
+    auto pl = [&](int k) { return lab16[(int64_t)k * L + l]; };
+    const float area = pl(3), sv = pl(7);
+    const int64_t N = L - 1, j = l - 1;
     float col[CLE_STAT_COLUMNS];
     col[0] = (float)l;
-    col[1] = minx; col[2] = miny; col[3] = minz;
-    col[4] = maxx; col[5] = maxy; col[6] = maxz;
-    col[7] = maxx - minx + 1.f; col[8] = maxy - miny + 1.f; col[9] = maxz - minz + 1.f;
-    col[10] = present ? o2f(a.minv[l]) : FLT_MAX;
-    col[11] = present ? o2f(a.maxv[l]) : -FLT_MAX;
-    col[12] = a.img[6 * L + l];
+    col[1] = pl(10); col[2] = pl(12); col[3] = pl(14);
+    col[4] = pl(11); col[5] = pl(13); col[6] = pl(15);
+    col[7] = col[4] - col[1] + 1.f; col[8] = col[5] - col[2] + 1.f; col[9] = col[6] - col[3] + 1.f;
+    col[10] = pl(8); col[11] = pl(9);
+    col[12] = img[6 * L + l];
     col[13] = sv;
-    col[14] = sqrtf((float)a.ssq[l]);
+    col[14] = sqrtf(s2);
     col[15] = area;
-    col[16] = (float)a.sxv[l]; col[17] = (float)a.syv[l]; col[18] = (float)a.szv[l];
-    col[19] = a.img[3 * L + l]; col[20] = a.img[4 * L + l]; col[21] = a.img[5 * L + l];
-    col[22] = (float)a.sx[l]; col[23] = (float)a.sy[l]; col[24] = (float)a.sz[l];
-    col[25] = a.img[0 * L + l]; col[26] = a.img[1 * L + l]; col[27] = a.img[2 * L + l];
-    col[28] = sdm; col[29] = meandm; col[30] = maxdm; col[31] = maxdm / meandm;
-    col[32] = sdc; col[33] = meandc; col[34] = maxdc; col[35] = maxdc / meandc;
+    col[16] = pl(4); col[17] = pl(5); col[18] = pl(6);
+    col[19] = img[3 * L + l]; col[20] = img[4 * L + l]; col[21] = img[5 * L + l];
+    col[22] = pl(0); col[23] = pl(1); col[24] = pl(2);
+    col[25] = img[0 * L + l]; col[26] = img[1 * L + l]; col[27] = img[2 * L + l];
+    col[28] = s1; col[29] = s1 / area; col[30] = m5; col[31] = m5 / col[29];
+    col[32] = s0; col[33] = s0 / area; col[34] = m4; col[35] = m4 / col[33];
     for (int c = 0; c < CLE_STAT_COLUMNS; ++c) out[c * N + j] = col[c];
+}
+
+static int bits_for(uint64_t maxval) {
+    int b = 1;
+    while (b < 64 && (maxval >> b) != 0) ++b;
+    return b;
+}
+
+template <typename LAB>
+static int stats_run(DBuf inten, LAB lab, const Geom& g, int64_t L, float* table, hipStream_t s) {
+    KeyBits kb;
+    kb.xb = bits_for((uint64_t)(g.W > 0 ? g.W - 1 : 0));
+    kb.zb = bits_for((uint64_t)(g.D > 0 ? g.D - 1 : 0));
+    kb.yb = bits_for((uint64_t)(g.H > 0 ? g.H - 1 : 0));
+    kb.lb = bits_for((uint64_t)(L - 1));
+    if (kb.total() > 64) return fail(CLE_E_SHAPE, "statistics: %d key bits exceed 64", kb.total());
+    const int64_t spans = g.D * g.H * ((g.W + SPAN - 1) / SPAN);
+    unsigned gr = grid_for((size_t)spans, 4);
+    if (gr > 65536) gr = 65536;
+    // (1) count the runs, size the buffers, write the runs
+    Scratch cur;
+    CLE_TRY(cur.alloc(8, s));
+    CLE_TRY(fill_u64(cur.get<unsigned long long>(), 0ull, 1, s));
+    if (g.n() > 0) k_runs<false><<<gr, 256, 0, s>>>(lab, g, L, kb, cur.get<unsigned long long>(), nullptr, nullptr);
+    CLE_LAUNCHED();
+    unsigned long long nruns = 0;
+    CLE_HIP(hipMemcpyAsync(&nruns, cur.p, 8, hipMemcpyDeviceToHost, s));
+    CLE_HIP(hipStreamSynchronize(s));
+    if (nruns >= 0x7FFFFFFFull) return fail(CLE_E_SHAPE, "statistics: %llu label runs exceed 2^31", nruns);
+    const int64_t n = (int64_t)nruns;
+    const int64_t cap = n > 0 ? n : 1;
+    Scratch k0, k1, e0, e1, tmp, flags, cidx, cstart, ncell, lst, len_, acc, acc2, lab16, img;
+    CLE_TRY(k0.alloc((size_t)cap * 8, s));
+    CLE_TRY(k1.alloc((size_t)cap * 8, s));
+    CLE_TRY(e0.alloc((size_t)cap * 4, s));
+    CLE_TRY(e1.alloc((size_t)cap * 4, s));
+    CLE_TRY(fill_u64(cur.get<unsigned long long>(), 0ull, 1, s));
+    if (n > 0)
+        k_runs<true><<<gr, 256, 0, s>>>(lab, g, L, kb, cur.get<unsigned long long>(), k0.get<uint64_t>(),
+                                        e0.get<uint32_t>());
+    CLE_LAUNCHED();
+    // (2) sort the runs by (label, y, z, x0)
+    size_t tbytes = 0;
+    CLE_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, k0.get<uint64_t>(), k1.get<uint64_t>(),
+                                               e0.get<uint32_t>(), e1.get<uint32_t>(), (int)cap, 0, kb.total(), s));
+    CLE_TRY(tmp.alloc(tbytes, s));
+    if (n > 0)
+        CLE_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tbytes, k0.get<uint64_t>(), k1.get<uint64_t>(),
+                                                   e0.get<uint32_t>(), e1.get<uint32_t>(), (int)n, 0, kb.total(), s));
+    const uint64_t* keys = k1.get<uint64_t>();
+    const uint32_t* ends = e1.get<uint32_t>();
+    // cells and label ranges
+    CLE_TRY(flags.alloc((size_t)cap * 4, s));
+    CLE_TRY(cidx.alloc((size_t)cap * 4, s));
+    CLE_TRY(cstart.alloc((size_t)(cap + 1) * 4, s));
+    CLE_TRY(ncell.alloc(4, s));
+    CLE_TRY(fill_u32(ncell.get<uint32_t>(), 0u, 1, s));
+    CLE_TRY(lst.alloc((size_t)L * 4, s));
+    CLE_TRY(len_.alloc((size_t)L * 4, s));
+    CLE_TRY(fill_u32(lst.get<uint32_t>(), 0u, (size_t)L, s));
+    CLE_TRY(fill_u32(len_.get<uint32_t>(), 0u, (size_t)L, s));
+    const unsigned gn = grid_for((size_t)cap, 256) > 65536 ? 65536 : grid_for((size_t)cap, 256);
+    if (n > 0) {
+        k_cell_flags<<<gn, 256, 0, s>>>(keys, n, kb.zb + kb.xb, flags.get<uint32_t>());
+        size_t sbytes = 0;
+        CLE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, sbytes, flags.get<uint32_t>(), cidx.get<uint32_t>(), (int)n, s));
+        Scratch stmp;
+        CLE_TRY(stmp.alloc(sbytes, s));
+        CLE_HIP(hipcub::DeviceScan::ExclusiveSum(stmp.p, sbytes, flags.get<uint32_t>(), cidx.get<uint32_t>(), (int)n, s));
+        k_cell_scatter<<<gn, 256, 0, s>>>(flags.get<uint32_t>(), cidx.get<uint32_t>(), n, cstart.get<uint32_t>(),
+                                          ncell.get<uint32_t>());
+        k_label_ranges<<<gn, 256, 0, s>>>(keys, cstart.get<uint32_t>(), ncell.get<uint32_t>(), cap, kb,
+                                          lst.get<uint32_t>(), len_.get<uint32_t>());
+    }
+    CLE_LAUNCHED();
+    // (3) + (4): cells, then labels, twice
+    CLE_TRY(acc.alloc((size_t)cap * 16 * 4, s));
+    CLE_TRY(acc2.alloc((size_t)cap * 6 * 4, s));
+    CLE_TRY(lab16.alloc((size_t)L * 16 * 4, s));
+    CLE_TRY(img.alloc((size_t)L * 8 * 4, s));
+    const CellGeom cg{kb, g.H, g.W};
+    if (n > 0)
+        k_cells_pass1<<<gn, 256, 0, s>>>(inten, cg, keys, ends, cstart.get<uint32_t>(), ncell.get<uint32_t>(), cap,
+                                         acc.get<float>());
+    k_labels_pass1<<<grid_for((size_t)L, 256), 256, 0, s>>>(acc.get<float>(), cap, lst.get<uint32_t>(),
+                                                            len_.get<uint32_t>(), L, lab16.get<float>(), img.get<float>());
+    if (n > 0)
+        k_cells_pass2<<<gn, 256, 0, s>>>(inten, cg, keys, ends, cstart.get<uint32_t>(), ncell.get<uint32_t>(), cap,
+                                         img.get<float>(), L, acc2.get<float>());
+    if (L > 1)
+        k_labels_table<<<grid_for((size_t)(L - 1), 256), 256, 0, s>>>(acc2.get<float>(), cap, lst.get<uint32_t>(),
+                                                                      len_.get<uint32_t>(), lab16.get<float>(),
+                                                                      img.get<float>(), L, table);
+    CLE_LAUNCHED();
+    return CLE_OK;
 }
 
 }  // namespace cle
@@ -287,47 +449,15 @@ int cle_statistics_of_labelled_pixels(const cle_buf* intensity, const cle_buf* l
         return fail(CLE_E_SHAPE, "statistics table holds %lld < %lld values", (long long)go.n(),
                     (long long)CLE_STAT_COLUMNS * (L - 1));
     hipStream_t s = (hipStream_t)stream;
-    // accumulators: 11 arrays of 8-byte sums, then 10 arrays of 4-byte extremes and
-    // the 8-row float32 image of pass-1 results
-    const size_t n8 = 11, n4 = 10 + 8;
-    Scratch acc;
-    CLE_TRY(acc.alloc((size_t)L * (n8 * 8 + n4 * 4), s));
-    char* p = acc.get<char>();
-    Acc a;
-    auto take8 = [&](void) { void* r = p; p += (size_t)L * 8; return r; };
-    auto take4 = [&](void) { void* r = p; p += (size_t)L * 4; return r; };
-    a.cnt = (unsigned long long*)take8(); a.sx = (unsigned long long*)take8();
-    a.sy = (unsigned long long*)take8(); a.sz = (unsigned long long*)take8();
-    a.sv = (double*)take8(); a.sxv = (double*)take8(); a.syv = (double*)take8(); a.szv = (double*)take8();
-    a.sdc = (double*)take8(); a.sdm = (double*)take8(); a.ssq = (double*)take8();
-    a.minv = (uint32_t*)take4(); a.maxv = (uint32_t*)take4();
-    a.bminx = (uint32_t*)take4(); a.bmaxx = (uint32_t*)take4(); a.bminy = (uint32_t*)take4();
-    a.bmaxy = (uint32_t*)take4(); a.bminz = (uint32_t*)take4(); a.bmaxz = (uint32_t*)take4();
-    a.maxdc = (uint32_t*)take4(); a.maxdm = (uint32_t*)take4();
-    a.img = (float*)p;
-    // zero the sums, set min/max sentinels
-    CLE_TRY(fill_u64((unsigned long long*)acc.p, 0ull, (size_t)L * n8, s));
-    CLE_TRY(fill_u32(a.minv, 0xFFFFFFFFu, (size_t)L, s));
-    CLE_TRY(fill_u32(a.maxv, 0u, (size_t)L, s));
-    CLE_TRY(fill_u32(a.bminx, 0xFFFFFFFFu, (size_t)L, s));
-    CLE_TRY(fill_u32(a.bmaxx, 0u, (size_t)L, s));
-    CLE_TRY(fill_u32(a.bminy, 0xFFFFFFFFu, (size_t)L, s));
-    CLE_TRY(fill_u32(a.bmaxy, 0u, (size_t)L, s));
-    CLE_TRY(fill_u32(a.bminz, 0xFFFFFFFFu, (size_t)L, s));
-    CLE_TRY(fill_u32(a.bmaxz, 0u, (size_t)L, s));
-    CLE_TRY(fill_u32(a.maxdc, 0u, (size_t)L, s));
-    CLE_TRY(fill_u32(a.maxdm, 0u, (size_t)L, s));
     const DBuf in = dbuf(intensity);
+    float* table = (float*)out_table->data;
     switch (labels->dtype) {
-        case CLE_I32: stats_passes(in, LabT<int32_t>{(const int32_t*)labels->data}, gi, L, a, s); break;
-        case CLE_U32: stats_passes(in, LabT<uint32_t>{(const uint32_t*)labels->data}, gi, L, a, s); break;
-        case CLE_F32: stats_passes(in, LabT<float>{(const float*)labels->data}, gi, L, a, s); break;
-        case CLE_U16: stats_passes(in, LabT<uint16_t>{(const uint16_t*)labels->data}, gi, L, a, s); break;
-        default: stats_passes(in, LabAny{dbuf(labels)}, gi, L, a, s); break;
+        case CLE_I32: return stats_run(in, LabT<int32_t>{(const int32_t*)labels->data}, gi, L, table, s);
+        case CLE_U32: return stats_run(in, LabT<uint32_t>{(const uint32_t*)labels->data}, gi, L, table, s);
+        case CLE_F32: return stats_run(in, LabT<float>{(const float*)labels->data}, gi, L, table, s);
+        case CLE_U16: return stats_run(in, LabT<uint16_t>{(const uint16_t*)labels->data}, gi, L, table, s);
+        default: return stats_run(in, LabAny{dbuf(labels)}, gi, L, table, s);
     }
-    if (L > 1) k_stats_table<<<grid_for((size_t)(L - 1), 256), 256, 0, s>>>(a, L, (float*)out_table->data);
-    CLE_LAUNCHED();
-    return CLE_OK;
 }
 
 }  // extern "C"

@@ -457,3 +457,38 @@ def test_statistics_label_dtypes_and_spans(cle, dtype):
     assert d.dtype == np.dtype(dtype)
     got = cle.statistics_of_labelled_pixels(img, d)
     _assert_stats_close(got, O.statistics_of_labelled_pixels(img, lab))
+
+
+def _assert_stats_identical(got, want):
+    for k in want:
+        g, w = np.asarray(got[k]), np.asarray(want[k])
+        same = (g == w) | (np.isnan(g.astype(np.float64)) & np.isnan(w.astype(np.float64)))
+        assert np.all(same), (k, g[~same][:5], w[~same][:5])
+
+
+@pytest.mark.parametrize("shape", [(60, 70), (8, 30, 40), (3, 5, 1500)])
+def test_statistics_bit_exact(cle, shape):
+    """The device statistics follow the reference's float32 accumulation order (cells
+    per (row y, label) in (z, x) order, then y order), so they equal the oracle's."""
+    rng = np.random.default_rng(26)
+    if shape == (3, 5, 1500):
+        lab = np.zeros(shape, np.uint32)
+        lab[0, 1, 10:1400] = 1
+        lab[1, 2:4, 1000:1100] = 2
+        lab[:, :, 200:203] = 4
+    else:
+        img0 = blobs_image(shape, 8, 27).astype(np.float32)
+        lab = O.voronoi_otsu_labeling(img0, 2.0, 1.0)
+    img = (rng.random(shape) * 1000).astype(np.float32)
+    _assert_stats_identical(cle.statistics_of_labelled_pixels(img, lab), O.statistics_of_labelled_pixels(img, lab))
+
+
+def test_ccl_large_vs_composite_rule(cle):
+    """4M voxels at a percolating density: labels equal the rank of each component's
+    minimum x-major key (scipy components)."""
+    rng = np.random.default_rng(28)
+    b = (rng.random((64, 256, 256)) < 0.12).astype(np.uint8)
+    want, n, _ = _ccl_expected_fp32_keys(b)
+    got = cle.pull(cle.connected_components_labeling_box(b))
+    assert got.max() == n
+    assert np.array_equal(got, want)
