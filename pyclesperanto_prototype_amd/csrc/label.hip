@@ -157,11 +157,16 @@ __global__ __launch_bounds__(CT_THREADS) void k_ccl_local(FG fg, Geom g, TileGri
         if (!f[k]) continue;
         const int li = threadIdx.x + k * CT_THREADS;
         const int lx = li % TX, ly = (li / TX) % TY, lz = li / (TX * TY);
+        // box: when the left neighbour (same row) is foreground it has already been
+        // linked to the dx = -1 and dx = 0 neighbours of the other rows, so only the
+        // dx = +1 column is new
+        const bool left_fg = !diamond && lx > 0 && lds_load(lab + li - 1) != NONE;
         for (int dz = -zr; dz <= 0; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx) {
                     if (dz == 0 && (dy > 0 || (dy == 0 && dx >= 0))) continue;
                     if (diamond && (abs(dx) + abs(dy) + abs(dz)) != 1) continue;
+                    if (left_fg && dx != 1 && !(dz == 0 && dy == 0)) continue;
                     const int qx = lx + dx, qy = ly + dy, qz = lz + dz;
                     if (qx < 0 || qx >= TX || qy < 0 || qy >= TY || qz < 0) continue;
                     const int lq = (qz * TY + qy) * TX + qx;
@@ -224,11 +229,14 @@ __global__ __launch_bounds__(CT_THREADS) void k_ccl_border(FG fg, Geom g, TileGr
         if (!fg(i)) continue;
         const uint32_t li_root = ld_relaxed(L + i);
         uint32_t last = NONE;
+        // box: a foreground left neighbour (in any tile) covers the dx = -1, 0 columns
+        const bool left_fg = !diamond && x > 0 && fg(i - 1);
         for (int dz = -zr; dz <= 0; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx) {
                     if (dz == 0 && (dy > 0 || (dy == 0 && dx >= 0))) continue;
                     if (diamond && (abs(dx) + abs(dy) + abs(dz)) != 1) continue;
+                    if (left_fg && dx != 1 && !(dz == 0 && dy == 0)) continue;
                     // only neighbours outside this tile
                     if (lz + dz >= 0 && ly + dy >= 0 && lx + dx >= 0 && lx + dx < TX && ly + dy < TY) continue;
                     const int64_t qx = x + dx, qy = y + dy, qz = z + dz;
@@ -391,6 +399,19 @@ struct RankTable {
     }
 };
 
+// final label of every local root: F[lr] = rank of the key bit of its global root
+template <typename TS>
+__global__ __launch_bounds__(CT_THREADS) void k_ccl_final(const uint32_t* L, const uint32_t* K, const uint32_t* roots,
+                                                          const uint32_t* nroots, int compat, const uint32_t* bm,
+                                                          const uint32_t* prefix, uint32_t* F) {
+    const int64_t tile = blockIdx.x;
+    const uint32_t n = nroots[tile];
+    for (uint32_t k = threadIdx.x; k < n; k += CT_THREADS) {
+        const uint32_t lr = roots[tile * TS::ROOTS + k];
+        F[lr] = bit_rank(bm, prefix, key_bit(K[L[lr]], compat));
+    }
+}
+
 struct StLabI32 {
     int32_t* p;
     __device__ __forceinline__ void operator()(int64_t i, float v) const {
@@ -413,25 +434,18 @@ struct StLabAny {
 };
 
 // labels are ranks (>= 0) held as float, as the reference stores them, then
-// converted with saturation to the destination type
+// converted with saturation to the destination type; a foreground voxel's label is
+// F[its local root]
 template <typename FG, typename ST>
-__global__ void k_ccl_write(FG fg, const uint32_t* L, const uint32_t* K, int64_t n, int compat, const uint32_t* bm,
-                            const uint32_t* prefix, ST out) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float lab = 0.f;
-        if (fg(i)) {
-            const uint32_t r = L[L[i]];
-            lab = (float)bit_rank(bm, prefix, key_bit(K[r], compat));
-        }
-        out(i, lab);
-    }
+__global__ void k_ccl_write(FG fg, const uint32_t* L, const uint32_t* F, int64_t n, ST out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out(i, fg(i) ? (float)F[L[i]] : 0.f);
 }
 
-// uint8 binary, 32-bit destination: four voxels per thread (one 4-byte load, four
-// 4-byte stores as one 16-byte store when all four are background)
+// uint8 binary, 32-bit destination: four voxels per thread (one 4-byte load; one
+// 16-byte store of zeros when all four are background)
 template <typename T, typename ST>
-__global__ void k_ccl_write_u8x4(const uint8_t* bin, const uint32_t* L, const uint32_t* K, int64_t n, int compat,
-                                 const uint32_t* bm, const uint32_t* prefix, T* dst, ST out) {
+__global__ void k_ccl_write_u8x4(const uint8_t* bin, const uint32_t* L, const uint32_t* F, int64_t n, T* dst, ST out) {
     const int64_t nq = n / 4;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t b4 = ((const uint32_t*)bin)[q];
@@ -442,47 +456,36 @@ __global__ void k_ccl_write_u8x4(const uint8_t* bin, const uint32_t* L, const ui
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int64_t i = q * 4 + k;
-            float lab = 0.f;
-            if ((b4 >> (8 * k)) & 0xFFu) {
-                const uint32_t r = L[L[i]];
-                lab = (float)bit_rank(bm, prefix, key_bit(K[r], compat));
-            }
-            out(i, lab);
+            out(i, ((b4 >> (8 * k)) & 0xFFu) ? (float)F[L[i]] : 0.f);
         }
     }
     for (int64_t i = nq * 4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        float lab = 0.f;
-        if (bin[i]) {
-            const uint32_t r = L[L[i]];
-            lab = (float)bit_rank(bm, prefix, key_bit(K[r], compat));
-        }
-        out(i, lab);
-    }
+         i += (int64_t)gridDim.x * blockDim.x)
+        out(i, bin[i] ? (float)F[L[i]] : 0.f);
 }
 
 template <typename FG>
-static void launch_ccl_write(FG fg, const uint32_t* L, const uint32_t* K, int64_t n, int compat, const uint32_t* bm,
-                             const uint32_t* prefix, DBuf dst, unsigned grid, hipStream_t s) {
+static void launch_ccl_write(FG fg, const uint32_t* L, const uint32_t* F, int64_t n, DBuf dst, unsigned grid,
+                             hipStream_t s) {
     switch (dst.dt) {
-        case CLE_I32: k_ccl_write<<<grid, 256, 0, s>>>(fg, L, K, n, compat, bm, prefix, StLabI32{(int32_t*)dst.p}); break;
-        case CLE_U32: k_ccl_write<<<grid, 256, 0, s>>>(fg, L, K, n, compat, bm, prefix, StLabU32{(uint32_t*)dst.p}); break;
-        case CLE_F32: k_ccl_write<<<grid, 256, 0, s>>>(fg, L, K, n, compat, bm, prefix, StLabF32{(float*)dst.p}); break;
-        default: k_ccl_write<<<grid, 256, 0, s>>>(fg, L, K, n, compat, bm, prefix, StLabAny{dst}); break;
+        case CLE_I32: k_ccl_write<<<grid, 256, 0, s>>>(fg, L, F, n, StLabI32{(int32_t*)dst.p}); break;
+        case CLE_U32: k_ccl_write<<<grid, 256, 0, s>>>(fg, L, F, n, StLabU32{(uint32_t*)dst.p}); break;
+        case CLE_F32: k_ccl_write<<<grid, 256, 0, s>>>(fg, L, F, n, StLabF32{(float*)dst.p}); break;
+        default: k_ccl_write<<<grid, 256, 0, s>>>(fg, L, F, n, StLabAny{dst}); break;
     }
 }
 
-static void launch_ccl_write(FgU8 fg, const uint32_t* L, const uint32_t* K, int64_t n, int compat, const uint32_t* bm,
-                             const uint32_t* prefix, DBuf dst, unsigned grid, hipStream_t s) {
+static void launch_ccl_write(FgU8 fg, const uint32_t* L, const uint32_t* F, int64_t n, DBuf dst, unsigned grid,
+                             hipStream_t s) {
     const bool aligned = ((uintptr_t)fg.p & 3) == 0 && ((uintptr_t)dst.p & 15) == 0;
     if (aligned && dst.dt == CLE_I32)
-        k_ccl_write_u8x4<<<grid, 256, 0, s>>>(fg.p, L, K, n, compat, bm, prefix, (int32_t*)dst.p, StLabI32{(int32_t*)dst.p});
+        k_ccl_write_u8x4<<<grid, 256, 0, s>>>(fg.p, L, F, n, (int32_t*)dst.p, StLabI32{(int32_t*)dst.p});
     else if (aligned && dst.dt == CLE_U32)
-        k_ccl_write_u8x4<<<grid, 256, 0, s>>>(fg.p, L, K, n, compat, bm, prefix, (uint32_t*)dst.p, StLabU32{(uint32_t*)dst.p});
+        k_ccl_write_u8x4<<<grid, 256, 0, s>>>(fg.p, L, F, n, (uint32_t*)dst.p, StLabU32{(uint32_t*)dst.p});
     else if (aligned && dst.dt == CLE_F32)
-        k_ccl_write_u8x4<<<grid, 256, 0, s>>>(fg.p, L, K, n, compat, bm, prefix, (float*)dst.p, StLabF32{(float*)dst.p});
+        k_ccl_write_u8x4<<<grid, 256, 0, s>>>(fg.p, L, F, n, (float*)dst.p, StLabF32{(float*)dst.p});
     else
-        launch_ccl_write<FgU8>(fg, L, K, n, compat, bm, prefix, dst, grid, s);
+        launch_ccl_write<FgU8>(fg, L, F, n, dst, grid, s);
 }
 
 static unsigned lg(int64_t n) {
@@ -498,9 +501,10 @@ static int ccl_run(FG fg, const Geom& g, DBuf dst, int connectivity, int options
     const int diamond = connectivity == CLE_CONN_DIAMOND;
     const TileGrid T = tile_grid<TS>(g);
     const int64_t nt = T.count();
-    Scratch Ls, Ks, Rs, Ns;
+    Scratch Ls, Ks, Rs, Ns, Fs;
     CLE_TRY(Ls.alloc((size_t)n * 4, s));
     CLE_TRY(Ks.alloc((size_t)n * 4, s));
+    CLE_TRY(Fs.alloc((size_t)n * 4, s));
     CLE_TRY(Rs.alloc((size_t)nt * TS::ROOTS * 4, s));
     CLE_TRY(Ns.alloc((size_t)nt * 4, s));
     uint32_t *l = Ls.get<uint32_t>(), *k = Ks.get<uint32_t>(), *roots = Rs.get<uint32_t>(), *nr = Ns.get<uint32_t>();
@@ -515,7 +519,9 @@ static int ccl_run(FG fg, const Geom& g, DBuf dst, int connectivity, int options
     k_ccl_mark_roots<TS><<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>());
     CLE_LAUNCHED();
     CLE_TRY(rt.scan(n_labels_dev, s));
-    launch_ccl_write(fg, l, k, n, compat, rt.bm.get<uint32_t>(), rt.prefix.get<uint32_t>(), dst, lg(n / 4 + 1), s);
+    k_ccl_final<TS><<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>(),
+                                                        rt.prefix.get<uint32_t>(), Fs.get<uint32_t>());
+    launch_ccl_write(fg, l, Fs.get<uint32_t>(), n, dst, lg(n / 4 + 1), s);
     CLE_LAUNCHED();
     return CLE_OK;
 }

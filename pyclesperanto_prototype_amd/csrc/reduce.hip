@@ -83,7 +83,23 @@ __global__ __launch_bounds__(RB) void k_minmax_vec(const T* __restrict__ p, int6
     float mn = INFINITY, mx = -INFINITY;
     const int64_t nv = n / E;
     const uint4* pv = (const uint4*)p;
-    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < nv; i += (int64_t)gridDim.x * RB) {
+    const int64_t step = (int64_t)gridDim.x * RB;
+    int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
+    // four independent 16-byte loads in flight per thread
+    for (; i + 3 * step < nv; i += 4 * step) {
+        uint4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = pv[i + u * step];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const float v = vec_elem<T>(q[u], k);
+                mn = fminf(mn, v);
+                mx = fmaxf(mx, v);
+            }
+    }
+    for (; i < nv; i += step) {
         const uint4 q = pv[i];
 #pragma unroll
         for (int k = 0; k < E; ++k) {
@@ -136,7 +152,7 @@ int minmax_impl(DBuf src, int64_t n, float* out2, hipStream_t s) {
     CLE_TRY(tmp.alloc(8, s));
     k_minmax_init<<<1, 1, 0, s>>>(tmp.get<uint32_t>());
     unsigned g = grid_for((size_t)n, RB * 16);
-    if (g > 4096) g = 4096;
+    if (g > 2048) g = 2048;
     const bool aligned = ((uintptr_t)src.p & 15) == 0;
     if (n > 0) {
         uint32_t* o = tmp.get<uint32_t>();
@@ -182,16 +198,18 @@ __device__ __forceinline__ uint32_t bin_of(float v, float mn, float range, int b
     return (uint32_t)t;
 }
 
-// One block = one row y over slices [z0, z0 + zc).  Each wavefront owns HREP
-// replicas of the histogram (lane l counts into replica l % HREP; replicas padded to
-// an odd stride so equal bins of different replicas sit in different LDS banks), and
-// each lane first merges runs of equal bins among its own consecutive voxels.
+// One block = one row y over slices [z0, z0 + zc).  Thread t takes the float4 column
+// x = 4t (+ 4*256 k) of every slice -- coalesced across the block -- and counts runs of
+// equal bins in registers across all its voxels (blurred backgrounds fall into one or
+// two bins), flushing a run into one of HREP LDS replicas per wavefront (lane % HREP;
+// replicas padded to an odd stride so equal bins of different replicas sit in
+// different LDS banks).  Rows whose length is not a multiple of 4 or whose start is
+// not 16-byte aligned take the scalar loads.
 constexpr int HREP = 8;
-constexpr int HVPT = 4;  // consecutive voxels per lane
 
 template <typename LD>
-__global__ __launch_bounds__(HB) void k_hist_rows(LD in, Geom g, int bins, int zc, int zchunks, float mn_arg,
-                                                 float mx_arg, const float* mm, uint32_t* hist2d) {
+__global__ __launch_bounds__(HB) void k_hist_rows(LD in, const float* f4base, Geom g, int bins, int zc, int zchunks,
+                                                 float mn_arg, float mx_arg, const float* mm, uint32_t* hist2d) {
     extern __shared__ uint32_t sh[];
     const int stride = bins | 1;
     const int64_t y = blockIdx.x / zchunks;
@@ -205,26 +223,31 @@ __global__ __launch_bounds__(HB) void k_hist_rows(LD in, Geom g, int bins, int z
     __syncthreads();
     uint32_t* mine = sh + (wave * HREP + (lane % HREP)) * stride;
     const int64_t z1 = (z0 + zc < g.D) ? z0 + zc : g.D;
+    uint32_t cur = 0xFFFFFFFFu, run = 0;
+    auto add = [&](float v) {
+        const uint32_t b = bin_of(v, mn, range, bins);
+        if (b == cur) {
+            ++run;
+        } else {
+            if (run) atomicAdd(&mine[cur], run);
+            cur = b;
+            run = 1;
+        }
+    };
+    const bool vec = f4base != nullptr && (g.W & 3) == 0;
     for (int64_t z = z0; z < z1; ++z) {
         const int64_t base = (z * g.H + y) * g.W;
-        for (int64_t x = (int64_t)threadIdx.x * HVPT; x < g.W; x += (int64_t)HB * HVPT) {
-            uint32_t cur = 0xFFFFFFFFu, run = 0;
-#pragma unroll
-            for (int k = 0; k < HVPT; ++k) {
-                if (x + k < g.W) {
-                    const uint32_t b = bin_of(in(base + x + k), mn, range, bins);
-                    if (b == cur) {
-                        ++run;
-                    } else {
-                        if (run) atomicAdd(&mine[cur], run);
-                        cur = b;
-                        run = 1;
-                    }
-                }
+        if (vec) {
+            const float4* row = (const float4*)(f4base + base);
+            for (int64_t q = threadIdx.x; q < g.W / 4; q += HB) {
+                const float4 v = row[q];
+                add(v.x); add(v.y); add(v.z); add(v.w);
             }
-            if (run) atomicAdd(&mine[cur], run);
+        } else {
+            for (int64_t x = threadIdx.x; x < g.W; x += HB) add(in(base + x));
         }
     }
+    if (run) atomicAdd(&mine[cur], run);
     __syncthreads();
     for (int b = threadIdx.x; b < bins; b += HB) {
         uint32_t c = 0;
@@ -273,8 +296,8 @@ __global__ __launch_bounds__(64) void k_hist_rowsum(const uint32_t* hist2d, int6
 }
 
 template <typename LD>
-static int hist_launch(LD in, const Geom& g, int bins, float mn, float mx, const float* mm, uint32_t* h2,
-                       hipStream_t s) {
+static int hist_launch(LD in, const float* f4base, const Geom& g, int bins, float mn, float mx, const float* mm,
+                       uint32_t* h2, hipStream_t s) {
     if (bins <= HIST_LDS_MAX_BINS) {
         const int64_t per_block = 16384;
         int zc = (int)(per_block / (g.W > 0 ? g.W : 1));
@@ -282,7 +305,7 @@ static int hist_launch(LD in, const Geom& g, int bins, float mn, float mx, const
         if (zc > g.D) zc = (int)g.D;
         const int zchunks = (int)((g.D + zc - 1) / zc);
         const size_t lds = (size_t)(bins | 1) * HWAVES * HREP * sizeof(uint32_t);
-        k_hist_rows<<<(unsigned)(g.H * zchunks), HB, lds, s>>>(in, g, bins, zc, zchunks, mn, mx, mm, h2);
+        k_hist_rows<<<(unsigned)(g.H * zchunks), HB, lds, s>>>(in, f4base, g, bins, zc, zchunks, mn, mx, mm, h2);
     } else {
         unsigned gr = grid_for((size_t)g.n(), 256);
         if (gr > 65536) gr = 65536;
@@ -299,8 +322,11 @@ static int histogram_impl(DBuf src, const Geom& g, int bins, float mn, float mx,
     CLE_TRY(h2.alloc(cells * sizeof(uint32_t), s));
     CLE_TRY(fill_u32(h2.get<uint32_t>(), 0u, cells, s));
     if (g.n() > 0) {
-        if (src.dt == CLE_F32) CLE_TRY(hist_launch(LdF32R{(const float*)src.p}, g, bins, mn, mx, mm, h2.get<uint32_t>(), s));
-        else CLE_TRY(hist_launch(LdAnyR{src}, g, bins, mn, mx, mm, h2.get<uint32_t>(), s));
+        const bool f4 = src.dt == CLE_F32 && ((uintptr_t)src.p & 15) == 0;
+        if (src.dt == CLE_F32)
+            CLE_TRY(hist_launch(LdF32R{(const float*)src.p}, f4 ? (const float*)src.p : nullptr, g, bins, mn, mx, mm,
+                                h2.get<uint32_t>(), s));
+        else CLE_TRY(hist_launch(LdAnyR{src}, nullptr, g, bins, mn, mx, mm, h2.get<uint32_t>(), s));
     }
     k_hist_rowsum<<<(unsigned)bins, 64, 0, s>>>(h2.get<uint32_t>(), g.H, bins, out);
     CLE_LAUNCHED();

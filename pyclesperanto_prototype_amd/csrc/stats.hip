@@ -10,10 +10,10 @@
 // mass centre and with (v - mean)^2 / area.  Column layout of the output table:
 // _tier9/_statistics_entry.py:3-46.
 //
-// MI355X design.  (1) Label runs: each wavefront takes a 1024-voxel row span, tests it
-// for labels with coalesced loads and skips it if empty; lanes emit the runs of equal
-// non-zero labels in their 16 voxels as (label, y, z, x0) keys.  A first pass counts
-// the runs (one synchronisation to size the buffers), a second writes them.  (2) A
+// MI355X design.  (1) Label runs: a block stages a 1024-voxel row span in LDS with
+// coalesced loads and finds the runs of equal measured labels as (label, y, z, x0)
+// keys.  A first pass counts them per span, an exclusive scan gives every span its
+// offset (one synchronisation sizes the buffers), a second pass writes them.  (2) A
 // radix sort orders the runs by key, i.e. by cell (label, y) and, inside a cell, in
 // the reference's (z, x) order.  (3) One thread per cell accumulates its voxels in
 // float32 with the reference's per-voxel operations (products and sums rounded
@@ -27,9 +27,6 @@
 #include "common.hpp"
 
 namespace cle {
-
-constexpr int RUN = 16;
-constexpr int SPAN = 64 * RUN;
 
 template <typename T>
 struct LabT {
@@ -53,76 +50,100 @@ __device__ __forceinline__ uint64_t run_key(const KeyBits& kb, uint64_t l, uint6
     return (((l << kb.yb | y) << kb.zb | z) << kb.xb) | x0;
 }
 
-// Runs of one lane's voxels [x0, x1) of row `base`; calls emit(label, xs, xe).
-template <typename LAB, typename F>
-__device__ __forceinline__ void lane_runs(LAB lab, int64_t base, int64_t x0, int64_t x1, int64_t L, F emit) {
-    int cur = 0;
-    int64_t xs = x0;
-    for (int64_t x = x0; x < x1; ++x) {
-        const int l = lab(base + x);
-        if (l != cur) {
-            if (cur > 0 && cur < L) emit(cur, xs, x);
-            cur = l;
-            xs = x;
-        }
-    }
-    if (cur > 0 && cur < L) emit(cur, xs, x1);
-}
+// Label runs of a row span, one span per block iteration.  The span's labels are
+// staged in LDS with coalesced loads (thread t reads x = sx0 + t + 256 k); a voxel
+// starts a run if its label is a measured label (0 < l < L) and differs from its
+// left neighbour inside the span; the run ends where the label changes.  Runs are
+// split at span borders, which keeps them in x order.
+constexpr int RB_THREADS = 256;
+constexpr int RB_SPAN = 1024;
+constexpr int RB_PER = RB_SPAN / RB_THREADS;
 
 template <typename LAB>
-__device__ __forceinline__ bool span_has_labels(LAB lab, int64_t base, int64_t x0, int64_t x1, int lane) {
+__device__ __forceinline__ bool stage_span(LAB lab, int64_t base, int64_t sx0, int64_t sx1, int* sl) {
     bool any = false;
-#pragma unroll 4
-    for (int k = 0; k < RUN; ++k) {
-        const int64_t x = x0 + lane + 64 * k;
-        if (x < x1) any |= lab.nz(base + x);
+#pragma unroll
+    for (int k = 0; k < RB_PER; ++k) {
+        const int64_t x = sx0 + threadIdx.x + (int64_t)RB_THREADS * k;
+        int l = 0;
+        if (x < sx1) l = lab(base + x);
+        sl[threadIdx.x + RB_THREADS * k] = l;
+        any |= l != 0;
     }
-    return __any(any);
+    return __syncthreads_or(any ? 1 : 0) != 0;
 }
 
-// EMIT = false: count runs into *cursor; EMIT = true: write keys / ends at slots
-// reserved from *cursor (one atomic per wavefront).
-template <bool EMIT, typename LAB>
-__global__ __launch_bounds__(256) void k_runs(LAB lab, Geom g, int64_t L, KeyBits kb, unsigned long long* cursor,
-                                              uint64_t* keys, uint32_t* ends) {
-    const int64_t spans = (g.W + SPAN - 1) / SPAN;
+// block-wide exclusive prefix of v (all threads call it); *total receives the sum
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* wsum, uint32_t* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(inc, off);
+        if (lane >= off) inc += t;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < RB_THREADS / 64; ++w) {
+        if (w < wave) before += wsum[w];
+        all += wsum[w];
+    }
+    __syncthreads();
+    *total = all;
+    return before + inc - v;
+}
+
+// PASS 0: span_count[sp] = number of runs; PASS 1: write them at span_off[sp] + prefix
+template <int PASS, typename LAB>
+__global__ __launch_bounds__(RB_THREADS) void k_runs(LAB lab, Geom g, int64_t L, KeyBits kb, uint32_t* span_count,
+                                                     const uint32_t* span_off, uint64_t* keys, uint32_t* ends) {
+    __shared__ int sl[RB_SPAN];
+    __shared__ uint32_t wsum[RB_THREADS / 64];
+    const int64_t spans = (g.W + RB_SPAN - 1) / RB_SPAN;
     const int64_t total = g.D * g.H * spans;
-    const int lane = threadIdx.x & 63;
-    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t sp = wave0; sp < total; sp += nwaves) {
+    for (int64_t sp = blockIdx.x; sp < total; sp += gridDim.x) {
         const int64_t row = sp / spans;
-        const int64_t sx0 = (sp - row * spans) * SPAN;
-        const int64_t sx1 = sx0 + SPAN < g.W ? sx0 + SPAN : g.W;
+        const int64_t sx0 = (sp - row * spans) * RB_SPAN;
+        const int64_t sx1 = sx0 + RB_SPAN < g.W ? sx0 + RB_SPAN : g.W;
         const int64_t base = row * g.W;
-        if (!span_has_labels(lab, base, sx0, sx1, lane)) continue;
-        const int64_t x0 = sx0 + (int64_t)lane * RUN;
-        const int64_t x1 = x0 + RUN < sx1 ? x0 + RUN : sx1;
-        uint32_t cnt = 0;
-        if (x0 < x1) lane_runs(lab, base, x0, x1, L, [&](int, int64_t, int64_t) { ++cnt; });
-        // inclusive prefix of the lane counts
-        uint32_t inc = cnt;
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t t = __shfl_up(inc, off);
-            if (lane >= off) inc += t;
-        }
-        const uint32_t wave_total = __shfl(inc, 63);
-        if (!EMIT) {
-            if (lane == 0 && wave_total) atomicAdd(cursor, (unsigned long long)wave_total);
+        const int n = (int)(sx1 - sx0);
+        if (!stage_span(lab, base, sx0, sx1, sl)) {
+            if (PASS == 0 && threadIdx.x == 0) span_count[sp] = 0;
+            __syncthreads();
             continue;
         }
-        unsigned long long wbase = 0;
-        if (lane == 0 && wave_total) wbase = atomicAdd(cursor, (unsigned long long)wave_total);
-        wbase = __shfl(wbase, 0);
-        uint64_t slot = wbase + inc - cnt;
-        if (x0 < x1) {
-            const uint64_t y = (uint64_t)(row % g.H), z = (uint64_t)(row / g.H);
-            lane_runs(lab, base, x0, x1, L, [&](int l, int64_t xs, int64_t xe) {
-                keys[slot] = run_key(kb, (uint64_t)l, y, z, (uint64_t)xs);
-                ends[slot] = (uint32_t)xe;
-                ++slot;
-            });
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < RB_PER; ++k) {
+            const int i = threadIdx.x + RB_THREADS * k;
+            if (i < n) {
+                const int l = sl[i];
+                if (l > 0 && l < L && (i == 0 || sl[i - 1] != l)) ++cnt;
+            }
         }
+        uint32_t tot;
+        const uint32_t pre = block_exclusive(cnt, wsum, &tot);
+        if (PASS == 0) {
+            if (threadIdx.x == 0) span_count[sp] = tot;
+        } else if (cnt) {
+            uint64_t slot = (uint64_t)span_off[sp] + pre;
+            const uint64_t y = (uint64_t)(row % g.H), z = (uint64_t)(row / g.H);
+#pragma unroll
+            for (int k = 0; k < RB_PER; ++k) {
+                const int i = threadIdx.x + RB_THREADS * k;
+                if (i < n) {
+                    const int l = sl[i];
+                    if (l > 0 && l < L && (i == 0 || sl[i - 1] != l)) {
+                        int e = i + 1;
+                        while (e < n && sl[e] == l) ++e;
+                        keys[slot] = run_key(kb, (uint64_t)l, y, z, (uint64_t)(sx0 + i));
+                        ends[slot] = (uint32_t)(sx0 + e);
+                        ++slot;
+                    }
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -334,6 +355,10 @@ __global__ void k_labels_table(const float* acc2, int64_t cap, const uint32_t* l
     for (int c = 0; c < CLE_STAT_COLUMNS; ++c) out[c * N + j] = col[c];
 }
 
+__global__ void k_span_total(const uint32_t* cnt, const uint32_t* off, int64_t n, uint32_t* total) {
+    *total = off[n - 1] + cnt[n - 1];
+}
+
 static int bits_for(uint64_t maxval) {
     int b = 1;
     while (b < 64 && (maxval >> b) != 0) ++b;
@@ -348,19 +373,29 @@ static int stats_run(DBuf inten, LAB lab, const Geom& g, int64_t L, float* table
     kb.yb = bits_for((uint64_t)(g.H > 0 ? g.H - 1 : 0));
     kb.lb = bits_for((uint64_t)(L - 1));
     if (kb.total() > 64) return fail(CLE_E_SHAPE, "statistics: %d key bits exceed 64", kb.total());
-    const int64_t spans = g.D * g.H * ((g.W + SPAN - 1) / SPAN);
-    unsigned gr = grid_for((size_t)spans, 4);
-    if (gr > 65536) gr = 65536;
-    // (1) count the runs, size the buffers, write the runs
-    Scratch cur;
-    CLE_TRY(cur.alloc(8, s));
-    CLE_TRY(fill_u64(cur.get<unsigned long long>(), 0ull, 1, s));
-    if (g.n() > 0) k_runs<false><<<gr, 256, 0, s>>>(lab, g, L, kb, cur.get<unsigned long long>(), nullptr, nullptr);
+    const int64_t spans = g.D * g.H * ((g.W + RB_SPAN - 1) / RB_SPAN);
+    unsigned gr = spans > 65536 ? 65536u : (unsigned)(spans > 0 ? spans : 1);
+    // (1) count the runs per span, offsets by an exclusive scan (one synchronisation to
+    //     size the buffers), then write every span's runs at its offset
+    Scratch scount, soff, stot;
+    const int64_t scap = spans > 0 ? spans : 1;
+    CLE_TRY(scount.alloc((size_t)scap * 4, s));
+    CLE_TRY(soff.alloc((size_t)scap * 4, s));
+    CLE_TRY(stot.alloc(8, s));
+    CLE_TRY(fill_u32(stot.get<uint32_t>(), 0u, 2, s));
+    if (spans > 0) {
+        k_runs<0><<<gr, RB_THREADS, 0, s>>>(lab, g, L, kb, scount.get<uint32_t>(), nullptr, nullptr, nullptr);
+        size_t sb = 0;
+        CLE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, scount.get<uint32_t>(), soff.get<uint32_t>(), (int)spans, s));
+        Scratch st;
+        CLE_TRY(st.alloc(sb, s));
+        CLE_HIP(hipcub::DeviceScan::ExclusiveSum(st.p, sb, scount.get<uint32_t>(), soff.get<uint32_t>(), (int)spans, s));
+        k_span_total<<<1, 1, 0, s>>>(scount.get<uint32_t>(), soff.get<uint32_t>(), spans, stot.get<uint32_t>());
+    }
     CLE_LAUNCHED();
-    unsigned long long nruns = 0;
-    CLE_HIP(hipMemcpyAsync(&nruns, cur.p, 8, hipMemcpyDeviceToHost, s));
+    uint32_t nruns = 0;
+    CLE_HIP(hipMemcpyAsync(&nruns, stot.p, 4, hipMemcpyDeviceToHost, s));
     CLE_HIP(hipStreamSynchronize(s));
-    if (nruns >= 0x7FFFFFFFull) return fail(CLE_E_SHAPE, "statistics: %llu label runs exceed 2^31", nruns);
     const int64_t n = (int64_t)nruns;
     const int64_t cap = n > 0 ? n : 1;
     Scratch k0, k1, e0, e1, tmp, flags, cidx, cstart, ncell, lst, len_, acc, acc2, lab16, img;
@@ -368,10 +403,9 @@ static int stats_run(DBuf inten, LAB lab, const Geom& g, int64_t L, float* table
     CLE_TRY(k1.alloc((size_t)cap * 8, s));
     CLE_TRY(e0.alloc((size_t)cap * 4, s));
     CLE_TRY(e1.alloc((size_t)cap * 4, s));
-    CLE_TRY(fill_u64(cur.get<unsigned long long>(), 0ull, 1, s));
     if (n > 0)
-        k_runs<true><<<gr, 256, 0, s>>>(lab, g, L, kb, cur.get<unsigned long long>(), k0.get<uint64_t>(),
-                                        e0.get<uint32_t>());
+        k_runs<1><<<gr, RB_THREADS, 0, s>>>(lab, g, L, kb, nullptr, soff.get<uint32_t>(), k0.get<uint64_t>(),
+                                            e0.get<uint32_t>());
     CLE_LAUNCHED();
     // (2) sort the runs by (label, y, z, x0)
     size_t tbytes = 0;
