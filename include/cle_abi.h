@@ -53,7 +53,8 @@ typedef struct cle_buf {
 const char* cle_version(void);
 /* message of the last failed call on this thread ("" if none) */
 const char* cle_last_error(void);
-/* release cached scratch memory held by the pool (synchronises the device) */
+/* free the scratch arenas of the current device (synchronises the device; call it
+   while no other call on that device is in flight) */
 int cle_release_pool(void);
 
 /* ---- fills and copies -------------------------------------------------- */

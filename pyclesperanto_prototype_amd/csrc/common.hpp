@@ -53,9 +53,9 @@ bool dtype_supported(int dt);
 size_t dtype_size(int dt);
 inline DBuf dbuf(const cle_buf* b) { return DBuf{b->data, b->dtype}; }
 
-// ----------------------------------------------------------------- scratch pool
-// Stream-ordered device scratch (hipMallocAsync on the default memory pool, kept
-// cached by a high release threshold).
+// ----------------------------------------------------------------- scratch arenas
+// Stream-ordered device scratch from a grow-only arena per (device, stream); memory
+// released by a call is reused by the next call on the same stream.
 void* scratch_alloc(size_t bytes, hipStream_t s);
 void scratch_free(void* p, hipStream_t s);
 

@@ -1,8 +1,10 @@
-// Library runtime: error reporting, argument validation, the scratch pool, fills,
+// Library runtime: error reporting, argument validation, the scratch arenas, fills,
 // and the set / copy entry points.
 #include <stdarg.h>
 #include <stdio.h>
+#include <map>
 #include <mutex>
+#include <vector>
 
 #include "common.hpp"
 
@@ -69,31 +71,115 @@ int check_same_shape(const Geom& a, const Geom& b, const char* na, const char* n
     return CLE_OK;
 }
 
-// ----------------------------------------------------------------- pool
-static std::once_flag g_pool_once;
+// ----------------------------------------------------------------- scratch arenas
+// One grow-only arena per (device, stream).  Calls on one stream run in stream order,
+// so memory a call released on the host can be handed to the next call on the same
+// stream at once: allocation is a bump of an offset, release pops a LIFO stack, and no
+// driver allocator (whose free can block the host until the device drains) runs on the
+// steady-state path.  An arena grows by adding a chunk; when it is idle again and holds
+// more than one chunk, its stream is drained and the chunks are merged into one.
+namespace {
+constexpr size_t ARENA_ALIGN = 256;
+constexpr size_t ARENA_MIN_CHUNK = 64ull << 20;
 
-static void pool_init() {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return;
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-        uint64_t thr = ~0ull;  // keep freed blocks cached in the pool
-        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+struct ArenaBlock {
+    int chunk;
+    size_t off, size;
+    bool freed;
+};
+
+struct Arena {
+    hipStream_t s = nullptr;
+    std::vector<std::pair<char*, size_t>> chunks;
+    std::vector<ArenaBlock> stack;
+    int cur = 0;      // chunk the bump offset points into
+    size_t top = 0;   // bump offset in chunks[cur]
+    size_t capacity() const {
+        size_t t = 0;
+        for (auto& c : chunks) t += c.second;
+        return t;
     }
+};
+
+std::mutex g_arena_mu;
+std::map<std::pair<int, hipStream_t>, Arena*> g_arenas;
+
+// caller holds g_arena_mu
+Arena* arena_for(hipStream_t s) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    auto& a = g_arenas[{dev, s}];
+    if (!a) {
+        a = new Arena;
+        a->s = s;
+    }
+    return a;
 }
 
-void* scratch_alloc(size_t bytes, hipStream_t s) {
-    std::call_once(g_pool_once, pool_init);
+bool arena_merge(Arena* a) {
+    const size_t total = a->capacity();
+    if (hipStreamSynchronize(a->s) != hipSuccess) return false;
+    for (auto& c : a->chunks) (void)hipFree(c.first);
+    a->chunks.clear();
     void* p = nullptr;
-    if (hipMallocAsync(&p, bytes, s) != hipSuccess) {
-        (void)hipGetLastError();
-        return nullptr;
+    if (hipMalloc(&p, total) != hipSuccess) return false;
+    a->chunks.push_back({(char*)p, total});
+    return true;
+}
+}  // namespace
+
+void* scratch_alloc(size_t bytes, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    Arena* a = arena_for(s);
+    if (!a) return nullptr;
+    const size_t need = (bytes + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN;
+    if (a->stack.empty()) {
+        if (a->chunks.size() > 1 && !arena_merge(a)) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        a->cur = 0;
+        a->top = 0;
     }
+    while (a->cur < (int)a->chunks.size() && a->top + need > a->chunks[a->cur].second) {
+        ++a->cur;
+        a->top = 0;
+    }
+    if (a->cur == (int)a->chunks.size()) {
+        size_t sz = a->capacity();
+        if (sz < ARENA_MIN_CHUNK) sz = ARENA_MIN_CHUNK;
+        if (sz < need) sz = need;
+        void* p = nullptr;
+        if (hipMalloc(&p, sz) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        a->chunks.push_back({(char*)p, sz});
+        a->top = 0;
+    }
+    char* p = a->chunks[a->cur].first + a->top;
+    a->stack.push_back({a->cur, a->top, need, false});
+    a->top += need;
     return p;
 }
 
 void scratch_free(void* p, hipStream_t s) {
-    if (p) (void)hipFreeAsync(p, s);
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    Arena* a = arena_for(s);
+    if (!a) return;
+    for (size_t i = a->stack.size(); i-- > 0;) {
+        ArenaBlock& b = a->stack[i];
+        if (a->chunks[b.chunk].first + b.off == (char*)p) {
+            b.freed = true;
+            break;
+        }
+    }
+    while (!a->stack.empty() && a->stack.back().freed) {
+        a->cur = a->stack.back().chunk;
+        a->top = a->stack.back().off;
+        a->stack.pop_back();
+    }
 }
 
 // ----------------------------------------------------------------- fills
@@ -145,9 +231,17 @@ int cle_release_pool(void) {
     int dev = 0;
     CLE_HIP(hipGetDevice(&dev));
     CLE_HIP(hipDeviceSynchronize());
-    hipMemPool_t pool;
-    CLE_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
-    CLE_HIP(hipMemPoolTrimTo(pool, 0));
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    for (auto it = g_arenas.begin(); it != g_arenas.end();) {
+        Arena* a = it->second;
+        if (it->first.first != dev || !a->stack.empty()) {
+            ++it;
+            continue;
+        }
+        for (auto& c : a->chunks) CLE_HIP(hipFree(c.first));
+        delete a;
+        it = g_arenas.erase(it);
+    }
     return CLE_OK;
 }
 

@@ -10,10 +10,11 @@
 // mass centre and with (v - mean)^2 / area.  Column layout of the output table:
 // _tier9/_statistics_entry.py:3-46.
 //
-// MI355X design.  (1) Label runs: a block stages a 1024-voxel row span in LDS with
-// coalesced loads and finds the runs of equal measured labels as (label, y, z, x0)
-// keys.  A first pass counts them per span, an exclusive scan gives every span its
-// offset (one synchronisation sizes the buffers), a second pass writes them.  (2) A
+// MI355X design.  (1) Label runs: a wavefront holds a 1024-voxel row span in registers
+// (coalesced loads), finds the runs of equal measured labels with shuffles and numbers
+// them with ballots, as (label, y, z, x0) keys.  A first pass counts them per span, an
+// exclusive scan gives every span its offset (one synchronisation sizes the buffers),
+// a second pass writes them.  (2) A
 // radix sort orders the runs by key, i.e. by cell (label, y) and, inside a cell, in
 // the reference's (z, x) order.  (3) One thread per cell accumulates its voxels in
 // float32 with the reference's per-voxel operations (products and sums rounded
@@ -50,100 +51,80 @@ __device__ __forceinline__ uint64_t run_key(const KeyBits& kb, uint64_t l, uint6
     return (((l << kb.yb | y) << kb.zb | z) << kb.xb) | x0;
 }
 
-// Label runs of a row span, one span per block iteration.  The span's labels are
-// staged in LDS with coalesced loads (thread t reads x = sx0 + t + 256 k); a voxel
-// starts a run if its label is a measured label (0 < l < L) and differs from its
-// left neighbour inside the span; the run ends where the label changes.  Runs are
-// split at span borders, which keeps them in x order.
+// Label runs of a row span, one span per wavefront, no LDS and no block barrier.
+// Lane l holds the labels x = sx0 + l + 64 k (k = 0..15; coalesced loads).  A voxel
+// starts a run if its label is measured (0 < l < L) and differs from its left
+// neighbour inside the span, and ends one if it differs from its right neighbour;
+// ballots number the starts and the ends in x order, so start j and end j bound run j.
+// Runs are split at span borders, which keeps them in x order.
 constexpr int RB_THREADS = 256;
-constexpr int RB_SPAN = 1024;
-constexpr int RB_PER = RB_SPAN / RB_THREADS;
+constexpr int RB_K = 16;
+constexpr int RB_SPAN = 64 * RB_K;
 
-template <typename LAB>
-__device__ __forceinline__ bool stage_span(LAB lab, int64_t base, int64_t sx0, int64_t sx1, int* sl) {
-    bool any = false;
-#pragma unroll
-    for (int k = 0; k < RB_PER; ++k) {
-        const int64_t x = sx0 + threadIdx.x + (int64_t)RB_THREADS * k;
-        int l = 0;
-        if (x < sx1) l = lab(base + x);
-        sl[threadIdx.x + RB_THREADS * k] = l;
-        any |= l != 0;
-    }
-    return __syncthreads_or(any ? 1 : 0) != 0;
+__device__ __forceinline__ uint64_t lanes_below() {
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// block-wide exclusive prefix of v (all threads call it); *total receives the sum
-__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* wsum, uint32_t* total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t inc = v;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t t = __shfl_up(inc, off);
-        if (lane >= off) inc += t;
-    }
-    if (lane == 63) wsum[wave] = inc;
-    __syncthreads();
-    uint32_t before = 0, all = 0;
-    for (int w = 0; w < RB_THREADS / 64; ++w) {
-        if (w < wave) before += wsum[w];
-        all += wsum[w];
-    }
-    __syncthreads();
-    *total = all;
-    return before + inc - v;
-}
-
-// PASS 0: span_count[sp] = number of runs; PASS 1: write them at span_off[sp] + prefix
+// PASS 0: span_count[sp] = number of runs; PASS 1: write them at span_off[sp] + rank
 template <int PASS, typename LAB>
 __global__ __launch_bounds__(RB_THREADS) void k_runs(LAB lab, Geom g, int64_t L, KeyBits kb, uint32_t* span_count,
                                                      const uint32_t* span_off, uint64_t* keys, uint32_t* ends) {
-    __shared__ int sl[RB_SPAN];
-    __shared__ uint32_t wsum[RB_THREADS / 64];
     const int64_t spans = (g.W + RB_SPAN - 1) / RB_SPAN;
     const int64_t total = g.D * g.H * spans;
-    for (int64_t sp = blockIdx.x; sp < total; sp += gridDim.x) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = lanes_below();
+    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t sp = wave0; sp < total; sp += nwaves) {
         const int64_t row = sp / spans;
         const int64_t sx0 = (sp - row * spans) * RB_SPAN;
-        const int64_t sx1 = sx0 + RB_SPAN < g.W ? sx0 + RB_SPAN : g.W;
-        const int64_t base = row * g.W;
-        const int n = (int)(sx1 - sx0);
-        if (!stage_span(lab, base, sx0, sx1, sl)) {
-            if (PASS == 0 && threadIdx.x == 0) span_count[sp] = 0;
-            __syncthreads();
+        const int n = (int)((sx0 + RB_SPAN < g.W ? sx0 + RB_SPAN : g.W) - sx0);
+        const int64_t base = row * g.W + sx0;
+        int v[RB_K];
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < RB_K; ++k) {
+            const int i = lane + 64 * k;
+            v[k] = i < n ? lab(base + i) : 0;
+            any |= v[k] != 0;
+        }
+        if (!__any(any)) {
+            if (PASS == 0 && lane == 0) span_count[sp] = 0;
             continue;
         }
-        uint32_t cnt = 0;
-#pragma unroll
-        for (int k = 0; k < RB_PER; ++k) {
-            const int i = threadIdx.x + RB_THREADS * k;
-            if (i < n) {
-                const int l = sl[i];
-                if (l > 0 && l < L && (i == 0 || sl[i - 1] != l)) ++cnt;
-            }
+        uint32_t nstart = 0, nend = 0;
+        uint64_t slot0 = 0;
+        uint64_t y = 0, z = 0;
+        if (PASS == 1) {
+            slot0 = span_off[sp];
+            y = (uint64_t)(row % g.H);
+            z = (uint64_t)(row / g.H);
         }
-        uint32_t tot;
-        const uint32_t pre = block_exclusive(cnt, wsum, &tot);
-        if (PASS == 0) {
-            if (threadIdx.x == 0) span_count[sp] = tot;
-        } else if (cnt) {
-            uint64_t slot = (uint64_t)span_off[sp] + pre;
-            const uint64_t y = (uint64_t)(row % g.H), z = (uint64_t)(row / g.H);
+        int prev_last = 0;  // label of voxel 64 k - 1 (lane 63 of the previous k)
 #pragma unroll
-            for (int k = 0; k < RB_PER; ++k) {
-                const int i = threadIdx.x + RB_THREADS * k;
-                if (i < n) {
-                    const int l = sl[i];
-                    if (l > 0 && l < L && (i == 0 || sl[i - 1] != l)) {
-                        int e = i + 1;
-                        while (e < n && sl[e] == l) ++e;
-                        keys[slot] = run_key(kb, (uint64_t)l, y, z, (uint64_t)(sx0 + i));
-                        ends[slot] = (uint32_t)(sx0 + e);
-                        ++slot;
-                    }
-                }
+        for (int k = 0; k < RB_K; ++k) {
+            const int i = lane + 64 * k;
+            const int l = v[k];
+            int left = __shfl_up(l, 1);
+            if (lane == 0) left = prev_last;
+            int right = __shfl_down(l, 1);
+            const int next_first = k + 1 < RB_K ? __shfl(v[k + 1 < RB_K ? k + 1 : k], 0) : 0;
+            if (lane == 63) right = next_first;
+            prev_last = __shfl(l, 63);
+            const bool valid = i < n && l > 0 && l < L;
+            const bool is_start = valid && (i == 0 || left != l);
+            const bool is_end = valid && (i == n - 1 || right != l);
+            const uint64_t ms = __ballot(is_start), me = __ballot(is_end);
+            if (PASS == 1) {
+                if (is_start)
+                    keys[slot0 + nstart + __popcll(ms & below)] = run_key(kb, (uint64_t)l, y, z, (uint64_t)(sx0 + i));
+                if (is_end) ends[slot0 + nend + __popcll(me & below)] = (uint32_t)(sx0 + i + 1);
             }
+            nstart += (uint32_t)__popcll(ms);
+            nend += (uint32_t)__popcll(me);
         }
-        __syncthreads();
+        if (PASS == 0 && lane == 0) span_count[sp] = nstart;
     }
 }
 
@@ -374,7 +355,8 @@ static int stats_run(DBuf inten, LAB lab, const Geom& g, int64_t L, float* table
     kb.lb = bits_for((uint64_t)(L - 1));
     if (kb.total() > 64) return fail(CLE_E_SHAPE, "statistics: %d key bits exceed 64", kb.total());
     const int64_t spans = g.D * g.H * ((g.W + RB_SPAN - 1) / RB_SPAN);
-    unsigned gr = spans > 65536 ? 65536u : (unsigned)(spans > 0 ? spans : 1);
+    unsigned gr = grid_for((size_t)(spans > 0 ? spans : 1), RB_THREADS / 64);  // one wavefront per span
+    if (gr > 32768) gr = 32768;
     // (1) count the runs per span, offsets by an exclusive scan (one synchronisation to
     //     size the buffers), then write every span's runs at its offset
     Scratch scount, soff, stot;
