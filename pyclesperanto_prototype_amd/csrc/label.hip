@@ -526,16 +526,317 @@ static int ccl_run(FG fg, const Geom& g, DBuf dst, int connectivity, int options
     return CLE_OK;
 }
 
+// ----------------------------------------------------------------- 2x2x2-block union-find (box)
+// With box (26-) connectivity every pair of voxels inside one 2 x 2 x 2 block (2 x 2 in
+// 2D) is adjacent, so the foreground voxels of a block always form one component and
+// the union-find can run over blocks: 8x fewer elements than voxels.  A block carries
+// an 8-bit mask of its foreground voxels, bit (dz * 4 + dy * 2 + dx).  Two blocks at
+// offset e are connected iff the block has a foreground voxel on its side facing the
+// neighbour (coordinate 0 on an axis where e = -1, 1 where e = +1, any where e = 0) and
+// the neighbour one on its side facing back: any two such voxels are adjacent.  The
+// component key of a block is the smallest x-major key of its foreground voxels.
+// Tiles of TBZ x TBY x TBX blocks run the same two-level scheme as the voxel version
+// (LDS union-find per tile, then unions across tile faces in global memory); the
+// final gather gives each foreground voxel the label of its block's local root.
+template <int TBZ_, int TBY_, int TBX_, int BZ_>
+struct BlkTile {
+    static constexpr int TZ = TBZ_, TY = TBY_, TX = TBX_, BZ = BZ_, N = TBZ_ * TBY_ * TBX_;
+    static constexpr int PER = N / CT_THREADS;
+    static constexpr int ROOTS = N;  // blocks side by side can still be disconnected
+};
+using BlkTile3D = BlkTile<4, 8, 16, 2>;   // 8 x 16 x 32 voxels
+using BlkTile2D = BlkTile<1, 32, 32, 1>;  // 64 x 64 pixels
+
+struct BGrid {
+    int64_t D, H, W;  // blocks along z, y, x
+};
+
+// mask bits on the side of a block facing offset e along one axis
+__device__ __forceinline__ uint32_t side_bits(int e, uint32_t lo, uint32_t hi) {
+    return e < 0 ? lo : (e > 0 ? hi : 0xFFu);
+}
+__device__ __forceinline__ uint32_t facing(int ex, int ey, int ez) {
+    return side_bits(ex, 0x55u, 0xAAu) & side_bits(ey, 0x33u, 0xCCu) & side_bits(ez, 0x0Fu, 0xF0u);
+}
+__device__ __forceinline__ bool blocks_touch(uint32_t m, uint32_t qm, int ex, int ey, int ez) {
+    return (m & facing(ex, ey, ez)) != 0 && (qm & facing(-ex, -ey, -ez)) != 0;
+}
+
+// smallest 0-based x-major key x * H * D + y * D + z among the block's voxels in m
+__device__ __forceinline__ uint32_t block_min_key(uint32_t m, const Geom& g, int64_t bx, int64_t by, int64_t bz,
+                                                  int BZ) {
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dz = 0; dz < 2; ++dz)
+                if (m & (1u << (dz * 4 + dy * 2 + dx)))
+                    return (uint32_t)((2 * bx + dx) * g.H * g.D + (2 * by + dy) * g.D + (bz * BZ + dz));
+    return NONE;
+}
+
+template <typename BT, typename FG>
+__global__ __launch_bounds__(CT_THREADS) void k_bccl_local(FG fg, Geom g, BGrid B, TileGrid T, uint8_t* M,
+                                                          uint32_t* L, uint32_t* K, uint32_t* roots,
+                                                          uint32_t* nroots) {
+    constexpr int N = BT::N, PER = BT::PER, TX = BT::TX, TY = BT::TY, BZ = BT::BZ;
+    __shared__ uint32_t lab[N];
+    __shared__ uint32_t kmin[N];
+    __shared__ uint8_t msk[N];
+    __shared__ uint32_t cnt;
+    const int64_t tile = blockIdx.x;
+    const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
+    const int64_t bx0 = tx * TX, by0 = ty * TY, bz0 = tz * BT::TZ;
+    if (threadIdx.x == 0) cnt = 0;
+    int64_t gb[PER];
+    uint32_t m[PER];
+    bool anyf = false;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int li = threadIdx.x + k * CT_THREADS;
+        const int lx = li % TX, ly = (li / TX) % TY, lz = li / (TX * TY);
+        const int64_t bx = bx0 + lx, by = by0 + ly, bz = bz0 + lz;
+        const bool in = bx < B.W && by < B.H && bz < B.D;
+        uint32_t mk = 0;
+        if (in) {
+#pragma unroll
+            for (int dz = 0; dz < BZ; ++dz) {
+                const int64_t z = bz * BZ + dz;
+                if (z >= g.D) continue;
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy) {
+                    const int64_t y = 2 * by + dy;
+                    if (y >= g.H) continue;
+                    const int64_t row = (z * g.H + y) * g.W;
+#pragma unroll
+                    for (int dx = 0; dx < 2; ++dx) {
+                        const int64_t x = 2 * bx + dx;
+                        if (x < g.W && fg(row + x)) mk |= 1u << (dz * 4 + dy * 2 + dx);
+                    }
+                }
+            }
+        }
+        gb[k] = in ? (bz * B.H + by) * B.W + bx : 0;
+        if (in) M[gb[k]] = (uint8_t)mk;  // neighbouring tiles read the masks in the border pass
+        m[k] = mk;
+        msk[li] = (uint8_t)mk;
+        lab[li] = mk ? (uint32_t)li : NONE;
+        kmin[li] = NONE;
+        anyf |= mk != 0;
+    }
+    if (!__syncthreads_or(anyf ? 1 : 0)) {
+        if (threadIdx.x == 0) nroots[tile] = 0;
+        return;
+    }
+    const int zr = BZ == 2 ? 1 : 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (!m[k]) continue;
+        const int li = threadIdx.x + k * CT_THREADS;
+        const int lx = li % TX, ly = (li / TX) % TY, lz = li / (TX * TY);
+        for (int ez = -zr; ez <= 0; ++ez)
+            for (int ey = -1; ey <= 1; ++ey)
+                for (int ex = -1; ex <= 1; ++ex) {
+                    if (ez == 0 && (ey > 0 || (ey == 0 && ex >= 0))) continue;
+                    const int qx = lx + ex, qy = ly + ey, qz = lz + ez;
+                    if (qx < 0 || qx >= TX || qy < 0 || qy >= TY || qz < 0) continue;
+                    const int lq = (qz * TY + qy) * TX + qx;
+                    const uint32_t qm = msk[lq];
+                    if (!qm || !blocks_touch(m[k], qm, ex, ey, ez)) continue;
+                    lunite(lab, (uint32_t)li, (uint32_t)lq);
+                }
+    }
+    __syncthreads();
+    uint32_t r[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int li = threadIdx.x + k * CT_THREADS;
+        r[k] = m[k] ? lfind(lab, (uint32_t)li) : NONE;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (!m[k]) continue;
+        const int li = threadIdx.x + k * CT_THREADS;
+        const int lx = li % TX, ly = (li / TX) % TY, lz = li / (TX * TY);
+        atomicMin(kmin + r[k], block_min_key(m[k], g, bx0 + lx, by0 + ly, bz0 + lz, BZ));
+        const int rx = (int)(r[k] % TX), ry = (int)((r[k] / TX) % TY), rz = (int)(r[k] / (TX * TY));
+        L[gb[k]] = (uint32_t)(((bz0 + rz) * B.H + (by0 + ry)) * B.W + (bx0 + rx));
+    }
+    __syncthreads();
+    uint32_t* myroots = roots + tile * BT::ROOTS;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int li = threadIdx.x + k * CT_THREADS;
+        if (m[k] && r[k] == (uint32_t)li) {
+            K[gb[k]] = kmin[li];
+            const uint32_t slot = atomicAdd(&cnt, 1u);
+            myroots[slot] = (uint32_t)gb[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) nroots[tile] = cnt;
+}
+
+// foreground blocks on the faces of each tile unite with the touching foreground
+// blocks (in the 13 earlier directions) that lie in another tile
+template <typename BT>
+__global__ __launch_bounds__(CT_THREADS) void k_bccl_border(BGrid B, TileGrid T, const uint8_t* M, uint32_t* L,
+                                                           const uint32_t* nroots) {
+    constexpr int N = BT::N, TX = BT::TX, TY = BT::TY;
+    const int64_t tile = blockIdx.x;
+    if (nroots[tile] == 0) return;  // no foreground in this tile
+    const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
+    const int64_t bx0 = tx * TX, by0 = ty * TY, bz0 = tz * BT::TZ;
+    const int zr = BT::BZ == 2 ? 1 : 0;
+    for (int li = threadIdx.x; li < N; li += CT_THREADS) {
+        const int lx = li % TX, ly = (li / TX) % TY, lz = li / (TX * TY);
+        // low z/y/x faces, and the high y/x faces (their ez = -1 / ey = -1 neighbours
+        // reach into the next y / x tile)
+        if (!((zr && lz == 0) || ly == 0 || lx == 0 || ly == TY - 1 || lx == TX - 1)) continue;
+        const int64_t bx = bx0 + lx, by = by0 + ly, bz = bz0 + lz;
+        if (bx >= B.W || by >= B.H || bz >= B.D) continue;
+        const int64_t b = (bz * B.H + by) * B.W + bx;
+        const uint32_t m = M[b];
+        if (!m) continue;
+        const uint32_t lroot = ld_relaxed(L + b);
+        uint32_t last = NONE;
+        for (int ez = -zr; ez <= 0; ++ez)
+            for (int ey = -1; ey <= 1; ++ey)
+                for (int ex = -1; ex <= 1; ++ex) {
+                    if (ez == 0 && (ey > 0 || (ey == 0 && ex >= 0))) continue;
+                    if (lz + ez >= 0 && ly + ey >= 0 && lx + ex >= 0 && lx + ex < TX && ly + ey < TY) continue;
+                    const int64_t qx = bx + ex, qy = by + ey, qz = bz + ez;
+                    if (qx < 0 || qx >= B.W || qy < 0 || qy >= B.H || qz < 0) continue;
+                    const int64_t q = (qz * B.H + qy) * B.W + qx;
+                    const uint32_t qm = M[q];
+                    if (!qm || !blocks_touch(m, qm, ex, ey, ez)) continue;
+                    const uint32_t lq = ld_relaxed(L + q);
+                    if (lq == last) continue;  // same local component as the previous neighbour
+                    last = lq;
+                    unite(L, lroot, lq);
+                }
+    }
+}
+
+// a foreground voxel's label is F[local root of its block], held as float and
+// converted with saturation like the voxel version; one row per block iteration
+template <int BZ, typename FG, typename ST>
+__global__ void k_bccl_write(FG fg, Geom g, BGrid B, const uint32_t* L, const uint32_t* F, ST out) {
+    const int64_t rows = g.D * g.H;
+    for (int64_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        const int64_t z = row / g.H, y = row - z * g.H;
+        const int64_t rb = ((z / BZ) * B.H + (y >> 1)) * B.W;
+        for (int64_t x = threadIdx.x; x < g.W; x += blockDim.x) {
+            const int64_t i = row * g.W + x;
+            out(i, fg(i) ? (float)F[L[rb + (x >> 1)]] : 0.f);
+        }
+    }
+}
+
+// uint8 binary, 32-bit destination, rows of a multiple of 4 voxels: four voxels
+// (two blocks) per thread, one 4-byte load and one 16-byte store
+template <int BZ, typename T, typename ST>
+__global__ void k_bccl_write_u8x4(const uint8_t* bin, Geom g, BGrid B, const uint32_t* L, const uint32_t* F, T* dst,
+                                  ST out) {
+    const int64_t rows = g.D * g.H;
+    for (int64_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        const int64_t z = row / g.H, y = row - z * g.H;
+        const int64_t rb = ((z / BZ) * B.H + (y >> 1)) * B.W;
+        for (int64_t x = 4 * (int64_t)threadIdx.x; x < g.W; x += 4 * (int64_t)blockDim.x) {
+            const int64_t i = row * g.W + x;
+            const uint32_t b4 = *(const uint32_t*)(bin + i);
+            if (b4 == 0) {
+                *(uint4*)(dst + i) = make_uint4(0u, 0u, 0u, 0u);
+                continue;
+            }
+            const float f0 = (b4 & 0xFFFFu) ? (float)F[L[rb + (x >> 1)]] : 0.f;
+            const float f1 = (b4 >> 16) ? (float)F[L[rb + (x >> 1) + 1]] : 0.f;
+            out(i, (b4 & 0xFFu) ? f0 : 0.f);
+            out(i + 1, (b4 & 0xFF00u) ? f0 : 0.f);
+            out(i + 2, (b4 & 0xFF0000u) ? f1 : 0.f);
+            out(i + 3, (b4 & 0xFF000000u) ? f1 : 0.f);
+        }
+    }
+}
+
+template <int BZ, typename FG>
+static void launch_bccl_write(FG fg, const Geom& g, const BGrid& B, const uint32_t* L, const uint32_t* F, DBuf dst,
+                              unsigned grid, hipStream_t s) {
+    switch (dst.dt) {
+        case CLE_I32: k_bccl_write<BZ><<<grid, 256, 0, s>>>(fg, g, B, L, F, StLabI32{(int32_t*)dst.p}); break;
+        case CLE_U32: k_bccl_write<BZ><<<grid, 256, 0, s>>>(fg, g, B, L, F, StLabU32{(uint32_t*)dst.p}); break;
+        case CLE_F32: k_bccl_write<BZ><<<grid, 256, 0, s>>>(fg, g, B, L, F, StLabF32{(float*)dst.p}); break;
+        default: k_bccl_write<BZ><<<grid, 256, 0, s>>>(fg, g, B, L, F, StLabAny{dst}); break;
+    }
+}
+
+template <int BZ>
+static void launch_bccl_write(FgU8 fg, const Geom& g, const BGrid& B, const uint32_t* L, const uint32_t* F, DBuf dst,
+                              unsigned grid, hipStream_t s) {
+    const bool vec = (g.W & 3) == 0 && ((uintptr_t)fg.p & 3) == 0 && ((uintptr_t)dst.p & 15) == 0;
+    if (vec && dst.dt == CLE_I32)
+        k_bccl_write_u8x4<BZ><<<grid, 256, 0, s>>>(fg.p, g, B, L, F, (int32_t*)dst.p, StLabI32{(int32_t*)dst.p});
+    else if (vec && dst.dt == CLE_U32)
+        k_bccl_write_u8x4<BZ><<<grid, 256, 0, s>>>(fg.p, g, B, L, F, (uint32_t*)dst.p, StLabU32{(uint32_t*)dst.p});
+    else if (vec && dst.dt == CLE_F32)
+        k_bccl_write_u8x4<BZ><<<grid, 256, 0, s>>>(fg.p, g, B, L, F, (float*)dst.p, StLabF32{(float*)dst.p});
+    else
+        launch_bccl_write<BZ, FgU8>(fg, g, B, L, F, dst, grid, s);
+}
+
+template <typename BT, typename FG>
+static int bccl_run(FG fg, const Geom& g, DBuf dst, int options, uint32_t* n_labels_dev, hipStream_t s) {
+    const int compat = (options & CLE_FP32_KEY_COMPAT) ? 1 : 0;
+    const BGrid B{(g.D + BT::BZ - 1) / BT::BZ, (g.H + 1) / 2, (g.W + 1) / 2};
+    const int64_t nb = B.D * B.H * B.W;
+    const TileGrid T{(B.D + BT::TZ - 1) / BT::TZ, (B.H + BT::TY - 1) / BT::TY, (B.W + BT::TX - 1) / BT::TX};
+    const int64_t nt = T.count();
+    Scratch Ms, Ls, Ks, Rs, Ns, Fs;
+    CLE_TRY(Ms.alloc((size_t)nb, s));
+    CLE_TRY(Ls.alloc((size_t)nb * 4, s));
+    CLE_TRY(Ks.alloc((size_t)nb * 4, s));
+    CLE_TRY(Fs.alloc((size_t)nb * 4, s));
+    CLE_TRY(Rs.alloc((size_t)nt * BT::ROOTS * 4, s));
+    CLE_TRY(Ns.alloc((size_t)nt * 4, s));
+    uint32_t *l = Ls.get<uint32_t>(), *k = Ks.get<uint32_t>(), *roots = Rs.get<uint32_t>(), *nr = Ns.get<uint32_t>();
+    k_bccl_local<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(fg, g, B, T, Ms.get<uint8_t>(), l, k, roots, nr);
+    k_bccl_border<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(B, T, Ms.get<uint8_t>(), l, nr);
+    k_ccl_resolve<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr);
+    CLE_LAUNCHED();
+    const int64_t n = g.n();
+    const uint64_t maxbit = compat ? (uint64_t)(float)(uint64_t)n : (uint64_t)n;
+    RankTable rt;
+    CLE_TRY(rt.init(maxbit + 1, s));
+    k_ccl_mark_roots<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>());
+    CLE_LAUNCHED();
+    CLE_TRY(rt.scan(n_labels_dev, s));
+    k_ccl_final<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>(),
+                                                        rt.prefix.get<uint32_t>(), Fs.get<uint32_t>());
+    const int64_t rows = g.D * g.H;
+    const unsigned grid = (unsigned)(rows < 262144 ? rows : 262144);
+    launch_bccl_write<BT::BZ>(fg, g, B, l, Fs.get<uint32_t>(), dst, grid, s);
+    CLE_LAUNCHED();
+    return CLE_OK;
+}
+
 int ccl_impl(DBuf binary, const Geom& g, DBuf dst, int connectivity, int options, uint32_t* n_labels_dev,
              hipStream_t s) {
     if (g.n() == 0) return n_labels_dev ? fill_u32(n_labels_dev, 0u, 1, s) : CLE_OK;
     const bool three_d = g.ndim == 3 && g.D > 1;
     if (binary.dt == CLE_U8 || binary.dt == CLE_I8) {
         FgU8 fg{(const uint8_t*)binary.p};
+        if (connectivity == CLE_CONN_BOX)
+            return three_d ? bccl_run<BlkTile3D>(fg, g, dst, options, n_labels_dev, s)
+                           : bccl_run<BlkTile2D>(fg, g, dst, options, n_labels_dev, s);
         return three_d ? ccl_run<Tile3D>(fg, g, dst, connectivity, options, n_labels_dev, s)
                        : ccl_run<Tile2D>(fg, g, dst, connectivity, options, n_labels_dev, s);
     }
     FgAny fg{binary};
+    if (connectivity == CLE_CONN_BOX)
+        return three_d ? bccl_run<BlkTile3D>(fg, g, dst, options, n_labels_dev, s)
+                       : bccl_run<BlkTile2D>(fg, g, dst, options, n_labels_dev, s);
     return three_d ? ccl_run<Tile3D>(fg, g, dst, connectivity, options, n_labels_dev, s)
                    : ccl_run<Tile2D>(fg, g, dst, connectivity, options, n_labels_dev, s);
 }

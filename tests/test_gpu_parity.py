@@ -404,6 +404,33 @@ def test_ccl_multi_tile(cle, shape, kind, p):
     assert np.array_equal(cle.pull(fn(b)), O.connected_components_labeling(b, kind))
 
 
+@pytest.mark.parametrize("shape,p", [((16, 48, 128), 0.25), ((9, 34, 68), 0.35), ((130, 256), 0.4),
+                                     ((7, 19, 44), 0.3), ((2, 64, 64), 0.5)])
+@pytest.mark.parametrize("dst_dtype", [np.uint32, np.int32, np.float32, np.uint16])
+def test_ccl_box_uint8_device_input(cle, shape, p, dst_dtype):
+    """uint8 binaries kept on the device (the 4-voxel gather for 32-bit destinations
+    when rows hold a multiple of 4 voxels), across 2x2x2-block tile borders."""
+    rng = np.random.default_rng(33)
+    b = (rng.random(shape) < p).astype(np.uint8)
+    want = O.connected_components_labeling(b, "box")
+    d = cle.create(b.shape, dst_dtype)
+    cle.connected_components_labeling_box(cle.asarray(b), d)
+    assert np.array_equal(cle.pull(d), want.astype(dst_dtype))
+
+
+def test_ccl_box_blocks_side_by_side_not_touching(cle):
+    """Neighbouring 2x2x2 blocks whose foreground voxels are two apart stay separate."""
+    b = np.zeros((6, 8, 12), np.uint8)
+    b[0, 0, 1] = 1; b[0, 0, 4] = 1          # x = 1 and x = 4: blocks 0 and 2 (gap block 1)
+    b[2, 1, 2] = 1; b[2, 1, 5] = 1          # blocks 1 and 2 along x, 3 apart
+    b[3, 2, 3] = 1; b[3, 4, 3] = 1          # y = 2 and 4: gap of one row
+    b[5, 6, 8] = 1; b[4, 7, 9] = 1          # diagonal across block corner: touching
+    b[1, 5, 10] = 1; b[3, 6, 11] = 1        # z 1 and 3: not touching
+    want = O.connected_components_labeling(b, "box")
+    assert np.array_equal(cle.pull(cle.connected_components_labeling_box(cle.asarray(b))), want)
+    assert np.array_equal(cle.pull(cle.connected_components_labeling_box(b)), want)
+
+
 def test_ccl_diagonal_tile_crossings(cle):
     """Diagonal chains crossing tile corners (+x/-y, +x/-z, +y/-z neighbours)."""
     b = np.zeros((12, 24, 140), np.uint8)
