@@ -160,13 +160,18 @@ int cle_gauss_otsu_labeling(const cle_buf* src, const cle_buf* dst, float outlin
  * MAX_MEAN_DISTANCE_TO_CENTROID_RATIO */
 #define CLE_STAT_COLUMNS 36
 /* _tier9/_statistics_of_labelled_pixels.py:7-293 + statistics_per_label_x.cl +
- * standard_deviation_per_label_x.cl.  num_labels = max(labels) + 1 (the caller
- * determines it, e.g. with cle_minimum_maximum).  out_table is a device float32
- * buffer of shape (CLE_STAT_COLUMNS, num_labels - 1); column j describes label
- * j + 1.  Label 0 (background) and negative labels are not measured, as in the
- * reference (sum_background = 0). */
+ * standard_deviation_per_label_x.cl.  num_labels = max(labels) + 1 as the reference
+ * computes it (_statistics_of_labelled_pixels.py:53), or 0: the library derives it
+ * from the label image in the same pass that finds the label runs.  out_table is a
+ * device float32 buffer of shape (CLE_STAT_COLUMNS, columns), columns >= num_labels -
+ * 1; column j describes label j + 1.  *out_num_labels (if not NULL) receives
+ * num_labels; with num_labels = 0 and a table that is too small the call fails with
+ * CLE_E_SHAPE after setting it, so the caller can grow the table and call again.
+ * Label 0 (background) and negative labels are not measured, as in the reference
+ * (sum_background = 0).  Synchronises the stream once (to size the run buffers). */
 int cle_statistics_of_labelled_pixels(const cle_buf* intensity, const cle_buf* labels,
-                                      int64_t num_labels, const cle_buf* out_table, void* stream);
+                                      int64_t num_labels, const cle_buf* out_table,
+                                      int64_t* out_num_labels, void* stream);
 
 #ifdef __cplusplus
 }

@@ -68,7 +68,7 @@ SIGNATURES = {
     "cle_masked_voronoi_labeling": [_P, _P, _P, ctypes.POINTER(ctypes.c_int), _V],
     "cle_voronoi_otsu_labeling": [_P, _P, _F, _F, _V],
     "cle_gauss_otsu_labeling": [_P, _P, _F, _V],
-    "cle_statistics_of_labelled_pixels": [_P, _P, ctypes.c_int64, _P, _V],
+    "cle_statistics_of_labelled_pixels": [_P, _P, ctypes.c_int64, _P, ctypes.POINTER(ctypes.c_int64), _V],
 }
 _STR = {"cle_version", "cle_last_error"}
 
@@ -115,3 +115,8 @@ def check(rc: int, what: str) -> None:
 
 def call(name: str, *args) -> None:
     check(getattr(load(), name)(*args), name)
+
+
+def call_rc(name: str, *args) -> int:
+    """Call an entry point and return its status without raising."""
+    return getattr(load(), name)(*args)

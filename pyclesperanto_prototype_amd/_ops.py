@@ -310,15 +310,32 @@ STATISTICS_KEYS = [
 ]
 
 
+_STAT_COLUMNS_HINT = [4096]  # table columns allocated when the label count is derived
+
+
 def statistics_table(intensity_image: Array, label_image: Array, num_labels: int = None):
-    """Device float32 table (36, num_labels - 1) in STATISTICS_ENTRY column order."""
-    if num_labels is None:
-        num_labels = int(maximum_of_all_pixels(label_image)) + 1
-    n = max(int(num_labels) - 1, 0)
-    table = create((_lib.CLE_STAT_COLUMNS, max(n, 1)), np.float32)
-    _lib.call("cle_statistics_of_labelled_pixels", intensity_image.buf(), label_image.buf(), int(num_labels),
-              table.buf(), _S())
-    return table, n
+    """Device float32 table (36, columns >= num_labels - 1) in STATISTICS_ENTRY column
+    order, and num_labels - 1.  Without num_labels the library derives max(labels) + 1
+    in its run pass (no separate maximum pass); a table found too small is grown once."""
+    out_n = ctypes.c_int64(0)
+    if num_labels is not None:
+        n = max(int(num_labels) - 1, 0)
+        table = create((_lib.CLE_STAT_COLUMNS, max(n, 1)), np.float32)
+        _lib.call("cle_statistics_of_labelled_pixels", intensity_image.buf(), label_image.buf(), int(num_labels),
+                  table.buf(), ctypes.byref(out_n), _S())
+        return table, n
+    for _ in range(2):
+        cols = _STAT_COLUMNS_HINT[0]
+        table = create((_lib.CLE_STAT_COLUMNS, cols), np.float32)
+        rc = _lib.call_rc("cle_statistics_of_labelled_pixels", intensity_image.buf(), label_image.buf(), 0,
+                          table.buf(), ctypes.byref(out_n), _S())
+        need = int(out_n.value) - 1
+        if rc == _lib.CLE_E_SHAPE and need > cols:
+            _STAT_COLUMNS_HINT[0] = 1 << max(need - 1, 1).bit_length()
+            continue
+        _lib.check(rc, "cle_statistics_of_labelled_pixels")
+        return table, max(need, 0)
+    raise RuntimeError("statistics table could not be sized")
 
 
 @plugin_function(output_creator=create_none, categories=['bia-bob-suggestion'])
@@ -333,9 +350,9 @@ def statistics_of_labelled_pixels(intensity_image: Image = None, label_image: Im
         set(label_image, 1)
     if intensity_image is None:
         intensity_image = label_image
-    num_labels = int(maximum_of_all_pixels(label_image)) + 1
-    table, n = statistics_table(intensity_image, label_image, num_labels)
-    host = pull(table)[:, :n]
+    table, n = statistics_table(intensity_image, label_image)
+    num_labels = n + 1
+    host = table.tensor[:, :n].cpu().numpy()
     cols = {k: host[i].copy() for i, k in enumerate(_TABLE_COLUMNS)}
     out = {}
     for k in STATISTICS_KEYS:

@@ -23,6 +23,9 @@
 // the foreground.
 #include <float.h>
 
+#include <atomic>
+#include <type_traits>
+
 #include <hipcub/hipcub.hpp>
 
 #include "common.hpp"
@@ -31,6 +34,7 @@ namespace cle {
 
 template <typename T>
 struct LabT {
+    using type = T;
     const T* p;
     // the reference reads a label as (int) of the float it loads
     __device__ __forceinline__ int operator()(int64_t i) const { return (int)(float)p[i]; }
@@ -51,80 +55,205 @@ __device__ __forceinline__ uint64_t run_key(const KeyBits& kb, uint64_t l, uint6
     return (((l << kb.yb | y) << kb.zb | z) << kb.xb) | x0;
 }
 
-// Label runs of a row span, one span per wavefront, no LDS and no block barrier.
-// Lane l holds the labels x = sx0 + l + 64 k (k = 0..15; coalesced loads).  A voxel
-// starts a run if its label is measured (0 < l < L) and differs from its left
-// neighbour inside the span, and ends one if it differs from its right neighbour;
-// ballots number the starts and the ends in x order, so start j and end j bound run j.
-// Runs are split at span borders, which keeps them in x order.
+// Label runs of a row span, one span of 1024 voxels per wavefront, no LDS, no atomics.
+// VEC layout (rows of a multiple of 4 voxels, 16-byte aligned): lane l holds the four
+// voxels x = sx0 + 256 k + 4 l + j (k, j = 0..3; one 16-byte load per k); scalar
+// layout: x = sx0 + 64 k + l (k = 0..15).  A voxel starts a run if its label is
+// measured (0 < label < lcap) and differs from its left neighbour inside the span,
+// and ends one if it differs from its right neighbour; ballots number the starts and
+// the ends in x order, so start i and end i bound run i.  Runs are split at span
+// borders, which keeps them in x order.  A span writes its runs into its own
+// `scap` slots (slot mode, off == nullptr; runs past scap are counted, not written)
+// or at off[span] (exact mode, once the offsets are known); count[span] = its runs.
+// bmax[block] = (largest span count, largest measured label) of the block.
 constexpr int RB_THREADS = 256;
-constexpr int RB_K = 16;
-constexpr int RB_SPAN = 64 * RB_K;
+constexpr int RB_SPAN = 1024;
 
 __device__ __forceinline__ uint64_t lanes_below() {
     const int lane = threadIdx.x & 63;
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// PASS 0: span_count[sp] = number of runs; PASS 1: write them at span_off[sp] + rank
-template <int PASS, typename LAB>
-__global__ __launch_bounds__(RB_THREADS) void k_runs(LAB lab, Geom g, int64_t L, KeyBits kb, uint32_t* span_count,
-                                                     const uint32_t* span_off, uint64_t* keys, uint32_t* ends) {
+template <typename T> struct Vec4;
+template <> struct Vec4<int32_t> { using V = int4; };
+template <> struct Vec4<uint32_t> { using V = uint4; };
+template <> struct Vec4<float> { using V = float4; };
+
+template <bool VEC, typename LAB>
+__global__ __launch_bounds__(RB_THREADS) void k_runs(LAB lab, Geom g, int64_t lcap, KeyBits kb, uint32_t scap,
+                                                     const uint32_t* off, uint32_t* count, uint64_t* keys,
+                                                     uint32_t* ends, uint32_t* bmax) {
+    constexpr int E = VEC ? 4 : 1;         // consecutive voxels per lane
+    constexpr int K = RB_SPAN / (64 * E);  // blocks of 64 * E voxels
     const int64_t spans = (g.W + RB_SPAN - 1) / RB_SPAN;
     const int64_t total = g.D * g.H * spans;
     const int lane = threadIdx.x & 63;
     const uint64_t below = lanes_below();
     const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    int lmax = 0;
+    uint32_t cmax = 0;
     for (int64_t sp = wave0; sp < total; sp += nwaves) {
         const int64_t row = sp / spans;
         const int64_t sx0 = (sp - row * spans) * RB_SPAN;
         const int n = (int)((sx0 + RB_SPAN < g.W ? sx0 + RB_SPAN : g.W) - sx0);
         const int64_t base = row * g.W + sx0;
-        int v[RB_K];
+        int v[K][E];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i0 = 64 * E * k + E * lane;
+            if constexpr (VEC) {
+                if (i0 < n) {  // n is a multiple of 4: all four or none
+                    const auto q = *(const typename Vec4<typename LAB::type>::V*)(lab.p + base + i0);
+                    v[k][0] = (int)(float)q.x; v[k][1] = (int)(float)q.y;
+                    v[k][2] = (int)(float)q.z; v[k][3] = (int)(float)q.w;
+                } else {
+                    v[k][0] = v[k][1] = v[k][2] = v[k][3] = 0;
+                }
+            } else {
+                v[k][0] = i0 < n ? lab(base + i0) : 0;
+            }
+        }
         bool any = false;
 #pragma unroll
-        for (int k = 0; k < RB_K; ++k) {
-            const int i = lane + 64 * k;
-            v[k] = i < n ? lab(base + i) : 0;
-            any |= v[k] != 0;
-        }
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                const bool valid = v[k][j] > 0 && v[k][j] < lcap;
+                any |= valid;
+                lmax = valid && v[k][j] > lmax ? v[k][j] : lmax;
+            }
         if (!__any(any)) {
-            if (PASS == 0 && lane == 0) span_count[sp] = 0;
+            if (lane == 0) count[sp] = 0;
             continue;
         }
-        uint32_t nstart = 0, nend = 0;
-        uint64_t slot0 = 0;
-        uint64_t y = 0, z = 0;
-        if (PASS == 1) {
-            slot0 = span_off[sp];
-            y = (uint64_t)(row % g.H);
-            z = (uint64_t)(row / g.H);
-        }
-        int prev_last = 0;  // label of voxel 64 k - 1 (lane 63 of the previous k)
+        uint64_t ms[K][E], me[K][E];
+        uint32_t nrun = 0;
+        int prev_last = 0;  // last voxel of the previous block of 64 * E
 #pragma unroll
-        for (int k = 0; k < RB_K; ++k) {
-            const int i = lane + 64 * k;
-            const int l = v[k];
-            int left = __shfl_up(l, 1);
+        for (int k = 0; k < K; ++k) {
+            int left = __shfl_up(v[k][E - 1], 1);
             if (lane == 0) left = prev_last;
-            int right = __shfl_down(l, 1);
-            const int next_first = k + 1 < RB_K ? __shfl(v[k + 1 < RB_K ? k + 1 : k], 0) : 0;
-            if (lane == 63) right = next_first;
-            prev_last = __shfl(l, 63);
-            const bool valid = i < n && l > 0 && l < L;
-            const bool is_start = valid && (i == 0 || left != l);
-            const bool is_end = valid && (i == n - 1 || right != l);
-            const uint64_t ms = __ballot(is_start), me = __ballot(is_end);
-            if (PASS == 1) {
-                if (is_start)
-                    keys[slot0 + nstart + __popcll(ms & below)] = run_key(kb, (uint64_t)l, y, z, (uint64_t)(sx0 + i));
-                if (is_end) ends[slot0 + nend + __popcll(me & below)] = (uint32_t)(sx0 + i + 1);
+            int right = __shfl_down(v[k][0], 1);
+            const int next_first = __shfl(v[k + 1 < K ? k + 1 : k][0], 0);
+            if (lane == 63) right = k + 1 < K ? next_first : 0;
+            prev_last = __shfl(v[k][E - 1], 63);
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                const int i = 64 * E * k + E * lane + j;
+                const int l = v[k][j];
+                const int lf = j == 0 ? left : v[k][j - 1];
+                const int rt = j == E - 1 ? right : v[k][j + 1];
+                const bool valid = i < n && l > 0 && l < lcap;
+                ms[k][j] = __ballot(valid && (i == 0 || lf != l));
+                me[k][j] = __ballot(valid && (i == n - 1 || rt != l));
+                nrun += (uint32_t)__popcll(ms[k][j]);
             }
-            nstart += (uint32_t)__popcll(ms);
-            nend += (uint32_t)__popcll(me);
         }
-        if (PASS == 0 && lane == 0) span_count[sp] = nstart;
+        cmax = nrun > cmax ? nrun : cmax;
+        if (lane == 0) count[sp] = nrun;
+        uint64_t slot0;
+        uint32_t lim;
+        if (off) {
+            slot0 = off[sp];
+            lim = nrun;
+        } else {
+            slot0 = (uint64_t)sp * scap;
+            lim = scap;
+        }
+        const uint64_t y = (uint64_t)(row % g.H), z = (uint64_t)(row / g.H);
+        uint32_t ns = 0, ne = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            // runs of this block before this lane (all of its E voxels), then inside it
+            uint32_t ps = ns, pe = ne;
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                ps += (uint32_t)__popcll(ms[k][j] & below);
+                pe += (uint32_t)__popcll(me[k][j] & below);
+            }
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                const int i = 64 * E * k + E * lane + j;
+                if ((ms[k][j] >> lane) & 1) {
+                    if (ps < lim) keys[slot0 + ps] = run_key(kb, (uint64_t)v[k][j], y, z, (uint64_t)(sx0 + i));
+                    ++ps;
+                }
+                if ((me[k][j] >> lane) & 1) {
+                    if (pe < lim) ends[slot0 + pe] = (uint32_t)(sx0 + i + 1);
+                    ++pe;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                ns += (uint32_t)__popcll(ms[k][j]);
+                ne += (uint32_t)__popcll(me[k][j]);
+            }
+        }
+    }
+    // block maxima of (span count, label), one store per block
+    __shared__ uint32_t red[2][RB_THREADS / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        const int a = __shfl_xor(lmax, o);
+        const uint32_t c = __shfl_xor(cmax, o);
+        lmax = a > lmax ? a : lmax;
+        cmax = c > cmax ? c : cmax;
+    }
+    if (lane == 0) {
+        red[0][threadIdx.x >> 6] = cmax;
+        red[1][threadIdx.x >> 6] = (uint32_t)lmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t c = 0, l = 0;
+        for (int w = 0; w < RB_THREADS / 64; ++w) {
+            c = red[0][w] > c ? red[0][w] : c;
+            l = red[1][w] > l ? red[1][w] : l;
+        }
+        bmax[2 * blockIdx.x] = c;
+        bmax[2 * blockIdx.x + 1] = l;
+    }
+}
+
+// sum[0] = runs (off[last] + count[last]), sum[1] = largest span count, sum[2] =
+// largest measured label (one block)
+__global__ __launch_bounds__(1024) void k_runs_summary(const uint32_t* count, const uint32_t* off, int64_t spans,
+                                                      const uint32_t* bmax, int nblocks, uint32_t* sum) {
+    __shared__ uint32_t sc[1024], sl[1024];
+    uint32_t c = 0, l = 0;
+    for (int b = threadIdx.x; b < nblocks; b += 1024) {
+        c = bmax[2 * b] > c ? bmax[2 * b] : c;
+        l = bmax[2 * b + 1] > l ? bmax[2 * b + 1] : l;
+    }
+    sc[threadIdx.x] = c;
+    sl[threadIdx.x] = l;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sc[threadIdx.x] = sc[threadIdx.x] > sc[threadIdx.x + o] ? sc[threadIdx.x] : sc[threadIdx.x + o];
+            sl[threadIdx.x] = sl[threadIdx.x] > sl[threadIdx.x + o] ? sl[threadIdx.x] : sl[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sum[0] = spans > 0 ? off[spans - 1] + count[spans - 1] : 0u;
+        sum[1] = sc[0];
+        sum[2] = sl[0];
+    }
+}
+
+// slot mode -> contiguous: one wavefront per span
+__global__ void k_runs_compact(const uint64_t* keys, const uint32_t* ends, const uint32_t* count,
+                               const uint32_t* off, int64_t spans, uint32_t scap, uint64_t* okeys, uint32_t* oends) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t sp = wave0; sp < spans; sp += nwaves) {
+        const uint32_t c = count[sp], o = off[sp];
+        for (uint32_t i = lane; i < c; i += 64) {
+            okeys[o + i] = keys[(uint64_t)sp * scap + i];
+            oends[o + i] = ends[(uint64_t)sp * scap + i];
+        }
     }
 }
 
@@ -169,9 +298,28 @@ struct CellGeom {
     }
 };
 
+// intensity loaders
+struct InU16 {
+    const uint16_t* p;
+    __device__ __forceinline__ float operator()(int64_t i) const { return (float)p[i]; }
+};
+struct InF32 {
+    const float* p;
+    __device__ __forceinline__ float operator()(int64_t i) const { return p[i]; }
+};
+struct InAny {
+    DBuf b;
+    __device__ __forceinline__ float operator()(int64_t i) const { return ld(b, i); }
+};
+
+// Voxels of a cell are visited in the reference's (z, x) order; loads go out CB at a
+// time (independent), the float32 accumulation then runs over them in order.
+constexpr int CB = 8;
+
 // pass 1, one thread per cell: 16 float32 planes in the reference's order:
 // 0-2 sum x,y,z  3 count  4-6 sum x*v, y*v, z*v  7 sum v  8/9 min/max v  10-15 min/max x,y,z
-__global__ __launch_bounds__(256) void k_cells_pass1(DBuf inten, CellGeom cg, const uint64_t* keys,
+template <typename IN>
+__global__ __launch_bounds__(256) void k_cells_pass1(IN in, CellGeom cg, const uint64_t* keys,
                                                      const uint32_t* ends, const uint32_t* cell_start,
                                                      const uint32_t* ncells_p, int64_t cap, float* acc) {
 #pragma clang fp contract(off)
@@ -182,35 +330,47 @@ __global__ __launch_bounds__(256) void k_cells_pass1(DBuf inten, CellGeom cg, co
         float s[16];
         for (int k = 0; k < 8; ++k) s[k] = 0.f;
         for (int k = 8; k < 16; k += 2) { s[k] = FLT_MAX; s[k + 1] = -FLT_MAX; }
+        uint64_t key = keys[r0];
+        uint32_t x1 = ends[r0];
         for (uint32_t r = r0; r < r1; ++r) {
             int64_t y, z, x0;
-            cg.decode(keys[r], y, z, x0);
-            const int64_t x1 = ends[r];
+            cg.decode(key, y, z, x0);
+            const int64_t xe = x1;
+            if (r + 1 < r1) {  // next run's bounds while this one is summed
+                key = keys[r + 1];
+                x1 = ends[r + 1];
+            }
             const float yf = (float)y, zf = (float)z;
             const int64_t base = (z * cg.H + y) * cg.W;
-            for (int64_t x = x0; x < x1; ++x) {
-                const float v = ld(inten, base + x);
-                const float xf = (float)x;
-                if (s[3] == 0.f) {
-                    s[8] = v; s[9] = v; s[10] = xf; s[11] = xf; s[12] = yf; s[13] = yf; s[14] = zf; s[15] = zf;
-                } else {
-                    s[8] = s[8] > v ? v : s[8];
-                    s[9] = s[9] < v ? v : s[9];
-                    s[10] = s[10] > xf ? xf : s[10];
-                    s[11] = s[11] < xf ? xf : s[11];
-                    s[12] = s[12] > yf ? yf : s[12];
-                    s[13] = s[13] < yf ? yf : s[13];
-                    s[14] = s[14] > zf ? zf : s[14];
-                    s[15] = s[15] < zf ? zf : s[15];
+            for (int64_t x = x0; x < xe; x += CB) {
+                float v[CB];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) v[j] = x + j < xe ? in(base + x + j) : 0.f;
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    if (x + j >= xe) break;
+                    const float xf = (float)(x + j);
+                    if (s[3] == 0.f) {
+                        s[8] = v[j]; s[9] = v[j]; s[10] = xf; s[11] = xf; s[12] = yf; s[13] = yf; s[14] = zf; s[15] = zf;
+                    } else {
+                        s[8] = s[8] > v[j] ? v[j] : s[8];
+                        s[9] = s[9] < v[j] ? v[j] : s[9];
+                        s[10] = s[10] > xf ? xf : s[10];
+                        s[11] = s[11] < xf ? xf : s[11];
+                        s[12] = s[12] > yf ? yf : s[12];
+                        s[13] = s[13] < yf ? yf : s[13];
+                        s[14] = s[14] > zf ? zf : s[14];
+                        s[15] = s[15] < zf ? zf : s[15];
+                    }
+                    s[0] = s[0] + xf;
+                    s[1] = s[1] + yf;
+                    s[2] = s[2] + zf;
+                    s[3] = s[3] + 1.f;
+                    s[4] = s[4] + xf * v[j];
+                    s[5] = s[5] + yf * v[j];
+                    s[6] = s[6] + zf * v[j];
+                    s[7] = s[7] + v[j];
                 }
-                s[0] = s[0] + xf;
-                s[1] = s[1] + yf;
-                s[2] = s[2] + zf;
-                s[3] = s[3] + 1.f;
-                s[4] = s[4] + xf * v;
-                s[5] = s[5] + yf * v;
-                s[6] = s[6] + zf * v;
-                s[7] = s[7] + v;
             }
         }
         for (int k = 0; k < 16; ++k) acc[(int64_t)k * cap + c] = s[k];
@@ -251,7 +411,8 @@ __global__ void k_labels_pass1(const float* acc, int64_t cap, const uint32_t* ls
 
 // pass 2, one thread per cell: 0 sum distance to centroid, 1 to mass centre,
 // 2 sum (v - mean)^2 / area, 3 count, 4/5 max distances
-__global__ __launch_bounds__(256) void k_cells_pass2(DBuf inten, CellGeom cg, const uint64_t* keys,
+template <typename IN>
+__global__ __launch_bounds__(256) void k_cells_pass2(IN in, CellGeom cg, const uint64_t* keys,
                                                      const uint32_t* ends, const uint32_t* cell_start,
                                                      const uint32_t* ncells_p, int64_t cap, const float* img,
                                                      int64_t L, float* acc2) {
@@ -261,32 +422,45 @@ __global__ __launch_bounds__(256) void k_cells_pass2(DBuf inten, CellGeom cg, co
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < cap && c < ncells;
          c += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t r0 = cell_start[c], r1 = cell_start[c + 1];
-        const int64_t l = (int64_t)(keys[r0] >> lshift);
+        uint64_t key = keys[r0];
+        uint32_t x1 = ends[r0];
+        const int64_t l = (int64_t)(key >> lshift);
         const float cx = img[0 * L + l], cy = img[1 * L + l], cz = img[2 * L + l];
         const float mx = img[3 * L + l], my = img[4 * L + l], mz = img[5 * L + l];
         const float mean = img[6 * L + l], area = img[7 * L + l];
         float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, m4 = 0.f, m5 = 0.f;
         for (uint32_t r = r0; r < r1; ++r) {
             int64_t y, z, x0;
-            cg.decode(keys[r], y, z, x0);
-            const int64_t x1 = ends[r];
+            cg.decode(key, y, z, x0);
+            const int64_t xe = x1;
+            if (r + 1 < r1) {
+                key = keys[r + 1];
+                x1 = ends[r + 1];
+            }
             const float yf = (float)y, zf = (float)z;
+            const float dy = yf - cy, dz = zf - cz, ey = yf - my, ez = zf - mz;
             const int64_t base = (z * cg.H + y) * cg.W;
-            for (int64_t x = x0; x < x1; ++x) {
-                const float v = ld(inten, base + x);
-                const float xf = (float)x;
-                const float dx = xf - cx, dy = yf - cy, dz = zf - cz;
-                const float dc = sqrtf((dx * dx + dy * dy) + dz * dz);
-                const float ex = xf - mx, ey = yf - my, ez = zf - mz;
-                const float dm = sqrtf((ex * ex + ey * ey) + ez * ez);
-                const float dv = v - mean;
-                const float sq = (dv * dv) / area;
-                if (s3 == 0.f) { m4 = dc; m5 = dm; }
-                else { m4 = m4 < dc ? dc : m4; m5 = m5 < dm ? dm : m5; }
-                s0 = s0 + dc;
-                s1 = s1 + dm;
-                s2 = s2 + sq;
-                s3 = s3 + 1.f;
+            for (int64_t x = x0; x < xe; x += CB) {
+                float v[CB];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) v[j] = x + j < xe ? in(base + x + j) : 0.f;
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    if (x + j >= xe) break;
+                    const float xf = (float)(x + j);
+                    const float dx = xf - cx;
+                    const float dc = sqrtf((dx * dx + dy * dy) + dz * dz);
+                    const float ex = xf - mx;
+                    const float dm = sqrtf((ex * ex + ey * ey) + ez * ez);
+                    const float dv = v[j] - mean;
+                    const float sq = (dv * dv) / area;
+                    if (s3 == 0.f) { m4 = dc; m5 = dm; }
+                    else { m4 = m4 < dc ? dc : m4; m5 = m5 < dm ? dm : m5; }
+                    s0 = s0 + dc;
+                    s1 = s1 + dm;
+                    s2 = s2 + sq;
+                    s3 = s3 + 1.f;
+                }
             }
         }
         acc2[0 * cap + c] = s0;
@@ -300,7 +474,7 @@ __global__ __launch_bounds__(256) void k_cells_pass2(DBuf inten, CellGeom cg, co
 
 // y-reduction of pass 2 and the STATISTICS_ENTRY table, one thread per label 1..L-1
 __global__ void k_labels_table(const float* acc2, int64_t cap, const uint32_t* lstart, const uint32_t* lend,
-                               const float* lab16, const float* img, int64_t L, float* out) {
+                               const float* lab16, const float* img, int64_t L, int64_t stride, float* out) {
 #pragma clang fp contract(off)
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (l >= L) return;
@@ -316,7 +490,7 @@ __global__ void k_labels_table(const float* acc2, int64_t cap, const uint32_t* l
     }
     auto pl = [&](int k) { return lab16[(int64_t)k * L + l]; };
     const float area = pl(3), sv = pl(7);
-    const int64_t N = L - 1, j = l - 1;
+    const int64_t j = l - 1;
     float col[CLE_STAT_COLUMNS];
     col[0] = (float)l;
     col[1] = pl(10); col[2] = pl(12); col[3] = pl(14);
@@ -333,11 +507,7 @@ __global__ void k_labels_table(const float* acc2, int64_t cap, const uint32_t* l
     col[25] = img[0 * L + l]; col[26] = img[1 * L + l]; col[27] = img[2 * L + l];
     col[28] = s1; col[29] = s1 / area; col[30] = m5; col[31] = m5 / col[29];
     col[32] = s0; col[33] = s0 / area; col[34] = m4; col[35] = m4 / col[33];
-    for (int c = 0; c < CLE_STAT_COLUMNS; ++c) out[c * N + j] = col[c];
-}
-
-__global__ void k_span_total(const uint32_t* cnt, const uint32_t* off, int64_t n, uint32_t* total) {
-    *total = off[n - 1] + cnt[n - 1];
+    for (int c = 0; c < CLE_STAT_COLUMNS; ++c) out[c * stride + j] = col[c];
 }
 
 static int bits_for(uint64_t maxval) {
@@ -346,70 +516,125 @@ static int bits_for(uint64_t maxval) {
     return b;
 }
 
+// run slots per span, sized from the previous call's busiest span
+static std::atomic<uint32_t> g_span_slots{32};
+
+// 32-bit label types have the 16-byte layout; it needs rows of a multiple of 4
+// labels and an aligned buffer
 template <typename LAB>
-static int stats_run(DBuf inten, LAB lab, const Geom& g, int64_t L, float* table, hipStream_t s) {
+constexpr bool kVecLab = std::is_same<LAB, LabT<int32_t>>::value || std::is_same<LAB, LabT<uint32_t>>::value ||
+                         std::is_same<LAB, LabT<float>>::value;
+
+// L > 0: labels 1..L-1 are measured (the caller passes max(labels) + 1).  L == 0:
+// every positive label is measured and L = max(labels) + 1 comes from the run pass.
+// The table holds CLE_STAT_COLUMNS rows of tcols >= L - 1 columns; a smaller table
+// fails with CLE_E_SHAPE after *out_L is set, so the caller can grow it and retry.
+template <typename LAB>
+static int stats_run(DBuf inten, LAB lab, const Geom& g, int64_t L, float* table, int64_t tcols, int64_t* out_L,
+                     hipStream_t s) {
     KeyBits kb;
     kb.xb = bits_for((uint64_t)(g.W > 0 ? g.W - 1 : 0));
     kb.zb = bits_for((uint64_t)(g.D > 0 ? g.D - 1 : 0));
     kb.yb = bits_for((uint64_t)(g.H > 0 ? g.H - 1 : 0));
-    kb.lb = bits_for((uint64_t)(L - 1));
+    kb.lb = L > 0 ? bits_for((uint64_t)(L - 1)) : 31;  // labels are ints: at most 31 bits
     if (kb.total() > 64) return fail(CLE_E_SHAPE, "statistics: %d key bits exceed 64", kb.total());
     const int64_t spans = g.D * g.H * ((g.W + RB_SPAN - 1) / RB_SPAN);
     unsigned gr = grid_for((size_t)(spans > 0 ? spans : 1), RB_THREADS / 64);  // one wavefront per span
     if (gr > 32768) gr = 32768;
-    // (1) count the runs per span, offsets by an exclusive scan (one synchronisation to
-    //     size the buffers), then write every span's runs at its offset
-    Scratch scount, soff, stot;
-    const int64_t scap = spans > 0 ? spans : 1;
-    CLE_TRY(scount.alloc((size_t)scap * 4, s));
-    CLE_TRY(soff.alloc((size_t)scap * 4, s));
-    CLE_TRY(stot.alloc(8, s));
-    CLE_TRY(fill_u32(stot.get<uint32_t>(), 0u, 2, s));
+    const int64_t lcap = L > 0 ? L : (int64_t)1 << 40;
+    // (1) the runs: every span writes into scap slots of its own (sized from the
+    //     previous call), an exclusive scan of the span counts gives the offsets, one
+    //     synchronisation reads the totals, and the slots are compacted; a span with
+    //     more runs than slots reruns the pass writing at the exact offsets
+    const int64_t scount = spans > 0 ? spans : 1;
+    uint32_t scap = (uint32_t)g_span_slots.load(std::memory_order_relaxed);
+    Scratch cnt, off, bmx, sum, k0s, e0s;
+    CLE_TRY(cnt.alloc((size_t)scount * 4, s));
+    CLE_TRY(off.alloc((size_t)scount * 4, s));
+    CLE_TRY(bmx.alloc((size_t)gr * 8, s));
+    CLE_TRY(sum.alloc(16, s));
+    CLE_TRY(k0s.alloc((size_t)scount * scap * 8, s));
+    CLE_TRY(e0s.alloc((size_t)scount * scap * 4, s));
+    CLE_TRY(fill_u32(sum.get<uint32_t>(), 0u, 4, s));
+    auto runs = [&](const uint32_t* o, uint64_t* kk, uint32_t* ee) {
+        if constexpr (kVecLab<LAB>) {
+            if ((g.W & 3) == 0 && ((uintptr_t)lab.p & 15) == 0) {
+                k_runs<true><<<gr, RB_THREADS, 0, s>>>(lab, g, lcap, kb, scap, o, cnt.get<uint32_t>(), kk, ee,
+                                                       bmx.get<uint32_t>());
+                return;
+            }
+        }
+        k_runs<false><<<gr, RB_THREADS, 0, s>>>(lab, g, lcap, kb, scap, o, cnt.get<uint32_t>(), kk, ee,
+                                                bmx.get<uint32_t>());
+    };
     if (spans > 0) {
-        k_runs<0><<<gr, RB_THREADS, 0, s>>>(lab, g, L, kb, scount.get<uint32_t>(), nullptr, nullptr, nullptr);
+        runs(nullptr, k0s.get<uint64_t>(), e0s.get<uint32_t>());
         size_t sb = 0;
-        CLE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, scount.get<uint32_t>(), soff.get<uint32_t>(), (int)spans, s));
+        CLE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, cnt.get<uint32_t>(), off.get<uint32_t>(), (int)spans, s));
         Scratch st;
         CLE_TRY(st.alloc(sb, s));
-        CLE_HIP(hipcub::DeviceScan::ExclusiveSum(st.p, sb, scount.get<uint32_t>(), soff.get<uint32_t>(), (int)spans, s));
-        k_span_total<<<1, 1, 0, s>>>(scount.get<uint32_t>(), soff.get<uint32_t>(), spans, stot.get<uint32_t>());
+        CLE_HIP(hipcub::DeviceScan::ExclusiveSum(st.p, sb, cnt.get<uint32_t>(), off.get<uint32_t>(), (int)spans, s));
+        k_runs_summary<<<1, 1024, 0, s>>>(cnt.get<uint32_t>(), off.get<uint32_t>(), spans, bmx.get<uint32_t>(),
+                                          (int)gr, sum.get<uint32_t>());
     }
     CLE_LAUNCHED();
-    uint32_t nruns = 0;
-    CLE_HIP(hipMemcpyAsync(&nruns, stot.p, 4, hipMemcpyDeviceToHost, s));
+    uint32_t hs[3] = {0, 0, 0};
+    CLE_HIP(hipMemcpyAsync(hs, sum.p, 12, hipMemcpyDeviceToHost, s));
     CLE_HIP(hipStreamSynchronize(s));
-    const int64_t n = (int64_t)nruns;
-    const int64_t cap = n > 0 ? n : 1;
-    Scratch k0, k1, e0, e1, tmp, flags, cidx, cstart, ncell, lst, len_, acc, acc2, lab16, img;
-    CLE_TRY(k0.alloc((size_t)cap * 8, s));
-    CLE_TRY(k1.alloc((size_t)cap * 8, s));
-    CLE_TRY(e0.alloc((size_t)cap * 4, s));
-    CLE_TRY(e1.alloc((size_t)cap * 4, s));
-    if (n > 0)
-        k_runs<1><<<gr, RB_THREADS, 0, s>>>(lab, g, L, kb, nullptr, soff.get<uint32_t>(), k0.get<uint64_t>(),
-                                            e0.get<uint32_t>());
+    const int64_t n = (int64_t)hs[0];
+    {
+        uint32_t want = 16;
+        while (want < hs[1] && want < 128) want <<= 1;
+        g_span_slots.store(want, std::memory_order_relaxed);
+    }
+    if (L <= 0) {
+        L = (int64_t)hs[2] + 1;
+        kb.lb = bits_for((uint64_t)(L - 1));
+    }
+    if (out_L) *out_L = L;
+    if (tcols < L - 1)
+        return fail(CLE_E_SHAPE, "statistics table holds %lld columns, %lld labels need %lld", (long long)tcols,
+                    (long long)(L - 1), (long long)(L - 1));
+    Scratch k0c, e0c, k1, e1, tmp, flags, cidx, cstart, ncell, lst, len_, acc, acc2, lab16, img;
+    const int64_t rcap = n > 0 ? n : 1;
+    CLE_TRY(k0c.alloc((size_t)rcap * 8, s));
+    CLE_TRY(e0c.alloc((size_t)rcap * 4, s));
+    CLE_TRY(k1.alloc((size_t)rcap * 8, s));
+    CLE_TRY(e1.alloc((size_t)rcap * 4, s));
+    uint64_t* k0 = k0c.get<uint64_t>();
+    uint32_t* e0 = e0c.get<uint32_t>();
+    if (n > 0) {
+        if (hs[1] > scap) {
+            runs(off.get<uint32_t>(), k0, e0);  // counts are those of the first pass
+        } else {
+            unsigned gc = grid_for((size_t)spans, RB_THREADS / 64);
+            if (gc > 32768) gc = 32768;
+            k_runs_compact<<<gc, RB_THREADS, 0, s>>>(k0s.get<uint64_t>(), e0s.get<uint32_t>(), cnt.get<uint32_t>(),
+                                                     off.get<uint32_t>(), spans, scap, k0, e0);
+        }
+    }
     CLE_LAUNCHED();
     // (2) sort the runs by (label, y, z, x0)
     size_t tbytes = 0;
-    CLE_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, k0.get<uint64_t>(), k1.get<uint64_t>(),
-                                               e0.get<uint32_t>(), e1.get<uint32_t>(), (int)cap, 0, kb.total(), s));
+    CLE_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, k0, k1.get<uint64_t>(), e0, e1.get<uint32_t>(),
+                                               (int)rcap, 0, kb.total(), s));
     CLE_TRY(tmp.alloc(tbytes, s));
     if (n > 0)
-        CLE_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tbytes, k0.get<uint64_t>(), k1.get<uint64_t>(),
-                                                   e0.get<uint32_t>(), e1.get<uint32_t>(), (int)n, 0, kb.total(), s));
+        CLE_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tbytes, k0, k1.get<uint64_t>(), e0, e1.get<uint32_t>(),
+                                                   (int)n, 0, kb.total(), s));
     const uint64_t* keys = k1.get<uint64_t>();
     const uint32_t* ends = e1.get<uint32_t>();
     // cells and label ranges
-    CLE_TRY(flags.alloc((size_t)cap * 4, s));
-    CLE_TRY(cidx.alloc((size_t)cap * 4, s));
-    CLE_TRY(cstart.alloc((size_t)(cap + 1) * 4, s));
+    CLE_TRY(flags.alloc((size_t)rcap * 4, s));
+    CLE_TRY(cidx.alloc((size_t)rcap * 4, s));
+    CLE_TRY(cstart.alloc((size_t)(rcap + 1) * 4, s));
     CLE_TRY(ncell.alloc(4, s));
     CLE_TRY(fill_u32(ncell.get<uint32_t>(), 0u, 1, s));
     CLE_TRY(lst.alloc((size_t)L * 4, s));
     CLE_TRY(len_.alloc((size_t)L * 4, s));
     CLE_TRY(fill_u32(lst.get<uint32_t>(), 0u, (size_t)L, s));
     CLE_TRY(fill_u32(len_.get<uint32_t>(), 0u, (size_t)L, s));
-    const unsigned gn = grid_for((size_t)cap, 256) > 65536 ? 65536 : grid_for((size_t)cap, 256);
+    const unsigned gn = grid_for((size_t)rcap, 256) > 65536 ? 65536 : grid_for((size_t)rcap, 256);
     if (n > 0) {
         k_cell_flags<<<gn, 256, 0, s>>>(keys, n, kb.zb + kb.xb, flags.get<uint32_t>());
         size_t sbytes = 0;
@@ -419,28 +644,34 @@ static int stats_run(DBuf inten, LAB lab, const Geom& g, int64_t L, float* table
         CLE_HIP(hipcub::DeviceScan::ExclusiveSum(stmp.p, sbytes, flags.get<uint32_t>(), cidx.get<uint32_t>(), (int)n, s));
         k_cell_scatter<<<gn, 256, 0, s>>>(flags.get<uint32_t>(), cidx.get<uint32_t>(), n, cstart.get<uint32_t>(),
                                           ncell.get<uint32_t>());
-        k_label_ranges<<<gn, 256, 0, s>>>(keys, cstart.get<uint32_t>(), ncell.get<uint32_t>(), cap, kb,
+        k_label_ranges<<<gn, 256, 0, s>>>(keys, cstart.get<uint32_t>(), ncell.get<uint32_t>(), rcap, kb,
                                           lst.get<uint32_t>(), len_.get<uint32_t>());
     }
     CLE_LAUNCHED();
     // (3) + (4): cells, then labels, twice
-    CLE_TRY(acc.alloc((size_t)cap * 16 * 4, s));
-    CLE_TRY(acc2.alloc((size_t)cap * 6 * 4, s));
+    CLE_TRY(acc.alloc((size_t)rcap * 16 * 4, s));
+    CLE_TRY(acc2.alloc((size_t)rcap * 6 * 4, s));
     CLE_TRY(lab16.alloc((size_t)L * 16 * 4, s));
     CLE_TRY(img.alloc((size_t)L * 8 * 4, s));
     const CellGeom cg{kb, g.H, g.W};
-    if (n > 0)
-        k_cells_pass1<<<gn, 256, 0, s>>>(inten, cg, keys, ends, cstart.get<uint32_t>(), ncell.get<uint32_t>(), cap,
-                                         acc.get<float>());
-    k_labels_pass1<<<grid_for((size_t)L, 256), 256, 0, s>>>(acc.get<float>(), cap, lst.get<uint32_t>(),
-                                                            len_.get<uint32_t>(), L, lab16.get<float>(), img.get<float>());
-    if (n > 0)
-        k_cells_pass2<<<gn, 256, 0, s>>>(inten, cg, keys, ends, cstart.get<uint32_t>(), ncell.get<uint32_t>(), cap,
-                                         img.get<float>(), L, acc2.get<float>());
+    auto cells = [&](auto in) {
+        if (n > 0)
+            k_cells_pass1<<<gn, 256, 0, s>>>(in, cg, keys, ends, cstart.get<uint32_t>(), ncell.get<uint32_t>(), rcap,
+                                             acc.get<float>());
+        k_labels_pass1<<<grid_for((size_t)L, 256), 256, 0, s>>>(acc.get<float>(), rcap, lst.get<uint32_t>(),
+                                                                len_.get<uint32_t>(), L, lab16.get<float>(),
+                                                                img.get<float>());
+        if (n > 0)
+            k_cells_pass2<<<gn, 256, 0, s>>>(in, cg, keys, ends, cstart.get<uint32_t>(), ncell.get<uint32_t>(), rcap,
+                                             img.get<float>(), L, acc2.get<float>());
+    };
+    if (inten.dt == CLE_U16) cells(InU16{(const uint16_t*)inten.p});
+    else if (inten.dt == CLE_F32) cells(InF32{(const float*)inten.p});
+    else cells(InAny{inten});
     if (L > 1)
-        k_labels_table<<<grid_for((size_t)(L - 1), 256), 256, 0, s>>>(acc2.get<float>(), cap, lst.get<uint32_t>(),
+        k_labels_table<<<grid_for((size_t)(L - 1), 256), 256, 0, s>>>(acc2.get<float>(), rcap, lst.get<uint32_t>(),
                                                                       len_.get<uint32_t>(), lab16.get<float>(),
-                                                                      img.get<float>(), L, table);
+                                                                      img.get<float>(), L, tcols, table);
     CLE_LAUNCHED();
     return CLE_OK;
 }
@@ -452,27 +683,31 @@ using namespace cle;
 extern "C" {
 
 int cle_statistics_of_labelled_pixels(const cle_buf* intensity, const cle_buf* labels, int64_t num_labels,
-                                      const cle_buf* out_table, void* stream) {
+                                      const cle_buf* out_table, int64_t* out_num_labels, void* stream) {
     Geom gi, gl, go;
     CLE_TRY(check_buf(intensity, "intensity_image", &gi));
     CLE_TRY(check_buf(labels, "label_image", &gl));
     CLE_TRY(check_same_shape(gi, gl, "intensity_image", "label_image"));
     CLE_TRY(check_buf(out_table, "table", &go));
     if (out_table->dtype != CLE_F32) return fail(CLE_E_TYPE, "statistics table must be float32");
-    if (num_labels < 1) return fail(CLE_E_VALUE, "num_labels must be >= 1");
-    const int64_t L = num_labels;
-    if (go.n() < (int64_t)CLE_STAT_COLUMNS * (L - 1))
-        return fail(CLE_E_SHAPE, "statistics table holds %lld < %lld values", (long long)go.n(),
-                    (long long)CLE_STAT_COLUMNS * (L - 1));
+    if (out_table->ndim != 2 || out_table->shape[0] != CLE_STAT_COLUMNS)
+        return fail(CLE_E_SHAPE, "statistics table must have shape (%d, columns)", CLE_STAT_COLUMNS);
+    if (num_labels < 0) return fail(CLE_E_VALUE, "num_labels must be >= 0");
+    const int64_t tcols = out_table->shape[1];
+    if (num_labels > 0 && tcols < num_labels - 1)
+        return fail(CLE_E_SHAPE, "statistics table holds %lld columns < %lld labels", (long long)tcols,
+                    (long long)(num_labels - 1));
     hipStream_t s = (hipStream_t)stream;
     const DBuf in = dbuf(intensity);
     float* table = (float*)out_table->data;
+    int64_t* oL = out_num_labels;
+    const int64_t L = num_labels;
     switch (labels->dtype) {
-        case CLE_I32: return stats_run(in, LabT<int32_t>{(const int32_t*)labels->data}, gi, L, table, s);
-        case CLE_U32: return stats_run(in, LabT<uint32_t>{(const uint32_t*)labels->data}, gi, L, table, s);
-        case CLE_F32: return stats_run(in, LabT<float>{(const float*)labels->data}, gi, L, table, s);
-        case CLE_U16: return stats_run(in, LabT<uint16_t>{(const uint16_t*)labels->data}, gi, L, table, s);
-        default: return stats_run(in, LabAny{dbuf(labels)}, gi, L, table, s);
+        case CLE_I32: return stats_run(in, LabT<int32_t>{(const int32_t*)labels->data}, gi, L, table, tcols, oL, s);
+        case CLE_U32: return stats_run(in, LabT<uint32_t>{(const uint32_t*)labels->data}, gi, L, table, tcols, oL, s);
+        case CLE_F32: return stats_run(in, LabT<float>{(const float*)labels->data}, gi, L, table, tcols, oL, s);
+        case CLE_U16: return stats_run(in, LabT<uint16_t>{(const uint16_t*)labels->data}, gi, L, table, tcols, oL, s);
+        default: return stats_run(in, LabAny{dbuf(labels)}, gi, L, table, tcols, oL, s);
     }
 }
 

@@ -59,8 +59,17 @@ def test_validation_errors_map_to_reference_exceptions():
     with pytest.raises(TypeError):
         tab = _buf(_lib.CLE_U32, (36, 4))
         _lib.call("cle_statistics_of_labelled_pixels", _buf(_lib.CLE_F32, (3, 3)), _buf(_lib.CLE_U32, (3, 3)), 5,
-                  tab, None)
+                  tab, None, None)
     assert "differ in shape" in _lib.last_error() or "float32" in _lib.last_error()
+    with pytest.raises(ValueError):  # table rows must be the 36 STATISTICS_ENTRY columns
+        _lib.call("cle_statistics_of_labelled_pixels", _buf(_lib.CLE_F32, (3, 3)), _buf(_lib.CLE_U32, (3, 3)), 5,
+                  _buf(_lib.CLE_F32, (35, 4)), None, None)
+    with pytest.raises(ValueError):  # explicit label count larger than the table
+        _lib.call("cle_statistics_of_labelled_pixels", _buf(_lib.CLE_F32, (3, 3)), _buf(_lib.CLE_U32, (3, 3)), 9,
+                  _buf(_lib.CLE_F32, (36, 4)), None, None)
+    with pytest.raises(ValueError):
+        _lib.call("cle_statistics_of_labelled_pixels", _buf(_lib.CLE_F32, (3, 3)), _buf(_lib.CLE_U32, (3, 3)), -1,
+                  _buf(_lib.CLE_F32, (36, 4)), ctypes.byref(ctypes.c_int64(0)), None)
 
 
 def test_api_surface_matches_reference_names():

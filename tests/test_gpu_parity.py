@@ -519,3 +519,49 @@ def test_ccl_large_vs_composite_rule(cle):
     got = cle.pull(cle.connected_components_labeling_box(b))
     assert got.max() == n
     assert np.array_equal(got, want)
+
+
+def test_statistics_label_count_derived_and_table_growth(cle):
+    """num_labels derived in the run pass equals max(labels) + 1; 5000 labels exceed
+    the default 4096-column table, which is grown and the call repeated; the explicit
+    count gives the same table."""
+    rng = np.random.default_rng(40)
+    lab = np.zeros((100, 120), np.uint32)
+    idx = rng.choice(lab.size, 5000, replace=False)
+    lab.flat[idx] = np.arange(1, 5001, dtype=np.uint32)
+    img = (rng.random(lab.shape) * 100).astype(np.float32)
+    got = cle.statistics_of_labelled_pixels(img, lab)
+    assert len(got["label"]) == 5000
+    _assert_stats_identical(got, O.statistics_of_labelled_pixels(img, lab))
+    t_explicit, n = cle.statistics_table(cle.push(img), cle.push(lab), 5001)
+    t_derived, n2 = cle.statistics_table(cle.push(img), cle.push(lab))
+    assert n == n2 == 5000
+    assert np.array_equal(t_explicit.tensor.cpu().numpy(), t_derived.tensor[:, :n].cpu().numpy(), equal_nan=True)
+    # no labels at all
+    empty = cle.statistics_of_labelled_pixels(img, np.zeros_like(lab))
+    assert all(len(v) == 0 for v in empty.values())
+
+
+def test_statistics_run_buffer_overflow_rerun(cle):
+    """A label image whose every pixel is its own run (16.7M runs) overflows the run
+    buffer sized from the previous call; the pass is rerun at the exact size and the
+    result equals the next call's (sized from the hint) and numpy's counts / bounds."""
+    H, W = 4096, 4096
+    small = np.ones((8, 8), np.uint32)
+    cle.statistics_of_labelled_pixels(np.ones((8, 8), np.float32), small)  # small run-count hint
+    x = np.arange(W, dtype=np.uint32)
+    lab = np.broadcast_to((x % 7 + 1)[None, :], (H, W)).copy()
+    lab[::2] = np.roll(lab[::2], 3, axis=1)
+    img = (np.random.default_rng(41).random((H, W)) * 50).astype(np.float32)
+    g_img, g_lab = cle.push(img), cle.asarray(lab)
+    first = cle.statistics_of_labelled_pixels(g_img, g_lab)
+    second = cle.statistics_of_labelled_pixels(g_img, g_lab)
+    _assert_stats_identical(first, second)
+    counts = np.bincount(lab.ravel(), minlength=8)[1:]
+    assert np.array_equal(first["area"], counts.astype(np.float32))
+    for k in range(7):
+        ys, xs = np.nonzero(lab == k + 1)
+        assert first["bbox_min_x"][k] == xs.min() and first["bbox_max_x"][k] == xs.max()
+        assert first["bbox_min_y"][k] == ys.min() and first["bbox_max_y"][k] == ys.max()
+        s = img[lab == k + 1].astype(np.float64).sum()
+        assert abs(first["sum_intensity"][k] - s) <= 1e-3 * s
