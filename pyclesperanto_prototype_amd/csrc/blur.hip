@@ -286,11 +286,17 @@ template <int C, bool Z3, typename LD>
 __global__ __launch_bounds__(FB_THREADS) void k_blur_iso(LD in, float* __restrict__ out, Geom g, int zc, int ntx,
                                                         int nty, const IsoW<C> w) {
     constexpr int AW = FB_TX + 2 * C, AH = FB_TY + 2 * C, NA = AW * AH;
+    constexpr int AWP = (AW + 3) / 4 * 4;   // row pitch: rows start 16-byte aligned
     constexpr int PF = (NA + FB_THREADS - 1) / FB_THREADS;
     constexpr int CZ = Z3 ? C : 0;
-    constexpr int XO = (AH * FB_TX + FB_THREADS - 1) / FB_THREADS;  // x-pass outputs per thread
-    __shared__ float A[NA];
-    __shared__ float B[AH * FB_TX];
+    constexpr int U = C <= 10 ? 4 : (C <= 14 ? 2 : 1);  // slices per ring shift (register budget)
+    constexpr int RL = 2 * CZ + U;          // ring length
+    constexpr int XQ = FB_TX / 4;           // x-pass quads per row
+    constexpr int XT = AH * XQ;             // x-pass tasks (4 outputs each)
+    constexpr int XR = (XT + FB_THREADS - 1) / FB_THREADS;
+    constexpr int NW4 = (2 * C + 4 + 3) / 4;  // float4 reads per x-pass window
+    __shared__ __align__(16) float A[AH * AWP];
+    __shared__ __align__(16) float B[AH * FB_TX];
     const int64_t nb = (int64_t)gridDim.x;
     const int64_t bid = xcd_tile(blockIdx.x, nb);
     const int64_t tx = bid % ntx, rest = bid / ntx;
@@ -301,7 +307,7 @@ __global__ __launch_bounds__(FB_THREADS) void k_blur_iso(LD in, float* __restric
     const int row = threadIdx.x / (FB_TX / 2);
     const int col = (threadIdx.x % (FB_TX / 2)) * 2;
     const int Hm = (int)g.H - 1, Wm = (int)g.W - 1, Wi = (int)g.W;
-    int off[PF];
+    int off[PF], lds[PF];
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
         const int idx = threadIdx.x + k * FB_THREADS;
@@ -310,11 +316,12 @@ __global__ __launch_bounds__(FB_THREADS) void k_blur_iso(LD in, float* __restric
         gy = gy < 0 ? 0 : (gy > Hm ? Hm : gy);
         gx = gx < 0 ? 0 : (gx > Wm ? Wm : gx);
         off[k] = gy * Wi + gx;
+        lds[k] = ay * AWP + ax;
     }
     float pa[PF], pb[PF];
-    float r0[2 * CZ + 1], r1[2 * CZ + 1];
+    float r0[RL], r1[RL];
 #pragma unroll
-    for (int j = 0; j < 2 * CZ + 1; ++j) { r0[j] = 0.f; r1[j] = 0.f; }
+    for (int j = 0; j < RL; ++j) { r0[j] = 0.f; r1[j] = 0.f; }
     const int64_t zlast = ze + CZ;  // one past the last input slice
 
     auto load = [&](float (&pf)[PF], int64_t zi) {
@@ -325,43 +332,81 @@ __global__ __launch_bounds__(FB_THREADS) void k_blur_iso(LD in, float* __restric
         for (int k = 0; k < PF; ++k)
             if (threadIdx.x + k * FB_THREADS < NA) pf[k] = in(base + off[k]);
     };
-    auto step = [&](float (&pf)[PF], int64_t zi) {
+    // one input slice: stage it, x pass (LDS -> LDS, four outputs per task from one
+    // window of 2C + 4 values), y pass (LDS -> registers, the thread's two columns),
+    // ring slot u; emits the output slice zi - CZ once the ring covers it
+    auto step = [&](float (&pf)[PF], int64_t zi, int u) {
 #pragma unroll
         for (int k = 0; k < PF; ++k)
-            if (threadIdx.x + k * FB_THREADS < NA) A[threadIdx.x + k * FB_THREADS] = pf[k];
+            if (threadIdx.x + k * FB_THREADS < NA) A[lds[k]] = pf[k];
         __syncthreads();
         load(pf, zi + 2);
+        if constexpr (C > 12) {
+            // wide kernels: one output per task, taps read from LDS (keeps the window
+            // out of registers)
+            constexpr int XO = (AH * FB_TX + FB_THREADS - 1) / FB_THREADS;
 #pragma unroll
-        for (int k = 0; k < XO; ++k) {
-            const int idx = threadIdx.x + k * FB_THREADS;
-            if (idx < AH * FB_TX) {
-                const int ay = idx / FB_TX, bx = idx - ay * FB_TX;
-                const float* a = A + ay * AW + bx + C;
-                float acc = w.w[0] * a[0];
+            for (int k = 0; k < XO; ++k) {
+                const int idx = threadIdx.x + k * FB_THREADS;
+                if (idx < AH * FB_TX) {
+                    const int ay = idx / FB_TX, bx = idx - ay * FB_TX;
+                    const float* a = A + ay * AWP + bx + C;
+                    float acc = w.w[0] * a[0];
 #pragma unroll
-                for (int v = 1; v <= C; ++v) acc = fmaf(w.w[v], a[-v] + a[v], acc);
-                B[idx] = acc;
+                    for (int v = 1; v <= C; ++v) acc = fmaf(w.w[v], a[-v] + a[v], acc);
+                    B[idx] = acc;
+                }
             }
+        } else {
+#pragma unroll
+        for (int k = 0; k < XR; ++k) {
+            const int t = threadIdx.x + k * FB_THREADS;
+            if (t < XT) {
+                const int ay = t / XQ, q = t - ay * XQ;
+                const float4* a4 = (const float4*)(A + ay * AWP + 4 * q);
+                float a[4 * NW4];
+#pragma unroll
+                for (int m = 0; m < NW4; ++m) {
+                    const float4 f = a4[m];
+                    a[4 * m] = f.x; a[4 * m + 1] = f.y; a[4 * m + 2] = f.z; a[4 * m + 3] = f.w;
+                }
+                float o[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float acc = w.w[0] * a[C + j];
+#pragma unroll
+                    for (int v = 1; v <= C; ++v) acc = fmaf(w.w[v], a[C + j - v] + a[C + j + v], acc);
+                    o[j] = acc;
+                }
+                *(float4*)(B + ay * FB_TX + 4 * q) = make_float4(o[0], o[1], o[2], o[3]);
+            }
+        }
         }
         __syncthreads();
         const float* b = B + (row + C) * FB_TX + col;
-        float v0 = w.w[0] * b[0], v1 = w.w[0] * b[1];
+        const float2 bc = *(const float2*)b;
+        float v0 = w.w[0] * bc.x, v1 = w.w[0] * bc.y;
 #pragma unroll
         for (int v = 1; v <= C; ++v) {
-            v0 = fmaf(w.w[v], b[-v * FB_TX] + b[v * FB_TX], v0);
-            v1 = fmaf(w.w[v], b[-v * FB_TX + 1] + b[v * FB_TX + 1], v1);
+            const float2 lo = *(const float2*)(b - v * FB_TX), hi = *(const float2*)(b + v * FB_TX);
+            v0 = fmaf(w.w[v], lo.x + hi.x, v0);
+            v1 = fmaf(w.w[v], lo.y + hi.y, v1);
         }
-#pragma unroll
-        for (int j = 0; j < 2 * CZ; ++j) { r0[j] = r0[j + 1]; r1[j] = r1[j + 1]; }
-        r0[2 * CZ] = v0;
-        r1[2 * CZ] = v1;
+        r0[2 * CZ + u] = v0;
+        r1[2 * CZ + u] = v1;
         if (zi >= zb + CZ) {
-            float o0 = w.w[0] * r0[CZ], o1 = w.w[0] * r1[CZ];
-            if (CZ == 0) { o0 = r0[0]; o1 = r1[0]; }
+            float o0, o1;
+            if (CZ == 0) {
+                o0 = r0[u];
+                o1 = r1[u];
+            } else {
+                o0 = w.w[0] * r0[u + CZ];
+                o1 = w.w[0] * r1[u + CZ];
 #pragma unroll
-            for (int k = 1; k <= CZ; ++k) {
-                o0 = fmaf(w.w[k], r0[CZ - k] + r0[CZ + k], o0);
-                o1 = fmaf(w.w[k], r1[CZ - k] + r1[CZ + k], o1);
+                for (int k = 1; k <= CZ; ++k) {
+                    o0 = fmaf(w.w[k], r0[u + CZ - k] + r0[u + CZ + k], o0);
+                    o1 = fmaf(w.w[k], r1[u + CZ - k] + r1[u + CZ + k], o1);
+                }
             }
             const int64_t zo = zi - CZ, y = y0 + row, x = x0 + col;
             if (y < g.H) {
@@ -374,9 +419,20 @@ __global__ __launch_bounds__(FB_THREADS) void k_blur_iso(LD in, float* __restric
 
     load(pa, zb - CZ);
     load(pb, zb - CZ + 1);
-    for (int64_t zi = zb - CZ; zi < zlast; zi += 2) {
-        step(pa, zi);
-        if (zi + 1 < zlast) step(pb, zi + 1);
+    // the prefetch buffers alternate pa, pb, pa, ... across consecutive slices
+    int64_t zi = zb - CZ;
+    bool even = true;
+    for (; zi < zlast; zi += U) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (zi + u < zlast) {
+                if (even == ((u & 1) == 0)) step(pa, zi + u, u);
+                else step(pb, zi + u, u);
+            }
+        }
+        if (U & 1) even = !even;
+#pragma unroll
+        for (int j = 0; j < 2 * CZ; ++j) { r0[j] = r0[j + U]; r1[j] = r1[j + U]; }
     }
 }
 
@@ -474,7 +530,7 @@ static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hip
     k_blur_iso<C, Z3, LD><<<(unsigned)(nxy * nzc), FB_THREADS, 0, s>>>(in, out, g, zc, ntx, nty, w);
 }
 
-constexpr int ISO_MAXC = 20;
+constexpr int ISO_MAXC = 16;  // wider kernels would spill the ring; they take k_blur_fused
 
 template <bool Z3, typename LD>
 static int launch_iso(LD in, float* out, const Geom& g, int c, const float* wts, hipStream_t s) {
@@ -482,7 +538,6 @@ static int launch_iso(LD in, float* out, const Geom& g, int c, const float* wts,
 #define ISO_CASE(n) case n: launch_iso_c<n, Z3>(in, out, g, wts, s); break;
         ISO_CASE(1) ISO_CASE(2) ISO_CASE(3) ISO_CASE(4) ISO_CASE(5) ISO_CASE(6) ISO_CASE(7) ISO_CASE(8)
         ISO_CASE(9) ISO_CASE(10) ISO_CASE(11) ISO_CASE(12) ISO_CASE(13) ISO_CASE(14) ISO_CASE(15) ISO_CASE(16)
-        ISO_CASE(17) ISO_CASE(18) ISO_CASE(19) ISO_CASE(20)
 #undef ISO_CASE
         default: return fail(CLE_E_INTERNAL, "isotropic blur: half-width %d out of range", c);
     }

@@ -80,11 +80,33 @@ template <> struct Vec4<uint32_t> { using V = uint4; };
 template <> struct Vec4<float> { using V = float4; };
 
 template <bool VEC, typename LAB>
+__device__ __forceinline__ void runs_load(const LAB& lab, int64_t base, int n, int lane, int (&v)[RB_SPAN / 64]) {
+    constexpr int E = VEC ? 4 : 1;
+    constexpr int K = RB_SPAN / (64 * E);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i0 = 64 * E * k + E * lane;
+        if constexpr (VEC) {
+            if (i0 < n) {  // n is a multiple of 4: all four or none
+                const auto q = *(const typename Vec4<typename LAB::type>::V*)(lab.p + base + i0);
+                v[4 * k + 0] = (int)(float)q.x; v[4 * k + 1] = (int)(float)q.y;
+                v[4 * k + 2] = (int)(float)q.z; v[4 * k + 3] = (int)(float)q.w;
+            } else {
+                v[4 * k + 0] = v[4 * k + 1] = v[4 * k + 2] = v[4 * k + 3] = 0;
+            }
+        } else {
+            v[k] = i0 < n ? lab(base + i0) : 0;
+        }
+    }
+}
+
+template <bool VEC, typename LAB>
 __global__ __launch_bounds__(RB_THREADS) void k_runs(LAB lab, Geom g, int64_t lcap, KeyBits kb, uint32_t scap,
                                                      const uint32_t* off, uint32_t* count, uint64_t* keys,
                                                      uint32_t* ends, uint32_t* bmax) {
     constexpr int E = VEC ? 4 : 1;         // consecutive voxels per lane
     constexpr int K = RB_SPAN / (64 * E);  // blocks of 64 * E voxels
+    constexpr int NV = RB_SPAN / 64;       // voxels per lane
     const int64_t spans = (g.W + RB_SPAN - 1) / RB_SPAN;
     const int64_t total = g.D * g.H * spans;
     const int lane = threadIdx.x & 63;
@@ -93,103 +115,90 @@ __global__ __launch_bounds__(RB_THREADS) void k_runs(LAB lab, Geom g, int64_t lc
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     int lmax = 0;
     uint32_t cmax = 0;
+    // the next span's labels are loaded while this span is processed
+    int nv[NV];
+    auto span_of = [&](int64_t sp, int64_t& row, int64_t& sx0, int& n) {
+        row = sp / spans;
+        sx0 = (sp - row * spans) * RB_SPAN;
+        n = (int)((sx0 + RB_SPAN < g.W ? sx0 + RB_SPAN : g.W) - sx0);
+    };
+    if (wave0 < total) {
+        int64_t row, sx0;
+        int n;
+        span_of(wave0, row, sx0, n);
+        runs_load<VEC>(lab, row * g.W + sx0, n, lane, nv);
+    }
     for (int64_t sp = wave0; sp < total; sp += nwaves) {
-        const int64_t row = sp / spans;
-        const int64_t sx0 = (sp - row * spans) * RB_SPAN;
-        const int n = (int)((sx0 + RB_SPAN < g.W ? sx0 + RB_SPAN : g.W) - sx0);
-        const int64_t base = row * g.W + sx0;
-        int v[K][E];
+        int64_t row, sx0;
+        int n;
+        span_of(sp, row, sx0, n);
+        int v[NV];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int i0 = 64 * E * k + E * lane;
-            if constexpr (VEC) {
-                if (i0 < n) {  // n is a multiple of 4: all four or none
-                    const auto q = *(const typename Vec4<typename LAB::type>::V*)(lab.p + base + i0);
-                    v[k][0] = (int)(float)q.x; v[k][1] = (int)(float)q.y;
-                    v[k][2] = (int)(float)q.z; v[k][3] = (int)(float)q.w;
-                } else {
-                    v[k][0] = v[k][1] = v[k][2] = v[k][3] = 0;
-                }
-            } else {
-                v[k][0] = i0 < n ? lab(base + i0) : 0;
-            }
+        for (int k = 0; k < NV; ++k) v[k] = nv[k];
+        if (sp + nwaves < total) {
+            int64_t row2, sx2;
+            int n2;
+            span_of(sp + nwaves, row2, sx2, n2);
+            runs_load<VEC>(lab, row2 * g.W + sx2, n2, lane, nv);
         }
         bool any = false;
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-#pragma unroll
-            for (int j = 0; j < E; ++j) {
-                const bool valid = v[k][j] > 0 && v[k][j] < lcap;
-                any |= valid;
-                lmax = valid && v[k][j] > lmax ? v[k][j] : lmax;
-            }
+        for (int k = 0; k < NV; ++k) {
+            const bool valid = v[k] > 0 && v[k] < lcap;
+            any |= valid;
+            lmax = valid && v[k] > lmax ? v[k] : lmax;
+        }
         if (!__any(any)) {
             if (lane == 0) count[sp] = 0;
             continue;
         }
-        uint64_t ms[K][E], me[K][E];
-        uint32_t nrun = 0;
+        // both modes know where the span's runs go before they are found
+        const uint64_t slot0 = off ? (uint64_t)off[sp] : (uint64_t)sp * scap;
+        const uint32_t lim = off ? 0xFFFFFFFFu : scap;
+        const uint64_t y = (uint64_t)(row % g.H), z = (uint64_t)(row / g.H);
+        uint32_t ns = 0, ne = 0;
         int prev_last = 0;  // last voxel of the previous block of 64 * E
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            int left = __shfl_up(v[k][E - 1], 1);
+            int left = __shfl_up(v[E * k + E - 1], 1);
             if (lane == 0) left = prev_last;
-            int right = __shfl_down(v[k][0], 1);
-            const int next_first = __shfl(v[k + 1 < K ? k + 1 : k][0], 0);
+            int right = __shfl_down(v[E * k], 1);
+            const int next_first = __shfl(v[k + 1 < K ? E * (k + 1) : E * k], 0);
             if (lane == 63) right = k + 1 < K ? next_first : 0;
-            prev_last = __shfl(v[k][E - 1], 63);
-#pragma unroll
-            for (int j = 0; j < E; ++j) {
-                const int i = 64 * E * k + E * lane + j;
-                const int l = v[k][j];
-                const int lf = j == 0 ? left : v[k][j - 1];
-                const int rt = j == E - 1 ? right : v[k][j + 1];
-                const bool valid = i < n && l > 0 && l < lcap;
-                ms[k][j] = __ballot(valid && (i == 0 || lf != l));
-                me[k][j] = __ballot(valid && (i == n - 1 || rt != l));
-                nrun += (uint32_t)__popcll(ms[k][j]);
-            }
-        }
-        cmax = nrun > cmax ? nrun : cmax;
-        if (lane == 0) count[sp] = nrun;
-        uint64_t slot0;
-        uint32_t lim;
-        if (off) {
-            slot0 = off[sp];
-            lim = nrun;
-        } else {
-            slot0 = (uint64_t)sp * scap;
-            lim = scap;
-        }
-        const uint64_t y = (uint64_t)(row % g.H), z = (uint64_t)(row / g.H);
-        uint32_t ns = 0, ne = 0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            // runs of this block before this lane (all of its E voxels), then inside it
+            prev_last = __shfl(v[E * k + E - 1], 63);
+            bool st[E], en[E];
             uint32_t ps = ns, pe = ne;
 #pragma unroll
             for (int j = 0; j < E; ++j) {
-                ps += (uint32_t)__popcll(ms[k][j] & below);
-                pe += (uint32_t)__popcll(me[k][j] & below);
+                const int i = 64 * E * k + E * lane + j;
+                const int l = v[E * k + j];
+                const int lf = j == 0 ? left : v[E * k + j - 1];
+                const int rt = j == E - 1 ? right : v[E * k + j + 1];
+                const bool valid = i < n && l > 0 && l < lcap;
+                st[j] = valid && (i == 0 || lf != l);
+                en[j] = valid && (i == n - 1 || rt != l);
+                const uint64_t ms = __ballot(st[j]), me = __ballot(en[j]);
+                ps += (uint32_t)__popcll(ms & below);
+                pe += (uint32_t)__popcll(me & below);
+                ns += (uint32_t)__popcll(ms);
+                ne += (uint32_t)__popcll(me);
             }
+            // ps / pe: runs of this block in earlier lanes; then this lane's own, in order
 #pragma unroll
             for (int j = 0; j < E; ++j) {
                 const int i = 64 * E * k + E * lane + j;
-                if ((ms[k][j] >> lane) & 1) {
-                    if (ps < lim) keys[slot0 + ps] = run_key(kb, (uint64_t)v[k][j], y, z, (uint64_t)(sx0 + i));
+                if (st[j]) {
+                    if (ps < lim) keys[slot0 + ps] = run_key(kb, (uint64_t)v[E * k + j], y, z, (uint64_t)(sx0 + i));
                     ++ps;
                 }
-                if ((me[k][j] >> lane) & 1) {
+                if (en[j]) {
                     if (pe < lim) ends[slot0 + pe] = (uint32_t)(sx0 + i + 1);
                     ++pe;
                 }
             }
-#pragma unroll
-            for (int j = 0; j < E; ++j) {
-                ns += (uint32_t)__popcll(ms[k][j]);
-                ne += (uint32_t)__popcll(me[k][j]);
-            }
         }
+        cmax = ns > cmax ? ns : cmax;
+        if (lane == 0) count[sp] = ns;
     }
     // block maxima of (span count, label), one store per block
     __shared__ uint32_t red[2][RB_THREADS / 64];
@@ -614,14 +623,16 @@ static int stats_run(DBuf inten, LAB lab, const Geom& g, int64_t L, float* table
         }
     }
     CLE_LAUNCHED();
-    // (2) sort the runs by (label, y, z, x0)
+    // (2) sort the runs by (label, y, z, x0): they are in raster (z, y, x0) order, so a
+    //     stable radix sort on the (label, y) bits alone is enough
+    const int bit0 = kb.zb + kb.xb;
     size_t tbytes = 0;
     CLE_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, k0, k1.get<uint64_t>(), e0, e1.get<uint32_t>(),
-                                               (int)rcap, 0, kb.total(), s));
+                                               (int)rcap, bit0, kb.total(), s));
     CLE_TRY(tmp.alloc(tbytes, s));
     if (n > 0)
         CLE_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tbytes, k0, k1.get<uint64_t>(), e0, e1.get<uint32_t>(),
-                                                   (int)n, 0, kb.total(), s));
+                                                   (int)n, bit0, kb.total(), s));
     const uint64_t* keys = k1.get<uint64_t>();
     const uint32_t* ends = e1.get<uint32_t>();
     // cells and label ranges
