@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-slices", type=int, default=256, help="depth of the slab timed on the CPU")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 1 GiB float32 cle_copy first (known bytes, to scale PMC byte counters)")
     return ap.parse_args()
 
 
@@ -71,6 +73,13 @@ def main():
     blurred = cle.create(SHAPE, np.float32)
     binary = cle.create(SHAPE, np.uint8)
     labels = cle.create(SHAPE, np.int32)
+    if args.copy_calibration:
+        # a plain copy of known size: reads 1 GiB and writes 1 GiB
+        cle.set(blurred, 1.0)
+        cal = cle.create(SHAPE, np.float32)
+        cle.copy(blurred, cal)
+        torch.cuda.synchronize()
+        del cal
 
     stages = ["gaussian_blur", "threshold_otsu", "connected_components_labeling_box", "statistics_of_labelled_pixels"]
 

@@ -173,16 +173,6 @@ struct FusedW {
     float wx[FB_MAXC + 1], wy[FB_MAXC + 1], wz[FB_MAXC + 1];
 };
 
-// consecutive tiles (sharing halos) go to the same XCD: hardware dispatches block b
-// to XCD b % 8, so logical tile = (b % 8) * per_xcd + b / 8
-__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
-    const int64_t per = nb / 8, rem = nb % 8;
-    const int64_t xcd = b % 8, k = b / 8;
-    // the first `rem` XCDs own one extra tile
-    const int64_t start = xcd * per + (xcd < rem ? xcd : rem);
-    return start + k;
-}
-
 template <int CZ, typename LD>
 __global__ __launch_bounds__(FB_THREADS) void k_blur_fused(LD in, float* __restrict__ out, Geom g, int cx, int cy,
                                                           int zc, int ntx, int nty, const FusedW w) {

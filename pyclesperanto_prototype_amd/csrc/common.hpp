@@ -82,6 +82,17 @@ int fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t 
 int fill_f32(float* p, float v, size_t n, hipStream_t s);
 int fill_f64(double* p, double v, size_t n, hipStream_t s);
 
+// Consecutive tiles (which share halos and cache lines) go to the same XCD: the
+// hardware hands block b to XCD b % 8, each XCD has its own L2, so block b works on
+// logical tile (b % 8) * per_xcd + b / 8.  A bijection on [0, nb).
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
+    const int64_t per = nb / 8, rem = nb % 8;
+    const int64_t xcd = b % 8, k = b / 8;
+    // the first `rem` XCDs own one extra tile
+    const int64_t start = xcd * per + (xcd < rem ? xcd : rem);
+    return start + k;
+}
+
 inline unsigned grid_for(size_t n, unsigned block) {
     size_t g = (n + block - 1) / block;
     return (unsigned)(g == 0 ? 1 : g);

@@ -128,7 +128,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_ccl_local(FG fg, Geom g, TileGri
     __shared__ uint32_t lab[N];
     __shared__ uint32_t kmin[N];
     __shared__ uint32_t cnt;
-    const int64_t tile = blockIdx.x;
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
     const int64_t x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
     const int zr = g.ndim == 3 ? 1 : 0;
@@ -213,7 +213,7 @@ template <typename TS, typename FG>
 __global__ __launch_bounds__(CT_THREADS) void k_ccl_border(FG fg, Geom g, TileGrid T, int diamond, uint32_t* L,
                                                           const uint32_t* nroots) {
     constexpr int N = TS::N, TX = TS::TX, TY = TS::TY, TZ = TS::TZ;
-    const int64_t tile = blockIdx.x;
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     if (nroots[tile] == 0) return;  // no foreground in this tile
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
     const int64_t x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_ccl_border(FG fg, Geom g, TileGr
 template <typename TS>
 __global__ __launch_bounds__(CT_THREADS) void k_ccl_resolve(uint32_t* L, uint32_t* K, const uint32_t* roots,
                                                             const uint32_t* nroots) {
-    const int64_t tile = blockIdx.x;
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t n = nroots[tile];
     for (uint32_t k = threadIdx.x; k < n; k += CT_THREADS) {
         const uint32_t lr = roots[tile * TS::ROOTS + k];
@@ -270,7 +270,7 @@ template <typename TS>
 __global__ __launch_bounds__(CT_THREADS) void k_ccl_mark_roots(const uint32_t* L, const uint32_t* K,
                                                                const uint32_t* roots, const uint32_t* nroots,
                                                                int compat, uint32_t* bm) {
-    const int64_t tile = blockIdx.x;
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t n = nroots[tile];
     for (uint32_t k = threadIdx.x; k < n; k += CT_THREADS) {
         const uint32_t lr = roots[tile * TS::ROOTS + k];
@@ -404,7 +404,7 @@ template <typename TS>
 __global__ __launch_bounds__(CT_THREADS) void k_ccl_final(const uint32_t* L, const uint32_t* K, const uint32_t* roots,
                                                           const uint32_t* nroots, int compat, const uint32_t* bm,
                                                           const uint32_t* prefix, uint32_t* F) {
-    const int64_t tile = blockIdx.x;
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t n = nroots[tile];
     for (uint32_t k = threadIdx.x; k < n; k += CT_THREADS) {
         const uint32_t lr = roots[tile * TS::ROOTS + k];
@@ -585,7 +585,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_bccl_local(FG fg, Geom g, BGrid 
     __shared__ uint32_t kmin[N];
     __shared__ uint8_t msk[N];
     __shared__ uint32_t cnt;
-    const int64_t tile = blockIdx.x;
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
     const int64_t bx0 = tx * TX, by0 = ty * TY, bz0 = tz * BT::TZ;
     if (threadIdx.x == 0) cnt = 0;
@@ -685,7 +685,7 @@ template <typename BT>
 __global__ __launch_bounds__(CT_THREADS) void k_bccl_border(BGrid B, TileGrid T, const uint8_t* M, uint32_t* L,
                                                            const uint32_t* nroots) {
     constexpr int N = BT::N, TX = BT::TX, TY = BT::TY;
-    const int64_t tile = blockIdx.x;
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     if (nroots[tile] == 0) return;  // no foreground in this tile
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
     const int64_t bx0 = tx * TX, by0 = ty * TY, bz0 = tz * BT::TZ;

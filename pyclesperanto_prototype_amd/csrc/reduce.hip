@@ -13,6 +13,8 @@
 // built from per-wavefront LDS sub-histograms (block = one row y over a chunk of z
 // slices), then each bin is summed over y in float32 in y order, which reproduces
 // the reference's float32 totals bit for bit (also above 2^24 per bin).
+#include <float.h>
+
 #include "common.hpp"
 
 namespace cle {
@@ -198,13 +200,53 @@ __device__ __forceinline__ uint32_t bin_of(float v, float mn, float range, int b
     return (uint32_t)t;
 }
 
-// One block = one row y over slices [z0, z0 + zc).  Thread t takes the float4 column
-// x = 4t (+ 4*256 k) of every slice -- coalesced across the block -- and counts runs of
-// equal bins in registers across all its voxels (blurred backgrounds fall into one or
-// two bins), flushing a run into one of HREP LDS replicas per wavefront (lane % HREP;
-// replicas padded to an odd stride so equal bins of different replicas sit in
-// different LDS banks).  Rows whose length is not a multiple of 4 or whose start is
-// not 16-byte aligned take the scalar loads.
+// Fast bin: bin_of needs floor(q + 0.5) of the correctly rounded float32 quotient q.
+// qa = a * (1 / range) is within 2^-23 relative of q (two roundings); t = qa + 0.5f
+// adds at most half an ulp of t.  For t < 2048 the total error stays below 4e-4, so
+// when t lies more than 2^-10 from an integer floor(t) = floor(q + 0.5) and the
+// division and float64 add are skipped; otherwise (near a bin boundary, NaN, huge or
+// tiny ranges) the exact bin_of decides.
+struct BinFast {
+    float mn, range, rinv, fb1;
+    int bins;
+    bool ok;  // range is a normal float with a finite reciprocal
+};
+
+__device__ __forceinline__ BinFast bin_fast_init(float mn, float range, int bins) {
+    BinFast c;
+    c.mn = mn;
+    c.range = range;
+    c.rinv = 1.0f / range;
+    c.fb1 = (float)(bins - 1);
+    c.bins = bins;
+    c.ok = range >= FLT_MIN && c.rinv <= FLT_MAX && bins + 2 <= 2048;
+    return c;
+}
+
+__device__ __forceinline__ uint32_t bin_fast(float v, const BinFast& c) {
+#pragma clang fp contract(off)
+    if (c.ok) {
+        const float a = (v - c.mn) * c.fb1;
+        const float t = a * c.rinv + 0.5f;
+        const float k = floorf(t);
+        const float d = t - k;
+        if (d > 0.0009765625f && d < 0.9990234375f && t < 2048.f)  // NaN fails these tests
+            return k <= 0.f ? 0u : (k >= c.fb1 ? (uint32_t)(c.bins - 1) : (uint32_t)k);
+        if (t >= 2048.f) return (uint32_t)(c.bins - 1);  // far above the last bin
+    }
+    return bin_of(v, c.mn, c.range, c.bins);
+}
+
+// One block = one row y over slices [z0, z0 + zc) (all slices when the image has
+// enough rows to fill the chip: then the block owns its row's counts and stores them;
+// otherwise the z chunks of a row add theirs atomically).  Thread t takes the float4
+// column x = 4t (+ 4*256 k) of every slice -- coalesced across the block -- eight slices
+// per iteration with the loads issued together, and counts runs of equal bins in
+// registers across all its voxels (blurred backgrounds fall into one or two bins),
+// flushing a run into one of HREP LDS replicas per wavefront (lane % HREP; replicas
+// padded to an odd stride so equal bins of different replicas sit in different LDS
+// banks).  Rows whose length is not a multiple of 4 or whose start is not 16-byte
+// aligned take the scalar loads.
 constexpr int HREP = 8;
 
 template <typename LD>
@@ -216,7 +258,7 @@ __global__ __launch_bounds__(HB) void k_hist_rows(LD in, const float* f4base, Ge
     const int64_t z0 = (int64_t)(blockIdx.x % zchunks) * zc;
     const float mn = mm ? mm[0] : mn_arg;
     const float mx = mm ? mm[1] : mx_arg;
-    const float range = mx - mn;
+    const BinFast bc = bin_fast_init(mn, mx - mn, bins);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nsub = HWAVES * HREP;
     for (int i = threadIdx.x; i < stride * nsub; i += HB) sh[i] = 0;
@@ -225,7 +267,7 @@ __global__ __launch_bounds__(HB) void k_hist_rows(LD in, const float* f4base, Ge
     const int64_t z1 = (z0 + zc < g.D) ? z0 + zc : g.D;
     uint32_t cur = 0xFFFFFFFFu, run = 0;
     auto add = [&](float v) {
-        const uint32_t b = bin_of(v, mn, range, bins);
+        const uint32_t b = bin_fast(v, bc);
         if (b == cur) {
             ++run;
         } else {
@@ -235,15 +277,30 @@ __global__ __launch_bounds__(HB) void k_hist_rows(LD in, const float* f4base, Ge
         }
     };
     const bool vec = f4base != nullptr && (g.W & 3) == 0;
-    for (int64_t z = z0; z < z1; ++z) {
-        const int64_t base = (z * g.H + y) * g.W;
-        if (vec) {
-            const float4* row = (const float4*)(f4base + base);
-            for (int64_t q = threadIdx.x; q < g.W / 4; q += HB) {
-                const float4 v = row[q];
+    const int64_t sstride = g.H * g.W;
+    if (vec) {
+        const int64_t nq = g.W / 4;
+        for (int64_t q = threadIdx.x; q < nq; q += HB) {
+            const float4* col = (const float4*)(f4base + (z0 * g.H + y) * g.W) + q;
+            const int64_t s4 = sstride / 4;
+            int64_t z = z0;
+            for (; z + 8 <= z1; z += 8) {
+                float4 v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = col[j * s4];
+                col += 8 * s4;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) { add(v[j].x); add(v[j].y); add(v[j].z); add(v[j].w); }
+            }
+            for (; z < z1; ++z) {
+                const float4 v = col[0];
+                col += s4;
                 add(v.x); add(v.y); add(v.z); add(v.w);
             }
-        } else {
+        }
+    } else {
+        for (int64_t z = z0; z < z1; ++z) {
+            const int64_t base = (z * g.H + y) * g.W;
             for (int64_t x = threadIdx.x; x < g.W; x += HB) add(in(base + x));
         }
     }
@@ -252,7 +309,8 @@ __global__ __launch_bounds__(HB) void k_hist_rows(LD in, const float* f4base, Ge
     for (int b = threadIdx.x; b < bins; b += HB) {
         uint32_t c = 0;
         for (int w = 0; w < nsub; ++w) c += sh[w * stride + b];
-        if (c) atomicAdd(&hist2d[y * bins + b], c);
+        if (zchunks == 1) hist2d[y * bins + b] = c;
+        else if (c) atomicAdd(&hist2d[y * bins + b], c);
     }
 }
 
@@ -299,10 +357,15 @@ template <typename LD>
 static int hist_launch(LD in, const float* f4base, const Geom& g, int bins, float mn, float mx, const float* mm,
                        uint32_t* h2, hipStream_t s) {
     if (bins <= HIST_LDS_MAX_BINS) {
-        const int64_t per_block = 16384;
-        int zc = (int)(per_block / (g.W > 0 ? g.W : 1));
-        if (zc < 1) zc = 1;
-        if (zc > g.D) zc = (int)g.D;
+        // whole rows per block when there are enough rows to fill the chip (no
+        // atomics); otherwise z chunks of about 16 K voxels
+        int zc = (int)g.D;
+        if (g.H < 1024) {
+            const int64_t per_block = 16384;
+            zc = (int)(per_block / (g.W > 0 ? g.W : 1));
+            if (zc < 1) zc = 1;
+            if (zc > g.D) zc = (int)g.D;
+        }
         const int zchunks = (int)((g.D + zc - 1) / zc);
         const size_t lds = (size_t)(bins | 1) * HWAVES * HREP * sizeof(uint32_t);
         k_hist_rows<<<(unsigned)(g.H * zchunks), HB, lds, s>>>(in, f4base, g, bins, zc, zchunks, mn, mx, mm, h2);

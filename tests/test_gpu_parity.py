@@ -565,3 +565,28 @@ def test_statistics_run_buffer_overflow_rerun(cle):
         assert first["bbox_min_y"][k] == ys.min() and first["bbox_max_y"][k] == ys.max()
         s = img[lab == k + 1].astype(np.float64).sum()
         assert abs(first["sum_intensity"][k] - s) <= 1e-3 * s
+
+
+@pytest.mark.parametrize("shape", [(8, 1100, 64), (3, 40, 256)])
+def test_histogram_bin_boundaries(cle, shape):
+    """Values on and next to the bin boundaries (bin = floor(q + 0.5) of the float32
+    quotient q = (v - min) * (bins - 1) / range): the device's reciprocal shortcut must
+    hand every near-boundary value to the exact rule.  Whole-row blocks (1100 rows)
+    and z-chunked blocks (40 rows) both run."""
+    rng = np.random.default_rng(42)
+    a = (rng.random(shape) * 255).astype(np.float32)
+    flat = a.reshape(-1)
+    k = rng.integers(0, 255, flat.size // 2)
+    eps = rng.choice(np.array([0, 1, -1, 2, -2, 3, -3], np.float32), flat.size // 2)
+    half = (k + 0.5).astype(np.float32)
+    flat[: flat.size // 2] = half + eps * np.spacing(half)  # k + 0.5 and up to 3 ulp either side
+    flat[0], flat[1] = 0.0, 255.0
+    for bins in (256, 17, 100):
+        assert np.array_equal(cle.pull(cle.histogram(a, num_bins=bins)), O.histogram(a, bins)), bins
+    # a range that is not a whole number, and one whose reciprocal is not exact
+    b = (a * np.float32(0.7313) + np.float32(3.1)).astype(np.float32)
+    assert np.array_equal(cle.pull(cle.histogram(b, num_bins=256)), O.histogram(b, 256))
+    # tiny range (subnormal): only the exact rule applies
+    c = np.zeros((4, 50, 64), np.float32)
+    c.flat[::3] = np.float32(1e-39)
+    assert np.array_equal(cle.pull(cle.histogram(c, num_bins=256)), O.histogram(c, 256))
