@@ -153,6 +153,16 @@ int cle_voronoi_otsu_labeling(const cle_buf* src, const cle_buf* dst,
                               float spot_sigma, float outline_sigma, void* stream);
 /* _tier9/_gauss_otsu_labeling.py:6-49: blur -> Otsu -> CCL box */
 int cle_gauss_otsu_labeling(const cle_buf* src, const cle_buf* dst, float outline_sigma, void* stream);
+/* Batched voronoi_otsu_labeling (_tier9/_voronoi_otsu_labeling.py:11-73 applied to
+ * every image): srcs[i] -> dsts[i] with spot_sigmas[i] / outline_sigmas[i], images of
+ * any sizes and dimensionalities in one call.  Each stage is one launch over the
+ * blocks of all images (the blur one launch per kernel half-width), the Otsu
+ * thresholds never leave the device, and the Voronoi growth of all images advances
+ * together.  Every dsts[i] equals what cle_voronoi_otsu_labeling gives for srcs[i]
+ * alone.  *n_steps (if not NULL) receives the number of growth steps the slowest
+ * image needed.  Returns after the work has finished (it synchronises the stream). */
+int cle_voronoi_otsu_labeling_batch(const cle_buf* srcs, const cle_buf* dsts, const float* spot_sigmas,
+                                    const float* outline_sigmas, int n_images, int* n_steps, void* stream);
 
 /* ---- statistics --------------------------------------------------------- */
 /* number of columns of the statistics table; column c holds STATISTICS_ENTRY c

@@ -68,6 +68,8 @@ SIGNATURES = {
     "cle_masked_voronoi_labeling": [_P, _P, _P, ctypes.POINTER(ctypes.c_int), _V],
     "cle_voronoi_otsu_labeling": [_P, _P, _F, _F, _V],
     "cle_gauss_otsu_labeling": [_P, _P, _F, _V],
+    "cle_voronoi_otsu_labeling_batch": [_P, _P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                        ctypes.c_int, ctypes.POINTER(ctypes.c_int), _V],
     "cle_statistics_of_labelled_pixels": [_P, _P, ctypes.c_int64, _P, ctypes.POINTER(ctypes.c_int64), _V],
 }
 _STR = {"cle_version", "cle_last_error"}

@@ -276,6 +276,32 @@ def voronoi_otsu_labeling(source: Image, label_image_destination: Image = None, 
     return label_image_destination
 
 
+def voronoi_otsu_labeling_batch(sources, spot_sigmas=2, outline_sigmas=2, destinations=None, return_steps=False):
+    """voronoi_otsu_labeling of many images at once (any sizes, 2D and 3D mixed).
+
+    `sources`: a list of images (numpy arrays are pushed as float32, as the reference
+    pushes them; device Arrays keep their type, so uint16 stacks are read natively);
+    `spot_sigmas` / `outline_sigmas`: one value for all or one per image.  Returns the
+    list of uint32 label images (or fills `destinations`).  Each result equals
+    voronoi_otsu_labeling of that image alone.
+    """
+    srcs = [s if isinstance(s, Array) else push(s) for s in sources]
+    n = len(srcs)
+    ss = np.broadcast_to(np.asarray(spot_sigmas, np.float32), (n,)).copy()
+    so = np.broadcast_to(np.asarray(outline_sigmas, np.float32), (n,)).copy()
+    if destinations is None:
+        destinations = [create_labels_like(s) for s in srcs]
+    if len(destinations) != n:
+        raise ValueError(f"{len(destinations)} destinations for {n} sources")
+    src_bufs = (_lib.cle_buf * max(n, 1))(*[s.buf() for s in srcs])
+    dst_bufs = (_lib.cle_buf * max(n, 1))(*[d.buf() for d in destinations])
+    steps = ctypes.c_int(0)
+    fp = ctypes.POINTER(ctypes.c_float)
+    _lib.call("cle_voronoi_otsu_labeling_batch", src_bufs, dst_bufs, ss.ctypes.data_as(fp), so.ctypes.data_as(fp),
+              n, ctypes.byref(steps), _S())
+    return (destinations, steps.value) if return_steps else destinations
+
+
 @plugin_function(categories=['label', 'in assistant'], output_creator=create_labels_like)
 def gauss_otsu_labeling(source: Image, label_image_destination: Image = None, outline_sigma: float = 2) -> Image:
     """_tier9/_gauss_otsu_labeling.py:6-49."""

@@ -272,11 +272,20 @@ struct IsoW {
     float w[C + 1];
 };
 
+template <int C>
+struct IsoLds {
+    static constexpr int AW = FB_TX + 2 * C, AH = FB_TY + 2 * C;
+    static constexpr int AWP = (AW + 3) / 4 * 4;  // row pitch: rows start 16-byte aligned
+};
+
+// One column tile (bid = tile index inside the image) of the isotropic blur; A and B
+// are the block's LDS buffers of IsoLds<C>::AH * AWP and AH * FB_TX floats.
 template <int C, bool Z3, typename LD>
-__global__ __launch_bounds__(FB_THREADS) void k_blur_iso(LD in, float* __restrict__ out, Geom g, int zc, int ntx,
-                                                        int nty, const IsoW<C> w) {
+__device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, const Geom& g, int zc, int ntx, int nty,
+                                              const IsoW<C>& w, int64_t bid, float* __restrict__ A,
+                                              float* __restrict__ B) {
     constexpr int AW = FB_TX + 2 * C, AH = FB_TY + 2 * C, NA = AW * AH;
-    constexpr int AWP = (AW + 3) / 4 * 4;   // row pitch: rows start 16-byte aligned
+    constexpr int AWP = IsoLds<C>::AWP;
     constexpr int PF = (NA + FB_THREADS - 1) / FB_THREADS;
     constexpr int CZ = Z3 ? C : 0;
     constexpr int U = C <= 10 ? 4 : (C <= 14 ? 2 : 1);  // slices per ring shift (register budget)
@@ -285,10 +294,6 @@ __global__ __launch_bounds__(FB_THREADS) void k_blur_iso(LD in, float* __restric
     constexpr int XT = AH * XQ;             // x-pass tasks (4 outputs each)
     constexpr int XR = (XT + FB_THREADS - 1) / FB_THREADS;
     constexpr int NW4 = (2 * C + 4 + 3) / 4;  // float4 reads per x-pass window
-    __shared__ __align__(16) float A[AH * AWP];
-    __shared__ __align__(16) float B[AH * FB_TX];
-    const int64_t nb = (int64_t)gridDim.x;
-    const int64_t bid = xcd_tile(blockIdx.x, nb);
     const int64_t tx = bid % ntx, rest = bid / ntx;
     const int64_t ty = rest % nty, tzc = rest / nty;
     const int64_t x0 = tx * FB_TX, y0 = ty * FB_TY;
@@ -424,6 +429,42 @@ __global__ __launch_bounds__(FB_THREADS) void k_blur_iso(LD in, float* __restric
 #pragma unroll
         for (int j = 0; j < 2 * CZ; ++j) { r0[j] = r0[j + U]; r1[j] = r1[j + U]; }
     }
+}
+
+template <int C, bool Z3, typename LD>
+__global__ __launch_bounds__(FB_THREADS) void k_blur_iso(LD in, float* __restrict__ out, Geom g, int zc, int ntx,
+                                                        int nty, const IsoW<C> w) {
+    __shared__ __align__(16) float A[IsoLds<C>::AH * IsoLds<C>::AWP];
+    __shared__ __align__(16) float B[IsoLds<C>::AH * FB_TX];
+    blur_iso_tile<C, Z3>(in, out, g, zc, ntx, nty, w, xcd_tile(blockIdx.x, gridDim.x), A, B);
+}
+
+// Batched form: one grid over the column tiles of many images that share the
+// half-width C, the dimensionality and the source type; image j owns the blocks
+// [bstart[j], bstart[j + 1]).
+template <int C>
+struct IsoJob {
+    const void* in;
+    float* out;
+    Geom g;
+    int zc, ntx, nty;
+    IsoW<C> w;
+};
+
+template <typename LD> __device__ __forceinline__ LD make_ld(const void* p);
+template <> __device__ __forceinline__ LdF32 make_ld<LdF32>(const void* p) { return LdF32{(const float*)p}; }
+template <> __device__ __forceinline__ LdU16 make_ld<LdU16>(const void* p) { return LdU16{(const uint16_t*)p}; }
+template <> __device__ __forceinline__ LdU8 make_ld<LdU8>(const void* p) { return LdU8{(const uint8_t*)p}; }
+
+template <int C, bool Z3, typename LD>
+__global__ __launch_bounds__(FB_THREADS) void k_blur_iso_batch(const IsoJob<C>* __restrict__ jobs,
+                                                              const int64_t* __restrict__ bstart, int njobs) {
+    __shared__ __align__(16) float A[IsoLds<C>::AH * IsoLds<C>::AWP];
+    __shared__ __align__(16) float B[IsoLds<C>::AH * FB_TX];
+    const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
+    const int j = find_seg(bstart, njobs, b);
+    const IsoJob<C>& jb = jobs[j];
+    blur_iso_tile<C, Z3>(make_ld<LD>(jb.in), jb.out, jb.g, jb.zc, jb.ntx, jb.nty, jb.w, b - bstart[j], A, B);
 }
 
 // ----------------------------------------------------------------- host side
@@ -674,6 +715,110 @@ static int separable_impl(const cle_buf* src, const cle_buf* dst, const Taps* ta
         CLE_TRY(pass_from(cur, nxt, gs, axes[i], *taps[axes[i]], s));
         cur = nxt;
     }
+    return CLE_OK;
+}
+
+// ----------------------------------------------------------------- batched blur
+// The isotropic requests of a batch are grouped by (half-width, dimensionality, source
+// type); each group is one launch of k_blur_iso_batch over the column tiles of all
+// its images.  The rest (sigma <= 0, wider kernels, other types) run blur_impl.
+template <int C, bool Z3, typename LD>
+static void plan_iso_group(const std::vector<BlurReq>& reqs, const std::vector<int>& idx, Blob& blob,
+                           std::vector<Launch>& out, hipStream_t s) {
+    constexpr int CZ = Z3 ? C : 0;
+    std::vector<IsoJob<C>> jobs(idx.size());
+    std::vector<int64_t> bst(idx.size() + 1);
+    int64_t nb = 0;
+    for (size_t k = 0; k < idx.size(); ++k) {
+        const BlurReq& r = reqs[idx[k]];
+        const Geom& g = r.g;
+        IsoJob<C>& jb = jobs[k];
+        jb.in = r.in;
+        jb.out = r.out;
+        jb.g = g;
+        jb.ntx = (int)((g.W + FB_TX - 1) / FB_TX);
+        jb.nty = (int)((g.H + FB_TY - 1) / FB_TY);
+        const int64_t nxy = (int64_t)jb.ntx * jb.nty;
+        // the batch fills the chip: z chunks only for images with few column tiles
+        int64_t nzc = (256 + nxy - 1) / nxy;
+        const int64_t max_nzc = g.D / (4 * CZ + 8) > 1 ? g.D / (4 * CZ + 8) : 1;
+        if (nzc > max_nzc) nzc = max_nzc;
+        if (nzc < 1) nzc = 1;
+        jb.zc = (int)((g.D + nzc - 1) / nzc);
+        nzc = (g.D + jb.zc - 1) / jb.zc;
+        Taps t;
+        gauss_taps(r.sigma, &t);
+        for (int v = 0; v <= C; ++v) jb.w.w[v] = t.w[v];
+        bst[k] = nb;
+        nb += nxy * nzc;
+    }
+    bst[idx.size()] = nb;
+    const size_t jo = blob.put_vec(jobs), bo = blob.put_vec(bst);
+    const int nj = (int)idx.size();
+    out.push_back([=](const char* dev) -> int {
+        k_blur_iso_batch<C, Z3, LD><<<(unsigned)nb, FB_THREADS, 0, s>>>((const IsoJob<C>*)(dev + jo),
+                                                                       (const int64_t*)(dev + bo), nj);
+        CLE_LAUNCHED();
+        return CLE_OK;
+    });
+}
+
+template <int C>
+static void plan_iso_c(bool z3, int dt, const std::vector<BlurReq>& reqs, const std::vector<int>& idx, Blob& blob,
+                       std::vector<Launch>& out, hipStream_t s) {
+    if (dt == CLE_U16) {
+        z3 ? plan_iso_group<C, true, LdU16>(reqs, idx, blob, out, s) : plan_iso_group<C, false, LdU16>(reqs, idx, blob, out, s);
+    } else if (dt == CLE_U8) {
+        z3 ? plan_iso_group<C, true, LdU8>(reqs, idx, blob, out, s) : plan_iso_group<C, false, LdU8>(reqs, idx, blob, out, s);
+    } else {
+        z3 ? plan_iso_group<C, true, LdF32>(reqs, idx, blob, out, s) : plan_iso_group<C, false, LdF32>(reqs, idx, blob, out, s);
+    }
+}
+
+int plan_blur_batch(const std::vector<BlurReq>& reqs, Blob& blob, std::vector<Launch>& out, hipStream_t s) {
+    std::vector<std::vector<int>> groups(2 * 3 * (ISO_MAXC + 1));
+    auto key = [](int c, bool z3, int dt) { return (c * 2 + (z3 ? 1 : 0)) * 3 + (dt == CLE_U16 ? 0 : dt == CLE_U8 ? 1 : 2); };
+    for (size_t i = 0; i < reqs.size(); ++i) {
+        const BlurReq& r = reqs[i];
+        if (r.g.n() == 0) continue;
+        Taps t;
+        const bool typed = r.in_dt == CLE_U16 || r.in_dt == CLE_U8 || r.in_dt == CLE_F32;
+        if (r.sigma > 0 && typed) {
+            gauss_taps(r.sigma, &t);
+            if (t.c >= 1 && t.c <= ISO_MAXC) {
+                groups[key(t.c, r.g.ndim == 3, r.in_dt)].push_back((int)i);
+                continue;
+            }
+        }
+        out.push_back([=](const char*) -> int {
+            cle_buf sb{}, db{};
+            sb.data = (void*)r.in;
+            sb.dtype = r.in_dt;
+            db.data = r.out;
+            db.dtype = CLE_F32;
+            const int nd = r.g.ndim;
+            const int64_t shp[3] = {r.g.D, r.g.H, r.g.W};
+            sb.ndim = db.ndim = nd;
+            for (int a = 0; a < nd; ++a) sb.shape[a] = db.shape[a] = shp[3 - nd + a];
+            const float sg[3] = {r.sigma, r.sigma, r.sigma};
+            return blur_impl(&sb, &db, sg, s);
+        });
+    }
+    for (int c = 1; c <= ISO_MAXC; ++c)
+        for (int z = 0; z < 2; ++z)
+            for (int d = 0; d < 3; ++d) {
+                const std::vector<int>& idx = groups[(c * 2 + z) * 3 + d];
+                if (idx.empty()) continue;
+                const int dt = d == 0 ? CLE_U16 : d == 1 ? CLE_U8 : CLE_F32;
+                switch (c) {
+#define ISO_PLAN(n) case n: plan_iso_c<n>(z == 1, dt, reqs, idx, blob, out, s); break;
+                    ISO_PLAN(1) ISO_PLAN(2) ISO_PLAN(3) ISO_PLAN(4) ISO_PLAN(5) ISO_PLAN(6) ISO_PLAN(7)
+                    ISO_PLAN(8) ISO_PLAN(9) ISO_PLAN(10) ISO_PLAN(11) ISO_PLAN(12) ISO_PLAN(13) ISO_PLAN(14)
+                    ISO_PLAN(15) ISO_PLAN(16)
+#undef ISO_PLAN
+                    default: break;
+                }
+            }
     return CLE_OK;
 }
 

@@ -5,7 +5,12 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
+
+#include <functional>
+#include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/cle_abi.h"
 
@@ -93,6 +98,19 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
     return start + k;
 }
 
+// Batches: one grid over the blocks of many independent images; image j owns the
+// blocks [start[j], start[j + 1]).  Returns the largest j with start[j] <= b (images
+// without blocks are skipped).
+__device__ __forceinline__ int find_seg(const int64_t* start, int n, int64_t b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 inline unsigned grid_for(size_t n, unsigned block) {
     size_t g = (n + block - 1) / block;
     return (unsigned)(g == 0 ? 1 : g);
@@ -148,6 +166,81 @@ __device__ __forceinline__ bool nz(const DBuf b, int64_t i) {
     }
 }
 
+// detect_maxima_box with radius 0 at voxel i: the neighbourhood is visited x outer,
+// then y, then z inner, starting from value(i) - 1, and a strictly larger value takes
+// over; the voxel is a maximum if the last take-over was its own value
+// (_tier1/detect_maxima_{2d,3d}_x.cl semantics; out-of-range coordinates are skipped
+// below 0 and clamped above the last index).
+template <typename LD>
+__device__ __forceinline__ bool maxima_voxel(const LD& in, const Geom& g, int64_t i) {
+    const int zr = g.ndim == 3 ? 1 : 0;
+    const int64_t x = i % g.W, y = (i / g.W) % g.H, z = i / (g.W * g.H);
+    float lm = in(i) - 1.f;
+    bool self = true;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -zr; dz <= zr; ++dz) {
+                const int64_t qx = x + dx, qy = y + dy, qz = z + dz;
+                if (qx < 0 || qy < 0 || qz < 0) continue;
+                const int64_t q = ((qz > g.D - 1 ? g.D - 1 : qz) * g.H + (qy > g.H - 1 ? g.H - 1 : qy)) * g.W +
+                                  (qx > g.W - 1 ? g.W - 1 : qx);
+                const float w = in(q);
+                if (w > lm) {
+                    lm = w;
+                    self = (dx == 0 && dy == 0 && dz == 0);
+                }
+            }
+    return self;
+}
+
+// ----------------------------------------------------------------- batches
+// One image of a batch.  Work images of all the batch's images are packed in shared
+// buffers, image j at voxel offset `off` (a multiple of 64).  Stages that walk voxels
+// give each image ceil(n / BV) blocks of BV voxels.
+constexpr int BV = 4096;
+
+struct BImg {
+    Geom g;
+    int64_t off;        // voxel offset in the packed work buffers
+    const void* src;    // caller's source image
+    void* dst;          // caller's label image
+    int src_dt, dst_dt;
+    int big;            // n >= 2^24: Otsu runs through the single-image path
+    int pad;
+    int64_t kbase;      // CCL: first key bit of the image (multiple of 32)
+    int64_t boff;       // CCL: first 2x2x2 block of the image
+    int64_t tbase;      // voronoi: first tile of the image
+};
+
+// Host-side staging of a batch's device tables: appended while the stages are
+// planned, uploaded with one copy; a table then lives at device base + its offset.
+struct Blob {
+    std::vector<char> h;
+    size_t put(const void* p, size_t bytes) {
+        const size_t o = (h.size() + 255) & ~(size_t)255;
+        h.resize(o + (bytes ? bytes : 1));
+        if (bytes) memcpy(h.data() + o, p, bytes);
+        return o;
+    }
+    template <typename T> size_t put_vec(const std::vector<T>& v) { return put(v.data(), v.size() * sizeof(T)); }
+};
+
+// A planned launch; runs with the device base of the uploaded blob.
+using Launch = std::function<int(const char* dev)>;
+
+// block-prefix table of the voxel stages: start[j] = sum over i < j of ceil(n_i / BV)
+inline std::vector<int64_t> voxel_block_starts(const std::vector<BImg>& imgs, int64_t* total) {
+    std::vector<int64_t> st(imgs.size() + 1);
+    int64_t t = 0;
+    for (size_t j = 0; j < imgs.size(); ++j) {
+        st[j] = t;
+        t += (imgs[j].g.n() + BV - 1) / BV;
+    }
+    st[imgs.size()] = t;
+    *total = t;
+    return st;
+}
+
 // ----------------------------------------------------------------- internal APIs
 // Used across translation units by the composite entry points.
 int blur_impl(const cle_buf* src, const cle_buf* dst, const float sig[3], hipStream_t s);
@@ -157,6 +250,31 @@ int minmax_impl(DBuf src, int64_t n, float* out2, hipStream_t s);
 int otsu_impl(DBuf src, const Geom& g, float* minmax_dev, double* thr_dev, hipStream_t s);
 int ccl_impl(DBuf binary, const Geom& g, DBuf dst, int connectivity, int options,
              uint32_t* n_labels_dev, hipStream_t s);
+// batched stages (defined next to their single-image kernels)
+struct BlurReq {
+    const void* in;
+    int in_dt;
+    float* out;
+    Geom g;
+    float sigma;
+};
+int plan_blur_batch(const std::vector<BlurReq>& reqs, Blob& blob, std::vector<Launch>& out, hipStream_t s);
+// per image: mm[2 j] = min, mm[2 j + 1] = max, thr[j] = Otsu threshold of the float32
+// work image F (images flagged big run otsu_impl one by one)
+int plan_otsu_batch(const std::vector<BImg>& imgs, size_t imgs_off, size_t vstart_off, int64_t vblocks,
+                    const float* F, float* mm, double* thr, uint32_t* scratch_u32, Blob& blob,
+                    std::vector<Launch>& out, hipStream_t s);
+// box CCL of the uint8 work image S of every image into the float32 work image F
+// (labels as the single-image path gives them, float32-key compatible).
+// ccl_batch_layout sets kbase / boff of every image before the image table is staged.
+struct CclBatchSizes {
+    int64_t nb = 0;       // 2x2x2 blocks of all images
+    int64_t bits = 0;     // key bits of all images
+};
+void ccl_batch_layout(std::vector<BImg>& imgs, CclBatchSizes* sz);
+int plan_ccl_batch(const std::vector<BImg>& imgs, size_t imgs_off, const CclBatchSizes& sz, const uint8_t* S,
+                   float* F, Blob& blob, std::vector<Launch>& out, std::vector<std::shared_ptr<void>>& keep,
+                   hipStream_t s);
 int voronoi_grow_impl(float* a, float* b, const Geom& g, int* n_iterations, int* final_in_a,
                       hipStream_t s);
 

@@ -112,26 +112,7 @@ __global__ void k_onlyzero_max(DBuf s, DBuf flag, DBuf d, Geom g, int diamond) {
 template <typename LD>
 __global__ void k_maxima(LD in, DBuf d, Geom g) {
     const int64_t n = g.n();
-    const int zr = g.ndim == 3 ? 1 : 0;
-    GRID_STRIDE(i, n) {
-        const int64_t x = i % g.W, y = (i / g.W) % g.H, z = i / (g.W * g.H);
-        float lm = in(i) - 1.f;
-        bool self = true;
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dz = -zr; dz <= zr; ++dz) {
-                    const int64_t qx = x + dx, qy = y + dy, qz = z + dz;
-                    if (qx < 0 || qy < 0 || qz < 0) continue;
-                    const int64_t q = ((qz > g.D - 1 ? g.D - 1 : qz) * g.H + (qy > g.H - 1 ? g.H - 1 : qy)) * g.W +
-                                      (qx > g.W - 1 ? g.W - 1 : qx);
-                    const float w = in(q);
-                    if (w > lm) {
-                        lm = w;
-                        self = (dx == 0 && dy == 0 && dz == 0);
-                    }
-                }
-        st(d, i, self ? 1.f : 0.f);
-    }
+    GRID_STRIDE(i, n) st(d, i, maxima_voxel(in, g, i) ? 1.f : 0.f);
 }
 
 struct LdAnyE {

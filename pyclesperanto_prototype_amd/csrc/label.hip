@@ -576,16 +576,28 @@ __device__ __forceinline__ uint32_t block_min_key(uint32_t m, const Geom& g, int
     return NONE;
 }
 
+// LDS of one local-pass block
+template <typename BT>
+struct BcclLds {
+    uint32_t lab[BT::N];
+    uint32_t kmin[BT::N];
+    uint8_t msk[BT::N];
+    uint32_t cnt;
+};
+
+// Local pass over tile `tile` (index inside the image) of an image whose blocks start
+// at global block boff; `gtile` is the tile's slot in the roots / nroots arrays.
+// Block indices stored in M, L, K and roots are global (boff + local).
 template <typename BT, typename FG>
-__global__ __launch_bounds__(CT_THREADS) void k_bccl_local(FG fg, Geom g, BGrid B, TileGrid T, uint8_t* M,
-                                                          uint32_t* L, uint32_t* K, uint32_t* roots,
-                                                          uint32_t* nroots) {
-    constexpr int N = BT::N, PER = BT::PER, TX = BT::TX, TY = BT::TY, BZ = BT::BZ;
-    __shared__ uint32_t lab[N];
-    __shared__ uint32_t kmin[N];
-    __shared__ uint8_t msk[N];
-    __shared__ uint32_t cnt;
-    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void bccl_local_tile(FG fg, const Geom& g, const BGrid& B, const TileGrid& T,
+                                                int64_t tile, int64_t gtile, int64_t boff, uint8_t* M,
+                                                uint32_t* L, uint32_t* K, uint32_t* roots, uint32_t* nroots,
+                                                BcclLds<BT>& sm) {
+    constexpr int PER = BT::PER, TX = BT::TX, TY = BT::TY, BZ = BT::BZ;
+    uint32_t* lab = sm.lab;
+    uint32_t* kmin = sm.kmin;
+    uint8_t* msk = sm.msk;
+    uint32_t& cnt = sm.cnt;
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
     const int64_t bx0 = tx * TX, by0 = ty * TY, bz0 = tz * BT::TZ;
     if (threadIdx.x == 0) cnt = 0;
@@ -617,7 +629,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_bccl_local(FG fg, Geom g, BGrid 
                 }
             }
         }
-        gb[k] = in ? (bz * B.H + by) * B.W + bx : 0;
+        gb[k] = boff + (in ? (bz * B.H + by) * B.W + bx : 0);
         if (in) M[gb[k]] = (uint8_t)mk;  // neighbouring tiles read the masks in the border pass
         m[k] = mk;
         msk[li] = (uint8_t)mk;
@@ -626,7 +638,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_bccl_local(FG fg, Geom g, BGrid 
         anyf |= mk != 0;
     }
     if (!__syncthreads_or(anyf ? 1 : 0)) {
-        if (threadIdx.x == 0) nroots[tile] = 0;
+        if (threadIdx.x == 0) nroots[gtile] = 0;
         return;
     }
     const int zr = BZ == 2 ? 1 : 0;
@@ -662,10 +674,10 @@ __global__ __launch_bounds__(CT_THREADS) void k_bccl_local(FG fg, Geom g, BGrid 
         const int lx = li % TX, ly = (li / TX) % TY, lz = li / (TX * TY);
         atomicMin(kmin + r[k], block_min_key(m[k], g, bx0 + lx, by0 + ly, bz0 + lz, BZ));
         const int rx = (int)(r[k] % TX), ry = (int)((r[k] / TX) % TY), rz = (int)(r[k] / (TX * TY));
-        L[gb[k]] = (uint32_t)(((bz0 + rz) * B.H + (by0 + ry)) * B.W + (bx0 + rx));
+        L[gb[k]] = (uint32_t)(boff + ((bz0 + rz) * B.H + (by0 + ry)) * B.W + (bx0 + rx));
     }
     __syncthreads();
-    uint32_t* myroots = roots + tile * BT::ROOTS;
+    uint32_t* myroots = roots + gtile * BT::ROOTS;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int li = threadIdx.x + k * CT_THREADS;
@@ -676,17 +688,26 @@ __global__ __launch_bounds__(CT_THREADS) void k_bccl_local(FG fg, Geom g, BGrid 
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) nroots[tile] = cnt;
+    if (threadIdx.x == 0) nroots[gtile] = cnt;
+}
+
+template <typename BT, typename FG>
+__global__ __launch_bounds__(CT_THREADS) void k_bccl_local(FG fg, Geom g, BGrid B, TileGrid T, uint8_t* M,
+                                                          uint32_t* L, uint32_t* K, uint32_t* roots,
+                                                          uint32_t* nroots) {
+    __shared__ BcclLds<BT> sm;
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    bccl_local_tile<BT>(fg, g, B, T, tile, tile, 0, M, L, K, roots, nroots, sm);
 }
 
 // foreground blocks on the faces of each tile unite with the touching foreground
 // blocks (in the 13 earlier directions) that lie in another tile
 template <typename BT>
-__global__ __launch_bounds__(CT_THREADS) void k_bccl_border(BGrid B, TileGrid T, const uint8_t* M, uint32_t* L,
-                                                           const uint32_t* nroots) {
+__device__ __forceinline__ void bccl_border_tile(const BGrid& B, const TileGrid& T, int64_t tile, int64_t gtile,
+                                                 int64_t boff, const uint8_t* M, uint32_t* L,
+                                                 const uint32_t* nroots) {
     constexpr int N = BT::N, TX = BT::TX, TY = BT::TY;
-    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    if (nroots[tile] == 0) return;  // no foreground in this tile
+    if (nroots[gtile] == 0) return;  // no foreground in this tile
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
     const int64_t bx0 = tx * TX, by0 = ty * TY, bz0 = tz * BT::TZ;
     const int zr = BT::BZ == 2 ? 1 : 0;
@@ -697,7 +718,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_bccl_border(BGrid B, TileGrid T,
         if (!((zr && lz == 0) || ly == 0 || lx == 0 || ly == TY - 1 || lx == TX - 1)) continue;
         const int64_t bx = bx0 + lx, by = by0 + ly, bz = bz0 + lz;
         if (bx >= B.W || by >= B.H || bz >= B.D) continue;
-        const int64_t b = (bz * B.H + by) * B.W + bx;
+        const int64_t b = boff + (bz * B.H + by) * B.W + bx;
         const uint32_t m = M[b];
         if (!m) continue;
         const uint32_t lroot = ld_relaxed(L + b);
@@ -709,7 +730,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_bccl_border(BGrid B, TileGrid T,
                     if (lz + ez >= 0 && ly + ey >= 0 && lx + ex >= 0 && lx + ex < TX && ly + ey < TY) continue;
                     const int64_t qx = bx + ex, qy = by + ey, qz = bz + ez;
                     if (qx < 0 || qx >= B.W || qy < 0 || qy >= B.H || qz < 0) continue;
-                    const int64_t q = (qz * B.H + qy) * B.W + qx;
+                    const int64_t q = boff + (qz * B.H + qy) * B.W + qx;
                     const uint32_t qm = M[q];
                     if (!qm || !blocks_touch(m, qm, ex, ey, ez)) continue;
                     const uint32_t lq = ld_relaxed(L + q);
@@ -718,6 +739,13 @@ __global__ __launch_bounds__(CT_THREADS) void k_bccl_border(BGrid B, TileGrid T,
                     unite(L, lroot, lq);
                 }
     }
+}
+
+template <typename BT>
+__global__ __launch_bounds__(CT_THREADS) void k_bccl_border(BGrid B, TileGrid T, const uint8_t* M, uint32_t* L,
+                                                           const uint32_t* nroots) {
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    bccl_border_tile<BT>(B, T, tile, tile, 0, M, L, nroots);
 }
 
 // a foreground voxel's label is F[local root of its block], held as float and
@@ -818,6 +846,199 @@ static int bccl_run(FG fg, const Geom& g, DBuf dst, int options, uint32_t* n_lab
     const unsigned grid = (unsigned)(rows < 262144 ? rows : 262144);
     launch_bccl_write<BT::BZ>(fg, g, B, l, Fs.get<uint32_t>(), dst, grid, s);
     CLE_LAUNCHED();
+    return CLE_OK;
+}
+
+// ----------------------------------------------------------------- batched box CCL
+// Every image of a batch runs the single-image block CCL on its own blocks, tiles and
+// key bits; the images' 2x2x2 blocks are numbered consecutively (image j from
+// boff_j), so the union-find arrays, the tile root lists and one rank bitmap serve
+// the whole batch.  2D and 3D images use their own tile shapes (one launch per kind
+// and pass); image j's label of key bit k is rank(kbase_j + k) - rank(kbase_j).
+__host__ __device__ inline bool is3d(const Geom& g) { return g.ndim == 3 && g.D > 1; }
+
+template <typename BT>
+__host__ __device__ inline BGrid bgrid_of(const Geom& g) {
+    return BGrid{(g.D + BT::BZ - 1) / BT::BZ, (g.H + 1) / 2, (g.W + 1) / 2};
+}
+template <typename BT>
+__host__ __device__ inline TileGrid tgrid_of(const BGrid& B) {
+    return TileGrid{(B.D + BT::TZ - 1) / BT::TZ, (B.H + BT::TY - 1) / BT::TY, (B.W + BT::TX - 1) / BT::TX};
+}
+
+void ccl_batch_layout(std::vector<BImg>& imgs, CclBatchSizes* sz) {
+    int64_t nb = 0, bits = 0;
+    for (auto& im : imgs) {
+        const bool d3 = is3d(im.g);
+        const BGrid B = d3 ? bgrid_of<BlkTile3D>(im.g) : bgrid_of<BlkTile2D>(im.g);
+        im.boff = nb;
+        nb += B.D * B.H * B.W;
+        const uint64_t n = (uint64_t)im.g.n();
+        const uint64_t maxbit = (uint64_t)(float)n;  // float32 keys (the voronoi pipeline's CCL)
+        im.kbase = bits;
+        bits += (int64_t)((maxbit + 1 + 31) / 32 * 32);
+    }
+    sz->nb = nb;
+    sz->bits = bits;
+}
+
+template <typename BT>
+__global__ __launch_bounds__(CT_THREADS) void k_bccl_local_batch(const BImg* __restrict__ imgs,
+                                                                const int64_t* __restrict__ tstart, int nimg,
+                                                                const uint8_t* S, uint8_t* M, uint32_t* L,
+                                                                uint32_t* K, uint32_t* roots, uint32_t* nroots) {
+    __shared__ BcclLds<BT> sm;
+    const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
+    const int j = find_seg(tstart, nimg, b);
+    const BImg& im = imgs[j];
+    const BGrid B = bgrid_of<BT>(im.g);
+    bccl_local_tile<BT>(FgU8{S + im.off}, im.g, B, tgrid_of<BT>(B), b - tstart[j], b, im.boff, M, L, K, roots,
+                        nroots, sm);
+}
+
+template <typename BT>
+__global__ __launch_bounds__(CT_THREADS) void k_bccl_border_batch(const BImg* __restrict__ imgs,
+                                                                 const int64_t* __restrict__ tstart, int nimg,
+                                                                 const uint8_t* M, uint32_t* L,
+                                                                 const uint32_t* nroots) {
+    const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
+    const int j = find_seg(tstart, nimg, b);
+    const BImg& im = imgs[j];
+    const BGrid B = bgrid_of<BT>(im.g);
+    bccl_border_tile<BT>(B, tgrid_of<BT>(B), b - tstart[j], b, im.boff, M, L, nroots);
+}
+
+template <typename BT>
+__global__ __launch_bounds__(CT_THREADS) void k_ccl_mark_roots_batch(const BImg* __restrict__ imgs,
+                                                                    const int64_t* __restrict__ tstart, int nimg,
+                                                                    const uint32_t* L, const uint32_t* K,
+                                                                    const uint32_t* roots, const uint32_t* nroots,
+                                                                    uint32_t* bm) {
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t kbase = imgs[find_seg(tstart, nimg, tile)].kbase;
+    const uint32_t n = nroots[tile];
+    for (uint32_t k = threadIdx.x; k < n; k += CT_THREADS) {
+        const uint32_t lr = roots[tile * BT::ROOTS + k];
+        if (L[lr] != lr) continue;  // not a global root
+        const uint64_t bit = (uint64_t)kbase + key_bit(K[lr], 1);
+        atomicOr(bm + (bit >> 5), 1u << (bit & 31));
+    }
+}
+
+template <typename BT>
+__global__ __launch_bounds__(CT_THREADS) void k_ccl_final_batch(const BImg* __restrict__ imgs,
+                                                               const int64_t* __restrict__ tstart, int nimg,
+                                                               const uint32_t* L, const uint32_t* K,
+                                                               const uint32_t* roots, const uint32_t* nroots,
+                                                               const uint32_t* bm, const uint32_t* prefix,
+                                                               uint32_t* F) {
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t kbase = imgs[find_seg(tstart, nimg, tile)].kbase;
+    const uint32_t before = prefix[kbase >> 5];  // set bits of the images before this one
+    const uint32_t n = nroots[tile];
+    for (uint32_t k = threadIdx.x; k < n; k += CT_THREADS) {
+        const uint32_t lr = roots[tile * BT::ROOTS + k];
+        F[lr] = bit_rank(bm, prefix, (uint64_t)kbase + key_bit(K[L[lr]], 1)) - before;
+    }
+}
+
+// float32 labels of the uint8 work image: one block per row of one image
+template <int BZ>
+__global__ void k_bccl_write_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ rstart, int nimg,
+                                   const uint8_t* S, const uint32_t* L, const uint32_t* F, float* out) {
+    const int64_t r = blockIdx.x;
+    const int j = find_seg(rstart, nimg, r);
+    const BImg& im = imgs[j];
+    const Geom g = im.g;
+    const int64_t row = r - rstart[j];
+    const int64_t z = row / g.H, y = row - z * g.H;
+    const BGrid B{(g.D + BZ - 1) / BZ, (g.H + 1) / 2, (g.W + 1) / 2};
+    const int64_t rb = im.boff + ((z / BZ) * B.H + (y >> 1)) * B.W;
+    const uint8_t* bin = S + im.off + row * g.W;
+    float* o = out + im.off + row * g.W;
+    for (int64_t x = threadIdx.x; x < g.W; x += blockDim.x) o[x] = bin[x] ? (float)F[L[rb + (x >> 1)]] : 0.f;
+}
+
+template <typename BT>
+static int plan_ccl_kind(const std::vector<BImg>& imgs, size_t imgs_off, bool want3d, const uint8_t* S, float* F,
+                         uint8_t* M, uint32_t* L, uint32_t* K, uint32_t* Fl, RankTable* rt, Blob& blob,
+                         std::vector<Launch>& pre, std::vector<Launch>& post, std::vector<std::shared_ptr<void>>& keep,
+                         hipStream_t s) {
+    const int nimg = (int)imgs.size();
+    std::vector<int64_t> tst(nimg + 1), rst(nimg + 1);
+    int64_t nt = 0, nr = 0;
+    for (int j = 0; j < nimg; ++j) {
+        tst[j] = nt;
+        rst[j] = nr;
+        if (is3d(imgs[j].g) != want3d || imgs[j].g.n() == 0) continue;
+        const BGrid B = bgrid_of<BT>(imgs[j].g);
+        nt += tgrid_of<BT>(B).count();
+        nr += imgs[j].g.D * imgs[j].g.H;
+    }
+    tst[nimg] = nt;
+    rst[nimg] = nr;
+    if (nt == 0) return CLE_OK;
+    auto Rs = std::make_shared<Scratch>();
+    auto Ns = std::make_shared<Scratch>();
+    keep.push_back(Rs);
+    keep.push_back(Ns);
+    CLE_TRY(Rs->alloc((size_t)nt * BT::ROOTS * 4, s));
+    CLE_TRY(Ns->alloc((size_t)nt * 4, s));
+    uint32_t* roots = Rs->get<uint32_t>();
+    uint32_t* nroots = Ns->get<uint32_t>();
+    const size_t to = blob.put_vec(tst), ro = blob.put_vec(rst);
+    // before the rank scan: local, border, resolve, mark
+    pre.push_back([=](const char* dev) -> int {
+        const BImg* im = (const BImg*)(dev + imgs_off);
+        const int64_t* ts = (const int64_t*)(dev + to);
+        k_bccl_local_batch<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(im, ts, nimg, S, M, L, K, roots, nroots);
+        k_bccl_border_batch<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(im, ts, nimg, M, L, nroots);
+        k_ccl_resolve<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(L, K, roots, nroots);
+        k_ccl_mark_roots_batch<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(im, ts, nimg, L, K, roots, nroots,
+                                                                      rt->bm.get<uint32_t>());
+        CLE_LAUNCHED();
+        return CLE_OK;
+    });
+    // after it: final labels of the local roots, then the float32 label rows
+    post.push_back([=](const char* dev) -> int {
+        const BImg* im = (const BImg*)(dev + imgs_off);
+        const int64_t* ts = (const int64_t*)(dev + to);
+        const int64_t* rs = (const int64_t*)(dev + ro);
+        k_ccl_final_batch<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(im, ts, nimg, L, K, roots, nroots,
+                                                                 rt->bm.get<uint32_t>(), rt->prefix.get<uint32_t>(),
+                                                                 Fl);
+        k_bccl_write_batch<BT::BZ><<<(unsigned)nr, 256, 0, s>>>(im, rs, nimg, S, L, Fl, F);
+        CLE_LAUNCHED();
+        return CLE_OK;
+    });
+    return CLE_OK;
+}
+
+int plan_ccl_batch(const std::vector<BImg>& imgs, size_t imgs_off, const CclBatchSizes& sz, const uint8_t* S,
+                   float* F, Blob& blob, std::vector<Launch>& out, std::vector<std::shared_ptr<void>>& keep, hipStream_t s) {
+    if (sz.nb == 0) return CLE_OK;
+    auto Ms = std::make_shared<Scratch>(), Ls = std::make_shared<Scratch>(), Ks = std::make_shared<Scratch>(),
+         Fs = std::make_shared<Scratch>();
+    keep.push_back(Ms);
+    keep.push_back(Ls);
+    keep.push_back(Ks);
+    keep.push_back(Fs);
+    CLE_TRY(Ms->alloc((size_t)sz.nb, s));
+    CLE_TRY(Ls->alloc((size_t)sz.nb * 4, s));
+    CLE_TRY(Ks->alloc((size_t)sz.nb * 4, s));
+    CLE_TRY(Fs->alloc((size_t)sz.nb * 4, s));
+    auto rts = std::make_shared<RankTable>();
+    keep.push_back(rts);
+    RankTable* rt = rts.get();
+    std::vector<Launch> pre, post;
+    CLE_TRY(rt->init((uint64_t)sz.bits, s));
+    CLE_TRY(plan_ccl_kind<BlkTile3D>(imgs, imgs_off, true, S, F, Ms->get<uint8_t>(), Ls->get<uint32_t>(),
+                                     Ks->get<uint32_t>(), Fs->get<uint32_t>(), rt, blob, pre, post, keep, s));
+    CLE_TRY(plan_ccl_kind<BlkTile2D>(imgs, imgs_off, false, S, F, Ms->get<uint8_t>(), Ls->get<uint32_t>(),
+                                     Ks->get<uint32_t>(), Fs->get<uint32_t>(), rt, blob, pre, post, keep, s));
+    for (auto& l : pre) out.push_back(l);
+    out.push_back([=](const char*) -> int { return rt->scan(nullptr, s); });
+    for (auto& l : post) out.push_back(l);
     return CLE_OK;
 }
 

@@ -401,10 +401,18 @@ static int histogram_impl(DBuf src, const Geom& g, int bins, float mn, float mx,
 // cumsum uses (sequential, float64, no contraction); the 255 between-class variances
 // are then formed in parallel and the first maximum (NaN counting as maximum, as
 // numpy.argmax does) is selected.
-__global__ __launch_bounds__(256) void k_otsu(const float* hist, const float* mm, double* thr) {
+struct OtsuLds {
+    double bc[256], w1[256], w2[256], m1[256], m2[256], var[256];
+};
+
+__device__ __forceinline__ void otsu_body(const float* hist, const float* mm, double* thr, OtsuLds& sm) {
 #pragma clang fp contract(off)
-    __shared__ double bc[256], w1[256], w2[256], m1[256], m2[256];
-    __shared__ double var[256];
+    double* bc = sm.bc;
+    double* w1 = sm.w1;
+    double* w2 = sm.w2;
+    double* m1 = sm.m1;
+    double* m2 = sm.m2;
+    double* var = sm.var;
     const int t = threadIdx.x;
     const float mnf = mm[0], mxf = mm[1];
     const float rng = mxf - mnf;  // float32 subtraction, as in the reference
@@ -449,6 +457,11 @@ __global__ __launch_bounds__(256) void k_otsu(const float* hist, const float* mm
     }
 }
 
+__global__ __launch_bounds__(256) void k_otsu(const float* hist, const float* mm, double* thr) {
+    __shared__ OtsuLds sm;
+    otsu_body(hist, mm, thr, sm);
+}
+
 // dst = src > float32(threshold); float32 -> uint8 with 16-byte loads, 4-byte stores
 __global__ __launch_bounds__(256) void k_threshold_f32_u8(const float* __restrict__ s, uint8_t* __restrict__ d,
                                                           const double* thr, int64_t n) {
@@ -477,6 +490,136 @@ int otsu_impl(DBuf src, const Geom& g, float* minmax_dev, double* thr_dev, hipSt
     CLE_TRY(histogram_impl(src, g, 256, 0.f, 0.f, minmax_dev, DBuf{hist.p, CLE_F32}, s));
     k_otsu<<<1, 256, 0, s>>>(hist.get<float>(), minmax_dev, thr_dev);
     CLE_LAUNCHED();
+    return CLE_OK;
+}
+
+// ----------------------------------------------------------------- batched Otsu
+// Per image of a batch: min/max, the 256-bin histogram and the threshold, each stage
+// one launch over the BV-voxel blocks of all images.  An image below 2^24 voxels has
+// every partial histogram sum below 2^24, so its float32 totals equal the exact
+// integer counts whatever the summation order: the counts are accumulated as uint32
+// (LDS per block, then atomics).  Larger images take otsu_impl (per-row partials in
+// y order) one by one.
+__global__ __launch_bounds__(256) void k_minmax_batch(const BImg* __restrict__ imgs,
+                                                     const int64_t* __restrict__ vstart, int nimg,
+                                                     const float* __restrict__ F, uint32_t* mmo) {
+    const int64_t b = blockIdx.x;
+    const int j = find_seg(vstart, nimg, b);
+    const BImg& im = imgs[j];
+    if (im.big) return;
+    const int64_t n = im.g.n();
+    const int64_t i0 = (b - vstart[j]) * BV;
+    const int64_t i1 = i0 + BV < n ? i0 + BV : n;
+    const float* p = F + im.off;
+    float mn = INFINITY, mx = -INFINITY;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+        const float v = p[i];
+        mn = fminf(mn, v);
+        mx = fmaxf(mx, v);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = fminf(mn, __shfl_xor(mn, off));
+        mx = fmaxf(mx, __shfl_xor(mx, off));
+    }
+    __shared__ float smn[4], smx[4];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) { smn[w] = mn; smx[w] = mx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; ++k) { mn = fminf(mn, smn[k]); mx = fmaxf(mx, smx[k]); }
+        atomicMin(mmo + 2 * j, f2o(mn));
+        atomicMax(mmo + 2 * j + 1, f2o(mx));
+    }
+}
+
+__global__ void k_mm_init_batch(uint32_t* mmo, int nimg) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nimg; j += gridDim.x * blockDim.x) {
+        mmo[2 * j] = 0xFFFFFFFFu;
+        mmo[2 * j + 1] = 0u;
+    }
+}
+
+__global__ void k_mm_fin_batch(const BImg* __restrict__ imgs, const uint32_t* mmo, float* mm, int nimg) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nimg; j += gridDim.x * blockDim.x) {
+        if (imgs[j].big) continue;
+        mm[2 * j] = o2f(mmo[2 * j]);
+        mm[2 * j + 1] = o2f(mmo[2 * j + 1]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_hist_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ vstart,
+                                                   int nimg, const float* __restrict__ F, const float* mm,
+                                                   uint32_t* cnt) {
+    __shared__ uint32_t h[4 * 257];
+    const int64_t b = blockIdx.x;
+    const int j = find_seg(vstart, nimg, b);
+    const BImg& im = imgs[j];
+    if (im.big) return;
+    for (int i = threadIdx.x; i < 4 * 257; i += 256) h[i] = 0;
+    __syncthreads();
+    const BinFast bc = bin_fast_init(mm[2 * j], mm[2 * j + 1] - mm[2 * j], 256);
+    const int64_t n = im.g.n();
+    const int64_t i0 = (b - vstart[j]) * BV;
+    const int64_t i1 = i0 + BV < n ? i0 + BV : n;
+    const float* p = F + im.off;
+    uint32_t* mine = h + (threadIdx.x >> 6) * 257;
+    uint32_t cur = 0xFFFFFFFFu, run = 0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+        const uint32_t bin = bin_fast(p[i], bc);
+        if (bin == cur) {
+            ++run;
+        } else {
+            if (run) atomicAdd(&mine[cur], run);
+            cur = bin;
+            run = 1;
+        }
+    }
+    if (run) atomicAdd(&mine[cur], run);
+    __syncthreads();
+    const int t = threadIdx.x;
+    const uint32_t c = h[t] + h[257 + t] + h[514 + t] + h[771 + t];
+    if (c) atomicAdd(&cnt[256 * j + t], c);
+}
+
+__global__ __launch_bounds__(256) void k_otsu_batch(const BImg* __restrict__ imgs, const uint32_t* cnt,
+                                                   float* histf, const float* mm, double* thr) {
+    __shared__ OtsuLds sm;
+    const int j = blockIdx.x;
+    if (imgs[j].big) return;
+    histf[256 * j + threadIdx.x] = (float)cnt[256 * j + threadIdx.x];
+    __syncthreads();
+    otsu_body(histf + 256 * j, mm + 2 * j, thr + j, sm);
+}
+
+int plan_otsu_batch(const std::vector<BImg>& imgs, size_t imgs_off, size_t vstart_off, int64_t vblocks,
+                    const float* F, float* mm, double* thr, uint32_t* scratch_u32, Blob& blob,
+                    std::vector<Launch>& out, hipStream_t s) {
+    (void)blob;
+    const int nimg = (int)imgs.size();
+    // scratch_u32: 2 * nimg ordered min/max, then 256 * nimg counts, then 256 * nimg floats
+    uint32_t* mmo = scratch_u32;
+    uint32_t* cnt = scratch_u32 + 2 * (size_t)nimg;
+    float* histf = (float*)(cnt + 256 * (size_t)nimg);
+    out.push_back([=](const char* dev) -> int {
+        const BImg* im = (const BImg*)(dev + imgs_off);
+        const int64_t* vs = (const int64_t*)(dev + vstart_off);
+        const unsigned gi = grid_for((size_t)nimg, 256);
+        k_mm_init_batch<<<gi, 256, 0, s>>>(mmo, nimg);
+        CLE_TRY(fill_u32(cnt, 0u, 256 * (size_t)nimg, s));
+        k_minmax_batch<<<(unsigned)vblocks, 256, 0, s>>>(im, vs, nimg, F, mmo);
+        k_mm_fin_batch<<<gi, 256, 0, s>>>(im, mmo, mm, nimg);
+        k_hist_batch<<<(unsigned)vblocks, 256, 0, s>>>(im, vs, nimg, F, mm, cnt);
+        k_otsu_batch<<<(unsigned)nimg, 256, 0, s>>>(im, cnt, histf, mm, thr);
+        CLE_LAUNCHED();
+        return CLE_OK;
+    });
+    for (int j = 0; j < nimg; ++j) {
+        if (!imgs[j].big) continue;
+        const BImg im = imgs[j];
+        out.push_back([=](const char*) -> int {
+            return otsu_impl(DBuf{(void*)(F + im.off), CLE_F32}, im.g, mm + 2 * j, thr + j, s);
+        });
+    }
     return CLE_OK;
 }
 

@@ -35,7 +35,7 @@ struct Tiles {
     __host__ __device__ int64_t count() const { return nz * ny * nx; }
 };
 
-static Tiles make_tiles(const Geom& g) {
+__host__ __device__ inline Tiles make_tiles(const Geom& g) {
     Tiles t;
     if (g.ndim == 3 && g.D > 1) { t.tz = 4; t.ty = 4; }
     else { t.tz = 1; t.ty = 16; }
@@ -49,10 +49,11 @@ __device__ __forceinline__ int64_t clampi(int64_t v, int64_t hi) { return v < 0 
 
 // A = B = (mask - 1) + label, and the tiles holding a positive value are flagged as
 // "changed" at steps -1 and -2 (slots 3 and 2 of the 4-slot ring).
-__global__ __launch_bounds__(VB) void k_vor_init(DBuf mask, float* A, float* B, Geom g, Tiles T, uint8_t* tflags) {
-    const int64_t tile = blockIdx.x;
+// tile = index inside the image, gtile = its slot in the flag arrays of ntiles slots
+template <typename MK>
+__device__ __forceinline__ void vor_init_tile(MK mask, float* A, float* B, const Geom& g, const Tiles& T,
+                                              int64_t tile, int64_t gtile, int64_t ntiles, uint8_t* tflags) {
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
-    const int64_t ntiles = T.count();
     bool pos = false;
     for (int k = 0; k < VPT; ++k) {
         const int li = threadIdx.x + k * VB;
@@ -61,7 +62,7 @@ __global__ __launch_bounds__(VB) void k_vor_init(DBuf mask, float* A, float* B, 
         const int64_t x = tx * VTX + lx, y = ty * T.ty + ly, z = tz * T.tz + lz;
         if (x >= g.W || y >= g.H || z >= g.D) continue;
         const int64_t i = (z * g.H + y) * g.W + x;
-        const float v = (ld(mask, i) - 1.f) + B[i];
+        const float v = (mask(i) - 1.f) + B[i];
         A[i] = v;
         B[i] = v;
         pos |= v > 0.f;
@@ -69,21 +70,35 @@ __global__ __launch_bounds__(VB) void k_vor_init(DBuf mask, float* A, float* B, 
     const int any = __syncthreads_or(pos ? 1 : 0);
     if (threadIdx.x == 0) {
         const uint8_t f = any ? 1 : 0;
-        tflags[0 * ntiles + tile] = 0;
-        tflags[1 * ntiles + tile] = 0;
-        tflags[2 * ntiles + tile] = f;
-        tflags[3 * ntiles + tile] = f;
+        tflags[0 * ntiles + gtile] = 0;
+        tflags[1 * ntiles + gtile] = 0;
+        tflags[2 * ntiles + gtile] = f;
+        tflags[3 * ntiles + gtile] = f;
     }
 }
 
+struct MaskAny {
+    DBuf b;
+    __device__ __forceinline__ float operator()(int64_t i) const { return ld(b, i); }
+};
+struct MaskU8 {
+    const uint8_t* p;
+    __device__ __forceinline__ float operator()(int64_t i) const { return (float)p[i]; }
+};
+
+__global__ __launch_bounds__(VB) void k_vor_init(DBuf mask, float* A, float* B, Geom g, Tiles T, uint8_t* tflags) {
+    vor_init_tile(MaskAny{mask}, A, B, g, T, blockIdx.x, blockIdx.x, T.count(), tflags);
+}
+
 // one growth step t: src -> dst, box (t even) or diamond (t odd)
-__global__ __launch_bounds__(VB) void k_vor_step(const float* __restrict__ src, float* __restrict__ dst, Geom g,
-                                                Tiles T, int t, uint8_t* tflags, uint32_t* gflags, int ring) {
-    if (t > 0 && gflags[(t - 1) % ring] == 0) return;  // the previous step changed nothing: done
-    const int64_t tile = blockIdx.x;
-    const int64_t ntiles = T.count();
+// One growth step on one tile.  gflag / gany: "changed" flags of this step for the
+// image and for the whole launch (the same word for a single image).
+__device__ __forceinline__ void vor_step_tile(const float* __restrict__ src, float* __restrict__ dst, const Geom& g,
+                                              const Tiles& T, int64_t tile, int64_t tbase, int64_t ntiles, int t,
+                                              uint8_t* tflags, uint32_t* gflag, uint32_t* gany) {
+    const int64_t gtile = tbase + tile;
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
-    if (threadIdx.x == 0) tflags[((t + 1) & 3) * ntiles + tile] = 0;
+    if (threadIdx.x == 0) tflags[((t + 1) & 3) * ntiles + gtile] = 0;
     // activity: any tile of the 3^d tile neighbourhood changed at t-1 or t-2
     const uint8_t* f1 = tflags + ((t + 3) & 3) * ntiles;
     const uint8_t* f2 = tflags + ((t + 2) & 3) * ntiles;
@@ -92,7 +107,7 @@ __global__ __launch_bounds__(VB) void k_vor_step(const float* __restrict__ src, 
         const int dx = threadIdx.x % 3 - 1, dy = (threadIdx.x / 3) % 3 - 1, dz = threadIdx.x / 9 - 1;
         const int64_t qx = tx + dx, qy = ty + dy, qz = tz + dz;
         if (qx >= 0 && qx < T.nx && qy >= 0 && qy < T.ny && qz >= 0 && qz < T.nz) {
-            const int64_t q = (qz * T.ny + qy) * T.nx + qx;
+            const int64_t q = tbase + (qz * T.ny + qy) * T.nx + qx;
             act = f1[q] | f2[q];
         }
     }
@@ -142,9 +157,17 @@ __global__ __launch_bounds__(VB) void k_vor_step(const float* __restrict__ src, 
         changed |= (out != v);
     }
     if (__syncthreads_or(changed ? 1 : 0) && threadIdx.x == 0) {
-        tflags[(t & 3) * ntiles + tile] = 1;
-        gflags[t % ring] = 1;
+        tflags[(t & 3) * ntiles + gtile] = 1;
+        *gflag = 1;
+        *gany = 1;
     }
+}
+
+__global__ __launch_bounds__(VB) void k_vor_step(const float* __restrict__ src, float* __restrict__ dst, Geom g,
+                                                Tiles T, int t, uint8_t* tflags, uint32_t* gflags, int ring) {
+    if (t > 0 && gflags[(t - 1) % ring] == 0) return;  // the previous step changed nothing: done
+    uint32_t* f = gflags + t % ring;
+    vor_step_tile(src, dst, g, T, blockIdx.x, 0, T.count(), t, tflags, f, f);
 }
 
 constexpr int VOR_BATCH = 8;
@@ -223,6 +246,222 @@ static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, 
     return CLE_OK;
 }
 
+// ----------------------------------------------------------------- batched pipeline
+// voronoi_otsu_labeling of many images of any sizes: every stage is one grid over the
+// blocks or tiles of all images (the blur one grid per half-width group), the Otsu
+// thresholds stay on the device, and the masked-Voronoi growth steps all images
+// together, each image stopping on its own convergence flag; the host checks one
+// "any image changed" word per batch of steps.  Each image's result equals
+// cle_voronoi_otsu_labeling of that image alone (same kernels, same arithmetic).
+struct F32In {
+    const float* p;
+    __device__ __forceinline__ float operator()(int64_t i) const { return p[i]; }
+};
+
+__global__ __launch_bounds__(256) void k_maxima_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ vs,
+                                                     int nimg, const float* F, uint8_t* S) {
+    const int64_t b = blockIdx.x;
+    const int j = find_seg(vs, nimg, b);
+    const BImg& im = imgs[j];
+    const int64_t n = im.g.n(), i0 = (b - vs[j]) * BV, i1 = i0 + BV < n ? i0 + BV : n;
+    const F32In in{F + im.off};
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) S[im.off + i] = maxima_voxel(in, im.g, i) ? 1 : 0;
+}
+
+// G = F > float32(thr[j]); S = S AND G
+__global__ __launch_bounds__(256) void k_seg_and_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ vs,
+                                                      int nimg, const float* F, const double* thr, uint8_t* S,
+                                                      uint8_t* G) {
+    const int64_t b = blockIdx.x;
+    const int j = find_seg(vs, nimg, b);
+    const BImg& im = imgs[j];
+    const int64_t n = im.g.n(), i0 = (b - vs[j]) * BV, i1 = i0 + BV < n ? i0 + BV : n;
+    const float t = (float)thr[j];
+    for (int64_t i = im.off + i0 + threadIdx.x; i < im.off + i1; i += 256) {
+        const uint8_t sg = F[i] > t ? 1 : 0;
+        G[i] = sg;
+        S[i] = (S[i] != 0 && sg) ? 1 : 0;
+    }
+}
+
+__global__ __launch_bounds__(VB) void k_vor_init_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ ts,
+                                                      int nimg, const uint8_t* G, float* A, float* B,
+                                                      uint8_t* tflags, int64_t ntiles) {
+    const int64_t b = blockIdx.x;
+    const int j = find_seg(ts, nimg, b);
+    const BImg& im = imgs[j];
+    vor_init_tile(MaskU8{G + im.off}, A + im.off, B + im.off, im.g, make_tiles(im.g), b - ts[j], b, ntiles, tflags);
+}
+
+__global__ __launch_bounds__(VB) void k_vor_step_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ ts,
+                                                      int nimg, const float* __restrict__ src,
+                                                      float* __restrict__ dst, int t, uint8_t* tflags,
+                                                      int64_t ntiles, uint32_t* gflags, uint32_t* gany,
+                                                      int ring) {
+    const int64_t b = blockIdx.x;
+    const int j = find_seg(ts, nimg, b);
+    if (t > 0 && gflags[(int64_t)((t - 1) % ring) * nimg + j] == 0) return;  // this image has converged
+    const BImg& im = imgs[j];
+    vor_step_tile(src + im.off, dst + im.off, im.g, make_tiles(im.g), b - ts[j], ts[j], ntiles, t, tflags,
+                  gflags + (int64_t)(t % ring) * nimg + j, gany + t % ring);
+}
+
+__global__ __launch_bounds__(256) void k_mask_out_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ vs,
+                                                       int nimg, const float* A, const uint8_t* G) {
+    const int64_t b = blockIdx.x;
+    const int j = find_seg(vs, nimg, b);
+    const BImg& im = imgs[j];
+    const int64_t n = im.g.n(), i0 = (b - vs[j]) * BV, i1 = i0 + BV < n ? i0 + BV : n;
+    const DBuf d{im.dst, im.dst_dt};
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) st(d, i, G[im.off + i] ? A[im.off + i] : 0.f);
+}
+
+// pinned host staging of the batch tables, released once the stream has drained
+// (also when a later step fails, the copy out of it may still be in flight)
+struct PinnedStage {
+    void* p = nullptr;
+    hipStream_t s = nullptr;
+    ~PinnedStage() {
+        if (p) {
+            (void)hipStreamSynchronize(s);
+            (void)hipHostFree(p);
+        }
+    }
+};
+
+static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const float* ss, const float* so, int n,
+                              int* n_steps, hipStream_t s) {
+    std::vector<BImg> imgs((size_t)n);
+    int64_t total = 0, ntiles = 0;
+    for (int j = 0; j < n; ++j) {
+        Geom g, gd;
+        CLE_TRY(check_buf(&srcs[j], "source", &g));
+        CLE_TRY(check_buf(&dsts[j], "label_image_destination", &gd));
+        CLE_TRY(check_same_shape(g, gd, "source", "label_image_destination"));
+        BImg& im = imgs[j];
+        memset(&im, 0, sizeof(im));
+        im.g = g;
+        im.off = total;
+        total += (g.n() + 63) / 64 * 64;
+        im.src = srcs[j].data;
+        im.src_dt = srcs[j].dtype;
+        im.dst = dsts[j].data;
+        im.dst_dt = dsts[j].dtype;
+        im.big = g.n() >= ((int64_t)1 << 24) ? 1 : 0;
+        im.tbase = ntiles;
+        ntiles += g.n() > 0 ? make_tiles(g).count() : 0;
+    }
+    if (total == 0) return CLE_OK;
+    CclBatchSizes csz;
+    ccl_batch_layout(imgs, &csz);
+    // packed work images: F (float32: blurs, CCL labels, growth buffer B), A (growth
+    // buffer A), S (uint8 seeds), G (uint8 segmentation)
+    Scratch F, A, S, G, aux, tf, gf;
+    CLE_TRY(F.alloc((size_t)total * 4, s));
+    CLE_TRY(A.alloc((size_t)total * 4, s));
+    CLE_TRY(S.alloc((size_t)total, s));
+    CLE_TRY(G.alloc((size_t)total, s));
+    const size_t otsu_words = (size_t)n * (2 + 256 + 256);
+    CLE_TRY(aux.alloc((size_t)n * 8 + (size_t)n * 8 + otsu_words * 4, s));
+    float* mm = aux.get<float>();
+    double* thr = (double*)(aux.get<char>() + (size_t)n * 8);
+    uint32_t* ou = (uint32_t*)(aux.get<char>() + (size_t)n * 16);
+    CLE_TRY(tf.alloc((size_t)(ntiles > 0 ? ntiles : 1) * 4, s));
+    CLE_TRY(gf.alloc(((size_t)VOR_RING * n + VOR_RING) * 4, s));
+    uint32_t* gflags = gf.get<uint32_t>();
+    uint32_t* gany = gflags + (size_t)VOR_RING * n;
+    float* Fp = F.get<float>();
+    float* Ap = A.get<float>();
+    uint8_t* Sp = S.get<uint8_t>();
+    uint8_t* Gp = G.get<uint8_t>();
+
+    Blob blob;
+    const size_t imgs_off = blob.put_vec(imgs);
+    int64_t vblocks = 0;
+    const std::vector<int64_t> vst = voxel_block_starts(imgs, &vblocks);
+    const size_t vs_off = blob.put_vec(vst);
+    std::vector<int64_t> tst((size_t)n + 1);
+    for (int j = 0; j < n; ++j) tst[j] = imgs[j].tbase;
+    tst[n] = ntiles;
+    const size_t ts_off = blob.put_vec(tst);
+
+    std::vector<Launch> L;
+    std::vector<std::shared_ptr<void>> keep;
+    std::vector<BlurReq> r1((size_t)n), r2((size_t)n);
+    for (int j = 0; j < n; ++j) {
+        r1[j] = BlurReq{imgs[j].src, imgs[j].src_dt, Fp + imgs[j].off, imgs[j].g, ss[j]};
+        r2[j] = BlurReq{imgs[j].src, imgs[j].src_dt, Fp + imgs[j].off, imgs[j].g, so[j]};
+    }
+    // spots = detect_maxima_box(gaussian_blur(src, spot_sigma))
+    CLE_TRY(plan_blur_batch(r1, blob, L, s));
+    L.push_back([=](const char* dev) -> int {
+        k_maxima_batch<<<(unsigned)vblocks, 256, 0, s>>>((const BImg*)(dev + imgs_off), (const int64_t*)(dev + vs_off),
+                                                         n, Fp, Sp);
+        CLE_LAUNCHED();
+        return CLE_OK;
+    });
+    // segmentation = threshold_otsu(gaussian_blur(src, outline_sigma)); seeds = spots AND segmentation
+    CLE_TRY(plan_blur_batch(r2, blob, L, s));
+    CLE_TRY(plan_otsu_batch(imgs, imgs_off, vs_off, vblocks, Fp, mm, thr, ou, blob, L, s));
+    L.push_back([=](const char* dev) -> int {
+        k_seg_and_batch<<<(unsigned)vblocks, 256, 0, s>>>((const BImg*)(dev + imgs_off),
+                                                          (const int64_t*)(dev + vs_off), n, Fp, thr, Sp, Gp);
+        CLE_LAUNCHED();
+        return CLE_OK;
+    });
+    // seed labels (float32, into F), then A = F = (segmentation - 1) + labels
+    CLE_TRY(plan_ccl_batch(imgs, imgs_off, csz, Sp, Fp, blob, L, keep, s));
+    uint8_t* tfp = tf.get<uint8_t>();
+    if (ntiles > 0)
+        L.push_back([=](const char* dev) -> int {
+            k_vor_init_batch<<<(unsigned)ntiles, VB, 0, s>>>((const BImg*)(dev + imgs_off),
+                                                             (const int64_t*)(dev + ts_off), n, Gp, Ap, Fp, tfp,
+                                                             ntiles);
+            CLE_LAUNCHED();
+            return CLE_OK;
+        });
+
+    // the tables go to the device with one copy
+    Scratch dblob;
+    CLE_TRY(dblob.alloc(blob.h.size(), s));
+    PinnedStage pin;
+    pin.s = s;
+    CLE_HIP(hipHostMalloc(&pin.p, blob.h.size()));
+    memcpy(pin.p, blob.h.data(), blob.h.size());
+    CLE_HIP(hipMemcpyAsync(dblob.p, pin.p, blob.h.size(), hipMemcpyHostToDevice, s));
+    const char* dev = dblob.get<char>();
+    for (auto& l : L) CLE_TRY(l(dev));
+
+    // growth: all images together, VOR_BATCH steps between host checks
+    const BImg* dimgs = (const BImg*)(dev + imgs_off);
+    const int64_t* dts = (const int64_t*)(dev + ts_off);
+    int t = 0, steps = -1;
+    uint32_t hany[VOR_BATCH];
+    while (ntiles > 0 && steps < 0) {
+        CLE_TRY(fill_u32(gflags + (size_t)(t % VOR_RING) * n, 0u, (size_t)VOR_BATCH * n, s));
+        CLE_TRY(fill_u32(gany + t % VOR_RING, 0u, VOR_BATCH, s));
+        for (int k = 0; k < VOR_BATCH; ++k) {
+            const int it = t + k;
+            const float* src = (it & 1) ? Fp : Ap;
+            float* dst = (it & 1) ? Ap : Fp;
+            k_vor_step_batch<<<(unsigned)ntiles, VB, 0, s>>>(dimgs, dts, n, src, dst, it, tf.get<uint8_t>(), ntiles,
+                                                             gflags, gany, VOR_RING);
+        }
+        CLE_LAUNCHED();
+        CLE_HIP(hipMemcpyAsync(hany, gany + t % VOR_RING, sizeof(hany), hipMemcpyDeviceToHost, s));
+        CLE_HIP(hipStreamSynchronize(s));
+        for (int k = 0; k < VOR_BATCH; ++k)
+            if (hany[k] == 0) { steps = t + k + 1; break; }
+        t += VOR_BATCH;
+    }
+    if (n_steps) *n_steps = steps;
+    k_mask_out_batch<<<(unsigned)vblocks, 256, 0, s>>>(dimgs, (const int64_t*)(dev + vs_off), n, Ap, Gp);
+    CLE_LAUNCHED();
+    // the pinned tables and the scratch are released once the work has run
+    CLE_HIP(hipStreamSynchronize(s));
+    return CLE_OK;
+}
+
 }  // namespace cle
 
 using namespace cle;
@@ -280,6 +519,14 @@ int cle_voronoi_otsu_labeling(const cle_buf* src, const cle_buf* dst, float spot
     CLE_TRY(a.alloc((size_t)n * 4, s));
     return masked_voronoi_core(DBuf{spots.p, CLE_U8}, DBuf{seg.p, CLE_U8}, dbuf(dst), g, a.get<float>(),
                                blurred.get<float>(), nullptr, s);
+}
+
+int cle_voronoi_otsu_labeling_batch(const cle_buf* srcs, const cle_buf* dsts, const float* spot_sigmas,
+                                    const float* outline_sigmas, int n_images, int* n_steps, void* stream) {
+    if (n_images < 0) return fail(CLE_E_VALUE, "n_images must be >= 0");
+    if (n_images == 0) return CLE_OK;
+    if (!srcs || !dsts || !spot_sigmas || !outline_sigmas) return fail(CLE_E_VALUE, "null batch argument");
+    return voronoi_otsu_batch(srcs, dsts, spot_sigmas, outline_sigmas, n_images, n_steps, (hipStream_t)stream);
 }
 
 int cle_gauss_otsu_labeling(const cle_buf* src, const cle_buf* dst, float outline_sigma, void* stream) {

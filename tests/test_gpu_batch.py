@@ -1,0 +1,64 @@
+"""Batched voronoi_otsu_labeling: every image of a mixed batch (2D and 3D, several
+sizes, source types and sigmas) must equal the single-image device path bit for bit,
+and small members must equal the oracle's pipeline fed with the same device blurs."""
+import numpy as np
+import pytest
+
+from conftest import blobs_image
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_batch(rng, cle):
+    shapes = [(96, 128), (64, 64), (300, 257), (17, 33, 65), (12, 40, 70), (1, 50, 60), (200, 200), (9, 64, 64),
+              (130, 70), (5, 20, 300)]
+    imgs, ss, so = [], [], []
+    for k, sh in enumerate(shapes):
+        a = blobs_image(sh, 6 + k % 5, 100 + k).astype(np.float32)
+        a = (a * 40 + rng.normal(0, 3, sh)).clip(0, 65535)
+        if k % 3 == 0:
+            imgs.append(cle.asarray(a.astype(np.uint16)))     # native uint16 source
+        else:
+            imgs.append(cle.push(a))                          # float32, as push gives it
+        ss.append(float(rng.uniform(1.0, 5.0)))
+        so.append(float(rng.uniform(0.5, 2.0)))
+    ss[2] = 4.6   # 37 taps: wider than the isotropic kernel, runs the general path
+    return imgs, ss, so
+
+
+def test_batch_equals_single_image_path(cle):
+    rng = np.random.default_rng(50)
+    imgs, ss, so = _mixed_batch(rng, cle)
+    out, steps = cle.voronoi_otsu_labeling_batch(imgs, ss, so, return_steps=True)
+    assert steps > 0
+    for k, (im, a, b) in enumerate(zip(imgs, ss, so)):
+        want = cle.pull(cle.voronoi_otsu_labeling(im, None, a, b))
+        got = cle.pull(out[k])
+        assert got.dtype == np.uint32 and got.shape == want.shape
+        assert np.array_equal(got, want), (k, im.shape, a, b)
+        assert got.max() > 0, k
+
+
+def test_batch_single_member_and_repeat(cle):
+    """A batch of one equals the single call; running a batch twice gives the same."""
+    rng = np.random.default_rng(51)
+    a = cle.push((blobs_image((40, 60, 50), 8, 7) * 30).astype(np.float32))
+    one = cle.pull(cle.voronoi_otsu_labeling_batch([a], 2.0, 1.0)[0])
+    assert np.array_equal(one, cle.pull(cle.voronoi_otsu_labeling(a, None, 2.0, 1.0)))
+    imgs, ss, so = _mixed_batch(rng, cle)
+    r1 = [cle.pull(x) for x in cle.voronoi_otsu_labeling_batch(imgs, ss, so)]
+    r2 = [cle.pull(x) for x in cle.voronoi_otsu_labeling_batch(imgs, ss, so)]
+    assert all(np.array_equal(x, y) for x, y in zip(r1, r2))
+
+
+def test_batch_large_member_uses_row_histogram(cle):
+    """A member of 2^24 voxels takes the per-row histogram of the single-image Otsu."""
+    import torch
+    from pyclesperanto_prototype_amd import synthetic
+    big, _ = synthetic.nuclei_3d((64, 512, 512), 400, seed=52)
+    small = cle.asarray((blobs_image((80, 90), 5, 3) * 50).astype(np.uint16))
+    imgs = [cle.asarray(big), small]
+    out = cle.voronoi_otsu_labeling_batch(imgs, [2.0, 1.5], [1.5, 1.0])
+    for k, (a, b) in enumerate([(2.0, 1.5), (1.5, 1.0)]):
+        want = cle.voronoi_otsu_labeling(imgs[k], None, a, b)
+        assert torch.equal(out[k].tensor, want.tensor), k

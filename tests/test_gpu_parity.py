@@ -6,7 +6,7 @@ Integer / binary / label outputs must be bit-exact; float32 blur within max-abs
 import numpy as np
 import pytest
 
-from conftest import load_npz
+from conftest import blobs_image, load_npz
 from oracle import cle_oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -15,35 +15,11 @@ BLUR_TOL = 1e-4    # max |gpu - oracle| <= BLUR_TOL * max(1, max |oracle|)
 STATS_RTOL = 1e-5
 
 
-@pytest.fixture(scope="module")
-def cle():
-    import torch
-    if not torch.cuda.is_available():
-        pytest.skip("no GPU")
-    import pyclesperanto_prototype_amd as cle
-    cle._load_library()
-    return cle
-
-
 def assert_blur_close(got, want):
     got, want = np.asarray(got, np.float64), np.asarray(want, np.float64)
     assert got.shape == want.shape
     err = np.abs(got - want).max() if got.size else 0.0
     assert err <= BLUR_TOL * max(1.0, np.abs(want).max()), f"max abs err {err}"
-
-
-def blobs_image(shape, n, seed, sig=(3, 6), amp=(100, 200), noise=5.0, dtype=np.uint8):
-    rng = np.random.default_rng(seed)
-    grids = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
-    img = np.full(shape, 10.0)
-    for _ in range(n):
-        c = [rng.uniform(0, s) for s in shape]
-        s = rng.uniform(*sig)
-        r2 = sum((g - ci) ** 2 for g, ci in zip(grids, c))
-        img += rng.uniform(*amp) * np.exp(-r2 / (2 * s * s))
-    img += rng.normal(0, noise, shape)
-    hi = 255 if dtype == np.uint8 else 65535
-    return np.clip(img, 0, hi).astype(dtype)
 
 
 # ----------------------------------------------------------------- reference test vectors
