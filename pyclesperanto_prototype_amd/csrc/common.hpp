@@ -166,31 +166,50 @@ __device__ __forceinline__ bool nz(const DBuf b, int64_t i) {
     }
 }
 
-// detect_maxima_box with radius 0 at voxel i: the neighbourhood is visited x outer,
-// then y, then z inner, starting from value(i) - 1, and a strictly larger value takes
-// over; the voxel is a maximum if the last take-over was its own value
-// (_tier1/detect_maxima_{2d,3d}_x.cl semantics; out-of-range coordinates are skipped
-// below 0 and clamped above the last index).
-template <typename LD>
-__device__ __forceinline__ bool maxima_voxel(const LD& in, const Geom& g, int64_t i) {
+// detect_maxima_box with radius 0 on row (z, y): the neighbourhood of voxel x is
+// visited x outer, then y, then z inner, starting from value - 1, and a strictly
+// larger value takes over; the voxel is a maximum if the last take-over was its own
+// value (_tier1/detect_maxima_{2d,3d}_x.cl semantics; coordinates below 0 are
+// skipped, above the last index clamped).  The 3 x 3 neighbour row offsets are
+// formed once per row; out(x, is_max) stores the flag.
+template <typename LD, typename OUT>
+__device__ __forceinline__ void maxima_row(const LD& in, const Geom& g, int64_t z, int64_t y, int tid, int nth,
+                                           const OUT& out) {
     const int zr = g.ndim == 3 ? 1 : 0;
-    const int64_t x = i % g.W, y = (i / g.W) % g.H, z = i / (g.W * g.H);
-    float lm = in(i) - 1.f;
-    bool self = true;
-    for (int dx = -1; dx <= 1; ++dx)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dz = -zr; dz <= zr; ++dz) {
-                const int64_t qx = x + dx, qy = y + dy, qz = z + dz;
-                if (qx < 0 || qy < 0 || qz < 0) continue;
-                const int64_t q = ((qz > g.D - 1 ? g.D - 1 : qz) * g.H + (qy > g.H - 1 ? g.H - 1 : qy)) * g.W +
-                                  (qx > g.W - 1 ? g.W - 1 : qx);
-                const float w = in(q);
-                if (w > lm) {
-                    lm = w;
-                    self = (dx == 0 && dy == 0 && dz == 0);
+    int64_t rb[3][3];
+    bool rv[3][3];
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int64_t qy = y + dy, qz = z + dz;
+            rv[dy + 1][dz + 1] = qy >= 0 && qz >= 0 && (dz == 0 || zr);
+            const int64_t cy = qy > g.H - 1 ? g.H - 1 : qy, cz = qz > g.D - 1 ? g.D - 1 : qz;
+            rb[dy + 1][dz + 1] = (cz * g.H + cy) * g.W;
+        }
+    const int64_t self_row = (z * g.H + y) * g.W;
+    for (int64_t x = tid; x < g.W; x += nth) {
+        float lm = in(self_row + x) - 1.f;
+        bool self = true;
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int64_t qx0 = x + dx;
+            if (qx0 < 0) continue;
+            const int64_t qx = qx0 > g.W - 1 ? g.W - 1 : qx0;
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int dz = 0; dz < 3; ++dz) {
+                    if (!rv[dy][dz]) continue;
+                    const float w = in(rb[dy][dz] + qx);
+                    if (w > lm) {
+                        lm = w;
+                        self = (dx == 0 && dy == 1 && dz == 1);
+                    }
                 }
-            }
-    return self;
+        }
+        out(x, self);
+    }
 }
 
 // ----------------------------------------------------------------- batches

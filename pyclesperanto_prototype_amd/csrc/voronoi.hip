@@ -51,7 +51,7 @@ __device__ __forceinline__ int64_t clampi(int64_t v, int64_t hi) { return v < 0 
 // "changed" at steps -1 and -2 (slots 3 and 2 of the 4-slot ring).
 // tile = index inside the image, gtile = its slot in the flag arrays of ntiles slots
 template <typename MK>
-__device__ __forceinline__ void vor_init_tile(MK mask, float* A, float* B, const Geom& g, const Tiles& T,
+__device__ __forceinline__ bool vor_init_tile(MK mask, float* A, float* B, const Geom& g, const Tiles& T,
                                               int64_t tile, int64_t gtile, int64_t ntiles, uint8_t* tflags) {
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
     bool pos = false;
@@ -68,13 +68,14 @@ __device__ __forceinline__ void vor_init_tile(MK mask, float* A, float* B, const
         pos |= v > 0.f;
     }
     const int any = __syncthreads_or(pos ? 1 : 0);
-    if (threadIdx.x == 0) {
+    if (tflags && threadIdx.x == 0) {
         const uint8_t f = any ? 1 : 0;
         tflags[0 * ntiles + gtile] = 0;
         tflags[1 * ntiles + gtile] = 0;
         tflags[2 * ntiles + gtile] = f;
         tflags[3 * ntiles + gtile] = f;
     }
+    return any != 0;
 }
 
 struct MaskAny {
@@ -93,26 +94,10 @@ __global__ __launch_bounds__(VB) void k_vor_init(DBuf mask, float* A, float* B, 
 // one growth step t: src -> dst, box (t even) or diamond (t odd)
 // One growth step on one tile.  gflag / gany: "changed" flags of this step for the
 // image and for the whole launch (the same word for a single image).
-__device__ __forceinline__ void vor_step_tile(const float* __restrict__ src, float* __restrict__ dst, const Geom& g,
-                                              const Tiles& T, int64_t tile, int64_t tbase, int64_t ntiles, int t,
-                                              uint8_t* tflags, uint32_t* gflag, uint32_t* gany) {
-    const int64_t gtile = tbase + tile;
+// step t on one tile (src -> dst); returns, in every thread, whether the tile changed
+__device__ __forceinline__ bool vor_tile_update(const float* __restrict__ src, float* __restrict__ dst, const Geom& g,
+                                                const Tiles& T, int64_t tile, int t) {
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
-    if (threadIdx.x == 0) tflags[((t + 1) & 3) * ntiles + gtile] = 0;
-    // activity: any tile of the 3^d tile neighbourhood changed at t-1 or t-2
-    const uint8_t* f1 = tflags + ((t + 3) & 3) * ntiles;
-    const uint8_t* f2 = tflags + ((t + 2) & 3) * ntiles;
-    bool act = false;
-    if (threadIdx.x < 27) {
-        const int dx = threadIdx.x % 3 - 1, dy = (threadIdx.x / 3) % 3 - 1, dz = threadIdx.x / 9 - 1;
-        const int64_t qx = tx + dx, qy = ty + dy, qz = tz + dz;
-        if (qx >= 0 && qx < T.nx && qy >= 0 && qy < T.ny && qz >= 0 && qz < T.nz) {
-            const int64_t q = tbase + (qz * T.ny + qy) * T.nx + qx;
-            act = f1[q] | f2[q];
-        }
-    }
-    if (!__syncthreads_or(act ? 1 : 0)) return;
-
     const bool diamond = (t & 1) != 0;
     const int zr = g.ndim == 3 ? 1 : 0;
     bool changed = false;
@@ -156,7 +141,29 @@ __device__ __forceinline__ void vor_step_tile(const float* __restrict__ src, flo
         dst[i] = out;
         changed |= (out != v);
     }
-    if (__syncthreads_or(changed ? 1 : 0) && threadIdx.x == 0) {
+    return __syncthreads_or(changed ? 1 : 0) != 0;
+}
+
+__device__ __forceinline__ void vor_step_tile(const float* __restrict__ src, float* __restrict__ dst, const Geom& g,
+                                              const Tiles& T, int64_t tile, int64_t tbase, int64_t ntiles, int t,
+                                              uint8_t* tflags, uint32_t* gflag, uint32_t* gany) {
+    const int64_t gtile = tbase + tile;
+    const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
+    if (threadIdx.x == 0) tflags[((t + 1) & 3) * ntiles + gtile] = 0;
+    // activity: any tile of the 3^d tile neighbourhood changed at t-1 or t-2
+    const uint8_t* f1 = tflags + ((t + 3) & 3) * ntiles;
+    const uint8_t* f2 = tflags + ((t + 2) & 3) * ntiles;
+    bool act = false;
+    if (threadIdx.x < 27) {
+        const int dx = threadIdx.x % 3 - 1, dy = (threadIdx.x / 3) % 3 - 1, dz = threadIdx.x / 9 - 1;
+        const int64_t qx = tx + dx, qy = ty + dy, qz = tz + dz;
+        if (qx >= 0 && qx < T.nx && qy >= 0 && qy < T.ny && qz >= 0 && qz < T.nz) {
+            const int64_t q = tbase + (qz * T.ny + qy) * T.nx + qx;
+            act = f1[q] | f2[q];
+        }
+    }
+    if (!__syncthreads_or(act ? 1 : 0)) return;
+    if (vor_tile_update(src, dst, g, T, tile, t) && threadIdx.x == 0) {
         tflags[(t & 3) * ntiles + gtile] = 1;
         *gflag = 1;
         *gany = 1;
@@ -258,14 +265,16 @@ struct F32In {
     __device__ __forceinline__ float operator()(int64_t i) const { return p[i]; }
 };
 
-__global__ __launch_bounds__(256) void k_maxima_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ vs,
+// one block per row of one image (rs = row-prefix table)
+__global__ __launch_bounds__(256) void k_maxima_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ rs,
                                                      int nimg, const float* F, uint8_t* S) {
-    const int64_t b = blockIdx.x;
-    const int j = find_seg(vs, nimg, b);
+    const int64_t r = blockIdx.x;
+    const int j = find_seg(rs, nimg, r);
     const BImg& im = imgs[j];
-    const int64_t n = im.g.n(), i0 = (b - vs[j]) * BV, i1 = i0 + BV < n ? i0 + BV : n;
-    const F32In in{F + im.off};
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) S[im.off + i] = maxima_voxel(in, im.g, i) ? 1 : 0;
+    const int64_t row = r - rs[j];
+    const int64_t z = row / im.g.H, y = row - z * im.g.H;
+    uint8_t* o = S + im.off + row * im.g.W;
+    maxima_row(F32In{F + im.off}, im.g, z, y, threadIdx.x, 256, [&](int64_t x, bool m) { o[x] = m ? 1 : 0; });
 }
 
 // G = F > float32(thr[j]); S = S AND G
@@ -284,27 +293,83 @@ __global__ __launch_bounds__(256) void k_seg_and_batch(const BImg* __restrict__ 
     }
 }
 
+// Work lists of the batched growth.  List t % 3 holds the tiles active at step t: the
+// 3^d tile neighbourhood of a tile changed at step t-1 or t-2 (at t = 0, 1: holds a
+// positive initial value).  A tile that changes at step t queues its neighbours for
+// t + 1 and t + 2; stamp[t & 1][tile] = last step the tile was queued for keeps every
+// list free of duplicates.  A step launch walks its list with a fixed grid, so its
+// cost follows the growth front instead of the volume.
+struct VorLists {
+    uint32_t* items;  // 3 lists of cap entries (global tile indices)
+    uint32_t* count;  // 3 counters
+    int32_t* stamp;   // 2 x cap
+    int64_t cap;      // tiles of all images
+};
+
+__device__ __forceinline__ uint64_t lanes_below_v() {
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// wavefront 0 of the block queues the neighbours of `tile` (inside its image) for
+// steps s1 and s2
+__device__ __forceinline__ void vor_queue(const Tiles& T, int64_t tile, int64_t tbase, int s1, int s2,
+                                          const VorLists& L) {
+    const int lane = threadIdx.x;
+    int64_t q = -1;
+    if (lane < 27) {
+        const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
+        const int64_t qx = tx + lane % 3 - 1, qy = ty + (lane / 3) % 3 - 1, qz = tz + lane / 9 - 1;
+        if (qx >= 0 && qx < T.nx && qy >= 0 && qy < T.ny && qz >= 0 && qz < T.nz)
+            q = tbase + (qz * T.ny + qy) * T.nx + qx;
+    }
+    for (int w = 0; w < 2; ++w) {
+        const int st = w ? s2 : s1;
+        const bool need = q >= 0 && atomicMax(&L.stamp[(int64_t)(st & 1) * L.cap + q], st) < st;
+        const uint64_t m = __ballot(need);
+        uint32_t base = 0;
+        if (lane == 0 && m) base = atomicAdd(&L.count[st % 3], (uint32_t)__popcll(m));
+        base = __shfl(base, 0);
+        if (need) L.items[(int64_t)(st % 3) * L.cap + base + __popcll(m & lanes_below_v())] = (uint32_t)q;
+    }
+}
+
+// A = B = (G - 1) + labels; tiles holding a positive value queue their neighbourhood
+// for steps 0 and 1
 __global__ __launch_bounds__(VB) void k_vor_init_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ ts,
-                                                      int nimg, const uint8_t* G, float* A, float* B,
-                                                      uint8_t* tflags, int64_t ntiles) {
+                                                      int nimg, const uint8_t* G, float* A, float* B, VorLists L) {
     const int64_t b = blockIdx.x;
     const int j = find_seg(ts, nimg, b);
     const BImg& im = imgs[j];
-    vor_init_tile(MaskU8{G + im.off}, A + im.off, B + im.off, im.g, make_tiles(im.g), b - ts[j], b, ntiles, tflags);
+    const Tiles T = make_tiles(im.g);
+    if (vor_init_tile(MaskU8{G + im.off}, A + im.off, B + im.off, im.g, T, b - ts[j], b, 0, nullptr) &&
+        threadIdx.x < 64)
+        vor_queue(T, b - ts[j], ts[j], 0, 1, L);
 }
 
 __global__ __launch_bounds__(VB) void k_vor_step_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ ts,
                                                       int nimg, const float* __restrict__ src,
-                                                      float* __restrict__ dst, int t, uint8_t* tflags,
-                                                      int64_t ntiles, uint32_t* gflags, uint32_t* gany,
-                                                      int ring) {
-    const int64_t b = blockIdx.x;
-    const int j = find_seg(ts, nimg, b);
-    if (t > 0 && gflags[(int64_t)((t - 1) % ring) * nimg + j] == 0) return;  // this image has converged
-    const BImg& im = imgs[j];
-    vor_step_tile(src + im.off, dst + im.off, im.g, make_tiles(im.g), b - ts[j], ts[j], ntiles, t, tflags,
-                  gflags + (int64_t)(t % ring) * nimg + j, gany + t % ring);
+                                                      float* __restrict__ dst, int t, VorLists L, uint32_t* gflags,
+                                                      uint32_t* gany, int ring) {
+    const uint32_t n = L.count[t % 3];
+    const uint32_t* items = L.items + (int64_t)(t % 3) * L.cap;
+    for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
+        const int64_t q = items[e];
+        const int j = find_seg(ts, nimg, q);
+        if (t > 0 && gflags[(int64_t)((t - 1) % ring) * nimg + j] == 0) continue;  // this image has converged
+        const BImg& im = imgs[j];
+        const Tiles T = make_tiles(im.g);
+        if (vor_tile_update(src + im.off, dst + im.off, im.g, T, q - ts[j], t)) {
+            if (threadIdx.x == 0) {
+                gflags[(int64_t)(t % ring) * nimg + j] = 1;
+                gany[t % ring] = 1;
+            }
+            if (threadIdx.x < 64) vor_queue(T, q - ts[j], ts[j], t + 1, t + 2, L);
+        }
+    }
 }
+
+__global__ void k_vor_list_reset(uint32_t* count, int which) { count[which] = 0; }
 
 __global__ __launch_bounds__(256) void k_mask_out_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ vs,
                                                        int nimg, const float* A, const uint8_t* G) {
@@ -366,7 +431,13 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     float* mm = aux.get<float>();
     double* thr = (double*)(aux.get<char>() + (size_t)n * 8);
     uint32_t* ou = (uint32_t*)(aux.get<char>() + (size_t)n * 16);
-    CLE_TRY(tf.alloc((size_t)(ntiles > 0 ? ntiles : 1) * 4, s));
+    const int64_t tcap = ntiles > 0 ? ntiles : 1;
+    CLE_TRY(tf.alloc((size_t)tcap * 3 * 4 + 16 + (size_t)tcap * 2 * 4, s));
+    VorLists VL;
+    VL.items = tf.get<uint32_t>();
+    VL.count = VL.items + (size_t)tcap * 3;
+    VL.stamp = (int32_t*)(VL.count + 4);
+    VL.cap = tcap;
     CLE_TRY(gf.alloc(((size_t)VOR_RING * n + VOR_RING) * 4, s));
     uint32_t* gflags = gf.get<uint32_t>();
     uint32_t* gany = gflags + (size_t)VOR_RING * n;
@@ -384,6 +455,14 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     for (int j = 0; j < n; ++j) tst[j] = imgs[j].tbase;
     tst[n] = ntiles;
     const size_t ts_off = blob.put_vec(tst);
+    std::vector<int64_t> rst((size_t)n + 1);
+    int64_t nrows = 0;
+    for (int j = 0; j < n; ++j) {
+        rst[j] = nrows;
+        nrows += imgs[j].g.n() > 0 ? imgs[j].g.D * imgs[j].g.H : 0;
+    }
+    rst[n] = nrows;
+    const size_t rs_off = blob.put_vec(rst);
 
     std::vector<Launch> L;
     std::vector<std::shared_ptr<void>> keep;
@@ -395,8 +474,8 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     // spots = detect_maxima_box(gaussian_blur(src, spot_sigma))
     CLE_TRY(plan_blur_batch(r1, blob, L, s));
     L.push_back([=](const char* dev) -> int {
-        k_maxima_batch<<<(unsigned)vblocks, 256, 0, s>>>((const BImg*)(dev + imgs_off), (const int64_t*)(dev + vs_off),
-                                                         n, Fp, Sp);
+        k_maxima_batch<<<(unsigned)nrows, 256, 0, s>>>((const BImg*)(dev + imgs_off), (const int64_t*)(dev + rs_off),
+                                                       n, Fp, Sp);
         CLE_LAUNCHED();
         return CLE_OK;
     });
@@ -411,12 +490,12 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     });
     // seed labels (float32, into F), then A = F = (segmentation - 1) + labels
     CLE_TRY(plan_ccl_batch(imgs, imgs_off, csz, Sp, Fp, blob, L, keep, s));
-    uint8_t* tfp = tf.get<uint8_t>();
     if (ntiles > 0)
         L.push_back([=](const char* dev) -> int {
+            CLE_TRY(fill_u32(VL.count, 0u, 3, s));
+            CLE_TRY(fill_u32((uint32_t*)VL.stamp, 0xFFFFFFFFu, (size_t)VL.cap * 2, s));
             k_vor_init_batch<<<(unsigned)ntiles, VB, 0, s>>>((const BImg*)(dev + imgs_off),
-                                                             (const int64_t*)(dev + ts_off), n, Gp, Ap, Fp, tfp,
-                                                             ntiles);
+                                                             (const int64_t*)(dev + ts_off), n, Gp, Ap, Fp, VL);
             CLE_LAUNCHED();
             return CLE_OK;
         });
@@ -437,6 +516,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     const int64_t* dts = (const int64_t*)(dev + ts_off);
     int t = 0, steps = -1;
     uint32_t hany[VOR_BATCH];
+    const unsigned vgrid = (unsigned)(ntiles < 8192 ? (ntiles > 0 ? ntiles : 1) : 8192);
     while (ntiles > 0 && steps < 0) {
         CLE_TRY(fill_u32(gflags + (size_t)(t % VOR_RING) * n, 0u, (size_t)VOR_BATCH * n, s));
         CLE_TRY(fill_u32(gany + t % VOR_RING, 0u, VOR_BATCH, s));
@@ -444,8 +524,8 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
             const int it = t + k;
             const float* src = (it & 1) ? Fp : Ap;
             float* dst = (it & 1) ? Ap : Fp;
-            k_vor_step_batch<<<(unsigned)ntiles, VB, 0, s>>>(dimgs, dts, n, src, dst, it, tf.get<uint8_t>(), ntiles,
-                                                             gflags, gany, VOR_RING);
+            k_vor_list_reset<<<1, 1, 0, s>>>(VL.count, (it + 2) % 3);  // consumed by step it - 1
+            k_vor_step_batch<<<vgrid, VB, 0, s>>>(dimgs, dts, n, src, dst, it, VL, gflags, gany, VOR_RING);
         }
         CLE_LAUNCHED();
         CLE_HIP(hipMemcpyAsync(hany, gany + t % VOR_RING, sizeof(hany), hipMemcpyDeviceToHost, s));

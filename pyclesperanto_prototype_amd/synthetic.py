@@ -6,6 +6,10 @@ nuclei_3d    3D uint16 nuclei stack: non-overlapping randomly oriented ellipsoid
              uniform intensity on a constant background, then Poisson shot noise and
              Gaussian read noise, rounded and clipped to uint16.  Rendered on the
              device for the large volumes.
+mixed_batch  many uint16 nuclei images of mixed sizes (70 % 2D log-uniform 64^2 ..
+             1024^2, 30 % 3D log-uniform 32x128x128 .. 96x512x512), densities uniform
+             in 1e-4 .. 1e-3 nuclei per voxel (overlap allowed), per-image spot sigma
+             uniform in 1 .. 5 and outline sigma in 0.5 .. 2.
 """
 from __future__ import annotations
 
@@ -145,3 +149,111 @@ def nuclei_3d(shape=(256, 1024, 1024), n_nuclei=5000, semi=(4.0, 10.0), intensit
     out = torch.clamp(torch.round(noisy), 0, 65535).to(torch.int32).to(torch.int16).view(torch.uint16)
     del clean, noisy
     return out, {"nuclei": n}
+
+
+def _render_nuclei(shape, centres, axes, rot, values, background, read_noise, seed, device, chunk=1024):
+    """Ellipsoids (ellipses for a 2-axis shape) of uniform values on a background, then
+    Poisson shot noise and Gaussian read noise, as a uint16 tensor on `device`."""
+    import torch
+
+    nd = len(shape)
+    clean = torch.full(shape, float(background), dtype=torch.float32, device=device)
+    R = int(math.ceil(float(np.max(axes)) if len(axes) else 1.0))
+    o = torch.arange(-R, R + 1, device=device, dtype=torch.float32)
+    grids = torch.meshgrid(*([o] * nd), indexing="ij")
+    offs = torch.stack([g.reshape(-1) for g in grids], 1)  # (P, nd)
+    n = len(centres)
+    for s0 in range(0, n, chunk):
+        e = min(n, s0 + chunk)
+        c = torch.tensor(centres[s0:e], dtype=torch.float32, device=device)
+        a = torch.tensor(axes[s0:e], dtype=torch.float32, device=device)
+        rt = torch.tensor(rot[s0:e], dtype=torch.float32, device=device)
+        v = torch.tensor(values[s0:e], dtype=torch.float32, device=device)
+        pos = torch.round(c)[:, None, :] + offs[None]
+        rel = pos - c[:, None, :]
+        loc = torch.matmul(rel, rt.contiguous())  # rel @ R = (R^T rel^T)^T: coordinates in the body frame
+        inside = ((loc / a[:, None, :]) ** 2).sum(-1) <= 1.0
+        inb = torch.ones_like(inside)
+        for d in range(nd):
+            inb &= (pos[..., d] >= 0) & (pos[..., d] < shape[d])
+        m = inside & inb
+        p = pos[m].long()
+        lin = p[:, 0]
+        for d in range(1, nd):
+            lin = lin * shape[d] + p[:, d]
+        clean.view(-1)[lin] = v[:, None].expand(-1, offs.shape[0])[m]
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    noisy = torch.poisson(clean, generator=gen)
+    noisy += torch.randn(shape, generator=gen, device=device) * read_noise
+    out = torch.clamp(torch.round(noisy), 0, 65535).to(torch.int32).to(torch.int16).view(torch.uint16)
+    del clean, noisy
+    return out
+
+
+def _random_rotations_2d(rng, n):
+    t = rng.uniform(0, 2 * np.pi, n)
+    R = np.empty((n, 2, 2))
+    R[:, 0, 0] = np.cos(t); R[:, 0, 1] = -np.sin(t); R[:, 1, 0] = np.sin(t); R[:, 1, 1] = np.cos(t)
+    return R
+
+
+def nuclei_overlapping(shape, density, semi=(4.0, 10.0), intensity=(800.0, 3000.0), background=200.0,
+                       read_noise=20.0, seed=4, device=None):
+    """uint16 nuclei image (2D or 3D) with `density` nuclei per voxel placed uniformly
+    at random, overlaps allowed (later nuclei overwrite earlier ones)."""
+    import torch
+
+    if device is None:
+        device = torch.device("cuda", torch.cuda.current_device())
+    rng = np.random.default_rng(seed)
+    nd = len(shape)
+    n = int(round(density * float(np.prod(shape))))
+    centres = rng.uniform(0, 1, (n, nd)) * np.array(shape, np.float64)
+    axes = rng.uniform(semi[0], semi[1], (n, nd))
+    rot = _random_rotations(rng, n) if nd == 3 else _random_rotations_2d(rng, n)
+    values = rng.uniform(*intensity, n)
+    return _render_nuclei(tuple(shape), centres, axes, rot, values, background, read_noise, seed, device), n
+
+
+def mixed_batch(n_images=2000, seed=4, device=None, levels=8):
+    """The mixed-size batch: returns (images, spot_sigmas, outline_sigmas, info).
+
+    Images are crops of one source image per density level (2D sources 1024 x 1024,
+    3D sources 96 x 512 x 512, `levels` densities evenly spaced over 1e-4 .. 1e-3), so
+    the whole set is rendered in a few seconds; each image gets its own size, crop
+    offset and sigmas.
+    """
+    import torch
+
+    if device is None:
+        device = torch.device("cuda", torch.cuda.current_device())
+    rng = np.random.default_rng(seed)
+    dens = np.linspace(1e-4, 1e-3, levels)
+    src2 = [nuclei_overlapping((1024, 1024), d, seed=seed * 100 + k, device=device)[0] for k, d in enumerate(dens)]
+    src3 = [nuclei_overlapping((96, 512, 512), d, seed=seed * 100 + 50 + k, device=device)[0]
+            for k, d in enumerate(dens)]
+    n2 = int(round(0.7 * n_images))
+    kinds = np.array([2] * n2 + [3] * (n_images - n2))
+    rng.shuffle(kinds)
+    images, ss, so, nvox = [], [], [], 0
+    for kind in kinds:
+        d = rng.uniform(1e-4, 1e-3)
+        lvl = int(np.argmin(np.abs(dens - d)))
+        if kind == 2:
+            side = int(round(math.exp(rng.uniform(math.log(64), math.log(1024)))))
+            y0, x0 = rng.integers(0, 1024 - side + 1, 2)
+            img = src2[lvl][y0:y0 + side, x0:x0 + side].contiguous()
+        else:
+            t = rng.uniform(0, 1)
+            zs, xy = int(round(32 * 3 ** t)), int(round(128 * 4 ** t))
+            z0 = rng.integers(0, 96 - zs + 1)
+            y0, x0 = rng.integers(0, 512 - xy + 1, 2)
+            img = src3[lvl][z0:z0 + zs, y0:y0 + xy, x0:x0 + xy].contiguous()
+        images.append(img)
+        nvox += img.numel()
+        ss.append(float(rng.uniform(1.0, 5.0)))
+        so.append(float(rng.uniform(0.5, 2.0)))
+    del src2, src3
+    return images, np.array(ss, np.float32), np.array(so, np.float32), {
+        "images": n_images, "voxels": nvox, "2d": int(n2), "3d": int(n_images - n2), "density_levels": levels}

@@ -160,7 +160,7 @@ def test_mean_box_parity(cle):
     assert_blur_close(got, O.mean_box(a, 1, 2, 1))
 
 
-@pytest.mark.parametrize("shape", [(31, 45), (6, 17, 23), (1, 9, 9)])
+@pytest.mark.parametrize("shape", [(31, 45), (6, 17, 23), (1, 9, 9), (3, 5, 300), (1, 1), (700, 3)])
 def test_maxima_parity(cle, shape):
     rng = np.random.default_rng(5)
     a = rng.integers(0, 5, shape).astype(np.float32)  # plateaus and ties
@@ -273,7 +273,7 @@ def test_relabel_and_spots_parity(cle):
     assert np.array_equal(cle.pull(cle.label_spots(spots)), O.label_spots(spots))
 
 
-@pytest.mark.parametrize("shape", [(50, 70), (12, 40, 45)])
+@pytest.mark.parametrize("shape", [(50, 70), (12, 40, 45), (1, 40, 45), (9, 13, 150), (130, 70), (5, 3, 200)])
 def test_masked_voronoi_parity(cle, shape):
     rng = np.random.default_rng(12)
     seeds = (rng.random(shape) < 0.01).astype(np.uint8)
