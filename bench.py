@@ -26,6 +26,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+VALU_PEAK_TFLOPS = 78.6  # MI355X float32 vector FMA without packed math (2 flops x 64 lanes x 4 SIMD x 256 CU x 2.4 GHz)
 
 SHAPE = (256, 1024, 1024)
 N_NUCLEI = 5000
@@ -46,6 +47,9 @@ def parse():
     ap.add_argument("--cpu-slices", type=int, default=256, help="depth of the slab timed on the CPU")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB float32 cle_copy first (known bytes, to scale PMC byte counters)")
+    ap.add_argument("--batch-images", type=int, default=2000, help="mixed-size batch (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=400, help="serving trace length (0 = skip)")
+    ap.add_argument("--serve-qps", type=float, default=300.0)
     return ap.parse_args()
 
 
@@ -121,6 +125,24 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "gaussian_blur (u16 -> f32, 17 taps/axis)", "algorithmic_bytes_per_launch": blur_bytes}
 
+    traffic = measured_traffic()
+    if traffic:
+        roofline["traffic"] = traffic["bytes"]
+        roofline["traffic_source"] = traffic["source"]
+    # the blur is VALU-bound (17 taps per axis): its place under the float32 vector roof
+    flops = nvox * 3 * 17 * 2
+    roofline["valu"] = {"achieved_tflops": round(flops / blur_s / 1e12, 2), "peak_tflops": VALU_PEAK_TFLOPS,
+                        "frac": round(flops / blur_s / 1e12 / VALU_PEAK_TFLOPS, 4),
+                        "flops_per_voxel": 3 * 17 * 2}
+    # algorithmic bytes per stage: inputs read once, outputs written once at their dtypes
+    # (Otsu reads the float32 image for min/max, histogram and compare: 3 x 4 + 1)
+    stage_bytes = {"threshold_otsu": nvox * 13, "connected_components_labeling_box": nvox * 5,
+                   "statistics_of_labelled_pixels": nvox * (4 + 2)}
+    stage_roof = {k: round(b / (stage_ms[k] / 1e3) / 1e9 / HBM_PEAK_GBS, 3) for k, b in stage_bytes.items()}
+
+    batched = batch_section(cle, torch, args.batch_images) if args.batch_images > 0 else None
+    serving = serve_section(cle, torch, vol, args.serve_requests, args.serve_qps) if args.serve_requests > 0 else None
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(vol_t, args.cpu_slices)
@@ -142,11 +164,125 @@ def main():
         "config": {"workload": "plain method 3D: 256x1024x1024 uint16 nuclei -> gaussian_blur(2,2,2) f32 -> "
                                "threshold_otsu -> connected_components_labeling_box int32 -> "
                                "statistics_of_labelled_pixels",
-                   "voxels": nvox, "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()}},
+                   "voxels": nvox, "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
+                   "stage_hbm_frac": stage_roof},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per launch of the blur from the committed PMC summary (FETCH_SIZE and
+    WRITE_SIZE, each scaled by what the counter reads for a copy of known size)."""
+    f = ROOT / "profiles" / "gaussian_blur_u16_256x1024x1024_pmc.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return {"bytes": int(line.split(":")[1]), "source": str(f.relative_to(ROOT))}
+    return None
+
+
+def batch_section(cle, torch, n_images):
+    """Mixed-size batch (70 % 2D 64^2..1024^2, 30 % 3D 32x128x128..96x512x512, per-image
+    sigmas): the plain per-image loop of voronoi_otsu_labeling against one batched call."""
+    import numpy as np
+    from pyclesperanto_prototype_amd import synthetic
+
+    t0 = time.time()
+    imgs_t, ss, so, info = synthetic.mixed_batch(n_images, seed=4)
+    torch.cuda.synchronize()
+    log(f"mixed batch: {info} generated in {time.time() - t0:.1f}s")
+    imgs = [cle.asarray(t) for t in imgs_t]
+    plain_out = [cle.create(t.shape, np.uint32) for t in imgs_t]
+    batch_out = [cle.create(t.shape, np.uint32) for t in imgs_t]
+    for k in range(min(20, n_images)):  # warm-up of the per-image path
+        cle.voronoi_otsu_labeling(imgs[k], plain_out[k], float(ss[k]), float(so[k]))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(n_images):
+        cle.voronoi_otsu_labeling(imgs[k], plain_out[k], float(ss[k]), float(so[k]))
+    torch.cuda.synchronize()
+    plain = time.perf_counter() - t0
+    cle.voronoi_otsu_labeling_batch(imgs, ss, so, destinations=batch_out)  # warm-up (scratch growth)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, steps = cle.voronoi_otsu_labeling_batch(imgs, ss, so, destinations=batch_out, return_steps=True)
+    torch.cuda.synchronize()
+    batched = time.perf_counter() - t0
+    same = sum(int(torch.equal(a.tensor, b.tensor)) for a, b in zip(plain_out, batch_out))
+    nv = info["voxels"]
+    log(f"batched: plain {plain * 1e3:.1f} ms, batched {batched * 1e3:.1f} ms, identical {same}/{n_images}")
+    del imgs, imgs_t, plain_out, batch_out
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    cle.release_pool()
+    return {"workload": "voronoi_otsu_labeling of a mixed-size batch (70% 2D 64^2..1024^2, 30% 3D "
+                        "32x128x128..96x512x512 uint16 nuclei, density 1e-4..1e-3, spot sigma 1-5, outline sigma "
+                        "0.5-2, seed 4)",
+            "images": n_images, "voxels": nv, "plain_ms": round(plain * 1e3, 1),
+            "plain_mvox_s": round(nv / plain / 1e6, 1), "batched_ms": round(batched * 1e3, 1),
+            "batched_mvox_s": round(nv / batched / 1e6, 1), "speedup_vs_plain": round(plain / batched, 2),
+            "growth_steps": steps, "identical_to_plain": f"{same}/{n_images}"}
+
+
+def serve_section(cle, torch, vol_c2, n_requests, qps):
+    """Poisson arrivals of requests drawn from the mixed-size distribution plus 5 %
+    256x1024x1024 nuclei stacks: the continuous-batching server against the plain
+    per-image path serving the same trace first-in first-out."""
+    import numpy as np
+    from pyclesperanto_prototype_amd import serving, synthetic
+
+    rng = np.random.default_rng(5)
+    n_large = max(1, int(round(0.05 * n_requests)))
+    small_t, ss, so, info = synthetic.mixed_batch(n_requests - n_large, seed=5)
+    reqs = [(cle.asarray(t), float(a), float(b)) for t, a, b in zip(small_t, ss, so)]
+    large = [(vol_c2, float(rng.uniform(1, 5)), float(rng.uniform(0.5, 2))) for _ in range(n_large)]
+    pos = np.sort(rng.choice(n_requests, n_large, replace=False))
+    for p, r in zip(pos, large):
+        reqs.insert(int(p), r)
+    arrivals = serving.poisson_trace(n_requests, qps, seed=5)
+    nv = int(sum(r[0].size for r in reqs))
+    torch.cuda.synchronize()
+    # continuous batching
+    with serving.SegmentationServer(workers=3) as srv:
+        served = serving.replay(srv, reqs, arrivals)
+        batch_sizes = [b[0] for b in srv.batches]
+    span = max(r.finished for r in served) - min(r.arrival for r in served)
+    # the plain per-image path, first-in first-out on the same arrivals
+    t0 = time.perf_counter()
+    plain_lat, plain_out = [], []
+    for (img, a, b), at in zip(reqs, arrivals):
+        delay = t0 + at - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        out = cle.voronoi_otsu_labeling(img, None, a, b)
+        torch.cuda.synchronize()
+        plain_lat.append(time.perf_counter() - (t0 + at))
+        plain_out.append(out)
+    plain_span = time.perf_counter() - t0
+    same = sum(int(torch.equal(r.result().tensor, o.tensor)) for r, o in zip(served, plain_out))
+    pl = np.array(plain_lat) * 1e3
+    lat = serving.latency_summary(served)
+    log(f"serving: {lat} batches {len(batch_sizes)}, plain p50 "
+        f"{np.percentile(pl, 50):.1f} p99 {np.percentile(pl, 99):.1f} ms, identical {same}/{n_requests}")
+    del reqs, served, plain_out, small_t
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    cle.release_pool()
+    return {"workload": f"Poisson arrivals at {qps:g} requests/s: mixed-size requests (as the batch) plus "
+                        f"{n_large} 256x1024x1024 nuclei stacks, voronoi_otsu_labeling per request",
+            "requests": n_requests, "voxels": nv, "served": {**lat,
+                                                              "mvox_s": round(nv / span / 1e6, 1),
+                                                              "batches": len(batch_sizes),
+                                                              "mean_batch": round(float(np.mean(batch_sizes)), 1)},
+            "plain": {"p50_ms": round(float(np.percentile(pl, 50)), 2),
+                      "p99_ms": round(float(np.percentile(pl, 99)), 2),
+                      "mvox_s": round(nv / plain_span / 1e6, 1)},
+            "identical_to_plain": f"{same}/{n_requests}"}
 
 
 def cpu_baseline(vol_t, slices):

@@ -15,7 +15,7 @@ from ._ops import (STATISTICS_KEYS, add_image_and_scalar, add_images, add_images
                    binary_intersection, close_index_gaps_in_label_map, connected_components_labeling_box,
                    connected_components_labeling_diamond, copy, detect_maxima_box, execute, gauss_otsu_labeling,
                    gaussian_blur, greater_constant, histogram, label, label_spots, logical_and, mask,
-                   masked_voronoi_labeling, maximum_of_all_pixels, mean_box, minimum_of_all_pixels,
+                   masked_voronoi_labeling, maximum_of_all_pixels, mean_box, minimum_of_all_pixels, release_pool,
                    nonzero_minimum_box, nonzero_minimum_diamond, onlyzero_overwrite_maximum_box,
                    onlyzero_overwrite_maximum_diamond, otsu_threshold, relabel_sequential, set,
                    set_fp32_key_compat, set_non_zero_pixels_to_pixel_index, set_nonzero_pixels_to_pixelindex,

@@ -409,6 +409,11 @@ logical_and = binary_and
 binary_intersection = binary_and
 
 
+def release_pool():
+    """Free the library's cached scratch memory on the current device (synchronises it)."""
+    _lib.call("cle_release_pool")
+
+
 def execute(*args, **kwargs):
     """Raw OpenCL kernel execution is not part of this build."""
     raise NotImplementedError("cle.execute (raw OpenCL kernels) is not supported by the MI355X build")

@@ -1,0 +1,42 @@
+"""Continuous batching: every request's labels are the same alone and in the mix."""
+import numpy as np
+import pytest
+
+from conftest import blobs_image
+
+pytestmark = pytest.mark.gpu
+
+
+def test_server_results_equal_alone(cle):
+    import torch
+    from pyclesperanto_prototype_amd import serving, synthetic
+
+    rng = np.random.default_rng(60)
+    reqs = []
+    for k in range(24):
+        if k % 4 == 3:
+            img, _ = synthetic.nuclei_overlapping((int(rng.integers(10, 30)), 96, 96), 4e-4, seed=600 + k)
+            img = cle.asarray(img)
+        else:
+            sh = (int(rng.integers(40, 200)), int(rng.integers(40, 200)))
+            img = cle.asarray((blobs_image(sh, 8, 600 + k).astype(np.float32) * 30).astype(np.uint16))
+        reqs.append((img, float(rng.uniform(1, 4)), float(rng.uniform(0.5, 2))))
+    alone = [cle.voronoi_otsu_labeling(im, None, a, b) for im, a, b in reqs]
+    arrivals = serving.poisson_trace(len(reqs), qps=2000.0, seed=61)  # bursts: several per batch
+    with serving.SegmentationServer(workers=2) as srv:
+        done = serving.replay(srv, reqs, arrivals)
+        sizes = [b[0] for b in srv.batches]
+    assert max(sizes) > 1, sizes  # requests were batched together
+    for r, want in zip(done, alone):
+        assert torch.equal(r.result().tensor, want.tensor)
+    lat = serving.latency_summary(done)
+    assert lat["p99_ms"] >= lat["p50_ms"] > 0
+
+
+def test_server_rejects_oversized_request(cle):
+    from pyclesperanto_prototype_amd import serving
+
+    srv = serving.SegmentationServer(workers=1, memory_budget=1000)
+    r = srv.submit(cle.create((64, 64), np.float32), 2.0, 1.0)
+    with pytest.raises(MemoryError):
+        r.result(timeout=1)
