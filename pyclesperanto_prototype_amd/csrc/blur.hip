@@ -288,12 +288,13 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
     constexpr int AWP = IsoLds<C>::AWP;
     constexpr int PF = (NA + FB_THREADS - 1) / FB_THREADS;
     constexpr int CZ = Z3 ? C : 0;
-    constexpr int U = C <= 10 ? 4 : (C <= 14 ? 2 : 1);  // slices per ring shift (register budget)
+    constexpr int U = C <= 10 ? 4 : 2;      // slices per ring shift (register budget); even
     constexpr int RL = 2 * CZ + U;          // ring length
     constexpr int XQ = FB_TX / 4;           // x-pass quads per row
     constexpr int XT = AH * XQ;             // x-pass tasks (4 outputs each)
     constexpr int XR = (XT + FB_THREADS - 1) / FB_THREADS;
     constexpr int NW4 = (2 * C + 4 + 3) / 4;  // float4 reads per x-pass window
+    constexpr int ASZ = AH * AWP, BSZ = AH * FB_TX;  // one buffer of each; A and B hold two
     const int64_t tx = bid % ntx, rest = bid / ntx;
     const int64_t ty = rest % nty, tzc = rest / nty;
     const int64_t x0 = tx * FB_TX, y0 = ty * FB_TY;
@@ -317,25 +318,25 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
     float r0[RL], r1[RL];
 #pragma unroll
     for (int j = 0; j < RL; ++j) { r0[j] = 0.f; r1[j] = 0.f; }
-    const int64_t zlast = ze + CZ;  // one past the last input slice
+    const int64_t z00 = zb - CZ;           // input slice k is z00 + k
+    const int64_t N = ze + CZ - z00;       // input slices
 
-    auto load = [&](float (&pf)[PF], int64_t zi) {
-        if (zi >= zlast) return;
+    auto load = [&](float (&pf)[PF], int64_t k) {
+        if (k >= N) return;
+        const int64_t zi = z00 + k;
         const int64_t zz = zi < 0 ? 0 : (zi >= g.D ? g.D - 1 : zi);
         const int64_t base = zz * g.H * g.W;
 #pragma unroll
-        for (int k = 0; k < PF; ++k)
-            if (threadIdx.x + k * FB_THREADS < NA) pf[k] = in(base + off[k]);
+        for (int m = 0; m < PF; ++m)
+            if (threadIdx.x + m * FB_THREADS < NA) pf[m] = in(base + off[m]);
     };
-    // one input slice: stage it, x pass (LDS -> LDS, four outputs per task from one
-    // window of 2C + 4 values), y pass (LDS -> registers, the thread's two columns),
-    // ring slot u; emits the output slice zi - CZ once the ring covers it
-    auto step = [&](float (&pf)[PF], int64_t zi, int u) {
+    auto store = [&](const float (&pf)[PF], float* a) {
 #pragma unroll
-        for (int k = 0; k < PF; ++k)
-            if (threadIdx.x + k * FB_THREADS < NA) A[lds[k]] = pf[k];
-        __syncthreads();
-        load(pf, zi + 2);
+        for (int m = 0; m < PF; ++m)
+            if (threadIdx.x + m * FB_THREADS < NA) a[lds[m]] = pf[m];
+    };
+    // x pass of one staged slice: a (LDS) -> b (LDS)
+    auto xpass = [&](const float* __restrict__ a, float* __restrict__ bo) {
         if constexpr (C > 12) {
             // wide kernels: one output per task, taps read from LDS (keeps the window
             // out of registers)
@@ -345,51 +346,54 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
                 const int idx = threadIdx.x + k * FB_THREADS;
                 if (idx < AH * FB_TX) {
                     const int ay = idx / FB_TX, bx = idx - ay * FB_TX;
-                    const float* a = A + ay * AWP + bx + C;
-                    float acc = w.w[0] * a[0];
+                    const float* ap = a + ay * AWP + bx + C;
+                    float acc = w.w[0] * ap[0];
 #pragma unroll
-                    for (int v = 1; v <= C; ++v) acc = fmaf(w.w[v], a[-v] + a[v], acc);
-                    B[idx] = acc;
+                    for (int v = 1; v <= C; ++v) acc = fmaf(w.w[v], ap[-v] + ap[v], acc);
+                    bo[idx] = acc;
                 }
             }
         } else {
 #pragma unroll
-        for (int k = 0; k < XR; ++k) {
-            const int t = threadIdx.x + k * FB_THREADS;
-            if (t < XT) {
-                const int ay = t / XQ, q = t - ay * XQ;
-                const float4* a4 = (const float4*)(A + ay * AWP + 4 * q);
-                float a[4 * NW4];
+            for (int k = 0; k < XR; ++k) {
+                const int t = threadIdx.x + k * FB_THREADS;
+                if (t < XT) {
+                    const int ay = t / XQ, q = t - ay * XQ;
+                    const float4* a4 = (const float4*)(a + ay * AWP + 4 * q);
+                    float av[4 * NW4];
 #pragma unroll
-                for (int m = 0; m < NW4; ++m) {
-                    const float4 f = a4[m];
-                    a[4 * m] = f.x; a[4 * m + 1] = f.y; a[4 * m + 2] = f.z; a[4 * m + 3] = f.w;
+                    for (int m = 0; m < NW4; ++m) {
+                        const float4 f = a4[m];
+                        av[4 * m] = f.x; av[4 * m + 1] = f.y; av[4 * m + 2] = f.z; av[4 * m + 3] = f.w;
+                    }
+                    float o[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        float acc = w.w[0] * av[C + j];
+#pragma unroll
+                        for (int v = 1; v <= C; ++v) acc = fmaf(w.w[v], av[C + j - v] + av[C + j + v], acc);
+                        o[j] = acc;
+                    }
+                    *(float4*)(bo + ay * FB_TX + 4 * q) = make_float4(o[0], o[1], o[2], o[3]);
                 }
-                float o[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    float acc = w.w[0] * a[C + j];
-#pragma unroll
-                    for (int v = 1; v <= C; ++v) acc = fmaf(w.w[v], a[C + j - v] + a[C + j + v], acc);
-                    o[j] = acc;
-                }
-                *(float4*)(B + ay * FB_TX + 4 * q) = make_float4(o[0], o[1], o[2], o[3]);
             }
         }
-        }
-        __syncthreads();
-        const float* b = B + (row + C) * FB_TX + col;
-        const float2 bc = *(const float2*)b;
+    };
+    // y pass of slice k (LDS -> the thread's two columns), ring slot u, output when
+    // the ring covers the slice k - CZ
+    auto ypass = [&](const float* __restrict__ bi, int64_t k, int u) {
+        const float* bp = bi + (row + C) * FB_TX + col;
+        const float2 bc = *(const float2*)bp;
         float v0 = w.w[0] * bc.x, v1 = w.w[0] * bc.y;
 #pragma unroll
         for (int v = 1; v <= C; ++v) {
-            const float2 lo = *(const float2*)(b - v * FB_TX), hi = *(const float2*)(b + v * FB_TX);
+            const float2 lo = *(const float2*)(bp - v * FB_TX), hi = *(const float2*)(bp + v * FB_TX);
             v0 = fmaf(w.w[v], lo.x + hi.x, v0);
             v1 = fmaf(w.w[v], lo.y + hi.y, v1);
         }
         r0[2 * CZ + u] = v0;
         r1[2 * CZ + u] = v1;
-        if (zi >= zb + CZ) {
+        if (k >= 2 * CZ) {
             float o0, o1;
             if (CZ == 0) {
                 o0 = r0[u];
@@ -398,12 +402,12 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
                 o0 = w.w[0] * r0[u + CZ];
                 o1 = w.w[0] * r1[u + CZ];
 #pragma unroll
-                for (int k = 1; k <= CZ; ++k) {
-                    o0 = fmaf(w.w[k], r0[u + CZ - k] + r0[u + CZ + k], o0);
-                    o1 = fmaf(w.w[k], r1[u + CZ - k] + r1[u + CZ + k], o1);
+                for (int kk = 1; kk <= CZ; ++kk) {
+                    o0 = fmaf(w.w[kk], r0[u + CZ - kk] + r0[u + CZ + kk], o0);
+                    o1 = fmaf(w.w[kk], r1[u + CZ - kk] + r1[u + CZ + kk], o1);
                 }
             }
-            const int64_t zo = zi - CZ, y = y0 + row, x = x0 + col;
+            const int64_t zo = z00 + k - CZ, y = y0 + row, x = x0 + col;
             if (y < g.H) {
                 float* o = out + (zo * g.H + y) * g.W + x;
                 if (x + 1 < g.W) { o[0] = o0; o[1] = o1; }
@@ -412,20 +416,39 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
         }
     };
 
-    load(pa, zb - CZ);
-    load(pb, zb - CZ + 1);
-    // the prefetch buffers alternate pa, pb, pa, ... across consecutive slices
-    int64_t zi = zb - CZ;
-    bool even = true;
-    for (; zi < zlast; zi += U) {
+    // Software pipeline over the input slices, one barrier per phase: phase p stages
+    // slice p (into A[p & 1]) and prefetches slice p + 2, runs the x pass of slice
+    // p - 1 (A[(p-1) & 1] -> B[(p-1) & 1]) and the y pass of slice p - 2 (from
+    // B[p & 1]).  The buffers a phase writes were last read one phase earlier.
+    load(pa, 0);
+    load(pb, 1);
+    store(pa, A);
+    load(pa, 2);
+    __syncthreads();
+    if (N > 1) {
+        store(pb, A + ASZ);
+        load(pb, 3);
+    }
+    xpass(A, B);
+    __syncthreads();
+    for (int64_t g0 = 0; g0 < N; g0 += U) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (zi + u < zlast) {
-                if (even == ((u & 1) == 0)) step(pa, zi + u, u);
-                else step(pb, zi + u, u);
+            const int64_t k = g0 + u;  // slice of the y pass; p = k + 2 has parity u & 1
+            if (k < N) {
+                float* Ap = A + ((u & 1) ? ASZ : 0);       // A[p & 1]
+                float* Aq = A + ((u & 1) ? 0 : ASZ);       // A[(p - 1) & 1]
+                float* Bq = B + ((u & 1) ? 0 : BSZ);       // B[(p - 1) & 1]
+                const float* Bk = B + ((u & 1) ? BSZ : 0);  // B[k & 1]
+                if (k + 2 < N) {
+                    if (u & 1) { store(pb, Ap); load(pb, k + 4); }
+                    else { store(pa, Ap); load(pa, k + 4); }
+                }
+                if (k + 1 < N) xpass(Aq, Bq);
+                ypass(Bk, k, u);
+                __syncthreads();
             }
         }
-        if (U & 1) even = !even;
 #pragma unroll
         for (int j = 0; j < 2 * CZ; ++j) { r0[j] = r0[j + U]; r1[j] = r1[j + U]; }
     }
@@ -434,8 +457,8 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
 template <int C, bool Z3, typename LD>
 __global__ __launch_bounds__(FB_THREADS) void k_blur_iso(LD in, float* __restrict__ out, Geom g, int zc, int ntx,
                                                         int nty, const IsoW<C> w) {
-    __shared__ __align__(16) float A[IsoLds<C>::AH * IsoLds<C>::AWP];
-    __shared__ __align__(16) float B[IsoLds<C>::AH * FB_TX];
+    __shared__ __align__(16) float A[2 * IsoLds<C>::AH * IsoLds<C>::AWP];
+    __shared__ __align__(16) float B[2 * IsoLds<C>::AH * FB_TX];
     blur_iso_tile<C, Z3>(in, out, g, zc, ntx, nty, w, xcd_tile(blockIdx.x, gridDim.x), A, B);
 }
 
@@ -459,8 +482,8 @@ template <> __device__ __forceinline__ LdU8 make_ld<LdU8>(const void* p) { retur
 template <int C, bool Z3, typename LD>
 __global__ __launch_bounds__(FB_THREADS) void k_blur_iso_batch(const IsoJob<C>* __restrict__ jobs,
                                                               const int64_t* __restrict__ bstart, int njobs) {
-    __shared__ __align__(16) float A[IsoLds<C>::AH * IsoLds<C>::AWP];
-    __shared__ __align__(16) float B[IsoLds<C>::AH * FB_TX];
+    __shared__ __align__(16) float A[2 * IsoLds<C>::AH * IsoLds<C>::AWP];
+    __shared__ __align__(16) float B[2 * IsoLds<C>::AH * FB_TX];
     const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
     const int j = find_seg(bstart, njobs, b);
     const IsoJob<C>& jb = jobs[j];

@@ -26,22 +26,23 @@ namespace cle {
 int maxima_impl(DBuf src, DBuf dst, const Geom& g, hipStream_t s);
 
 constexpr int VB = 256;  // threads per tile block
-constexpr int VTX = 64;  // tile extent along x
 constexpr int VPT = 4;   // voxels per thread (tile = 1024 voxels)
 
+// Tiles of 1024 voxels, close to cubes so that a growth front keeps as few voxels
+// active as possible: 16 x 8 x 8 in 3D, 32 x 32 in 2D (x rows of 64 / 128 bytes)
 struct Tiles {
-    int tz, ty;             // tile extents in z and y (tx = VTX)
+    int tx, ty, tz;         // tile extents
     int64_t nz, ny, nx;     // tile counts
     __host__ __device__ int64_t count() const { return nz * ny * nx; }
 };
 
 __host__ __device__ inline Tiles make_tiles(const Geom& g) {
     Tiles t;
-    if (g.ndim == 3 && g.D > 1) { t.tz = 4; t.ty = 4; }
-    else { t.tz = 1; t.ty = 16; }
+    if (g.ndim == 3 && g.D > 1) { t.tx = 16; t.ty = 8; t.tz = 8; }
+    else { t.tx = 32; t.ty = 32; t.tz = 1; }
     t.nz = (g.D + t.tz - 1) / t.tz;
     t.ny = (g.H + t.ty - 1) / t.ty;
-    t.nx = (g.W + VTX - 1) / VTX;
+    t.nx = (g.W + t.tx - 1) / t.tx;
     return t;
 }
 
@@ -57,9 +58,9 @@ __device__ __forceinline__ bool vor_init_tile(MK mask, float* A, float* B, const
     bool pos = false;
     for (int k = 0; k < VPT; ++k) {
         const int li = threadIdx.x + k * VB;
-        const int lx = li % VTX, ly = (li / VTX) % T.ty, lz = li / (VTX * T.ty);
+        const int lx = li % T.tx, ly = (li / T.tx) % T.ty, lz = li / (T.tx * T.ty);
         if (lz >= T.tz) continue;
-        const int64_t x = tx * VTX + lx, y = ty * T.ty + ly, z = tz * T.tz + lz;
+        const int64_t x = tx * T.tx + lx, y = ty * T.ty + ly, z = tz * T.tz + lz;
         if (x >= g.W || y >= g.H || z >= g.D) continue;
         const int64_t i = (z * g.H + y) * g.W + x;
         const float v = (mask(i) - 1.f) + B[i];
@@ -103,9 +104,9 @@ __device__ __forceinline__ bool vor_tile_update(const float* __restrict__ src, f
     bool changed = false;
     for (int k = 0; k < VPT; ++k) {
         const int li = threadIdx.x + k * VB;
-        const int lx = li % VTX, ly = (li / VTX) % T.ty, lz = li / (VTX * T.ty);
+        const int lx = li % T.tx, ly = (li / T.tx) % T.ty, lz = li / (T.tx * T.ty);
         if (lz >= T.tz) continue;
-        const int64_t x = tx * VTX + lx, y = ty * T.ty + ly, z = tz * T.tz + lz;
+        const int64_t x = tx * T.tx + lx, y = ty * T.ty + ly, z = tz * T.tz + lz;
         if (x >= g.W || y >= g.H || z >= g.D) continue;
         const int64_t i = (z * g.H + y) * g.W + x;
         const float v = src[i];
