@@ -584,7 +584,7 @@ static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hip
     k_blur_iso<C, Z3, LD><<<(unsigned)(nxy * nzc), FB_THREADS, 0, s>>>(in, out, g, zc, ntx, nty, w);
 }
 
-constexpr int ISO_MAXC = 16;  // wider kernels would spill the ring; they take k_blur_fused
+constexpr int ISO_MAXC = 20;
 
 template <bool Z3, typename LD>
 static int launch_iso(LD in, float* out, const Geom& g, int c, const float* wts, hipStream_t s) {
@@ -592,6 +592,7 @@ static int launch_iso(LD in, float* out, const Geom& g, int c, const float* wts,
 #define ISO_CASE(n) case n: launch_iso_c<n, Z3>(in, out, g, wts, s); break;
         ISO_CASE(1) ISO_CASE(2) ISO_CASE(3) ISO_CASE(4) ISO_CASE(5) ISO_CASE(6) ISO_CASE(7) ISO_CASE(8)
         ISO_CASE(9) ISO_CASE(10) ISO_CASE(11) ISO_CASE(12) ISO_CASE(13) ISO_CASE(14) ISO_CASE(15) ISO_CASE(16)
+        ISO_CASE(17) ISO_CASE(18) ISO_CASE(19) ISO_CASE(20)
 #undef ISO_CASE
         default: return fail(CLE_E_INTERNAL, "isotropic blur: half-width %d out of range", c);
     }
@@ -837,7 +838,7 @@ int plan_blur_batch(const std::vector<BlurReq>& reqs, Blob& blob, std::vector<La
 #define ISO_PLAN(n) case n: plan_iso_c<n>(z == 1, dt, reqs, idx, blob, out, s); break;
                     ISO_PLAN(1) ISO_PLAN(2) ISO_PLAN(3) ISO_PLAN(4) ISO_PLAN(5) ISO_PLAN(6) ISO_PLAN(7)
                     ISO_PLAN(8) ISO_PLAN(9) ISO_PLAN(10) ISO_PLAN(11) ISO_PLAN(12) ISO_PLAN(13) ISO_PLAN(14)
-                    ISO_PLAN(15) ISO_PLAN(16)
+                    ISO_PLAN(15) ISO_PLAN(16) ISO_PLAN(17) ISO_PLAN(18) ISO_PLAN(19) ISO_PLAN(20)
 #undef ISO_PLAN
                     default: break;
                 }

@@ -51,11 +51,14 @@ __device__ __forceinline__ int64_t clampi(int64_t v, int64_t hi) { return v < 0 
 // A = B = (mask - 1) + label, and the tiles holding a positive value are flagged as
 // "changed" at steps -1 and -2 (slots 3 and 2 of the 4-slot ring).
 // tile = index inside the image, gtile = its slot in the flag arrays of ntiles slots
+// A tile is "settled" once both buffers hold the same values there and none is zero:
+// it can never change again, so the steps skip it (settled[gtile] = 1).
 template <typename MK>
 __device__ __forceinline__ bool vor_init_tile(MK mask, float* A, float* B, const Geom& g, const Tiles& T,
-                                              int64_t tile, int64_t gtile, int64_t ntiles, uint8_t* tflags) {
+                                              int64_t tile, int64_t gtile, int64_t ntiles, uint8_t* tflags,
+                                              uint8_t* settled) {
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
-    bool pos = false;
+    bool pos = false, zero = false;
     for (int k = 0; k < VPT; ++k) {
         const int li = threadIdx.x + k * VB;
         const int lx = li % T.tx, ly = (li / T.tx) % T.ty, lz = li / (T.tx * T.ty);
@@ -67,8 +70,11 @@ __device__ __forceinline__ bool vor_init_tile(MK mask, float* A, float* B, const
         A[i] = v;
         B[i] = v;
         pos |= v > 0.f;
+        zero |= v == 0.f;
     }
     const int any = __syncthreads_or(pos ? 1 : 0);
+    const int anyzero = __syncthreads_or(zero ? 1 : 0);
+    if (threadIdx.x == 0) settled[gtile] = anyzero ? 0 : 1;
     if (tflags && threadIdx.x == 0) {
         const uint8_t f = any ? 1 : 0;
         tflags[0 * ntiles + gtile] = 0;
@@ -89,19 +95,20 @@ struct MaskU8 {
 };
 
 __global__ __launch_bounds__(VB) void k_vor_init(DBuf mask, float* A, float* B, Geom g, Tiles T, uint8_t* tflags) {
-    vor_init_tile(MaskAny{mask}, A, B, g, T, blockIdx.x, blockIdx.x, T.count(), tflags);
+    vor_init_tile(MaskAny{mask}, A, B, g, T, blockIdx.x, blockIdx.x, T.count(), tflags, tflags + 4 * T.count());
 }
 
 // one growth step t: src -> dst, box (t even) or diamond (t odd)
 // One growth step on one tile.  gflag / gany: "changed" flags of this step for the
 // image and for the whole launch (the same word for a single image).
-// step t on one tile (src -> dst); returns, in every thread, whether the tile changed
+// step t on one tile (src -> dst); returns, in every thread, whether the tile changed,
+// and marks the tile settled when it did not change and holds no zero
 __device__ __forceinline__ bool vor_tile_update(const float* __restrict__ src, float* __restrict__ dst, const Geom& g,
-                                                const Tiles& T, int64_t tile, int t) {
+                                                const Tiles& T, int64_t tile, int t, uint8_t* settled_slot) {
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
     const bool diamond = (t & 1) != 0;
     const int zr = g.ndim == 3 ? 1 : 0;
-    bool changed = false;
+    bool changed = false, zero = false;
     for (int k = 0; k < VPT; ++k) {
         const int li = threadIdx.x + k * VB;
         const int lx = li % T.tx, ly = (li / T.tx) % T.ty, lz = li / (T.tx * T.ty);
@@ -141,8 +148,12 @@ __device__ __forceinline__ bool vor_tile_update(const float* __restrict__ src, f
         }
         dst[i] = out;
         changed |= (out != v);
+        zero |= out == 0.f;
     }
-    return __syncthreads_or(changed ? 1 : 0) != 0;
+    const bool ch = __syncthreads_or(changed ? 1 : 0) != 0;
+    const bool zr0 = __syncthreads_or(zero ? 1 : 0) != 0;
+    if (!ch && !zr0 && threadIdx.x == 0) *settled_slot = 1;
+    return ch;
 }
 
 __device__ __forceinline__ void vor_step_tile(const float* __restrict__ src, float* __restrict__ dst, const Geom& g,
@@ -151,6 +162,8 @@ __device__ __forceinline__ void vor_step_tile(const float* __restrict__ src, flo
     const int64_t gtile = tbase + tile;
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
     if (threadIdx.x == 0) tflags[((t + 1) & 3) * ntiles + gtile] = 0;
+    uint8_t* settled = tflags + 4 * ntiles;
+    if (settled[gtile]) return;  // no zero left and both buffers equal: nothing can change
     // activity: any tile of the 3^d tile neighbourhood changed at t-1 or t-2
     const uint8_t* f1 = tflags + ((t + 3) & 3) * ntiles;
     const uint8_t* f2 = tflags + ((t + 2) & 3) * ntiles;
@@ -164,7 +177,7 @@ __device__ __forceinline__ void vor_step_tile(const float* __restrict__ src, flo
         }
     }
     if (!__syncthreads_or(act ? 1 : 0)) return;
-    if (vor_tile_update(src, dst, g, T, tile, t) && threadIdx.x == 0) {
+    if (vor_tile_update(src, dst, g, T, tile, t, settled + gtile) && threadIdx.x == 0) {
         tflags[(t & 3) * ntiles + gtile] = 1;
         *gflag = 1;
         *gany = 1;
@@ -245,7 +258,7 @@ static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, 
     CLE_TRY(ccl_impl(binary, g, DBuf{B, CLE_F32}, CLE_CONN_BOX, CLE_FP32_KEY_COMPAT, nullptr, s));
     const Tiles T = make_tiles(g);
     Scratch tf;
-    CLE_TRY(tf.alloc((size_t)T.count() * 4, s));
+    CLE_TRY(tf.alloc((size_t)T.count() * 5, s));  // 4 "changed" slots + settled
     k_vor_init<<<(unsigned)T.count(), VB, 0, s>>>(mask, A, B, g, T, tf.get<uint8_t>());
     CLE_LAUNCHED();
     CLE_TRY(grow(A, B, g, T, tf.get<uint8_t>(), n_iterations, s));
@@ -301,10 +314,11 @@ __global__ __launch_bounds__(256) void k_seg_and_batch(const BImg* __restrict__ 
 // list free of duplicates.  A step launch walks its list with a fixed grid, so its
 // cost follows the growth front instead of the volume.
 struct VorLists {
-    uint32_t* items;  // 3 lists of cap entries (global tile indices)
-    uint32_t* count;  // 3 counters
-    int32_t* stamp;   // 2 x cap
-    int64_t cap;      // tiles of all images
+    uint32_t* items;   // 3 lists of cap entries (global tile indices)
+    uint32_t* count;   // 3 counters
+    int32_t* stamp;    // 2 x cap
+    uint8_t* settled;  // cap
+    int64_t cap;       // tiles of all images
 };
 
 __device__ __forceinline__ uint64_t lanes_below_v() {
@@ -343,7 +357,7 @@ __global__ __launch_bounds__(VB) void k_vor_init_batch(const BImg* __restrict__ 
     const int j = find_seg(ts, nimg, b);
     const BImg& im = imgs[j];
     const Tiles T = make_tiles(im.g);
-    if (vor_init_tile(MaskU8{G + im.off}, A + im.off, B + im.off, im.g, T, b - ts[j], b, 0, nullptr) &&
+    if (vor_init_tile(MaskU8{G + im.off}, A + im.off, B + im.off, im.g, T, b - ts[j], b, 0, nullptr, L.settled) &&
         threadIdx.x < 64)
         vor_queue(T, b - ts[j], ts[j], 0, 1, L);
 }
@@ -358,9 +372,10 @@ __global__ __launch_bounds__(VB) void k_vor_step_batch(const BImg* __restrict__ 
         const int64_t q = items[e];
         const int j = find_seg(ts, nimg, q);
         if (t > 0 && gflags[(int64_t)((t - 1) % ring) * nimg + j] == 0) continue;  // this image has converged
+        if (L.settled[q]) continue;
         const BImg& im = imgs[j];
         const Tiles T = make_tiles(im.g);
-        if (vor_tile_update(src + im.off, dst + im.off, im.g, T, q - ts[j], t)) {
+        if (vor_tile_update(src + im.off, dst + im.off, im.g, T, q - ts[j], t, L.settled + q)) {
             if (threadIdx.x == 0) {
                 gflags[(int64_t)(t % ring) * nimg + j] = 1;
                 gany[t % ring] = 1;
@@ -433,11 +448,12 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     double* thr = (double*)(aux.get<char>() + (size_t)n * 8);
     uint32_t* ou = (uint32_t*)(aux.get<char>() + (size_t)n * 16);
     const int64_t tcap = ntiles > 0 ? ntiles : 1;
-    CLE_TRY(tf.alloc((size_t)tcap * 3 * 4 + 16 + (size_t)tcap * 2 * 4, s));
+    CLE_TRY(tf.alloc((size_t)tcap * 3 * 4 + 16 + (size_t)tcap * 2 * 4 + (size_t)tcap, s));
     VorLists VL;
     VL.items = tf.get<uint32_t>();
     VL.count = VL.items + (size_t)tcap * 3;
     VL.stamp = (int32_t*)(VL.count + 4);
+    VL.settled = (uint8_t*)(VL.stamp + (size_t)tcap * 2);
     VL.cap = tcap;
     CLE_TRY(gf.alloc(((size_t)VOR_RING * n + VOR_RING) * 4, s));
     uint32_t* gflags = gf.get<uint32_t>();
