@@ -16,6 +16,8 @@
 // merges above 2^24 voxels) are set in a bitmap over the key space, a popcount prefix
 // scan gives every key its rank, and one gather writes the labels.  Everything stays
 // on the device; no host synchronisation.
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace cle {
@@ -582,6 +584,7 @@ struct BcclLds {
     uint32_t lab[BT::N];
     uint32_t kmin[BT::N];
     uint8_t msk[BT::N];
+    alignas(16) uint8_t vox[BT::N * 4 * BT::BZ];  // the tile's binary voxels (uint8 fast path)
     uint32_t cnt;
 };
 
@@ -604,6 +607,36 @@ __device__ __forceinline__ void bccl_local_tile(FG fg, const Geom& g, const BGri
     int64_t gb[PER];
     uint32_t m[PER];
     bool anyf = false;
+    // uint8 binary, tile fully inside the image, rows 8-byte aligned: the tile's voxels
+    // come in with one 8-byte load per thread (rows of 2 TX bytes), an empty tile
+    // leaves after one block-wide test, and the block masks are read from LDS
+    bool staged = false;
+    if constexpr (std::is_same<FG, FgU8>::value) {
+        constexpr int RB = 2 * TX;                 // bytes per voxel row of the tile
+        constexpr int NROW = 2 * TY * BT::TZ * BZ;  // voxel rows of the tile
+        static_assert(RB % 8 == 0 && NROW * RB == 8 * CT_THREADS, "tile must be 8 bytes per thread");
+        const bool full = (bx0 + TX) * 2 <= g.W && (by0 + TY) * 2 <= g.H && (bz0 + BT::TZ) * BZ <= g.D &&
+                          (g.W & 7) == 0 && ((uintptr_t)fg.p & 7) == 0;
+        if (full) {
+            const int r = threadIdx.x / (RB / 8), part = threadIdx.x % (RB / 8);
+            const int ry = r % (2 * TY), rz = r / (2 * TY);
+            const int64_t z = bz0 * BZ + rz, y = by0 * 2 + ry;
+            const uint2 q = *(const uint2*)(fg.p + (z * g.H + y) * g.W + bx0 * 2 + part * 8);
+            if (!__syncthreads_or((q.x | q.y) != 0)) {
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    const int li = threadIdx.x + k * CT_THREADS;
+                    const int lx = li % TX, ly = (li / TX) % TY, lz = li / (TX * TY);
+                    M[boff + ((bz0 + lz) * B.H + (by0 + ly)) * B.W + (bx0 + lx)] = 0;
+                }
+                if (threadIdx.x == 0) nroots[gtile] = 0;
+                return;
+            }
+            *(uint2*)(sm.vox + r * RB + part * 8) = q;
+            __syncthreads();
+            staged = true;
+        }
+    }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int li = threadIdx.x + k * CT_THREADS;
@@ -611,7 +644,16 @@ __device__ __forceinline__ void bccl_local_tile(FG fg, const Geom& g, const BGri
         const int64_t bx = bx0 + lx, by = by0 + ly, bz = bz0 + lz;
         const bool in = bx < B.W && by < B.H && bz < B.D;
         uint32_t mk = 0;
-        if (in) {
+        if (staged) {
+#pragma unroll
+            for (int dz = 0; dz < BZ; ++dz)
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy) {
+                    const uint8_t* row = sm.vox + ((lz * BZ + dz) * 2 * TY + 2 * ly + dy) * (2 * TX) + 2 * lx;
+                    if (row[0]) mk |= 1u << (dz * 4 + dy * 2);
+                    if (row[1]) mk |= 1u << (dz * 4 + dy * 2 + 1);
+                }
+        } else if (in) {
 #pragma unroll
             for (int dz = 0; dz < BZ; ++dz) {
                 const int64_t z = bz * BZ + dz;
