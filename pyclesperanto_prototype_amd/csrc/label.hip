@@ -414,25 +414,39 @@ __global__ __launch_bounds__(CT_THREADS) void k_ccl_final(const uint32_t* L, con
     }
 }
 
+// store4 writes voxels i..i+3 with one 16-byte store (i a multiple of 4, p 16-byte aligned)
 struct StLabI32 {
     int32_t* p;
-    __device__ __forceinline__ void operator()(int64_t i, float v) const {
-        p[i] = v > 2147483647.f ? 2147483647 : (int32_t)v;
+    __device__ __forceinline__ static int32_t cv(float v) { return v > 2147483647.f ? 2147483647 : (int32_t)v; }
+    __device__ __forceinline__ void operator()(int64_t i, float v) const { p[i] = cv(v); }
+    __device__ __forceinline__ void store4(int64_t i, float a, float b, float c, float d) const {
+        *(int4*)(p + i) = make_int4(cv(a), cv(b), cv(c), cv(d));
     }
 };
 struct StLabU32 {
     uint32_t* p;
-    __device__ __forceinline__ void operator()(int64_t i, float v) const {
-        p[i] = v > 4294967295.f ? 4294967295u : (uint32_t)v;
+    __device__ __forceinline__ static uint32_t cv(float v) { return v > 4294967295.f ? 4294967295u : (uint32_t)v; }
+    __device__ __forceinline__ void operator()(int64_t i, float v) const { p[i] = cv(v); }
+    __device__ __forceinline__ void store4(int64_t i, float a, float b, float c, float d) const {
+        *(uint4*)(p + i) = make_uint4(cv(a), cv(b), cv(c), cv(d));
     }
 };
 struct StLabF32 {
     float* p;
     __device__ __forceinline__ void operator()(int64_t i, float v) const { p[i] = v; }
+    __device__ __forceinline__ void store4(int64_t i, float a, float b, float c, float d) const {
+        *(float4*)(p + i) = make_float4(a, b, c, d);
+    }
 };
 struct StLabAny {
     DBuf b;
     __device__ __forceinline__ void operator()(int64_t i, float v) const { st(b, i, v); }
+    __device__ __forceinline__ void store4(int64_t i, float a, float b_, float c, float d) const {
+        st(b, i, a);
+        st(b, i + 1, b_);
+        st(b, i + 2, c);
+        st(b, i + 3, d);
+    }
 };
 
 // labels are ranks (>= 0) held as float, as the reference stores them, then
@@ -790,70 +804,68 @@ __global__ __launch_bounds__(CT_THREADS) void k_bccl_border(BGrid B, TileGrid T,
     bccl_border_tile<BT>(B, T, tile, tile, 0, M, L, nroots);
 }
 
-// a foreground voxel's label is F[local root of its block], held as float and
-// converted with saturation like the voxel version; one row per block iteration
-template <int BZ, typename FG, typename ST>
-__global__ void k_bccl_write(FG fg, Geom g, BGrid B, const uint32_t* L, const uint32_t* F, ST out) {
-    const int64_t rows = g.D * g.H;
-    for (int64_t row = blockIdx.x; row < rows; row += gridDim.x) {
-        const int64_t z = row / g.H, y = row - z * g.H;
-        const int64_t rb = ((z / BZ) * B.H + (y >> 1)) * B.W;
-        for (int64_t x = threadIdx.x; x < g.W; x += blockDim.x) {
-            const int64_t i = row * g.W + x;
-            out(i, fg(i) ? (float)F[L[rb + (x >> 1)]] : 0.f);
+// A foreground voxel's label is F[local root of its block], held as float and
+// converted with saturation like the voxel version.  The voxels are written from
+// the blocks' masks (bit dz * 4 + dy * 2 + dx), so the binary image is not read
+// again and each block's label is looked up once: one thread per two blocks along
+// x writes their 4 x 2 x BZ voxels.  VEC: rows of a multiple of 4 voxels and a
+// 16-byte aligned destination, four voxels per store.
+template <int BZ, bool VEC, typename ST>
+__global__ void k_bccl_write(const uint8_t* __restrict__ M, Geom g, BGrid B, const uint32_t* __restrict__ L,
+                             const uint32_t* __restrict__ F, ST out) {
+    const int64_t brows = B.D * B.H, np = (B.W + 1) >> 1;
+    for (int64_t br = blockIdx.x; br < brows; br += gridDim.x) {
+        const int64_t bz = br / B.H, by = br - bz * B.H;
+        const int64_t z0 = bz * BZ, y0 = by * 2;
+        const int nz = (BZ == 2 && z0 + 1 < g.D) ? 2 : 1, ny = y0 + 1 < g.H ? 2 : 1;
+        for (int64_t p = threadIdx.x; p < np; p += blockDim.x) {
+            const int64_t b = br * B.W + 2 * p, x = 4 * p;
+            uint32_t m2;
+            if constexpr (VEC)
+                m2 = *(const uint16_t*)(M + b);  // B.W even
+            else
+                m2 = M[b] | (2 * p + 1 < B.W ? (uint32_t)M[b + 1] << 8 : 0u);
+            const float f0 = (m2 & 0xFFu) ? (float)F[L[b]] : 0.f;
+            const float f1 = (m2 >> 8) ? (float)F[L[b + 1]] : 0.f;
+            for (int dz = 0; dz < nz; ++dz)
+                for (int dy = 0; dy < ny; ++dy) {
+                    const int sh = dz * 4 + dy * 2;
+                    const float v0 = (m2 >> sh) & 1u ? f0 : 0.f, v1 = (m2 >> (sh + 1)) & 1u ? f0 : 0.f;
+                    const float v2 = (m2 >> (sh + 8)) & 1u ? f1 : 0.f, v3 = (m2 >> (sh + 9)) & 1u ? f1 : 0.f;
+                    const int64_t i = ((z0 + dz) * g.H + y0 + dy) * g.W + x;
+                    if constexpr (VEC) {
+                        out.store4(i, v0, v1, v2, v3);
+                    } else {
+                        out(i, v0);
+                        if (x + 1 < g.W) out(i + 1, v1);
+                        if (x + 2 < g.W) out(i + 2, v2);
+                        if (x + 3 < g.W) out(i + 3, v3);
+                    }
+                }
         }
     }
 }
 
-// uint8 binary, 32-bit destination, rows of a multiple of 4 voxels: four voxels
-// (two blocks) per thread, one 4-byte load and one 16-byte store
-template <int BZ, typename T, typename ST>
-__global__ void k_bccl_write_u8x4(const uint8_t* bin, Geom g, BGrid B, const uint32_t* L, const uint32_t* F, T* dst,
-                                  ST out) {
-    const int64_t rows = g.D * g.H;
-    for (int64_t row = blockIdx.x; row < rows; row += gridDim.x) {
-        const int64_t z = row / g.H, y = row - z * g.H;
-        const int64_t rb = ((z / BZ) * B.H + (y >> 1)) * B.W;
-        for (int64_t x = 4 * (int64_t)threadIdx.x; x < g.W; x += 4 * (int64_t)blockDim.x) {
-            const int64_t i = row * g.W + x;
-            const uint32_t b4 = *(const uint32_t*)(bin + i);
-            if (b4 == 0) {
-                *(uint4*)(dst + i) = make_uint4(0u, 0u, 0u, 0u);
-                continue;
-            }
-            const float f0 = (b4 & 0xFFFFu) ? (float)F[L[rb + (x >> 1)]] : 0.f;
-            const float f1 = (b4 >> 16) ? (float)F[L[rb + (x >> 1) + 1]] : 0.f;
-            out(i, (b4 & 0xFFu) ? f0 : 0.f);
-            out(i + 1, (b4 & 0xFF00u) ? f0 : 0.f);
-            out(i + 2, (b4 & 0xFF0000u) ? f1 : 0.f);
-            out(i + 3, (b4 & 0xFF000000u) ? f1 : 0.f);
-        }
-    }
-}
-
-template <int BZ, typename FG>
-static void launch_bccl_write(FG fg, const Geom& g, const BGrid& B, const uint32_t* L, const uint32_t* F, DBuf dst,
-                              unsigned grid, hipStream_t s) {
+template <int BZ, bool VEC>
+static void launch_bccl_write_v(const uint8_t* M, const Geom& g, const BGrid& B, const uint32_t* L, const uint32_t* F,
+                                DBuf dst, unsigned grid, hipStream_t s) {
     switch (dst.dt) {
-        case CLE_I32: k_bccl_write<BZ><<<grid, 256, 0, s>>>(fg, g, B, L, F, StLabI32{(int32_t*)dst.p}); break;
-        case CLE_U32: k_bccl_write<BZ><<<grid, 256, 0, s>>>(fg, g, B, L, F, StLabU32{(uint32_t*)dst.p}); break;
-        case CLE_F32: k_bccl_write<BZ><<<grid, 256, 0, s>>>(fg, g, B, L, F, StLabF32{(float*)dst.p}); break;
-        default: k_bccl_write<BZ><<<grid, 256, 0, s>>>(fg, g, B, L, F, StLabAny{dst}); break;
+        case CLE_I32: k_bccl_write<BZ, VEC><<<grid, 256, 0, s>>>(M, g, B, L, F, StLabI32{(int32_t*)dst.p}); break;
+        case CLE_U32: k_bccl_write<BZ, VEC><<<grid, 256, 0, s>>>(M, g, B, L, F, StLabU32{(uint32_t*)dst.p}); break;
+        case CLE_F32: k_bccl_write<BZ, VEC><<<grid, 256, 0, s>>>(M, g, B, L, F, StLabF32{(float*)dst.p}); break;
+        default: k_bccl_write<BZ, false><<<grid, 256, 0, s>>>(M, g, B, L, F, StLabAny{dst}); break;
     }
 }
 
 template <int BZ>
-static void launch_bccl_write(FgU8 fg, const Geom& g, const BGrid& B, const uint32_t* L, const uint32_t* F, DBuf dst,
-                              unsigned grid, hipStream_t s) {
-    const bool vec = (g.W & 3) == 0 && ((uintptr_t)fg.p & 3) == 0 && ((uintptr_t)dst.p & 15) == 0;
-    if (vec && dst.dt == CLE_I32)
-        k_bccl_write_u8x4<BZ><<<grid, 256, 0, s>>>(fg.p, g, B, L, F, (int32_t*)dst.p, StLabI32{(int32_t*)dst.p});
-    else if (vec && dst.dt == CLE_U32)
-        k_bccl_write_u8x4<BZ><<<grid, 256, 0, s>>>(fg.p, g, B, L, F, (uint32_t*)dst.p, StLabU32{(uint32_t*)dst.p});
-    else if (vec && dst.dt == CLE_F32)
-        k_bccl_write_u8x4<BZ><<<grid, 256, 0, s>>>(fg.p, g, B, L, F, (float*)dst.p, StLabF32{(float*)dst.p});
+static void launch_bccl_write(const uint8_t* M, const Geom& g, const BGrid& B, const uint32_t* L, const uint32_t* F,
+                              DBuf dst, hipStream_t s) {
+    const int64_t brows = B.D * B.H;
+    const unsigned grid = (unsigned)(brows < 262144 ? brows : 262144);
+    if ((g.W & 3) == 0 && ((uintptr_t)dst.p & 15) == 0)
+        launch_bccl_write_v<BZ, true>(M, g, B, L, F, dst, grid, s);
     else
-        launch_bccl_write<BZ, FgU8>(fg, g, B, L, F, dst, grid, s);
+        launch_bccl_write_v<BZ, false>(M, g, B, L, F, dst, grid, s);
 }
 
 template <typename BT, typename FG>
@@ -884,9 +896,7 @@ static int bccl_run(FG fg, const Geom& g, DBuf dst, int options, uint32_t* n_lab
     CLE_TRY(rt.scan(n_labels_dev, s));
     k_ccl_final<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>(),
                                                         rt.prefix.get<uint32_t>(), Fs.get<uint32_t>());
-    const int64_t rows = g.D * g.H;
-    const unsigned grid = (unsigned)(rows < 262144 ? rows : 262144);
-    launch_bccl_write<BT::BZ>(fg, g, B, l, Fs.get<uint32_t>(), dst, grid, s);
+    launch_bccl_write<BT::BZ>(Ms.get<uint8_t>(), g, B, l, Fs.get<uint32_t>(), dst, s);
     CLE_LAUNCHED();
     return CLE_OK;
 }
