@@ -29,17 +29,29 @@ struct Taps {
 };
 
 // ----------------------------------------------------------------- loaders/storers
+// raw(i) / cvt(r): the stored value and its conversion to float, for prefetches that
+// hold the loaded value in a register and convert it where it is used (a conversion
+// next to the load would make the load wait at once)
 struct LdF32 {
     const float* p;
+    using raw_t = float;
     __device__ __forceinline__ float operator()(int64_t i) const { return p[i]; }
+    __device__ __forceinline__ raw_t raw(int64_t i) const { return p[i]; }
+    __device__ __forceinline__ static float cvt(raw_t r) { return r; }
 };
 struct LdU16 {
     const uint16_t* p;
+    using raw_t = uint32_t;
     __device__ __forceinline__ float operator()(int64_t i) const { return (float)p[i]; }
+    __device__ __forceinline__ raw_t raw(int64_t i) const { return p[i]; }
+    __device__ __forceinline__ static float cvt(raw_t r) { return (float)r; }
 };
 struct LdU8 {
     const uint8_t* p;
+    using raw_t = uint32_t;
     __device__ __forceinline__ float operator()(int64_t i) const { return (float)p[i]; }
+    __device__ __forceinline__ raw_t raw(int64_t i) const { return p[i]; }
+    __device__ __forceinline__ static float cvt(raw_t r) { return (float)r; }
 };
 struct LdAny {
     DBuf b;
@@ -272,41 +284,51 @@ struct IsoW {
     float w[C + 1];
 };
 
+// Isotropic tile: ISO_TY x ISO_TX outputs per slice, two adjacent columns per thread.
+// A launch aims at ISO_BLOCKS blocks (two rounds of one block per CU): fewer, longer
+// z chunks re-read less of the z halo.
+constexpr int ISO_TX = 64, ISO_TY = 32, ISO_THREADS = ISO_TX * ISO_TY / 2;
+constexpr int ISO_BLOCKS = 512;
+
 template <int C>
 struct IsoLds {
-    static constexpr int AW = FB_TX + 2 * C, AH = FB_TY + 2 * C;
-    static constexpr int AWP = (AW + 3) / 4 * 4;  // row pitch: rows start 16-byte aligned
+    static constexpr int AW = ISO_TX + 2 * C, AH = ISO_TY + 2 * C;
+    static constexpr int AWP = (AW + 3) / 4 * 4;  // row pitch of both buffers (even: float2 reads)
+    static constexpr int ASZ = AH * AWP;          // one staged slice (with x and y halo)
+    static constexpr int BSZ = ISO_TY * AWP;      // one y-blurred slice (with x halo)
 };
 
 // One column tile (bid = tile index inside the image) of the isotropic blur; A and B
-// are the block's LDS buffers of IsoLds<C>::AH * AWP and AH * FB_TX floats.
+// hold two IsoLds<C>::ASZ and two BSZ floats (LDS).  Per input slice: the staged tile
+// (x and y halo, clamped reads) is blurred along y (A -> B, over the x halo columns
+// too), then along x into registers; the z pass runs over a register ring of the
+// last 2 * CZ + U xy-blurred values of the thread's two columns.  Doing y before x
+// blurs 1 + 2C / ISO_TX of the tile in the first pass instead of 1 + 2C / ISO_TY.
 template <int C, bool Z3, typename LD>
 __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, const Geom& g, int zc, int ntx, int nty,
                                               const IsoW<C>& w, int64_t bid, float* __restrict__ A,
                                               float* __restrict__ B) {
-    constexpr int AW = FB_TX + 2 * C, AH = FB_TY + 2 * C, NA = AW * AH;
-    constexpr int AWP = IsoLds<C>::AWP;
-    constexpr int PF = (NA + FB_THREADS - 1) / FB_THREADS;
+    constexpr int AW = IsoLds<C>::AW, AH = IsoLds<C>::AH, NA = AW * AH;
+    constexpr int AWP = IsoLds<C>::AWP, ASZ = IsoLds<C>::ASZ, BSZ = IsoLds<C>::BSZ;
+    constexpr int PF = (NA + ISO_THREADS - 1) / ISO_THREADS;
     constexpr int CZ = Z3 ? C : 0;
-    constexpr int U = C <= 10 ? 4 : 2;      // slices per ring shift (register budget); even
-    constexpr int RL = 2 * CZ + U;          // ring length
-    constexpr int XQ = FB_TX / 4;           // x-pass quads per row
-    constexpr int XT = AH * XQ;             // x-pass tasks (4 outputs each)
-    constexpr int XR = (XT + FB_THREADS - 1) / FB_THREADS;
-    constexpr int NW4 = (2 * C + 4 + 3) / 4;  // float4 reads per x-pass window
-    constexpr int ASZ = AH * AWP, BSZ = AH * FB_TX;  // one buffer of each; A and B hold two
+    constexpr int U = C <= 10 ? 4 : 2;   // slices per ring shift (register budget); even
+    constexpr int RL = 2 * CZ + U;       // ring length
+    constexpr int RY = 4;                // y-pass outputs per task (rows of one column)
+    constexpr int YT = AW * (ISO_TY / RY);
+    constexpr int YR = (YT + ISO_THREADS - 1) / ISO_THREADS;
     const int64_t tx = bid % ntx, rest = bid / ntx;
     const int64_t ty = rest % nty, tzc = rest / nty;
-    const int64_t x0 = tx * FB_TX, y0 = ty * FB_TY;
+    const int64_t x0 = tx * ISO_TX, y0 = ty * ISO_TY;
     const int64_t zb = tzc * zc;
     const int64_t ze = zb + zc < g.D ? zb + zc : g.D;
-    const int row = threadIdx.x / (FB_TX / 2);
-    const int col = (threadIdx.x % (FB_TX / 2)) * 2;
+    const int row = threadIdx.x / (ISO_TX / 2);
+    const int col = (threadIdx.x % (ISO_TX / 2)) * 2;
     const int Hm = (int)g.H - 1, Wm = (int)g.W - 1, Wi = (int)g.W;
     int off[PF], lds[PF];
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
-        const int idx = threadIdx.x + k * FB_THREADS;
+        const int idx = threadIdx.x + k * ISO_THREADS;
         const int ay = idx / AW, ax = idx - ay * AW;
         int gy = (int)y0 - C + ay, gx = (int)x0 - C + ax;
         gy = gy < 0 ? 0 : (gy > Hm ? Hm : gy);
@@ -314,82 +336,69 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
         off[k] = gy * Wi + gx;
         lds[k] = ay * AWP + ax;
     }
-    float pa[PF], pb[PF];
+    using R = typename LD::raw_t;
+    R pa[PF], pb[PF];
     float r0[RL], r1[RL];
 #pragma unroll
     for (int j = 0; j < RL; ++j) { r0[j] = 0.f; r1[j] = 0.f; }
     const int64_t z00 = zb - CZ;           // input slice k is z00 + k
     const int64_t N = ze + CZ - z00;       // input slices
 
-    auto load = [&](float (&pf)[PF], int64_t k) {
+    auto load = [&](R (&pf)[PF], int64_t k) {
         if (k >= N) return;
         const int64_t zi = z00 + k;
         const int64_t zz = zi < 0 ? 0 : (zi >= g.D ? g.D - 1 : zi);
         const int64_t base = zz * g.H * g.W;
 #pragma unroll
         for (int m = 0; m < PF; ++m)
-            if (threadIdx.x + m * FB_THREADS < NA) pf[m] = in(base + off[m]);
+            if (threadIdx.x + m * ISO_THREADS < NA) pf[m] = in.raw(base + off[m]);
     };
-    auto store = [&](const float (&pf)[PF], float* a) {
+    auto store = [&](const R (&pf)[PF], float* a) {
 #pragma unroll
         for (int m = 0; m < PF; ++m)
-            if (threadIdx.x + m * FB_THREADS < NA) a[lds[m]] = pf[m];
+            if (threadIdx.x + m * ISO_THREADS < NA) a[lds[m]] = LD::cvt(pf[m]);
     };
-    // x pass of one staged slice: a (LDS) -> b (LDS)
-    auto xpass = [&](const float* __restrict__ a, float* __restrict__ bo) {
-        if constexpr (C > 12) {
-            // wide kernels: one output per task, taps read from LDS (keeps the window
-            // out of registers)
-            constexpr int XO = (AH * FB_TX + FB_THREADS - 1) / FB_THREADS;
+    // The x and y passes take the taps in order (w[|j - C|] * in[j], j = 0 .. 2C),
+    // one fused multiply-add each, as the values stream in from LDS.
+    // y pass of one staged slice: a (LDS) -> bo (LDS); a task is RY rows of one
+    // column (consecutive lanes take consecutive columns)
+    auto ypass = [&](const float* __restrict__ a, float* __restrict__ bo) {
 #pragma unroll
-            for (int k = 0; k < XO; ++k) {
-                const int idx = threadIdx.x + k * FB_THREADS;
-                if (idx < AH * FB_TX) {
-                    const int ay = idx / FB_TX, bx = idx - ay * FB_TX;
-                    const float* ap = a + ay * AWP + bx + C;
-                    float acc = w.w[0] * ap[0];
+        for (int k = 0; k < YR; ++k) {
+            const int t = threadIdx.x + k * ISO_THREADS;
+            if (t < YT) {
+                const int q = t / AW, c = t - q * AW;
+                const float* ap = a + q * RY * AWP + c;
+                float acc[RY];
 #pragma unroll
-                    for (int v = 1; v <= C; ++v) acc = fmaf(w.w[v], ap[-v] + ap[v], acc);
-                    bo[idx] = acc;
-                }
-            }
-        } else {
+                for (int j = 0; j < RY; ++j) acc[j] = 0.f;
 #pragma unroll
-            for (int k = 0; k < XR; ++k) {
-                const int t = threadIdx.x + k * FB_THREADS;
-                if (t < XT) {
-                    const int ay = t / XQ, q = t - ay * XQ;
-                    const float4* a4 = (const float4*)(a + ay * AWP + 4 * q);
-                    float av[4 * NW4];
+                for (int m = 0; m < RY + 2 * C; ++m) {
+                    const float av = ap[m * AWP];
 #pragma unroll
-                    for (int m = 0; m < NW4; ++m) {
-                        const float4 f = a4[m];
-                        av[4 * m] = f.x; av[4 * m + 1] = f.y; av[4 * m + 2] = f.z; av[4 * m + 3] = f.w;
+                    for (int j = 0; j < RY; ++j) {
+                        const int d = m - j - C;  // tap offset of input row m for output j
+                        if (d >= -C && d <= C) acc[j] = fmaf(w.w[d < 0 ? -d : d], av, acc[j]);
                     }
-                    float o[4];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        float acc = w.w[0] * av[C + j];
-#pragma unroll
-                        for (int v = 1; v <= C; ++v) acc = fmaf(w.w[v], av[C + j - v] + av[C + j + v], acc);
-                        o[j] = acc;
-                    }
-                    *(float4*)(bo + ay * FB_TX + 4 * q) = make_float4(o[0], o[1], o[2], o[3]);
                 }
+#pragma unroll
+                for (int j = 0; j < RY; ++j) bo[(q * RY + j) * AWP + c] = acc[j];
             }
         }
     };
-    // y pass of slice k (LDS -> the thread's two columns), ring slot u, output when
-    // the ring covers the slice k - CZ
-    auto ypass = [&](const float* __restrict__ bi, int64_t k, int u) {
-        const float* bp = bi + (row + C) * FB_TX + col;
-        const float2 bc = *(const float2*)bp;
-        float v0 = w.w[0] * bc.x, v1 = w.w[0] * bc.y;
+    // x pass of slice k (LDS -> the thread's two columns), ring slot u, z pass and
+    // output when the ring covers the slice k - CZ
+    auto xpass = [&](const float* __restrict__ bi, int64_t k, int u) {
+        const float* bp = bi + row * AWP + col;  // B columns col .. col + 2C + 1
+        float v0 = 0.f, v1 = 0.f;
 #pragma unroll
-        for (int v = 1; v <= C; ++v) {
-            const float2 lo = *(const float2*)(bp - v * FB_TX), hi = *(const float2*)(bp + v * FB_TX);
-            v0 = fmaf(w.w[v], lo.x + hi.x, v0);
-            v1 = fmaf(w.w[v], lo.y + hi.y, v1);
+        for (int m = 0; m <= C; ++m) {
+            const float2 f = *(const float2*)(bp + 2 * m);
+            const int d0 = 2 * m - C, d1 = 2 * m + 1 - C;  // offsets of f.x, f.y for column col
+            if (d0 <= C) v0 = fmaf(w.w[d0 < 0 ? -d0 : d0], f.x, v0);
+            if (d1 <= C) v0 = fmaf(w.w[d1 < 0 ? -d1 : d1], f.y, v0);
+            if (d0 - 1 >= -C) v1 = fmaf(w.w[d0 - 1 < 0 ? 1 - d0 : d0 - 1], f.x, v1);
+            v1 = fmaf(w.w[d1 - 1 < 0 ? 1 - d1 : d1 - 1], f.y, v1);
         }
         r0[2 * CZ + u] = v0;
         r1[2 * CZ + u] = v1;
@@ -417,8 +426,8 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
     };
 
     // Software pipeline over the input slices, one barrier per phase: phase p stages
-    // slice p (into A[p & 1]) and prefetches slice p + 2, runs the x pass of slice
-    // p - 1 (A[(p-1) & 1] -> B[(p-1) & 1]) and the y pass of slice p - 2 (from
+    // slice p (into A[p & 1]) and prefetches slice p + 2, runs the y pass of slice
+    // p - 1 (A[(p-1) & 1] -> B[(p-1) & 1]) and the x pass of slice p - 2 (from
     // B[p & 1]).  The buffers a phase writes were last read one phase earlier.
     load(pa, 0);
     load(pb, 1);
@@ -429,12 +438,12 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
         store(pb, A + ASZ);
         load(pb, 3);
     }
-    xpass(A, B);
+    ypass(A, B);
     __syncthreads();
     for (int64_t g0 = 0; g0 < N; g0 += U) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int64_t k = g0 + u;  // slice of the y pass; p = k + 2 has parity u & 1
+            const int64_t k = g0 + u;  // slice of the x pass; p = k + 2 has parity u & 1
             if (k < N) {
                 float* Ap = A + ((u & 1) ? ASZ : 0);       // A[p & 1]
                 float* Aq = A + ((u & 1) ? 0 : ASZ);       // A[(p - 1) & 1]
@@ -444,8 +453,8 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
                     if (u & 1) { store(pb, Ap); load(pb, k + 4); }
                     else { store(pa, Ap); load(pa, k + 4); }
                 }
-                if (k + 1 < N) xpass(Aq, Bq);
-                ypass(Bk, k, u);
+                if (k + 1 < N) ypass(Aq, Bq);
+                xpass(Bk, k, u);
                 __syncthreads();
             }
         }
@@ -455,10 +464,10 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
 }
 
 template <int C, bool Z3, typename LD>
-__global__ __launch_bounds__(FB_THREADS) void k_blur_iso(LD in, float* __restrict__ out, Geom g, int zc, int ntx,
+__global__ __launch_bounds__(ISO_THREADS) void k_blur_iso(LD in, float* __restrict__ out, Geom g, int zc, int ntx,
                                                         int nty, const IsoW<C> w) {
-    __shared__ __align__(16) float A[2 * IsoLds<C>::AH * IsoLds<C>::AWP];
-    __shared__ __align__(16) float B[2 * IsoLds<C>::AH * FB_TX];
+    __shared__ __align__(16) float A[2 * IsoLds<C>::ASZ];
+    __shared__ __align__(16) float B[2 * IsoLds<C>::BSZ];
     blur_iso_tile<C, Z3>(in, out, g, zc, ntx, nty, w, xcd_tile(blockIdx.x, gridDim.x), A, B);
 }
 
@@ -480,10 +489,10 @@ template <> __device__ __forceinline__ LdU16 make_ld<LdU16>(const void* p) { ret
 template <> __device__ __forceinline__ LdU8 make_ld<LdU8>(const void* p) { return LdU8{(const uint8_t*)p}; }
 
 template <int C, bool Z3, typename LD>
-__global__ __launch_bounds__(FB_THREADS) void k_blur_iso_batch(const IsoJob<C>* __restrict__ jobs,
+__global__ __launch_bounds__(ISO_THREADS) void k_blur_iso_batch(const IsoJob<C>* __restrict__ jobs,
                                                               const int64_t* __restrict__ bstart, int njobs) {
-    __shared__ __align__(16) float A[2 * IsoLds<C>::AH * IsoLds<C>::AWP];
-    __shared__ __align__(16) float B[2 * IsoLds<C>::AH * FB_TX];
+    __shared__ __align__(16) float A[2 * IsoLds<C>::ASZ];
+    __shared__ __align__(16) float B[2 * IsoLds<C>::BSZ];
     const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
     const int j = find_seg(bstart, njobs, b);
     const IsoJob<C>& jb = jobs[j];
@@ -570,10 +579,10 @@ static int try_fused(const cle_buf* src, const cle_buf* dst, const Taps* taps[3]
 
 template <int C, bool Z3, typename LD>
 static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hipStream_t s) {
-    const int ntx = (int)((g.W + FB_TX - 1) / FB_TX), nty = (int)((g.H + FB_TY - 1) / FB_TY);
+    const int ntx = (int)((g.W + ISO_TX - 1) / ISO_TX), nty = (int)((g.H + ISO_TY - 1) / ISO_TY);
     const int64_t nxy = (int64_t)ntx * nty;
     constexpr int CZ = Z3 ? C : 0;
-    int64_t nzc = (2048 + nxy - 1) / nxy;
+    int64_t nzc = (ISO_BLOCKS + nxy - 1) / nxy;
     const int64_t max_nzc = g.D / (4 * CZ + 8) > 1 ? g.D / (4 * CZ + 8) : 1;
     if (nzc > max_nzc) nzc = max_nzc;
     if (nzc < 1) nzc = 1;
@@ -581,7 +590,7 @@ static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hip
     nzc = (g.D + zc - 1) / zc;
     IsoW<C> w;
     for (int v = 0; v <= C; ++v) w.w[v] = wts[v];
-    k_blur_iso<C, Z3, LD><<<(unsigned)(nxy * nzc), FB_THREADS, 0, s>>>(in, out, g, zc, ntx, nty, w);
+    k_blur_iso<C, Z3, LD><<<(unsigned)(nxy * nzc), ISO_THREADS, 0, s>>>(in, out, g, zc, ntx, nty, w);
 }
 
 constexpr int ISO_MAXC = 20;
@@ -760,8 +769,8 @@ static void plan_iso_group(const std::vector<BlurReq>& reqs, const std::vector<i
         jb.in = r.in;
         jb.out = r.out;
         jb.g = g;
-        jb.ntx = (int)((g.W + FB_TX - 1) / FB_TX);
-        jb.nty = (int)((g.H + FB_TY - 1) / FB_TY);
+        jb.ntx = (int)((g.W + ISO_TX - 1) / ISO_TX);
+        jb.nty = (int)((g.H + ISO_TY - 1) / ISO_TY);
         const int64_t nxy = (int64_t)jb.ntx * jb.nty;
         // the batch fills the chip: z chunks only for images with few column tiles
         int64_t nzc = (256 + nxy - 1) / nxy;
@@ -780,7 +789,7 @@ static void plan_iso_group(const std::vector<BlurReq>& reqs, const std::vector<i
     const size_t jo = blob.put_vec(jobs), bo = blob.put_vec(bst);
     const int nj = (int)idx.size();
     out.push_back([=](const char* dev) -> int {
-        k_blur_iso_batch<C, Z3, LD><<<(unsigned)nb, FB_THREADS, 0, s>>>((const IsoJob<C>*)(dev + jo),
+        k_blur_iso_batch<C, Z3, LD><<<(unsigned)nb, ISO_THREADS, 0, s>>>((const IsoJob<C>*)(dev + jo),
                                                                        (const int64_t*)(dev + bo), nj);
         CLE_LAUNCHED();
         return CLE_OK;

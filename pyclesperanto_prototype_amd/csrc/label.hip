@@ -994,21 +994,37 @@ __global__ __launch_bounds__(CT_THREADS) void k_ccl_final_batch(const BImg* __re
     }
 }
 
-// float32 labels of the uint8 work image: one block per row of one image
+// float32 labels of the work image from the block masks, as k_bccl_write: one
+// block per row of 2x2x2 blocks of one image
 template <int BZ>
-__global__ void k_bccl_write_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ rstart, int nimg,
-                                   const uint8_t* S, const uint32_t* L, const uint32_t* F, float* out) {
+__global__ void k_bccl_write_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ brstart, int nimg,
+                                   const uint8_t* __restrict__ M, const uint32_t* __restrict__ L,
+                                   const uint32_t* __restrict__ F, float* out) {
     const int64_t r = blockIdx.x;
-    const int j = find_seg(rstart, nimg, r);
+    const int j = find_seg(brstart, nimg, r);
     const BImg& im = imgs[j];
     const Geom g = im.g;
-    const int64_t row = r - rstart[j];
-    const int64_t z = row / g.H, y = row - z * g.H;
     const BGrid B{(g.D + BZ - 1) / BZ, (g.H + 1) / 2, (g.W + 1) / 2};
-    const int64_t rb = im.boff + ((z / BZ) * B.H + (y >> 1)) * B.W;
-    const uint8_t* bin = S + im.off + row * g.W;
-    float* o = out + im.off + row * g.W;
-    for (int64_t x = threadIdx.x; x < g.W; x += blockDim.x) o[x] = bin[x] ? (float)F[L[rb + (x >> 1)]] : 0.f;
+    const int64_t br = r - brstart[j];
+    const int64_t bz = br / B.H, by = br - bz * B.H;
+    const int64_t z0 = bz * BZ, y0 = by * 2;
+    const int nz = (BZ == 2 && z0 + 1 < g.D) ? 2 : 1, ny = y0 + 1 < g.H ? 2 : 1;
+    const int64_t np = (B.W + 1) >> 1;
+    for (int64_t p = threadIdx.x; p < np; p += blockDim.x) {
+        const int64_t b = im.boff + br * B.W + 2 * p, x = 4 * p;
+        const uint32_t m2 = M[b] | (2 * p + 1 < B.W ? (uint32_t)M[b + 1] << 8 : 0u);
+        const float f0 = (m2 & 0xFFu) ? (float)F[L[b]] : 0.f;
+        const float f1 = (m2 >> 8) ? (float)F[L[b + 1]] : 0.f;
+        for (int dz = 0; dz < nz; ++dz)
+            for (int dy = 0; dy < ny; ++dy) {
+                const int sh = dz * 4 + dy * 2;
+                float* o = out + im.off + ((z0 + dz) * g.H + y0 + dy) * g.W + x;
+                o[0] = (m2 >> sh) & 1u ? f0 : 0.f;
+                if (x + 1 < g.W) o[1] = (m2 >> (sh + 1)) & 1u ? f0 : 0.f;
+                if (x + 2 < g.W) o[2] = (m2 >> (sh + 8)) & 1u ? f1 : 0.f;
+                if (x + 3 < g.W) o[3] = (m2 >> (sh + 9)) & 1u ? f1 : 0.f;
+            }
+    }
 }
 
 template <typename BT>
@@ -1025,7 +1041,7 @@ static int plan_ccl_kind(const std::vector<BImg>& imgs, size_t imgs_off, bool wa
         if (is3d(imgs[j].g) != want3d || imgs[j].g.n() == 0) continue;
         const BGrid B = bgrid_of<BT>(imgs[j].g);
         nt += tgrid_of<BT>(B).count();
-        nr += imgs[j].g.D * imgs[j].g.H;
+        nr += B.D * B.H;  // rows of blocks
     }
     tst[nimg] = nt;
     rst[nimg] = nr;
@@ -1059,7 +1075,7 @@ static int plan_ccl_kind(const std::vector<BImg>& imgs, size_t imgs_off, bool wa
         k_ccl_final_batch<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(im, ts, nimg, L, K, roots, nroots,
                                                                  rt->bm.get<uint32_t>(), rt->prefix.get<uint32_t>(),
                                                                  Fl);
-        k_bccl_write_batch<BT::BZ><<<(unsigned)nr, 256, 0, s>>>(im, rs, nimg, S, L, Fl, F);
+        k_bccl_write_batch<BT::BZ><<<(unsigned)nr, 256, 0, s>>>(im, rs, nimg, M, L, Fl, F);
         CLE_LAUNCHED();
         return CLE_OK;
     });
