@@ -821,12 +821,21 @@ __global__ void k_bccl_write(const uint8_t* __restrict__ M, Geom g, BGrid B, con
         for (int64_t p = threadIdx.x; p < np; p += blockDim.x) {
             const int64_t b = br * B.W + 2 * p, x = 4 * p;
             uint32_t m2;
-            if constexpr (VEC)
-                m2 = *(const uint16_t*)(M + b);  // B.W even
-            else
+            float f0, f1;
+            if constexpr (VEC) {
+                // B.W even: the pair's mask and both roots load together (a root of
+                // an empty block is not used), then both labels together
+                m2 = *(const uint16_t*)(M + b);
+                const uint2 l2 = *(const uint2*)(L + b);
+                const uint32_t i0 = (m2 & 0xFFu) ? l2.x : 0u, i1 = (m2 >> 8) ? l2.y : 0u;
+                const uint32_t g0 = F[i0], g1 = F[i1];
+                f0 = (m2 & 0xFFu) ? (float)g0 : 0.f;
+                f1 = (m2 >> 8) ? (float)g1 : 0.f;
+            } else {
                 m2 = M[b] | (2 * p + 1 < B.W ? (uint32_t)M[b + 1] << 8 : 0u);
-            const float f0 = (m2 & 0xFFu) ? (float)F[L[b]] : 0.f;
-            const float f1 = (m2 >> 8) ? (float)F[L[b + 1]] : 0.f;
+                f0 = (m2 & 0xFFu) ? (float)F[L[b]] : 0.f;
+                f1 = (m2 >> 8) ? (float)F[L[b + 1]] : 0.f;
+            }
             for (int dz = 0; dz < nz; ++dz)
                 for (int dy = 0; dy < ny; ++dy) {
                     const int sh = dz * 4 + dy * 2;

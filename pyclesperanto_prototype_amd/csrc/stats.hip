@@ -32,18 +32,26 @@
 
 namespace cle {
 
+// raw(i) / cvt(r): the stored label and its conversion, so that a prefetch holds the
+// loaded value and converts it where it is used
 template <typename T>
 struct LabT {
     using type = T;
+    using raw_t = T;
     const T* p;
     // the reference reads a label as (int) of the float it loads
     __device__ __forceinline__ int operator()(int64_t i) const { return (int)(float)p[i]; }
     __device__ __forceinline__ bool nz(int64_t i) const { return p[i] != (T)0; }
+    __device__ __forceinline__ raw_t raw(int64_t i) const { return p[i]; }
+    __device__ __forceinline__ static int cvt(raw_t r) { return (int)(float)r; }
 };
 struct LabAny {
     DBuf b;
+    using raw_t = float;
     __device__ __forceinline__ int operator()(int64_t i) const { return (int)ld(b, i); }
     __device__ __forceinline__ bool nz(int64_t i) const { return cle::nz(b, i); }
+    __device__ __forceinline__ raw_t raw(int64_t i) const { return ld(b, i); }
+    __device__ __forceinline__ static int cvt(raw_t r) { return (int)r; }
 };
 
 struct KeyBits {
@@ -79,24 +87,36 @@ template <> struct Vec4<int32_t> { using V = int4; };
 template <> struct Vec4<uint32_t> { using V = uint4; };
 template <> struct Vec4<float> { using V = float4; };
 
+// A span's labels as stored (no branch: lanes past the row end load voxel 0 of the
+// span and are masked in runs_cvt), so the loads stay in flight until used.
 template <bool VEC, typename LAB>
-__device__ __forceinline__ void runs_load(const LAB& lab, int64_t base, int n, int lane, int (&v)[RB_SPAN / 64]) {
+__device__ __forceinline__ void runs_fetch(const LAB& lab, int64_t base, int n, int lane,
+                                           typename LAB::raw_t (&r)[RB_SPAN / 64]) {
     constexpr int E = VEC ? 4 : 1;
     constexpr int K = RB_SPAN / (64 * E);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int i0 = 64 * E * k + E * lane;
-        if constexpr (VEC) {
-            if (i0 < n) {  // n is a multiple of 4: all four or none
-                const auto q = *(const typename Vec4<typename LAB::type>::V*)(lab.p + base + i0);
-                v[4 * k + 0] = (int)(float)q.x; v[4 * k + 1] = (int)(float)q.y;
-                v[4 * k + 2] = (int)(float)q.z; v[4 * k + 3] = (int)(float)q.w;
-            } else {
-                v[4 * k + 0] = v[4 * k + 1] = v[4 * k + 2] = v[4 * k + 3] = 0;
-            }
+        const int j = i0 < n ? i0 : 0;
+        if constexpr (VEC) {  // n is a multiple of 4: all four or none
+            const auto q = *(const typename Vec4<typename LAB::type>::V*)(lab.p + base + j);
+            r[4 * k + 0] = q.x; r[4 * k + 1] = q.y; r[4 * k + 2] = q.z; r[4 * k + 3] = q.w;
         } else {
-            v[k] = i0 < n ? lab(base + i0) : 0;
+            r[k] = lab.raw(base + j);
         }
+    }
+}
+
+template <bool VEC, typename LAB>
+__device__ __forceinline__ void runs_cvt(const typename LAB::raw_t (&r)[RB_SPAN / 64], int n, int lane,
+                                         int (&v)[RB_SPAN / 64]) {
+    constexpr int E = VEC ? 4 : 1;
+    constexpr int K = RB_SPAN / (64 * E);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i0 = 64 * E * k + E * lane;
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[E * k + e] = i0 < n ? LAB::cvt(r[E * k + e]) : 0;
     }
 }
 
@@ -116,7 +136,7 @@ __global__ __launch_bounds__(RB_THREADS) void k_runs(LAB lab, Geom g, int64_t lc
     int lmax = 0;
     uint32_t cmax = 0;
     // the next span's labels are loaded while this span is processed
-    int nv[NV];
+    typename LAB::raw_t nr[NV];
     auto span_of = [&](int64_t sp, int64_t& row, int64_t& sx0, int& n) {
         row = sp / spans;
         sx0 = (sp - row * spans) * RB_SPAN;
@@ -126,20 +146,23 @@ __global__ __launch_bounds__(RB_THREADS) void k_runs(LAB lab, Geom g, int64_t lc
         int64_t row, sx0;
         int n;
         span_of(wave0, row, sx0, n);
-        runs_load<VEC>(lab, row * g.W + sx0, n, lane, nv);
+        runs_fetch<VEC>(lab, row * g.W + sx0, n, lane, nr);
     }
     for (int64_t sp = wave0; sp < total; sp += nwaves) {
         int64_t row, sx0;
         int n;
         span_of(sp, row, sx0, n);
         int v[NV];
-#pragma unroll
-        for (int k = 0; k < NV; ++k) v[k] = nv[k];
+        runs_cvt<VEC, LAB>(nr, n, lane, v);
+        // both modes know where the span's runs go before they are found (the offset
+        // is read before the next span's loads are issued, so waiting for it does not
+        // wait for them)
+        const uint64_t slot0 = off ? (uint64_t)off[sp] : (uint64_t)sp * scap;
         if (sp + nwaves < total) {
             int64_t row2, sx2;
             int n2;
             span_of(sp + nwaves, row2, sx2, n2);
-            runs_load<VEC>(lab, row2 * g.W + sx2, n2, lane, nv);
+            runs_fetch<VEC>(lab, row2 * g.W + sx2, n2, lane, nr);
         }
         bool any = false;
 #pragma unroll
@@ -152,8 +175,6 @@ __global__ __launch_bounds__(RB_THREADS) void k_runs(LAB lab, Geom g, int64_t lc
             if (lane == 0) count[sp] = 0;
             continue;
         }
-        // both modes know where the span's runs go before they are found
-        const uint64_t slot0 = off ? (uint64_t)off[sp] : (uint64_t)sp * scap;
         const uint32_t lim = off ? 0xFFFFFFFFu : scap;
         const uint64_t y = (uint64_t)(row % g.H), z = (uint64_t)(row / g.H);
         uint32_t ns = 0, ne = 0;
@@ -307,23 +328,49 @@ struct CellGeom {
     }
 };
 
-// intensity loaders
-struct InU16 {
-    const uint16_t* p;
-    __device__ __forceinline__ float operator()(int64_t i) const { return (float)p[i]; }
-};
-struct InF32 {
-    const float* p;
-    __device__ __forceinline__ float operator()(int64_t i) const { return p[i]; }
-};
-struct InAny {
-    DBuf b;
-    __device__ __forceinline__ float operator()(int64_t i) const { return ld(b, i); }
-};
-
 // Voxels of a cell are visited in the reference's (z, x) order; loads go out CB at a
 // time (independent), the float32 accumulation then runs over them in order.
 constexpr int CB = 8;
+
+// intensity loaders; raw(i) / cvt(r) as for the labels
+struct InU16 {
+    const uint16_t* p;
+    using raw_t = uint32_t;
+    __device__ __forceinline__ float operator()(int64_t i) const { return (float)p[i]; }
+    __device__ __forceinline__ raw_t raw(int64_t i) const { return p[i]; }
+    __device__ __forceinline__ static float cvt(raw_t r) { return (float)r; }
+};
+struct InF32 {
+    const float* p;
+    using raw_t = float;
+    __device__ __forceinline__ float operator()(int64_t i) const { return p[i]; }
+    __device__ __forceinline__ raw_t raw(int64_t i) const { return p[i]; }
+    __device__ __forceinline__ static float cvt(raw_t r) { return r; }
+};
+struct InAny {
+    DBuf b;
+    using raw_t = float;
+    __device__ __forceinline__ float operator()(int64_t i) const { return ld(b, i); }
+    __device__ __forceinline__ raw_t raw(int64_t i) const { return ld(b, i); }
+    __device__ __forceinline__ static float cvt(raw_t r) { return r; }
+};
+
+// CB voxels of a run from x (x < xe): branch-free loads (indices past the run end
+// repeat its last voxel), all in flight before the first is converted
+template <typename IN>
+__device__ __forceinline__ void cell_fetch(const IN& in, int64_t base, int64_t x, int64_t xe, float (&v)[CB]) {
+    typename IN::raw_t r[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) r[j] = in.raw(base + (x + j < xe ? x + j : xe - 1));
+    static_assert(CB == 8, "the register fence below names CB values");
+    // an empty fence that reads all CB values: the compiler issues every load before
+    // it (instead of sinking each one into the conditional code that uses it)
+    asm volatile("" : "+v"(r[0]), "+v"(r[1]), "+v"(r[2]), "+v"(r[3]), "+v"(r[4]), "+v"(r[5]), "+v"(r[6]),
+                 "+v"(r[7]));
+#pragma unroll
+    for (int j = 0; j < CB; ++j) v[j] = IN::cvt(r[j]);
+}
+
 
 // pass 1, one thread per cell: 16 float32 planes in the reference's order:
 // 0-2 sum x,y,z  3 count  4-6 sum x*v, y*v, z*v  7 sum v  8/9 min/max v  10-15 min/max x,y,z
@@ -353,8 +400,7 @@ __global__ __launch_bounds__(256) void k_cells_pass1(IN in, CellGeom cg, const u
             const int64_t base = (z * cg.H + y) * cg.W;
             for (int64_t x = x0; x < xe; x += CB) {
                 float v[CB];
-#pragma unroll
-                for (int j = 0; j < CB; ++j) v[j] = x + j < xe ? in(base + x + j) : 0.f;
+                cell_fetch(in, base, x, xe, v);
 #pragma unroll
                 for (int j = 0; j < CB; ++j) {
                     if (x + j >= xe) break;
@@ -451,8 +497,7 @@ __global__ __launch_bounds__(256) void k_cells_pass2(IN in, CellGeom cg, const u
             const int64_t base = (z * cg.H + y) * cg.W;
             for (int64_t x = x0; x < xe; x += CB) {
                 float v[CB];
-#pragma unroll
-                for (int j = 0; j < CB; ++j) v[j] = x + j < xe ? in(base + x + j) : 0.f;
+                cell_fetch(in, base, x, xe, v);
 #pragma unroll
                 for (int j = 0; j < CB; ++j) {
                     if (x + j >= xe) break;
