@@ -146,6 +146,10 @@ int cle_label_spots(const cle_buf* src, const cle_buf* dst, void* stream);
  * the number of growth steps is stored there (host). */
 int cle_masked_voronoi_labeling(const cle_buf* binary, const cle_buf* mask, const cle_buf* dst,
                                 int* n_iterations, void* stream);
+/* _tier4/_extend_labeling_via_voronoi.py:12-61: the labels (converted to float32)
+ * grown into the zero voxels by alternating box / diamond steps until no change,
+ * then stored in dst's type.  No mask.  n_iterations as above. */
+int cle_extend_labeling_via_voronoi(const cle_buf* src, const cle_buf* dst, int* n_iterations, void* stream);
 /* _tier9/_voronoi_otsu_labeling.py:11-73: blur(spot) -> maxima -> blur(outline) ->
  * Otsu -> AND -> masked Voronoi -> mask.  Synchronises only inside the Voronoi
  * convergence test. */

@@ -430,6 +430,39 @@ def masked_voronoi_labeling(binary: np.ndarray, mask_image: np.ndarray, out_dtyp
     return (out, it) if return_iterations else out
 
 
+def extend_labeling_via_voronoi(labels: np.ndarray, out_dtype=np.uint32, return_iterations: bool = False):
+    """flip = float32(labels); alternate box (even iterations, flip -> flop) / diamond
+    (odd, flop -> flip) growth until an iteration changes nothing; the last written
+    buffer is copied (saturating) into the destination type.  No mask.
+
+    Reference: pyclesperanto_prototype/_tier4/_extend_labeling_via_voronoi.py:39-61.
+    Pinned by tests/test_extend_labeling_via_voronoi.py:4-36 of the reference and by
+    voronoi_labeling (_tier5/_voronoi_labeling.py:33-37 = CCL box + this function),
+    whose reference test array it must reproduce.
+    """
+    ndim = labels.ndim
+    flip = np.asarray(labels).astype(np.float32)
+    flop = flip
+    it = 0
+    flag = True
+    while flag:
+        if it % 2 == 0:
+            flop, flag = onlyzero_overwrite_maximum(flip, "box", ndim)
+        else:
+            flip, flag = onlyzero_overwrite_maximum(flop, "diamond", ndim)
+        it += 1
+    last = flip if it % 2 == 0 else flop
+    out = convert_sat(last, out_dtype)
+    return (out, it) if return_iterations else out
+
+
+def voronoi_labeling(binary: np.ndarray, out_dtype=np.uint32) -> np.ndarray:
+    """CCL box into float32, then extend_labeling_via_voronoi.
+
+    Reference: pyclesperanto_prototype/_tier5/_voronoi_labeling.py:33-37."""
+    return extend_labeling_via_voronoi(connected_components_labeling(binary, "box", np.float32), out_dtype)
+
+
 def voronoi_otsu_labeling(src: np.ndarray, spot_sigma: float = 2, outline_sigma: float = 2,
                           out_dtype=np.uint32) -> np.ndarray:
     """Reference: pyclesperanto_prototype/_tier9/_voronoi_otsu_labeling.py:57-73."""

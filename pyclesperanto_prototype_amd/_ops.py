@@ -257,13 +257,21 @@ def masked_voronoi_labeling(binary_source: Image, mask_image: Image, labeling_de
     return labeling_destination
 
 
+@plugin_function(categories=['label processing', 'in assistant'], output_creator=create_labels_like)
+def extend_labeling_via_voronoi(labeling_source: Image, labeling_destination: Image = None) -> Image:
+    """_tier4/_extend_labeling_via_voronoi.py:12-61: the labels grown into the zero
+    voxels (alternating box / diamond steps, float32 buffers) until they touch."""
+    _lib.call("cle_extend_labeling_via_voronoi", labeling_source.buf(), labeling_destination.buf(), None, _S())
+    return labeling_destination
+
+
 @plugin_function(categories=['label', 'in assistant', 'bia-bob-suggestion'], output_creator=create_labels_like)
 def voronoi_labeling(binary_source: Image, labeling_destination: Image = None) -> Image:
-    """_tier5/_voronoi_labeling.py:7-39: CCL box then unmasked growth (an all-ones mask)."""
-    ones = create(binary_source.shape, np.uint8)
-    _lib.call("cle_set", ones.buf(), 1.0, _S())
-    _lib.call("cle_masked_voronoi_labeling", binary_source.buf(), ones.buf(), labeling_destination.buf(), None, _S())
-    return labeling_destination
+    """_tier5/_voronoi_labeling.py:7-39: CCL box into a float32 image, then
+    extend_labeling_via_voronoi."""
+    flip = create_like(labeling_destination)
+    connected_components_labeling_box(binary_source, flip)
+    return extend_labeling_via_voronoi(flip, labeling_destination)
 
 
 @plugin_function(categories=['label', 'in assistant', 'bia-bob-suggestion'], priority=1,

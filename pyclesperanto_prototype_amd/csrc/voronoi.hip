@@ -53,8 +53,9 @@ __device__ __forceinline__ int64_t clampi(int64_t v, int64_t hi) { return v < 0 
 // tile = index inside the image, gtile = its slot in the flag arrays of ntiles slots
 // A tile is "settled" once both buffers hold the same values there and none is zero:
 // it can never change again, so the steps skip it (settled[gtile] = 1).
-template <typename MK>
-__device__ __forceinline__ bool vor_init_tile(MK mask, float* A, float* B, const Geom& g, const Tiles& T,
+// `val(i)` is the initial value of voxel i (it may read B[i] before B[i] is written).
+template <typename VF>
+__device__ __forceinline__ bool vor_init_tile(VF val, float* A, float* B, const Geom& g, const Tiles& T,
                                               int64_t tile, int64_t gtile, int64_t ntiles, uint8_t* tflags,
                                               uint8_t* settled) {
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
@@ -66,7 +67,7 @@ __device__ __forceinline__ bool vor_init_tile(MK mask, float* A, float* B, const
         const int64_t x = tx * T.tx + lx, y = ty * T.ty + ly, z = tz * T.tz + lz;
         if (x >= g.W || y >= g.H || z >= g.D) continue;
         const int64_t i = (z * g.H + y) * g.W + x;
-        const float v = (mask(i) - 1.f) + B[i];
+        const float v = val(i);
         A[i] = v;
         B[i] = v;
         pos |= v > 0.f;
@@ -85,17 +86,32 @@ __device__ __forceinline__ bool vor_init_tile(MK mask, float* A, float* B, const
     return any != 0;
 }
 
-struct MaskAny {
-    DBuf b;
-    __device__ __forceinline__ float operator()(int64_t i) const { return ld(b, i); }
+// masked growth: (mask - 1) + CCL label, the label held in B (float32 add, as the
+// reference's add_image_and_scalar then add_images)
+struct MaskedAny {
+    DBuf mask;
+    const float* B;
+    __device__ __forceinline__ float operator()(int64_t i) const { return (ld(mask, i) - 1.f) + B[i]; }
 };
-struct MaskU8 {
-    const uint8_t* p;
-    __device__ __forceinline__ float operator()(int64_t i) const { return (float)p[i]; }
+struct MaskedU8 {
+    const uint8_t* mask;
+    const float* B;
+    __device__ __forceinline__ float operator()(int64_t i) const { return ((float)mask[i] - 1.f) + B[i]; }
+};
+// unmasked growth of an existing label image: the labels converted to float32 as the
+// reference's copy into a float32 buffer does
+struct LabelsAny {
+    DBuf src;
+    __device__ __forceinline__ float operator()(int64_t i) const { return ld(src, i); }
 };
 
 __global__ __launch_bounds__(VB) void k_vor_init(DBuf mask, float* A, float* B, Geom g, Tiles T, uint8_t* tflags) {
-    vor_init_tile(MaskAny{mask}, A, B, g, T, blockIdx.x, blockIdx.x, T.count(), tflags, tflags + 4 * T.count());
+    vor_init_tile(MaskedAny{mask, B}, A, B, g, T, blockIdx.x, blockIdx.x, T.count(), tflags, tflags + 4 * T.count());
+}
+
+__global__ __launch_bounds__(VB) void k_vor_init_labels(DBuf src, float* A, float* B, Geom g, Tiles T,
+                                                        uint8_t* tflags) {
+    vor_init_tile(LabelsAny{src}, A, B, g, T, blockIdx.x, blockIdx.x, T.count(), tflags, tflags + 4 * T.count());
 }
 
 // one growth step t: src -> dst, box (t even) or diamond (t odd)
@@ -222,6 +238,11 @@ static int grow(float* A, float* B, const Geom& g, const Tiles& T, uint8_t* tfla
     }
     if (n_iterations) *n_iterations = iterations;
     return CLE_OK;
+}
+
+__global__ void k_store_out(const float* A, DBuf dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        st(dst, i, A[i]);
 }
 
 __global__ void k_mask_out(const float* A, DBuf mask, DBuf dst, int64_t n) {
@@ -357,7 +378,7 @@ __global__ __launch_bounds__(VB) void k_vor_init_batch(const BImg* __restrict__ 
     const int j = find_seg(ts, nimg, b);
     const BImg& im = imgs[j];
     const Tiles T = make_tiles(im.g);
-    if (vor_init_tile(MaskU8{G + im.off}, A + im.off, B + im.off, im.g, T, b - ts[j], b, 0, nullptr, L.settled) &&
+    if (vor_init_tile(MaskedU8{G + im.off, B + im.off}, A + im.off, B + im.off, im.g, T, b - ts[j], b, 0, nullptr, L.settled) &&
         threadIdx.x < 64)
         vor_queue(T, b - ts[j], ts[j], 0, 1, L);
 }
@@ -580,6 +601,28 @@ int cle_masked_voronoi_labeling(const cle_buf* binary, const cle_buf* mask, cons
     CLE_TRY(a.alloc((size_t)n * 4, s));
     CLE_TRY(b.alloc((size_t)n * 4, s));
     return masked_voronoi_core(dbuf(binary), dbuf(mask), dbuf(dst), g, a.get<float>(), b.get<float>(), n_iterations, s);
+}
+
+int cle_extend_labeling_via_voronoi(const cle_buf* src, const cle_buf* dst, int* n_iterations, void* stream) {
+    Geom g, gd;
+    CLE_TRY(check_buf(src, "labeling_source", &g));
+    CLE_TRY(check_buf(dst, "labeling_destination", &gd));
+    CLE_TRY(check_same_shape(g, gd, "labeling_source", "labeling_destination"));
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = g.n();
+    if (!n) return CLE_OK;
+    Scratch a, b, tf;
+    CLE_TRY(a.alloc((size_t)n * 4, s));
+    CLE_TRY(b.alloc((size_t)n * 4, s));
+    const Tiles T = make_tiles(g);
+    CLE_TRY(tf.alloc((size_t)T.count() * 5, s));  // 4 "changed" slots + settled
+    k_vor_init_labels<<<(unsigned)T.count(), VB, 0, s>>>(dbuf(src), a.get<float>(), b.get<float>(), g, T,
+                                                        tf.get<uint8_t>());
+    CLE_LAUNCHED();
+    CLE_TRY(grow(a.get<float>(), b.get<float>(), g, T, tf.get<uint8_t>(), n_iterations, s));
+    k_store_out<<<vg(n), 256, 0, s>>>(a.get<float>(), dbuf(dst), n);
+    CLE_LAUNCHED();
+    return CLE_OK;
 }
 
 int cle_voronoi_otsu_labeling(const cle_buf* src, const cle_buf* dst, float spot_sigma, float outline_sigma,

@@ -80,7 +80,8 @@ def test_api_surface_matches_reference_names():
                  "connected_components_labeling_box", "relabel_sequential", "label_spots",
                  "statistics_of_labelled_pixels", "statistics_of_labeled_pixels", "label",
                  "close_index_gaps_in_label_map", "set_nonzero_pixels_to_pixelindex", "logical_and",
-                 "binary_intersection", "minimum_of_all_pixels", "maximum_of_all_pixels"):
+                 "binary_intersection", "minimum_of_all_pixels", "maximum_of_all_pixels",
+                 "extend_labeling_via_voronoi", "voronoi_labeling", "gauss_otsu_labeling"):
         assert callable(getattr(cle, name)), name
     assert cle.statistics_of_labeled_pixels is cle.statistics_of_labelled_pixels
 

@@ -73,6 +73,9 @@ def test_reference_vectors(cle, ref_tests):
     assert np.array_equal(cle.pull(cle.masked_voronoi_labeling(A(t["input"]), A(t["mask"]))), A(t["expected"]))
     t = ref_tests["voronoi_labeling"]
     assert np.array_equal(cle.pull(cle.voronoi_labeling(A(t["input"]))), A(t["expected"]))
+    t = ref_tests["extend_labeling_via_voronoi"]
+    out = cle.extend_labeling_via_voronoi(A(t["input"]))
+    assert out.dtype == np.uint32 and np.array_equal(cle.pull(out), A(t["expected"]))
     t = ref_tests["voronoi_otsu_labeling"]
     out = cle.voronoi_otsu_labeling(A(t["input"]), spot_sigma=t["spot_sigma"], outline_sigma=t["outline_sigma"])
     assert out.dtype == np.uint32 and np.array_equal(cle.pull(out), A(t["expected"]))
@@ -280,6 +283,28 @@ def test_masked_voronoi_parity(cle, shape):
     m = (rng.random(shape) < 0.8).astype(np.uint8)
     got = cle.pull(cle.masked_voronoi_labeling(seeds, m))
     assert np.array_equal(got, O.masked_voronoi_labeling(seeds, m))
+
+
+@pytest.mark.parametrize("shape", [(50, 70), (12, 40, 45), (1, 40, 45), (9, 13, 150), (130, 70), (5, 3, 200)])
+def test_extend_labeling_via_voronoi_parity(cle, shape):
+    # sparse labels with gaps and large values, float32 destination and an int32 device source
+    import torch
+    rng = np.random.default_rng(14)
+    lab = np.zeros(shape, np.uint32)
+    idx = rng.choice(lab.size, max(2, lab.size // 150), replace=False)
+    lab.flat[idx] = rng.integers(1, 100000, idx.size)
+    got = cle.pull(cle.extend_labeling_via_voronoi(lab))
+    assert got.dtype == np.uint32 and np.array_equal(got, O.extend_labeling_via_voronoi(lab))
+    dst = cle.create(shape, np.float32)
+    cle.extend_labeling_via_voronoi(cle.asarray(torch.from_numpy(lab.astype(np.int32)).cuda()), dst)
+    assert np.array_equal(cle.pull(dst), O.extend_labeling_via_voronoi(lab, np.float32))
+
+
+@pytest.mark.parametrize("shape", [(60, 50), (10, 30, 40)])
+def test_voronoi_labeling_parity(cle, shape):
+    rng = np.random.default_rng(15)
+    b = (rng.random(shape) < 0.02).astype(np.uint8)
+    assert np.array_equal(cle.pull(cle.voronoi_labeling(b)), O.voronoi_labeling(b))
 
 
 @pytest.mark.parametrize("shape,ss,so", [((128, 128), 3.5, 1.0), ((200, 180), 2.0, 1.5), ((24, 80, 90), 2.0, 1.0),

@@ -253,6 +253,36 @@ def test_voronoi_labeling_reference(ref_tests):
     assert np.array_equal(out, A(t["expected"]))
 
 
+def test_extend_labeling_via_voronoi_reference(ref_tests):
+    t = ref_tests["extend_labeling_via_voronoi"]
+    out = O.extend_labeling_via_voronoi(A(t["input"]))
+    assert out.dtype == np.uint32 and np.array_equal(out, A(t["expected"]))
+
+
+def test_voronoi_labeling_via_extend(ref_tests):
+    # the reference composes voronoi_labeling as CCL box + extend_labeling_via_voronoi
+    t = ref_tests["voronoi_labeling"]
+    assert np.array_equal(O.voronoi_labeling(A(t["input"])), A(t["expected"]))
+
+
+def test_extend_labeling_is_nearest_label_in_chebyshev_octagon():
+    # the growth never changes a non-zero voxel, fills every zero voxel that a
+    # label can reach, and is idempotent on its own output
+    rng = np.random.default_rng(7)
+    for shape in ((9, 13), (5, 7, 6)):
+        lab = np.zeros(shape, np.uint32)
+        idx = rng.choice(lab.size, 4, replace=False)
+        lab.flat[idx] = [3, 9, 1, 20]
+        out = O.extend_labeling_via_voronoi(lab)
+        assert np.array_equal(out.flat[idx], lab.flat[idx])
+        assert (out > 0).all()
+        assert np.array_equal(O.extend_labeling_via_voronoi(out), out)
+        # a single seed fills the whole image with its value
+        one = np.zeros(shape, np.uint32)
+        one.flat[idx[0]] = 5
+        assert (O.extend_labeling_via_voronoi(one) == 5).all()
+
+
 def test_voronoi_otsu_reference(ref_tests):
     t = ref_tests["voronoi_otsu_labeling"]
     out = O.voronoi_otsu_labeling(A(t["input"]), t["spot_sigma"], t["outline_sigma"])
