@@ -166,49 +166,82 @@ __device__ __forceinline__ bool nz(const DBuf b, int64_t i) {
     }
 }
 
-// detect_maxima_box with radius 0 on row (z, y): the neighbourhood of voxel x is
-// visited x outer, then y, then z inner, starting from value - 1, and a strictly
-// larger value takes over; the voxel is a maximum if the last take-over was its own
-// value (_tier1/detect_maxima_{2d,3d}_x.cl semantics; coordinates below 0 are
-// skipped, above the last index clamped).  The 3 x 3 neighbour row offsets are
-// formed once per row; out(x, is_max) stores the flag.
+// detect_maxima_box with radius 0: the 3^d neighbourhood of a voxel is visited x
+// outer, then y, then z inner, starting from value - 1, and a strictly larger value
+// takes over; the voxel is a maximum if the last take-over was its own value
+// (_tier1/detect_maxima_{2d,3d}_x.cl semantics; coordinates below 0 are skipped,
+// above the last index clamped).
+// A block of 256 threads handles a 64 x 16 (x, y) tile of MX_ZC consecutive slices,
+// the slices z-1, z, z+1 of the tile (+ 1-voxel x / y halo) held in an LDS ring of 3
+// planes; every read stays inside the image and inside the staged planes.
+// out(i, is_max) stores voxel i's flag.
+constexpr int MX_TX = 64, MX_TY = 16, MX_ZC = 16;
+constexpr int MX_RX = MX_TX + 2, MX_RY = MX_TY + 2, MX_RP = MX_RX * MX_RY;
+
+__host__ __device__ inline int64_t maxima_tiles(const Geom& g) {
+    const int64_t nzc = g.ndim == 3 ? (g.D + MX_ZC - 1) / MX_ZC : 1;
+    return ((g.W + MX_TX - 1) / MX_TX) * ((g.H + MX_TY - 1) / MX_TY) * nzc;
+}
+
 template <typename LD, typename OUT>
-__device__ __forceinline__ void maxima_row(const LD& in, const Geom& g, int64_t z, int64_t y, int tid, int nth,
-                                           const OUT& out) {
-    const int zr = g.ndim == 3 ? 1 : 0;
-    int64_t rb[3][3];
-    bool rv[3][3];
-#pragma unroll
-    for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-        for (int dz = -1; dz <= 1; ++dz) {
-            const int64_t qy = y + dy, qz = z + dz;
-            rv[dy + 1][dz + 1] = qy >= 0 && qz >= 0 && (dz == 0 || zr);
-            const int64_t cy = qy > g.H - 1 ? g.H - 1 : qy, cz = qz > g.D - 1 ? g.D - 1 : qz;
-            rb[dy + 1][dz + 1] = (cz * g.H + cy) * g.W;
+__device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t t, float* ring, const OUT& out) {
+    const int tid = threadIdx.x;
+    const int64_t ntx = (g.W + MX_TX - 1) / MX_TX, nty = (g.H + MX_TY - 1) / MX_TY;
+    const int64_t x0 = (t % ntx) * MX_TX, y0 = ((t / ntx) % nty) * MX_TY, zc = t / (ntx * nty);
+    const bool d3 = g.ndim == 3;
+    const int64_t z0 = d3 ? zc * MX_ZC : 0, z1 = d3 ? (z0 + MX_ZC < g.D ? z0 + MX_ZC : g.D) : 1;
+    // plane zz -> ring slot zz % 3 (voxels inside the image only)
+    auto stage = [&](int64_t zz) {
+        float* pl = ring + (zz % 3) * MX_RP;
+        for (int c = tid; c < MX_RP; c += 256) {
+            const int64_t xx = x0 - 1 + c % MX_RX, yy = y0 - 1 + c / MX_RX;
+            if (xx >= 0 && xx < g.W && yy >= 0 && yy < g.H) pl[c] = in((zz * g.H + yy) * g.W + xx);
         }
-    const int64_t self_row = (z * g.H + y) * g.W;
-    for (int64_t x = tid; x < g.W; x += nth) {
-        float lm = in(self_row + x) - 1.f;
-        bool self = true;
+    };
+    if (z0 > 0) stage(z0 - 1);
+    stage(z0);
+    if (z0 + 1 < g.D && d3) stage(z0 + 1);
+    for (int64_t z = z0; z < z1; ++z) {
+        __syncthreads();
+        // ring planes of z - 1 (skipped below 0), z, z + 1 (clamped to the last slice)
+        const int s0 = (int)(z % 3);
+        const int sm = z > 0 ? (s0 + 2) % 3 : -1;
+        const int sp = z + 1 < g.D ? (s0 + 1) % 3 : s0;
 #pragma unroll
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int64_t qx0 = x + dx;
-            if (qx0 < 0) continue;
-            const int64_t qx = qx0 > g.W - 1 ? g.W - 1 : qx0;
+        for (int k = 0; k < (MX_TX * MX_TY) / 256; ++k) {
+            const int c = tid + k * 256;
+            const int ix = c % MX_TX, iy = c / MX_TX;
+            if (x0 + ix >= g.W || y0 + iy >= g.H) continue;
+            // staged-plane offsets of x - 1 (skipped below 0), x, x + 1 (clamped), same for y
+            const int lx = ix + 1, ly = iy + 1;
+            const int xo[3] = {x0 + ix > 0 ? lx - 1 : -1, lx, x0 + ix + 1 < g.W ? lx + 1 : lx};
+            const int yo[3] = {y0 + iy > 0 ? (ly - 1) * MX_RX : -1, ly * MX_RX,
+                               y0 + iy + 1 < g.H ? (ly + 1) * MX_RX : ly * MX_RX};
+            const int zo[3] = {d3 ? (sm >= 0 ? sm * MX_RP : -1) : -1, s0 * MX_RP, d3 ? sp * MX_RP : -1};
+            const float v = ring[s0 * MX_RP + ly * MX_RX + lx];
+            float lm = v - 1.f;
+            bool self = true;
 #pragma unroll
-            for (int dy = 0; dy < 3; ++dy)
+            for (int dx = 0; dx < 3; ++dx) {
+                if (xo[dx] < 0) continue;
 #pragma unroll
-                for (int dz = 0; dz < 3; ++dz) {
-                    if (!rv[dy][dz]) continue;
-                    const float w = in(rb[dy][dz] + qx);
-                    if (w > lm) {
-                        lm = w;
-                        self = (dx == 0 && dy == 1 && dz == 1);
+                for (int dy = 0; dy < 3; ++dy) {
+                    if (yo[dy] < 0) continue;
+#pragma unroll
+                    for (int dz = 0; dz < 3; ++dz) {
+                        if (zo[dz] < 0) continue;
+                        const float w = ring[zo[dz] + yo[dy] + xo[dx]];
+                        if (w > lm) {
+                            lm = w;
+                            self = (dx == 1 && dy == 1 && dz == 1);
+                        }
                     }
                 }
+            }
+            out((z * g.H + y0 + iy) * g.W + x0 + ix, self);
         }
-        out(x, self);
+        __syncthreads();
+        if (d3 && z + 2 < z1 + 1 && z + 2 < g.D) stage(z + 2);
     }
 }
 
@@ -229,6 +262,8 @@ struct BImg {
     int64_t kbase;      // CCL: first key bit of the image (multiple of 32)
     int64_t boff;       // CCL: first 2x2x2 block of the image
     int64_t tbase;      // voronoi: first tile of the image
+    int vnx, vny, vnz;  // voronoi: tile counts along x, y, z
+    int pad2;
 };
 
 // Host-side staging of a batch's device tables: appended while the stages are

@@ -110,13 +110,9 @@ __global__ void k_onlyzero_max(DBuf s, DBuf flag, DBuf d, Geom g, int diamond) {
 // neighbours with a negative coordinate are skipped, upper ones read clamped; the
 // running maximum starts at v - 1 and a neighbour wins only if strictly greater.
 template <typename LD>
-__global__ __launch_bounds__(256) void k_maxima(LD in, DBuf d, Geom g) {
-    const int64_t rows = g.D * g.H;
-    for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
-        const int64_t z = r / g.H, y = r - z * g.H;
-        const int64_t base = r * g.W;
-        maxima_row(in, g, z, y, threadIdx.x, 256, [&](int64_t x, bool m) { st(d, base + x, m ? 1.f : 0.f); });
-    }
+__global__ __launch_bounds__(256) void k_maxima_t(LD in, DBuf d, Geom g) {
+    __shared__ float ring[3 * MX_RP];
+    maxima_tile(in, g, blockIdx.x, ring, [&](int64_t i, bool m) { st(d, i, m ? 1.f : 0.f); });
 }
 
 struct LdAnyE {
@@ -130,12 +126,11 @@ struct LdF32E {
 
 int maxima_impl(DBuf src, DBuf dst, const Geom& g, hipStream_t s) {
     if (g.n() == 0) return CLE_OK;
-    const int64_t rows = g.D * g.H;
-    const unsigned grid = (unsigned)(rows < 262144 ? rows : 262144);
+    const unsigned grid = (unsigned)maxima_tiles(g);
     if (src.dt == CLE_F32)
-        k_maxima<<<grid, 256, 0, s>>>(LdF32E{(const float*)src.p}, dst, g);
+        k_maxima_t<<<grid, 256, 0, s>>>(LdF32E{(const float*)src.p}, dst, g);
     else
-        k_maxima<<<grid, 256, 0, s>>>(LdAnyE{src}, dst, g);
+        k_maxima_t<<<grid, 256, 0, s>>>(LdAnyE{src}, dst, g);
     CLE_LAUNCHED();
     return CLE_OK;
 }

@@ -7,18 +7,39 @@
 //       non-zero voxels and gives a zero voxel the largest positive neighbour value
 //       (_tier1/_onlyzero_overwrite_maximum_{box,diamond}.py), until an iteration
 //       changes nothing; finally mask(last, mask).
+//   extend_labeling_via_voronoi  _tier4/_extend_labeling_via_voronoi.py:39-61 (the same
+//       growth from a label image, no mask)
 //   voronoi_otsu_labeling    _tier9/_voronoi_otsu_labeling.py:57-73
 //   gauss_otsu_labeling      _tier9/_gauss_otsu_labeling.py:6-49
 //
-// MI355X design.  The growth runs as a sequence of synchronous (Jacobi) steps on two
-// float32 buffers, but a step only touches the tiles that can change: a tile T is
-// processed at step t iff some tile of its 3^d tile neighbourhood changed at step
-// t-1 or t-2 (at t = 0, 1 the "changes" at -1, -2 are the tiles holding a positive
-// initial value).  If nothing near T changed in the last two steps, T's output at t
-// equals its output at t-2 (same neighbourhood kind, same input), which is already
-// in the destination buffer, so skipping T leaves both buffers exactly as the
-// reference's full sweeps would.  The global "changed" flag of each step lives on
-// the device; the host checks it once per batch of steps.
+// MI355X design.  One launch runs a pair of steps, box step 2b then diamond step
+// 2b + 1, on every active tile: the tile and a 2-voxel halo are staged in LDS from the
+// source buffer, the box step is computed on the tile plus a 1-voxel ring, the
+// diamond step on the tile, and only the tile is written to the destination buffer.
+// Across tiles this is exactly the reference's synchronous (Jacobi) steps.
+//
+// Stopping.  The reference stops after the first step that changes nothing.  A pair
+// launch reports, per image, whether its box step and its diamond step changed a
+// voxel.  If the box step changed nothing, no zero voxel has a positive box
+// neighbour, so the diamond step changes nothing either and the pair's output is the
+// reference's result; if only the diamond step changed nothing, the pair's output is
+// the state after it, again the reference's result.  So an image stops after the
+// first pair that is not "both changed", with reference iteration count
+// 2b + (box changed ? 2 : 1), and its result is the destination of that pair.
+//
+// Work lists.  Tile T is processed in pair b iff a tile of its 3^d tile neighbourhood
+// changed in pair b - 1 (pair 0: holds a positive initial value).  Otherwise T's
+// neighbourhood is what it was one pair earlier, so T's output equals the output of
+// the pair before, which the destination buffer (ping-pong) already holds.  A tile
+// that changes queues its neighbourhood for the next pair (deduplicated by a per-tile
+// stamp); a tile with no zero voxel that did not change holds the same values in both
+// buffers and is never visited again ("settled").  Each launch walks its list with a
+// fixed grid, so the cost follows the growth front.  Block 0 of each launch closes the
+// previous pair (per-image convergence, list and flag resets; three lists and three
+// flag sets rotate so that nothing it resets is in use), and a one-block launch closes
+// the last pair of each chunk of pairs with one "still growing" word for the host.
+// (No grid-wide "last block" ticket: the device-scope fence it needs writes back the
+// L2 in every block.)
 #include "common.hpp"
 
 namespace cle {
@@ -27,6 +48,7 @@ int maxima_impl(DBuf src, DBuf dst, const Geom& g, hipStream_t s);
 
 constexpr int VB = 256;  // threads per tile block
 constexpr int VPT = 4;   // voxels per thread (tile = 1024 voxels)
+constexpr int GH = 2;    // halo of the staged region = steps per launch
 
 // Tiles of 1024 voxels, close to cubes so that a growth front keeps as few voxels
 // active as possible: 16 x 8 x 8 in 3D, 32 x 32 in 2D (x rows of 64 / 128 bytes)
@@ -36,9 +58,11 @@ struct Tiles {
     __host__ __device__ int64_t count() const { return nz * ny * nx; }
 };
 
+__host__ __device__ inline bool vor_3d(const Geom& g) { return g.ndim == 3 && g.D > 1; }
+
 __host__ __device__ inline Tiles make_tiles(const Geom& g) {
     Tiles t;
-    if (g.ndim == 3 && g.D > 1) { t.tx = 16; t.ty = 8; t.tz = 8; }
+    if (vor_3d(g)) { t.tx = 16; t.ty = 8; t.tz = 8; }
     else { t.tx = 32; t.ty = 32; t.tz = 1; }
     t.nz = (g.D + t.tz - 1) / t.tz;
     t.ny = (g.H + t.ty - 1) / t.ty;
@@ -46,18 +70,105 @@ __host__ __device__ inline Tiles make_tiles(const Geom& g) {
     return t;
 }
 
-__device__ __forceinline__ int64_t clampi(int64_t v, int64_t hi) { return v < 0 ? 0 : (v > hi ? hi : v); }
+// A work-list entry: image j (< 2^16) and tile coordinates (each < 2^16) packed as
+// j << 48 | tz << 32 | ty << 16 | tx, and the tile's index in the per-tile arrays.
+struct VorItem {
+    uint64_t c;
+    int64_t gt;
+};
+__host__ __device__ inline uint64_t vor_pack(int j, int64_t tx, int64_t ty, int64_t tz) {
+    return ((uint64_t)j << 48) | ((uint64_t)tz << 32) | ((uint64_t)ty << 16) | (uint64_t)tx;
+}
+constexpr int64_t VOR_MAX_TILES_AXIS = 65535;
+constexpr int VOR_MAX_IMAGES = 65535;
 
-// A = B = (mask - 1) + label, and the tiles holding a positive value are flagged as
-// "changed" at steps -1 and -2 (slots 3 and 2 of the 4-slot ring).
-// tile = index inside the image, gtile = its slot in the flag arrays of ntiles slots
-// A tile is "settled" once both buffers hold the same values there and none is zero:
-// it can never change again, so the steps skip it (settled[gtile] = 1).
+// Work lists of the growth (one set for a single image or a whole batch).
+struct VorLists {
+    VorItem* items;    // 3 lists of cap entries
+    uint32_t* count;   // [0..2]: list sizes
+    int32_t* stamp;    // cap: last pair the tile was queued for
+    int32_t* mark;     // 2 x cap (raster mode): mark[b & 1][t] == b iff tile t is active in pair b
+    uint8_t* settled;  // cap
+    int64_t cap;       // tiles of all images
+};
+
+// Per-image state of the growth.
+struct VorState {
+    uint32_t* bits;    // 3 x nimg: per pair b % 3 and image, OR of the change bits (1 box, 2 diamond)
+    int32_t* iters;    // per image: reference iteration count once converged, else 0
+    uint32_t* word;    // [0]: an image still grows after the last pair; [1]: max iteration count
+};
+
+__device__ __forceinline__ uint64_t lanes_below_v() {
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// wavefront 0 of the block queues the tile neighbourhood of `tile` (of image j, whose
+// tiles start at tbase in the per-tile arrays) for pair `blk`.  `skip_settled`: leave
+// out settled tiles (not during init, where the settled flags are still being written)
+__device__ __forceinline__ void vor_queue(const Tiles& T, int64_t tile, int j, int64_t tbase, int blk,
+                                          const VorLists& L, bool skip_settled) {
+    const int lane = threadIdx.x;
+    int64_t q = -1;
+    if (lane < 27) {
+        const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
+        const int64_t qx = tx + lane % 3 - 1, qy = ty + (lane / 3) % 3 - 1, qz = tz + lane / 9 - 1;
+        if (qx >= 0 && qx < T.nx && qy >= 0 && qy < T.ny && qz >= 0 && qz < T.nz)
+            q = (qz * T.ny + qy) * T.nx + qx;
+    }
+    // a settled tile never changes again: not queued
+    const bool need =
+        q >= 0 && !(skip_settled && L.settled[tbase + q]) && atomicMax(&L.stamp[tbase + q], blk) < blk;
+    const uint64_t m = __ballot(need);
+    uint32_t base = 0;
+    if (lane == 0 && m) base = atomicAdd(&L.count[blk % 3], (uint32_t)__popcll(m));
+    base = __shfl(base, 0);
+    if (need) {
+        const int64_t qx = q % T.nx, qy = (q / T.nx) % T.ny, qz = q / (T.nx * T.ny);
+        L.items[(int64_t)(blk % 3) * L.cap + base + __popcll(m & lanes_below_v())] =
+            VorItem{vor_pack(j, qx, qy, qz), tbase + q};
+    }
+}
+
+constexpr int VOR_PEND = 64;  // changed tiles a block collects before queueing their neighbourhoods
+
+// All threads of the block queue the tile neighbourhoods of the np changed tiles in
+// `pend` (packed image / tile coordinates) for pair `blk`, leaving out settled tiles.
+__device__ __forceinline__ void vor_flush(const uint64_t* pend, int np, const BImg* __restrict__ imgs, int blk,
+                                          const VorLists& L) {
+    const int total = np * 27;
+    for (int base0 = 0; base0 < total; base0 += VB) {
+        const int idx = base0 + (int)threadIdx.x;
+        bool need = false;
+        VorItem item{0, 0};
+        if (idx < total) {
+            const uint64_t pe = pend[idx / 27];
+            const int k = idx % 27;
+            const int j = (int)(pe >> 48);
+            const int nx = imgs[j].vnx, ny = imgs[j].vny, nz = imgs[j].vnz;
+            const int qx = (int)(pe & 0xFFFF) + k % 3 - 1, qy = (int)((pe >> 16) & 0xFFFF) + (k / 3) % 3 - 1;
+            const int qz = (int)((pe >> 32) & 0xFFFF) + k / 9 - 1;
+            if (qx >= 0 && qx < nx && qy >= 0 && qy < ny && qz >= 0 && qz < nz) {
+                const int64_t gq = imgs[j].tbase + ((int64_t)qz * ny + qy) * nx + qx;
+                need = !L.settled[gq] && atomicMax(&L.stamp[gq], blk) < blk;
+                item = VorItem{vor_pack(j, qx, qy, qz), gq};
+            }
+        }
+        const uint64_t m = __ballot(need);
+        uint32_t b0 = 0;
+        if ((threadIdx.x & 63) == 0 && m) b0 = atomicAdd(&L.count[blk % 3], (uint32_t)__popcll(m));
+        b0 = __shfl(b0, 0);
+        if (need) L.items[(int64_t)(blk % 3) * L.cap + b0 + __popcll(m & lanes_below_v())] = item;
+    }
+}
+
+// A = B = val(i) on one tile; the tile is settled when it holds no zero; returns, in
+// every thread, whether the tile holds a positive value.
 // `val(i)` is the initial value of voxel i (it may read B[i] before B[i] is written).
 template <typename VF>
 __device__ __forceinline__ bool vor_init_tile(VF val, float* A, float* B, const Geom& g, const Tiles& T,
-                                              int64_t tile, int64_t gtile, int64_t ntiles, uint8_t* tflags,
-                                              uint8_t* settled) {
+                                              int64_t tile, uint8_t* settled) {
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
     bool pos = false, zero = false;
     for (int k = 0; k < VPT; ++k) {
@@ -75,14 +186,7 @@ __device__ __forceinline__ bool vor_init_tile(VF val, float* A, float* B, const 
     }
     const int any = __syncthreads_or(pos ? 1 : 0);
     const int anyzero = __syncthreads_or(zero ? 1 : 0);
-    if (threadIdx.x == 0) settled[gtile] = anyzero ? 0 : 1;
-    if (tflags && threadIdx.x == 0) {
-        const uint8_t f = any ? 1 : 0;
-        tflags[0 * ntiles + gtile] = 0;
-        tflags[1 * ntiles + gtile] = 0;
-        tflags[2 * ntiles + gtile] = f;
-        tflags[3 * ntiles + gtile] = f;
-    }
+    if (threadIdx.x == 0) *settled = anyzero ? 0 : 1;
     return any != 0;
 }
 
@@ -105,138 +209,533 @@ struct LabelsAny {
     __device__ __forceinline__ float operator()(int64_t i) const { return ld(src, i); }
 };
 
-__global__ __launch_bounds__(VB) void k_vor_init(DBuf mask, float* A, float* B, Geom g, Tiles T, uint8_t* tflags) {
-    vor_init_tile(MaskedAny{mask, B}, A, B, g, T, blockIdx.x, blockIdx.x, T.count(), tflags, tflags + 4 * T.count());
+// single image: init every tile, queue the neighbourhood of tiles holding a positive
+// value for pair 0
+template <typename VF>
+__global__ __launch_bounds__(VB) void k_vor_init(VF val, float* A, float* B, Geom g, VorLists L) {
+    const Tiles T = make_tiles(g);
+    if (vor_init_tile(val, A, B, g, T, blockIdx.x, L.settled + blockIdx.x) && threadIdx.x < 27) {
+        const int64_t t = blockIdx.x, k = threadIdx.x;
+        const int64_t tx = t % T.nx, ty = (t / T.nx) % T.ny, tz = t / (T.nx * T.ny);
+        const int64_t qx = tx + k % 3 - 1, qy = ty + (k / 3) % 3 - 1, qz = tz + k / 9 - 1;
+        if (qx >= 0 && qx < T.nx && qy >= 0 && qy < T.ny && qz >= 0 && qz < T.nz)
+            L.mark[(qz * T.ny + qy) * T.nx + qx] = 0;  // active in pair 0
+    }
 }
 
-__global__ __launch_bounds__(VB) void k_vor_init_labels(DBuf src, float* A, float* B, Geom g, Tiles T,
-                                                        uint8_t* tflags) {
-    vor_init_tile(LabelsAny{src}, A, B, g, T, blockIdx.x, blockIdx.x, T.count(), tflags, tflags + 4 * T.count());
+// The staged region of a tile: the tile and a GH-voxel halo (z halo only in 3D).
+// Tile extents: 16 x 8 x 8 (3D), 32 x 32 (2D).
+template <bool D3>
+struct PairGeo {
+    static constexpr int TX = D3 ? 16 : 32, TY = D3 ? 8 : 32, TZ = D3 ? 8 : 1;
+    static constexpr int HZ = D3 ? GH : 0;
+    static constexpr int RX = TX + 2 * GH, RY = TY + 2 * GH, RZ = TZ + 2 * HZ;  // region extents
+    static constexpr int RP = RX * RY;                                          // region plane
+    static constexpr int NR = RP * RZ;
+    // registers per thread holding a staged region in flight: a z column (3D) or
+    // cells tid + k * VB (2D)
+    static constexpr int KV = D3 ? RZ : (NR + VB - 1) / VB;
+    int64_t obase;                 // offset of the region origin (may lie outside the image)
+    int64_t plane;
+    int W;
+    int xlo, xhi, ylo, yhi, zlo, zhi;  // the part inside the image, region coordinates [lo, hi)
+    __device__ __forceinline__ PairGeo(const Geom& g, int tx, int ty, int tz) {
+        const int64_t x0 = (int64_t)tx * TX - GH, y0 = (int64_t)ty * TY - GH, z0 = (int64_t)tz * TZ - HZ;
+        xlo = x0 < 0 ? (int)-x0 : 0;
+        ylo = y0 < 0 ? (int)-y0 : 0;
+        zlo = z0 < 0 ? (int)-z0 : 0;
+        xhi = (int)(g.W - x0 < RX ? g.W - x0 : RX);
+        yhi = (int)(g.H - y0 < RY ? g.H - y0 : RY);
+        zhi = (int)(g.D - z0 < RZ ? g.D - z0 : RZ);
+        plane = g.H * g.W;
+        W = (int)g.W;
+        obase = (z0 * g.H + y0) * g.W + x0;
+    }
+};
+constexpr int VOR_KV = PairGeo<true>::KV > PairGeo<false>::KV ? PairGeo<true>::KV : PairGeo<false>::KV;
+
+// issue the loads of a tile's region (voxels inside the image) into registers
+template <bool D3>
+__device__ __forceinline__ void vor_stage_load(const float* __restrict__ src, const PairGeo<D3>& R, float* v) {
+    using P = PairGeo<D3>;
+    const int tid = threadIdx.x;
+    if (D3) {
+        const int rx = tid % P::RX, ry = tid / P::RX;
+        const bool col = tid < P::RP && rx >= R.xlo && rx < R.xhi && ry >= R.ylo && ry < R.yhi;
+        const float* c = src + R.obase + (ry * R.W + rx);
+#pragma unroll
+        for (int rz = 0; rz < P::KV; ++rz) v[rz] = (col && rz >= R.zlo && rz < R.zhi) ? c[rz * R.plane] : 0.f;
+    } else {
+#pragma unroll
+        for (int k = 0; k < P::KV; ++k) {
+            const int c = tid + k * VB;
+            const int rx = c % P::RX, ry = c / P::RX;
+            const bool in = c < P::NR && rx >= R.xlo && rx < R.xhi && ry >= R.ylo && ry < R.yhi;
+            v[k] = in ? src[R.obase + (ry * R.W + rx)] : 0.f;
+        }
+    }
 }
 
-// one growth step t: src -> dst, box (t even) or diamond (t odd)
-// One growth step on one tile.  gflag / gany: "changed" flags of this step for the
-// image and for the whole launch (the same word for a single image).
-// step t on one tile (src -> dst); returns, in every thread, whether the tile changed,
-// and marks the tile settled when it did not change and holds no zero
-__device__ __forceinline__ bool vor_tile_update(const float* __restrict__ src, float* __restrict__ dst, const Geom& g,
-                                                const Tiles& T, int64_t tile, int t, uint8_t* settled_slot) {
-    const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
-    const bool diamond = (t & 1) != 0;
-    const int zr = g.ndim == 3 ? 1 : 0;
-    bool changed = false, zero = false;
-    for (int k = 0; k < VPT; ++k) {
-        const int li = threadIdx.x + k * VB;
-        const int lx = li % T.tx, ly = (li / T.tx) % T.ty, lz = li / (T.tx * T.ty);
-        if (lz >= T.tz) continue;
-        const int64_t x = tx * T.tx + lx, y = ty * T.ty + ly, z = tz * T.tz + lz;
-        if (x >= g.W || y >= g.H || z >= g.D) continue;
-        const int64_t i = (z * g.H + y) * g.W + x;
-        const float v = src[i];
-        float out = v;
-        if (v == 0.f) {
-            float best = 0.f;
-            if (diamond) {
-                const int64_t xm = clampi(x - 1, g.W - 1), xp = clampi(x + 1, g.W - 1);
-                const int64_t ym = clampi(y - 1, g.H - 1), yp = clampi(y + 1, g.H - 1);
-                const int64_t row = (z * g.H + y) * g.W;
-                best = fmaxf(best, src[row + xm]);
-                best = fmaxf(best, src[row + xp]);
-                best = fmaxf(best, src[(z * g.H + ym) * g.W + x]);
-                best = fmaxf(best, src[(z * g.H + yp) * g.W + x]);
-                if (zr) {
-                    const int64_t zm = clampi(z - 1, g.D - 1), zp = clampi(z + 1, g.D - 1);
-                    best = fmaxf(best, src[(zm * g.H + y) * g.W + x]);
-                    best = fmaxf(best, src[(zp * g.H + y) * g.W + x]);
-                }
-            } else {
-                for (int dz = -zr; dz <= zr; ++dz) {
-                    const int64_t qz = clampi(z + dz, g.D - 1);
-                    for (int dy = -1; dy <= 1; ++dy) {
-                        const int64_t row = (qz * g.H + clampi(y + dy, g.H - 1)) * g.W;
-                        best = fmaxf(best, src[row + clampi(x - 1, g.W - 1)]);
-                        best = fmaxf(best, src[row + x]);
-                        best = fmaxf(best, src[row + clampi(x + 1, g.W - 1)]);
+template <bool D3>
+__device__ __forceinline__ void vor_stage_store(const float* v, float* lds) {
+    using P = PairGeo<D3>;
+    const int tid = threadIdx.x;
+    if (D3) {
+        if (tid < P::RP) {
+#pragma unroll
+            for (int rz = 0; rz < P::KV; ++rz) lds[rz * P::RP + tid] = v[rz];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < P::KV; ++k)
+            if (tid + k * VB < P::NR) lds[tid + k * VB] = v[k];
+    }
+}
+
+// One pair of steps (box, then diamond) on a staged tile: lds holds the region, lds2
+// receives the box step's result, the diamond step is written to dst.  Returns, in
+// every thread, bit 0: a tile voxel changed in the box step, bit 1: in the diamond
+// step, bit 2: a tile voxel is zero after the pair.  Neighbour coordinates are
+// clamped to the image, which keeps them inside the staged region.
+template <bool D3>
+__device__ __forceinline__ int vor_pair_compute(float* __restrict__ dst, const PairGeo<D3>& R, const float* lds,
+                                                float* lds2, int* sbits) {
+    using P = PairGeo<D3>;
+    constexpr int TX = P::TX, TY = P::TY, TZ = P::TZ, RX = P::RX, RZ = P::RZ, RP = P::RP;
+    const int tid = threadIdx.x;
+    const int xlo = R.xlo, xhi = R.xhi, ylo = R.ylo, yhi = R.yhi, zlo = R.zlo, zhi = R.zhi;
+    // a tile without a zero voxel keeps its values (non-zero voxels never change): its
+    // output is its input, whatever the halo holds
+    {
+        bool zero = false;
+#pragma unroll
+        for (int k = 0; k < (TX * TY * TZ) / VB; ++k) {
+            const int c = tid + k * VB;
+            const int rx = c % TX + GH, ry = (c / TX) % TY + GH, rz = c / (TX * TY) + P::HZ;
+            if (rx < xhi && ry < yhi && rz < zhi) zero |= lds[(rz * P::RY + ry) * RX + rx] == 0.f;
+        }
+        if (!__syncthreads_or(zero ? 1 : 0)) {
+#pragma unroll
+            for (int k = 0; k < (TX * TY * TZ) / VB; ++k) {
+                const int c = tid + k * VB;
+                const int rx = c % TX + GH, ry = (c / TX) % TY + GH, rz = c / (TX * TY) + P::HZ;
+                if (rx < xhi && ry < yhi && rz < zhi)
+                    dst[R.obase + rz * R.plane + (ry * R.W + rx)] = lds[(rz * P::RY + ry) * RX + rx];
+            }
+            __syncthreads();  // lds is read above; the next tile's staging overwrites it
+            return 0;         // no change, no zero
+        }
+    }
+    int bits = 0;
+    if (D3) {
+        // ---- box step on the tile + 1-voxel ring: one thread per column, the 3 x 3
+        // plane maximum m(z) slid along z, box(z) = max(0, m(z-1), m(z), m(z+1))
+        constexpr int SX = TX + 2, SY = TY + 2;
+        if (tid < SX * SY) {
+            const int rx = tid % SX + 1, ry = tid / SX + 1;
+            if (rx >= xlo && rx < xhi && ry >= ylo && ry < yhi) {
+                const int c0 = ry * RX + rx;
+                const int dxm = rx > xlo ? -1 : 0, dxp = rx < xhi - 1 ? 1 : 0;
+                const int dym = ry > ylo ? -RX : 0, dyp = ry < yhi - 1 ? RX : 0;
+                const int r0 = zlo > 1 ? zlo : 1, r1 = zhi < RZ - 1 ? zhi : RZ - 1;  // box cells: [r0, r1)
+                bool anyzero = false;
+#pragma unroll
+                for (int rz = 1; rz < RZ - 1; ++rz)
+                    if (rz >= r0 && rz < r1) anyzero |= lds[rz * RP + c0] == 0.f;
+                if (!anyzero) {
+#pragma unroll
+                    for (int rz = 1; rz < RZ - 1; ++rz)
+                        if (rz >= r0 && rz < r1) lds2[rz * RP + c0] = lds[rz * RP + c0];
+                } else {
+                    auto m9 = [&](int rz) {
+                        const float* p = lds + rz * RP + c0;
+                        float m = fmaxf(fmaxf(p[dym + dxm], p[dym]), p[dym + dxp]);
+                        m = fmaxf(m, fmaxf(fmaxf(p[dxm], p[0]), p[dxp]));
+                        return fmaxf(m, fmaxf(fmaxf(p[dyp + dxm], p[dyp]), p[dyp + dxp]));
+                    };
+                    // planes r0 - 1 .. r1 (clamped to the image) feed the box cells
+                    float mprev = m9(r0 - 1 >= zlo ? r0 - 1 : r0);
+                    float mcur = m9(r0);
+#pragma unroll
+                    for (int rz = 1; rz < RZ - 1; ++rz) {
+                        if (rz < r0 || rz >= r1) continue;
+                        const float mnext = rz + 1 < zhi ? m9(rz + 1) : mcur;
+                        const float v = lds[rz * RP + c0];
+                        float out = v;
+                        if (v == 0.f) {
+                            out = fmaxf(0.f, fmaxf(fmaxf(mprev, mcur), mnext));
+                            const bool inner = rx >= GH && rx < TX + GH && ry >= GH && ry < TY + GH &&
+                                               rz >= GH && rz < TZ + GH;
+                            if (inner && out != 0.f) bits |= 1;
+                        }
+                        lds2[rz * RP + c0] = out;
+                        mprev = mcur;
+                        mcur = mnext;
                     }
                 }
             }
-            out = best;
         }
-        dst[i] = out;
-        changed |= (out != v);
-        zero |= out == 0.f;
+        __syncthreads();
+        // ---- diamond step on the tile: two threads per column (4 slices each)
+        {
+            const int col = tid % (TX * TY), half = tid / (TX * TY);
+            const int rx = col % TX + GH, ry = col / TX + GH;
+            if (rx < xhi && ry < yhi) {
+                const int c0 = ry * RX + rx;
+                const int dxm = rx > xlo ? -1 : 0, dxp = rx < xhi - 1 ? 1 : 0;
+                const int dym = ry > ylo ? -RX : 0, dyp = ry < yhi - 1 ? RX : 0;
+                float* dcol = dst + R.obase + (ry * R.W + rx);
+#pragma unroll
+                for (int k = 0; k < TZ / 2; ++k) {
+                    const int rz = GH + half * (TZ / 2) + k;
+                    if (rz >= zhi) continue;
+                    const float* p = lds2 + rz * RP + c0;
+                    const float v = p[0];
+                    float out = v;
+                    if (v == 0.f) {
+                        float best = fmaxf(0.f, p[dxm]);
+                        best = fmaxf(best, p[dxp]);
+                        best = fmaxf(best, p[dym]);
+                        best = fmaxf(best, p[dyp]);
+                        best = fmaxf(best, p[rz > zlo ? -RP : 0]);
+                        best = fmaxf(best, p[rz < zhi - 1 ? RP : 0]);
+                        out = best;
+                        if (out != 0.f) bits |= 2;
+                        else bits |= 4;
+                    }
+                    dcol[rz * R.plane] = out;
+                }
+            }
+        }
+    } else {
+        constexpr int SX = TX + 2, SY = TY + 2, NS = SX * SY;
+#pragma unroll 2
+        for (int c = tid; c < NS; c += VB) {
+            const int rx = c % SX + 1, ry = c / SX + 1;
+            if (rx < xlo || rx >= xhi || ry < ylo || ry >= yhi) continue;
+            const int rc = ry * RX + rx;
+            const float v = lds[rc];
+            float out = v;
+            if (v == 0.f) {
+                const int dxm = rx > xlo ? -1 : 0, dxp = rx < xhi - 1 ? 1 : 0;
+                const int dym = ry > ylo ? -RX : 0, dyp = ry < yhi - 1 ? RX : 0;
+                const float* p = lds + rc;
+                float best = fmaxf(0.f, fmaxf(fmaxf(p[dym + dxm], p[dym]), p[dym + dxp]));
+                best = fmaxf(best, fmaxf(fmaxf(p[dxm], p[0]), p[dxp]));
+                best = fmaxf(best, fmaxf(fmaxf(p[dyp + dxm], p[dyp]), p[dyp + dxp]));
+                out = best;
+                const bool inner = rx >= GH && rx < TX + GH && ry >= GH && ry < TY + GH;
+                if (inner && out != 0.f) bits |= 1;
+            }
+            lds2[rc] = out;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < (TX * TY) / VB; ++k) {
+            const int c = tid + k * VB;
+            const int rx = c % TX + GH, ry = c / TX + GH;
+            if (rx >= xhi || ry >= yhi) continue;
+            const int rc = ry * RX + rx;
+            const float v = lds2[rc];
+            float out = v;
+            if (v == 0.f) {
+                float best = fmaxf(0.f, lds2[rc + (rx > xlo ? -1 : 0)]);
+                best = fmaxf(best, lds2[rc + (rx < xhi - 1 ? 1 : 0)]);
+                best = fmaxf(best, lds2[rc + (ry > ylo ? -RX : 0)]);
+                best = fmaxf(best, lds2[rc + (ry < yhi - 1 ? RX : 0)]);
+                out = best;
+                if (out != 0.f) bits |= 2;
+                else bits |= 4;
+            }
+            dst[R.obase + (ry * R.W + rx)] = out;
+        }
     }
-    const bool ch = __syncthreads_or(changed ? 1 : 0) != 0;
-    const bool zr0 = __syncthreads_or(zero ? 1 : 0) != 0;
-    if (!ch && !zr0 && threadIdx.x == 0) *settled_slot = 1;
-    return ch;
+    if (bits) atomicOr(sbits, bits);
+    __syncthreads();
+    return *sbits;
 }
 
-__device__ __forceinline__ void vor_step_tile(const float* __restrict__ src, float* __restrict__ dst, const Geom& g,
-                                              const Tiles& T, int64_t tile, int64_t tbase, int64_t ntiles, int t,
-                                              uint8_t* tflags, uint32_t* gflag, uint32_t* gany) {
-    const int64_t gtile = tbase + tile;
-    const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
-    if (threadIdx.x == 0) tflags[((t + 1) & 3) * ntiles + gtile] = 0;
-    uint8_t* settled = tflags + 4 * ntiles;
-    if (settled[gtile]) return;  // no zero left and both buffers equal: nothing can change
-    // activity: any tile of the 3^d tile neighbourhood changed at t-1 or t-2
-    const uint8_t* f1 = tflags + ((t + 3) & 3) * ntiles;
-    const uint8_t* f2 = tflags + ((t + 2) & 3) * ntiles;
-    bool act = false;
-    if (threadIdx.x < 27) {
-        const int dx = threadIdx.x % 3 - 1, dy = (threadIdx.x / 3) % 3 - 1, dz = threadIdx.x / 9 - 1;
-        const int64_t qx = tx + dx, qy = ty + dy, qz = tz + dz;
-        if (qx >= 0 && qx < T.nx && qy >= 0 && qy < T.ny && qz >= 0 && qz < T.nz) {
-            const int64_t q = tbase + (qz * T.ny + qy) * T.nx + qx;
-            act = f1[q] | f2[q];
-        }
+// A list entry to process.
+struct PairItem {
+    int j, tx, ty, tz;
+    int64_t gt;
+};
+
+// the next entry at or after e (stride gridDim.x) that is not skipped: its image
+// converged in an earlier pair (iteration count set, or pair b - 1 not "both
+// changed"), or the tile is settled.  One round trip after the entry itself.
+// Uniform over the block.
+__device__ __forceinline__ bool vor_next_item(uint32_t& e, uint32_t n, const VorItem* items, const uint32_t* prev,
+                                              int b, const VorLists& L, const VorState& S, PairItem& it) {
+    for (; e < n; e += gridDim.x) {
+        const VorItem v = items[e];
+        it.j = (int)(v.c >> 48);
+        it.tz = (int)((v.c >> 32) & 0xFFFF);
+        it.ty = (int)((v.c >> 16) & 0xFFFF);
+        it.tx = (int)(v.c & 0xFFFF);
+        it.gt = v.gt;
+        const bool conv = S.iters[it.j] != 0 || (b > 0 && prev[it.j] != 3u);
+        const bool settled = L.settled[it.gt] != 0;
+        if (!conv && !settled) return true;
     }
-    if (!__syncthreads_or(act ? 1 : 0)) return;
-    if (vor_tile_update(src, dst, g, T, tile, t, settled + gtile) && threadIdx.x == 0) {
-        tflags[(t & 3) * ntiles + gtile] = 1;
-        *gflag = 1;
-        *gany = 1;
+    return false;
+}
+
+// Pair b (steps 2b, 2b + 1) over list b % 3, src -> dst, for the images of `imgs`
+// (one image: the single-image path).  Block 0 first closes pair b - 1: images whose
+// pair b - 1 was not "both changed" get their iteration count; it also empties the
+// list and the change bits that pair b + 1 will fill (both last used by pair b - 2 /
+// b - 1, untouched during pair b).  Changed tiles are collected in LDS and their
+// neighbourhoods queued by the whole block at once.
+__global__ __launch_bounds__(VB, 4) void k_vor_pair(const BImg* __restrict__ imgs, int nimg,
+                                                    const float* __restrict__ src, float* __restrict__ dst, int b,
+                                                    VorLists L, VorState S) {
+    __shared__ float lds[PairGeo<true>::NR > PairGeo<false>::NR ? PairGeo<true>::NR : PairGeo<false>::NR];
+    __shared__ float lds2[PairGeo<true>::NR > PairGeo<false>::NR ? PairGeo<true>::NR : PairGeo<false>::NR];
+    __shared__ int sbits[2];
+    __shared__ uint64_t pend[VOR_PEND];  // changed tiles not yet queued
+    int np = 0;
+    const uint32_t* prev = S.bits + (size_t)((b + 2) % 3) * nimg;  // pair b - 1
+    uint32_t* cur = S.bits + (size_t)(b % 3) * nimg;
+    if (blockIdx.x == 0) {
+        uint32_t* next = S.bits + (size_t)((b + 1) % 3) * nimg;
+        int mx = 0;
+        for (int j = threadIdx.x; j < nimg; j += VB) {
+            if (b > 0 && S.iters[j] == 0 && prev[j] != 3u) {
+                const int it = 2 * (b - 1) + ((prev[j] & 1u) ? 2 : 1);
+                S.iters[j] = it;
+                mx = it > mx ? it : mx;
+            }
+            next[j] = 0;
+        }
+        if (mx) atomicMax((int*)&S.word[1], mx);
+        if (threadIdx.x == 0) L.count[(b + 2) % 3] = 0;
+    }
+    const uint32_t n = L.count[b % 3];
+    const VorItem* items = L.items + (int64_t)(b % 3) * L.cap;
+    PairItem it;
+    int par = 0;
+    for (uint32_t e = blockIdx.x; vor_next_item(e, n, items, prev, b, L, S, it); e += gridDim.x) {
+        const Geom g = imgs[it.j].g;
+        const int64_t off = imgs[it.j].off;
+        // sbits[par] was last read two tiles ago, before a barrier of the previous tile
+        if (threadIdx.x == 0) sbits[par] = 0;
+        int bits;
+        {
+            float v[VOR_KV];
+            if (vor_3d(g)) {
+                const PairGeo<true> R(g, it.tx, it.ty, it.tz);
+                vor_stage_load<true>(src + off, R, v);
+                vor_stage_store<true>(v, lds);
+                __syncthreads();
+                bits = vor_pair_compute<true>(dst + off, R, lds, lds2, &sbits[par]);
+            } else {
+                const PairGeo<false> R(g, it.tx, it.ty, it.tz);
+                vor_stage_load<false>(src + off, R, v);
+                vor_stage_store<false>(v, lds);
+                __syncthreads();
+                bits = vor_pair_compute<false>(dst + off, R, lds, lds2, &sbits[par]);
+            }
+        }
+        par ^= 1;
+        if (bits & 3) {
+            if (threadIdx.x == 0) {
+                atomicOr(&cur[it.j], (uint32_t)(bits & 3));
+                pend[np] = vor_pack(it.j, it.tx, it.ty, it.tz);
+            }
+            ++np;
+        } else if (!(bits & 4) && threadIdx.x == 0) {
+            L.settled[it.gt] = 1;
+        }
+        if (np == VOR_PEND) {
+            __syncthreads();
+            vor_flush(pend, np, imgs, b + 1, L);
+            __syncthreads();
+            np = 0;
+        }
+        // lds is rewritten by the next tile only after every thread passed the
+        // last barrier of vor_pair_compute (lds reads come before it)
+    }
+    if (np) {
+        __syncthreads();
+        vor_flush(pend, np, imgs, b + 1, L);
     }
 }
 
-__global__ __launch_bounds__(VB) void k_vor_step(const float* __restrict__ src, float* __restrict__ dst, Geom g,
-                                                Tiles T, int t, uint8_t* tflags, uint32_t* gflags, int ring) {
-    if (t > 0 && gflags[(t - 1) % ring] == 0) return;  // the previous step changed nothing: done
-    uint32_t* f = gflags + t % ring;
-    vor_step_tile(src, dst, g, T, blockIdx.x, 0, T.count(), t, tflags, f, f);
+// Pair b in raster mode (one image, one block per tile in raster order, so that the
+// halos of neighbouring tiles are read while still in L2): tile t runs iff
+// mark[b & 1][t] == b; a tile that changes marks its neighbourhood with b + 1 in the
+// other array (plain stores of one value: no atomics, no list).  Block 0 first closes
+// pair b - 1 as in the list mode.
+__global__ __launch_bounds__(VB, 4) void k_vor_pair_grid(const BImg* __restrict__ imgs,
+                                                         const float* __restrict__ src, float* __restrict__ dst,
+                                                         int b, VorLists L, VorState S) {
+    __shared__ float lds[PairGeo<true>::NR > PairGeo<false>::NR ? PairGeo<true>::NR : PairGeo<false>::NR];
+    __shared__ float lds2[PairGeo<true>::NR > PairGeo<false>::NR ? PairGeo<true>::NR : PairGeo<false>::NR];
+    __shared__ int sbits;
+    const uint32_t* prev = S.bits + (b + 2) % 3;  // pair b - 1 (one image)
+    uint32_t* cur = S.bits + b % 3;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (b > 0 && S.iters[0] == 0 && prev[0] != 3u) {
+            const int it = 2 * (b - 1) + ((prev[0] & 1u) ? 2 : 1);
+            S.iters[0] = it;
+            atomicMax((int*)&S.word[1], it);
+        }
+        S.bits[(b + 1) % 3] = 0;
+    }
+    const int64_t t = blockIdx.x;
+    if (L.mark[(int64_t)(b & 1) * L.cap + t] != b) return;
+    if (S.iters[0] != 0 || (b > 0 && prev[0] != 3u)) return;  // converged in an earlier pair
+    if (L.settled[t]) return;
+    const Geom g = imgs[0].g;
+    const int nx = imgs[0].vnx, ny = imgs[0].vny, nz = imgs[0].vnz;
+    const int txy = (int)(t / nx), tx = (int)(t - (int64_t)txy * nx), ty = txy % ny, tz = txy / ny;
+    if (threadIdx.x == 0) sbits = 0;
+    float v[VOR_KV];
+    int bits;
+    if (vor_3d(g)) {
+        const PairGeo<true> R(g, tx, ty, tz);
+        vor_stage_load<true>(src, R, v);
+        vor_stage_store<true>(v, lds);
+        __syncthreads();
+        bits = vor_pair_compute<true>(dst, R, lds, lds2, &sbits);
+    } else {
+        const PairGeo<false> R(g, tx, ty, tz);
+        vor_stage_load<false>(src, R, v);
+        vor_stage_store<false>(v, lds);
+        __syncthreads();
+        bits = vor_pair_compute<false>(dst, R, lds, lds2, &sbits);
+    }
+    if (bits & 3) {
+        if (threadIdx.x == 0) atomicOr(cur, (uint32_t)(bits & 3));
+        const int k = threadIdx.x;
+        if (k < 27) {
+            const int qx = tx + k % 3 - 1, qy = ty + (k / 3) % 3 - 1, qz = tz + k / 9 - 1;
+            if (qx >= 0 && qx < nx && qy >= 0 && qy < ny && qz >= 0 && qz < nz)
+                L.mark[(int64_t)((b + 1) & 1) * L.cap + ((int64_t)qz * ny + qy) * nx + qx] = b + 1;
+        }
+    } else if (!(bits & 4) && threadIdx.x == 0) {
+        L.settled[t] = 1;
+    }
 }
 
-constexpr int VOR_BATCH = 8;
-constexpr int VOR_RING = 2 * VOR_BATCH;
-
-// Grows the labels held (identically) in A and B until convergence.  On return
-// both buffers hold the converged image.
-static int grow(float* A, float* B, const Geom& g, const Tiles& T, uint8_t* tflags, int* n_iterations,
-                hipStream_t s) {
-    Scratch gf;
-    CLE_TRY(gf.alloc(VOR_RING * sizeof(uint32_t), s));
-    uint32_t* gflags = gf.get<uint32_t>();
-    const int64_t ntiles = T.count();
-    uint32_t hflags[VOR_BATCH];
-    int t = 0;
-    int iterations = -1;
-    while (iterations < 0) {
-        CLE_TRY(fill_u32(gflags + (t % VOR_RING), 0u, VOR_BATCH, s));
-        for (int j = 0; j < VOR_BATCH; ++j) {
-            const int it = t + j;
-            const float* src = (it & 1) ? B : A;
-            float* dst = (it & 1) ? A : B;
-            k_vor_step<<<(unsigned)ntiles, VB, 0, s>>>(src, dst, g, T, it, tflags, gflags, VOR_RING);
+// Closes the last pair b of a chunk (one block): iteration counts of the images that
+// converged in it, and the "an image still grows" word the host reads.
+__global__ __launch_bounds__(VB) void k_vor_close(int nimg, int b, VorState S) {
+    const uint32_t* last = S.bits + (size_t)(b % 3) * nimg;
+    bool grows = false;
+    int mx = 0;
+    for (int j = threadIdx.x; j < nimg; j += VB) {
+        if (S.iters[j] != 0) continue;
+        if (last[j] == 3u) {
+            grows = true;
+        } else {
+            const int it = 2 * b + ((last[j] & 1u) ? 2 : 1);
+            S.iters[j] = it;
+            mx = it > mx ? it : mx;
         }
+    }
+    grows = __syncthreads_or(grows ? 1 : 0) != 0;
+    if (mx) atomicMax((int*)&S.word[1], mx);
+    if (threadIdx.x == 0) S.word[0] = grows ? 1u : 0u;
+}
+
+constexpr int VOR_CHUNK = 8;  // pairs between host checks (the first check after 2)
+
+// scratch of the growth over `ntiles` tiles of `nimg` images
+struct VorScratch {
+    Scratch mem;
+    VorLists L;
+    VorState S;
+    int alloc(int64_t ntiles, int nimg, hipStream_t s) {
+        const int64_t cap = ntiles > 0 ? ntiles : 1;
+        const size_t bytes = (size_t)cap * 3 * 16 + (size_t)cap * 12 + 64 + (size_t)nimg * 16 + (size_t)cap;
+        CLE_TRY(mem.alloc(bytes, s));
+        char* p = mem.get<char>();
+        L.items = (VorItem*)p;
+        p += (size_t)cap * 3 * 16;
+        L.stamp = (int32_t*)p;
+        L.mark = L.stamp + cap;
+        p += (size_t)cap * 12;
+        L.count = (uint32_t*)p;        // count[0..2], word[0..1]: 5 words, padded to 16
+        S.word = L.count + 3;
+        p += 64;
+        S.bits = (uint32_t*)p;         // 3 x nimg
+        S.iters = (int32_t*)(S.bits + (size_t)3 * nimg);
+        p += (size_t)nimg * 16;
+        L.settled = (uint8_t*)p;
+        L.cap = cap;
+        // stamps -1, counters / words / per-image state 0 (contiguous from count)
+        CLE_TRY(fill_u32((uint32_t*)L.stamp, 0xFFFFFFFFu, (size_t)cap * 3, s));  // stamp and mark: -1
+        CLE_TRY(fill_u32(L.count, 0u, 16 + (size_t)nimg * 4, s));
+        return CLE_OK;
+    }
+};
+
+// Runs pairs until every image has converged; P0 holds the initial state (as does P1);
+// pair b reads P[b & 1] and writes P[(b + 1) & 1].  `max_iters` (host, may be null)
+// receives the largest reference iteration count over the images.
+static int grow_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t ntiles, const VorScratch& V,
+                      bool raster, int* max_iters, hipStream_t s) {
+    const unsigned grid = raster ? (unsigned)ntiles : (unsigned)(ntiles < 4096 ? (ntiles > 0 ? ntiles : 1) : 4096);
+    int b = 0;
+    uint32_t hw[2] = {0, 0};
+    for (;;) {
+        const int chunk = b == 0 ? 2 : VOR_CHUNK;
+        for (int k = 0; k < chunk; ++k, ++b) {
+            if (raster) k_vor_pair_grid<<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, V.L, V.S);
+            else k_vor_pair<<<grid, VB, 0, s>>>(dimgs, nimg, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, V.L, V.S);
+        }
+        k_vor_close<<<1, VB, 0, s>>>(nimg, b - 1, V.S);
         CLE_LAUNCHED();
-        CLE_HIP(hipMemcpyAsync(hflags, gflags + (t % VOR_RING), sizeof(hflags), hipMemcpyDeviceToHost, s));
+        CLE_HIP(hipMemcpyAsync(hw, V.S.word, sizeof(hw), hipMemcpyDeviceToHost, s));
         CLE_HIP(hipStreamSynchronize(s));
-        for (int j = 0; j < VOR_BATCH; ++j)
-            if (hflags[j] == 0) { iterations = t + j + 1; break; }
-        t += VOR_BATCH;
+        if (hw[0] == 0) break;
     }
-    if (n_iterations) *n_iterations = iterations;
+    if (max_iters) *max_iters = (int)hw[1];
+    return CLE_OK;
+}
+
+// the buffer holding the result of an image that converged with `iters` iterations
+__host__ __device__ inline int vor_result_buffer(int iters) { return (((iters - 1) >> 1) + 1) & 1; }
+
+__global__ void k_put_img(BImg* d, BImg im) { *d = im; }
+
+// growth tiling of an image into its table entry; fails for tile counts beyond the
+// packed list-entry fields
+static int vor_set_tiles(BImg& im) {
+    const Tiles T = make_tiles(im.g);
+    if (T.nx > VOR_MAX_TILES_AXIS || T.ny > VOR_MAX_TILES_AXIS || T.nz > VOR_MAX_TILES_AXIS)
+        return fail(CLE_E_VALUE, "image too large for the Voronoi growth tiling (%lld x %lld x %lld tiles, at most %lld "
+                    "per axis)", (long long)T.nx, (long long)T.ny, (long long)T.nz, (long long)VOR_MAX_TILES_AXIS);
+    im.vnx = (int)T.nx;
+    im.vny = (int)T.ny;
+    im.vnz = (int)T.nz;
+    return CLE_OK;
+}
+
+// single-image growth: P0 = P1 = init state from `val`, grown; returns the buffer
+// holding the result
+template <typename VF>
+static int grow_single(VF val, float* P0, float* P1, const Geom& g, int* n_iterations, const float** result,
+                       hipStream_t s) {
+    const Tiles T = make_tiles(g);
+    const int64_t ntiles = T.count();
+    BImg im;
+    memset(&im, 0, sizeof(im));
+    im.g = g;
+    CLE_TRY(vor_set_tiles(im));
+    VorScratch V;
+    CLE_TRY(V.alloc(ntiles, 1, s));
+    Scratch tab;
+    CLE_TRY(tab.alloc(sizeof(BImg), s));
+    k_put_img<<<1, 1, 0, s>>>(tab.get<BImg>(), im);
+    k_vor_init<<<(unsigned)ntiles, VB, 0, s>>>(val, P0, P1, g, V.L);
+    CLE_LAUNCHED();
+    int iters = 0;
+    CLE_TRY(grow_pairs(tab.get<BImg>(), 1, P0, P1, ntiles, V, true, &iters, s));
+    if (n_iterations) *n_iterations = iters;
+    *result = vor_result_buffer(iters) ? P1 : P0;
     return CLE_OK;
 }
 
@@ -277,13 +776,9 @@ static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, 
                                int* n_iterations, hipStream_t s) {
     const int64_t n = g.n();
     CLE_TRY(ccl_impl(binary, g, DBuf{B, CLE_F32}, CLE_CONN_BOX, CLE_FP32_KEY_COMPAT, nullptr, s));
-    const Tiles T = make_tiles(g);
-    Scratch tf;
-    CLE_TRY(tf.alloc((size_t)T.count() * 5, s));  // 4 "changed" slots + settled
-    k_vor_init<<<(unsigned)T.count(), VB, 0, s>>>(mask, A, B, g, T, tf.get<uint8_t>());
-    CLE_LAUNCHED();
-    CLE_TRY(grow(A, B, g, T, tf.get<uint8_t>(), n_iterations, s));
-    k_mask_out<<<vg(n), 256, 0, s>>>(A, mask, dst, n);
+    const float* res = nullptr;
+    CLE_TRY(grow_single(MaskedAny{mask, B}, A, B, g, n_iterations, &res, s));
+    k_mask_out<<<vg(n), 256, 0, s>>>(res, mask, dst, n);
     CLE_LAUNCHED();
     return CLE_OK;
 }
@@ -300,16 +795,15 @@ struct F32In {
     __device__ __forceinline__ float operator()(int64_t i) const { return p[i]; }
 };
 
-// one block per row of one image (rs = row-prefix table)
-__global__ __launch_bounds__(256) void k_maxima_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ rs,
+// one block per maxima tile (64 x 16 x 16 voxels) of one image (ms = tile-prefix table)
+__global__ __launch_bounds__(256) void k_maxima_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ ms,
                                                      int nimg, const float* F, uint8_t* S) {
-    const int64_t r = blockIdx.x;
-    const int j = find_seg(rs, nimg, r);
+    __shared__ float ring[3 * MX_RP];
+    const int64_t b = blockIdx.x;
+    const int j = find_seg(ms, nimg, b);
     const BImg& im = imgs[j];
-    const int64_t row = r - rs[j];
-    const int64_t z = row / im.g.H, y = row - z * im.g.H;
-    uint8_t* o = S + im.off + row * im.g.W;
-    maxima_row(F32In{F + im.off}, im.g, z, y, threadIdx.x, 256, [&](int64_t x, bool m) { o[x] = m ? 1 : 0; });
+    uint8_t* o = S + im.off;
+    maxima_tile(F32In{F + im.off}, im.g, b - ms[j], ring, [&](int64_t i, bool m) { o[i] = m ? 1 : 0; });
 }
 
 // G = F > float32(thr[j]); S = S AND G
@@ -328,93 +822,29 @@ __global__ __launch_bounds__(256) void k_seg_and_batch(const BImg* __restrict__ 
     }
 }
 
-// Work lists of the batched growth.  List t % 3 holds the tiles active at step t: the
-// 3^d tile neighbourhood of a tile changed at step t-1 or t-2 (at t = 0, 1: holds a
-// positive initial value).  A tile that changes at step t queues its neighbours for
-// t + 1 and t + 2; stamp[t & 1][tile] = last step the tile was queued for keeps every
-// list free of duplicates.  A step launch walks its list with a fixed grid, so its
-// cost follows the growth front instead of the volume.
-struct VorLists {
-    uint32_t* items;   // 3 lists of cap entries (global tile indices)
-    uint32_t* count;   // 3 counters
-    int32_t* stamp;    // 2 x cap
-    uint8_t* settled;  // cap
-    int64_t cap;       // tiles of all images
-};
-
-__device__ __forceinline__ uint64_t lanes_below_v() {
-    const int lane = threadIdx.x & 63;
-    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
-}
-
-// wavefront 0 of the block queues the neighbours of `tile` (inside its image) for
-// steps s1 and s2
-__device__ __forceinline__ void vor_queue(const Tiles& T, int64_t tile, int64_t tbase, int s1, int s2,
-                                          const VorLists& L) {
-    const int lane = threadIdx.x;
-    int64_t q = -1;
-    if (lane < 27) {
-        const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
-        const int64_t qx = tx + lane % 3 - 1, qy = ty + (lane / 3) % 3 - 1, qz = tz + lane / 9 - 1;
-        if (qx >= 0 && qx < T.nx && qy >= 0 && qy < T.ny && qz >= 0 && qz < T.nz)
-            q = tbase + (qz * T.ny + qy) * T.nx + qx;
-    }
-    for (int w = 0; w < 2; ++w) {
-        const int st = w ? s2 : s1;
-        const bool need = q >= 0 && atomicMax(&L.stamp[(int64_t)(st & 1) * L.cap + q], st) < st;
-        const uint64_t m = __ballot(need);
-        uint32_t base = 0;
-        if (lane == 0 && m) base = atomicAdd(&L.count[st % 3], (uint32_t)__popcll(m));
-        base = __shfl(base, 0);
-        if (need) L.items[(int64_t)(st % 3) * L.cap + base + __popcll(m & lanes_below_v())] = (uint32_t)q;
-    }
-}
-
 // A = B = (G - 1) + labels; tiles holding a positive value queue their neighbourhood
-// for steps 0 and 1
+// for pair 0
 __global__ __launch_bounds__(VB) void k_vor_init_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ ts,
                                                       int nimg, const uint8_t* G, float* A, float* B, VorLists L) {
     const int64_t b = blockIdx.x;
     const int j = find_seg(ts, nimg, b);
     const BImg& im = imgs[j];
     const Tiles T = make_tiles(im.g);
-    if (vor_init_tile(MaskedU8{G + im.off, B + im.off}, A + im.off, B + im.off, im.g, T, b - ts[j], b, 0, nullptr, L.settled) &&
+    if (vor_init_tile(MaskedU8{G + im.off, B + im.off}, A + im.off, B + im.off, im.g, T, b - ts[j], L.settled + b) &&
         threadIdx.x < 64)
-        vor_queue(T, b - ts[j], ts[j], 0, 1, L);
+        vor_queue(T, b - ts[j], j, ts[j], 0, L, false);
 }
 
-__global__ __launch_bounds__(VB) void k_vor_step_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ ts,
-                                                      int nimg, const float* __restrict__ src,
-                                                      float* __restrict__ dst, int t, VorLists L, uint32_t* gflags,
-                                                      uint32_t* gany, int ring) {
-    const uint32_t n = L.count[t % 3];
-    const uint32_t* items = L.items + (int64_t)(t % 3) * L.cap;
-    for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
-        const int64_t q = items[e];
-        const int j = find_seg(ts, nimg, q);
-        if (t > 0 && gflags[(int64_t)((t - 1) % ring) * nimg + j] == 0) continue;  // this image has converged
-        if (L.settled[q]) continue;
-        const BImg& im = imgs[j];
-        const Tiles T = make_tiles(im.g);
-        if (vor_tile_update(src + im.off, dst + im.off, im.g, T, q - ts[j], t, L.settled + q)) {
-            if (threadIdx.x == 0) {
-                gflags[(int64_t)(t % ring) * nimg + j] = 1;
-                gany[t % ring] = 1;
-            }
-            if (threadIdx.x < 64) vor_queue(T, q - ts[j], ts[j], t + 1, t + 2, L);
-        }
-    }
-}
-
-__global__ void k_vor_list_reset(uint32_t* count, int which) { count[which] = 0; }
-
+// dst = G ? result : 0, the result in the growth buffer of the image's last pair
 __global__ __launch_bounds__(256) void k_mask_out_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ vs,
-                                                       int nimg, const float* A, const uint8_t* G) {
+                                                       int nimg, const float* P0, const float* P1,
+                                                       const int32_t* __restrict__ iters, const uint8_t* G) {
     const int64_t b = blockIdx.x;
     const int j = find_seg(vs, nimg, b);
     const BImg& im = imgs[j];
     const int64_t n = im.g.n(), i0 = (b - vs[j]) * BV, i1 = i0 + BV < n ? i0 + BV : n;
     const DBuf d{im.dst, im.dst_dt};
+    const float* A = vor_result_buffer(iters[j]) ? P1 : P0;
     for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) st(d, i, G[im.off + i] ? A[im.off + i] : 0.f);
 }
 
@@ -433,6 +863,7 @@ struct PinnedStage {
 
 static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const float* ss, const float* so, int n,
                               int* n_steps, hipStream_t s) {
+    if (n > VOR_MAX_IMAGES) return fail(CLE_E_VALUE, "at most %d images per batch (got %d)", VOR_MAX_IMAGES, n);
     std::vector<BImg> imgs((size_t)n);
     int64_t total = 0, ntiles = 0;
     for (int j = 0; j < n; ++j) {
@@ -451,6 +882,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
         im.dst_dt = dsts[j].dtype;
         im.big = g.n() >= ((int64_t)1 << 24) ? 1 : 0;
         im.tbase = ntiles;
+        CLE_TRY(vor_set_tiles(im));
         ntiles += g.n() > 0 ? make_tiles(g).count() : 0;
     }
     if (total == 0) return CLE_OK;
@@ -458,7 +890,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     ccl_batch_layout(imgs, &csz);
     // packed work images: F (float32: blurs, CCL labels, growth buffer B), A (growth
     // buffer A), S (uint8 seeds), G (uint8 segmentation)
-    Scratch F, A, S, G, aux, tf, gf;
+    Scratch F, A, S, G, aux;
     CLE_TRY(F.alloc((size_t)total * 4, s));
     CLE_TRY(A.alloc((size_t)total * 4, s));
     CLE_TRY(S.alloc((size_t)total, s));
@@ -468,17 +900,9 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     float* mm = aux.get<float>();
     double* thr = (double*)(aux.get<char>() + (size_t)n * 8);
     uint32_t* ou = (uint32_t*)(aux.get<char>() + (size_t)n * 16);
-    const int64_t tcap = ntiles > 0 ? ntiles : 1;
-    CLE_TRY(tf.alloc((size_t)tcap * 3 * 4 + 16 + (size_t)tcap * 2 * 4 + (size_t)tcap, s));
-    VorLists VL;
-    VL.items = tf.get<uint32_t>();
-    VL.count = VL.items + (size_t)tcap * 3;
-    VL.stamp = (int32_t*)(VL.count + 4);
-    VL.settled = (uint8_t*)(VL.stamp + (size_t)tcap * 2);
-    VL.cap = tcap;
-    CLE_TRY(gf.alloc(((size_t)VOR_RING * n + VOR_RING) * 4, s));
-    uint32_t* gflags = gf.get<uint32_t>();
-    uint32_t* gany = gflags + (size_t)VOR_RING * n;
+    VorScratch V;
+    CLE_TRY(V.alloc(ntiles, n, s));
+    const VorLists VL = V.L;
     float* Fp = F.get<float>();
     float* Ap = A.get<float>();
     uint8_t* Sp = S.get<uint8_t>();
@@ -493,14 +917,14 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     for (int j = 0; j < n; ++j) tst[j] = imgs[j].tbase;
     tst[n] = ntiles;
     const size_t ts_off = blob.put_vec(tst);
-    std::vector<int64_t> rst((size_t)n + 1);
-    int64_t nrows = 0;
+    std::vector<int64_t> mst((size_t)n + 1);
+    int64_t nmt = 0;
     for (int j = 0; j < n; ++j) {
-        rst[j] = nrows;
-        nrows += imgs[j].g.n() > 0 ? imgs[j].g.D * imgs[j].g.H : 0;
+        mst[j] = nmt;
+        nmt += imgs[j].g.n() > 0 ? maxima_tiles(imgs[j].g) : 0;
     }
-    rst[n] = nrows;
-    const size_t rs_off = blob.put_vec(rst);
+    mst[n] = nmt;
+    const size_t ms_off = blob.put_vec(mst);
 
     std::vector<Launch> L;
     std::vector<std::shared_ptr<void>> keep;
@@ -512,8 +936,8 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     // spots = detect_maxima_box(gaussian_blur(src, spot_sigma))
     CLE_TRY(plan_blur_batch(r1, blob, L, s));
     L.push_back([=](const char* dev) -> int {
-        k_maxima_batch<<<(unsigned)nrows, 256, 0, s>>>((const BImg*)(dev + imgs_off), (const int64_t*)(dev + rs_off),
-                                                       n, Fp, Sp);
+        k_maxima_batch<<<(unsigned)nmt, 256, 0, s>>>((const BImg*)(dev + imgs_off), (const int64_t*)(dev + ms_off),
+                                                     n, Fp, Sp);
         CLE_LAUNCHED();
         return CLE_OK;
     });
@@ -530,8 +954,6 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     CLE_TRY(plan_ccl_batch(imgs, imgs_off, csz, Sp, Fp, blob, L, keep, s));
     if (ntiles > 0)
         L.push_back([=](const char* dev) -> int {
-            CLE_TRY(fill_u32(VL.count, 0u, 3, s));
-            CLE_TRY(fill_u32((uint32_t*)VL.stamp, 0xFFFFFFFFu, (size_t)VL.cap * 2, s));
             k_vor_init_batch<<<(unsigned)ntiles, VB, 0, s>>>((const BImg*)(dev + imgs_off),
                                                              (const int64_t*)(dev + ts_off), n, Gp, Ap, Fp, VL);
             CLE_LAUNCHED();
@@ -549,31 +971,12 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     const char* dev = dblob.get<char>();
     for (auto& l : L) CLE_TRY(l(dev));
 
-    // growth: all images together, VOR_BATCH steps between host checks
+    // growth: all images together, each stopping on its own convergence
     const BImg* dimgs = (const BImg*)(dev + imgs_off);
-    const int64_t* dts = (const int64_t*)(dev + ts_off);
-    int t = 0, steps = -1;
-    uint32_t hany[VOR_BATCH];
-    const unsigned vgrid = (unsigned)(ntiles < 8192 ? (ntiles > 0 ? ntiles : 1) : 8192);
-    while (ntiles > 0 && steps < 0) {
-        CLE_TRY(fill_u32(gflags + (size_t)(t % VOR_RING) * n, 0u, (size_t)VOR_BATCH * n, s));
-        CLE_TRY(fill_u32(gany + t % VOR_RING, 0u, VOR_BATCH, s));
-        for (int k = 0; k < VOR_BATCH; ++k) {
-            const int it = t + k;
-            const float* src = (it & 1) ? Fp : Ap;
-            float* dst = (it & 1) ? Ap : Fp;
-            k_vor_list_reset<<<1, 1, 0, s>>>(VL.count, (it + 2) % 3);  // consumed by step it - 1
-            k_vor_step_batch<<<vgrid, VB, 0, s>>>(dimgs, dts, n, src, dst, it, VL, gflags, gany, VOR_RING);
-        }
-        CLE_LAUNCHED();
-        CLE_HIP(hipMemcpyAsync(hany, gany + t % VOR_RING, sizeof(hany), hipMemcpyDeviceToHost, s));
-        CLE_HIP(hipStreamSynchronize(s));
-        for (int k = 0; k < VOR_BATCH; ++k)
-            if (hany[k] == 0) { steps = t + k + 1; break; }
-        t += VOR_BATCH;
-    }
+    int steps = 0;
+    if (ntiles > 0) CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V, false, &steps, s));
     if (n_steps) *n_steps = steps;
-    k_mask_out_batch<<<(unsigned)vblocks, 256, 0, s>>>(dimgs, (const int64_t*)(dev + vs_off), n, Ap, Gp);
+    k_mask_out_batch<<<(unsigned)vblocks, 256, 0, s>>>(dimgs, (const int64_t*)(dev + vs_off), n, Ap, Fp, V.S.iters, Gp);
     CLE_LAUNCHED();
     // the pinned tables and the scratch are released once the work has run
     CLE_HIP(hipStreamSynchronize(s));
@@ -611,16 +1014,12 @@ int cle_extend_labeling_via_voronoi(const cle_buf* src, const cle_buf* dst, int*
     hipStream_t s = (hipStream_t)stream;
     const int64_t n = g.n();
     if (!n) return CLE_OK;
-    Scratch a, b, tf;
+    Scratch a, b;
     CLE_TRY(a.alloc((size_t)n * 4, s));
     CLE_TRY(b.alloc((size_t)n * 4, s));
-    const Tiles T = make_tiles(g);
-    CLE_TRY(tf.alloc((size_t)T.count() * 5, s));  // 4 "changed" slots + settled
-    k_vor_init_labels<<<(unsigned)T.count(), VB, 0, s>>>(dbuf(src), a.get<float>(), b.get<float>(), g, T,
-                                                        tf.get<uint8_t>());
-    CLE_LAUNCHED();
-    CLE_TRY(grow(a.get<float>(), b.get<float>(), g, T, tf.get<uint8_t>(), n_iterations, s));
-    k_store_out<<<vg(n), 256, 0, s>>>(a.get<float>(), dbuf(dst), n);
+    const float* res = nullptr;
+    CLE_TRY(grow_single(LabelsAny{dbuf(src)}, a.get<float>(), b.get<float>(), g, n_iterations, &res, s));
+    k_store_out<<<vg(n), 256, 0, s>>>(res, dbuf(dst), n);
     CLE_LAUNCHED();
     return CLE_OK;
 }
