@@ -247,7 +247,7 @@ __device__ __forceinline__ uint32_t bin_fast(float v, const BinFast& c) {
 // padded to an odd stride so equal bins of different replicas sit in different LDS
 // banks).  Rows whose length is not a multiple of 4 or whose start is not 16-byte
 // aligned take the scalar loads.
-constexpr int HREP = 8;
+constexpr int HREP = 4;
 
 template <typename LD>
 __global__ __launch_bounds__(HB) void k_hist_rows(LD in, const float* f4base, Geom g, int bins, int zc, int zchunks,
@@ -357,10 +357,15 @@ template <typename LD>
 static int hist_launch(LD in, const float* f4base, const Geom& g, int bins, float mn, float mx, const float* mm,
                        uint32_t* h2, hipStream_t s) {
     if (bins <= HIST_LDS_MAX_BINS) {
-        // whole rows per block when there are enough rows to fill the chip (no
-        // atomics); otherwise z chunks of about 16 K voxels
+        // rows split into z chunks so that the grid holds at least 4096 blocks (the
+        // chunks of a row add their exact integer counts atomically; short images:
+        // chunks of about 16 K voxels)
         int zc = (int)g.D;
-        if (g.H < 1024) {
+        if (g.H >= 1024) {
+            const int64_t chunks = (4096 + g.H - 1) / g.H;
+            zc = (int)((g.D + chunks - 1) / chunks);
+            if (zc < 8 && g.D >= 8) zc = 8;
+        } else {
             const int64_t per_block = 16384;
             zc = (int)(per_block / (g.W > 0 ? g.W : 1));
             if (zc < 1) zc = 1;

@@ -42,6 +42,8 @@
 // L2 in every block.)
 #include "common.hpp"
 
+#include <mutex>
+
 namespace cle {
 
 int maxima_impl(DBuf src, DBuf dst, const Geom& g, hipStream_t s);
@@ -831,8 +833,17 @@ __global__ __launch_bounds__(VB) void k_vor_init_batch(const BImg* __restrict__ 
     const BImg& im = imgs[j];
     const Tiles T = make_tiles(im.g);
     if (vor_init_tile(MaskedU8{G + im.off, B + im.off}, A + im.off, B + im.off, im.g, T, b - ts[j], L.settled + b) &&
-        threadIdx.x < 64)
+        threadIdx.x < 64) {
         vor_queue(T, b - ts[j], j, ts[j], 0, L, false);
+        // raster-mode marks as well (a one-image batch grows in raster order)
+        if (threadIdx.x < 27) {
+            const int64_t t = b - ts[j], k = threadIdx.x;
+            const int64_t tx = t % T.nx, ty = (t / T.nx) % T.ny, tz = t / (T.nx * T.ny);
+            const int64_t qx = tx + k % 3 - 1, qy = ty + (k / 3) % 3 - 1, qz = tz + k / 9 - 1;
+            if (qx >= 0 && qx < T.nx && qy >= 0 && qy < T.ny && qz >= 0 && qz < T.nz)
+                L.mark[ts[j] + (qz * T.ny + qy) * T.nx + qx] = 0;
+        }
+    }
 }
 
 // dst = G ? result : 0, the result in the growth buffer of the image's last pair
@@ -848,15 +859,45 @@ __global__ __launch_bounds__(256) void k_mask_out_batch(const BImg* __restrict__
     for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) st(d, i, G[im.off + i] ? A[im.off + i] : 0.f);
 }
 
-// pinned host staging of the batch tables, released once the stream has drained
-// (also when a later step fails, the copy out of it may still be in flight)
+// Pinned host staging of the batch tables.  Buffers are taken from a process-wide
+// pool and given back when the call ends (after its stream has drained, also when a
+// later step fails: the copy out of the buffer may still be in flight); they are never
+// freed, so a call allocates pinned memory only when no free buffer is large enough.
+struct PinnedPool {
+    std::mutex m;
+    std::vector<std::pair<void*, size_t>> free;
+};
+static PinnedPool& pinned_pool() {
+    static PinnedPool* pool = new PinnedPool();  // outlives every caller (never destroyed)
+    return *pool;
+}
 struct PinnedStage {
     void* p = nullptr;
+    size_t cap = 0;
     hipStream_t s = nullptr;
+    int take(size_t bytes, hipStream_t stream) {
+        s = stream;
+        PinnedPool& pool = pinned_pool();
+        {
+            std::lock_guard<std::mutex> lk(pool.m);
+            for (size_t k = 0; k < pool.free.size(); ++k)
+                if (pool.free[k].second >= bytes) {
+                    p = pool.free[k].first;
+                    cap = pool.free[k].second;
+                    pool.free.erase(pool.free.begin() + (long)k);
+                    return CLE_OK;
+                }
+        }
+        cap = bytes < ((size_t)1 << 20) ? ((size_t)1 << 20) : bytes;
+        CLE_HIP(hipHostMalloc(&p, cap));
+        return CLE_OK;
+    }
     ~PinnedStage() {
         if (p) {
             (void)hipStreamSynchronize(s);
-            (void)hipHostFree(p);
+            PinnedPool& pool = pinned_pool();
+            std::lock_guard<std::mutex> lk(pool.m);
+            pool.free.emplace_back(p, cap);
         }
     }
 };
@@ -964,8 +1005,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     Scratch dblob;
     CLE_TRY(dblob.alloc(blob.h.size(), s));
     PinnedStage pin;
-    pin.s = s;
-    CLE_HIP(hipHostMalloc(&pin.p, blob.h.size()));
+    CLE_TRY(pin.take(blob.h.size(), s));
     memcpy(pin.p, blob.h.data(), blob.h.size());
     CLE_HIP(hipMemcpyAsync(dblob.p, pin.p, blob.h.size(), hipMemcpyHostToDevice, s));
     const char* dev = dblob.get<char>();
@@ -974,7 +1014,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     // growth: all images together, each stopping on its own convergence
     const BImg* dimgs = (const BImg*)(dev + imgs_off);
     int steps = 0;
-    if (ntiles > 0) CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V, false, &steps, s));
+    if (ntiles > 0) CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V, n == 1, &steps, s));
     if (n_steps) *n_steps = steps;
     k_mask_out_batch<<<(unsigned)vblocks, 256, 0, s>>>(dimgs, (const int64_t*)(dev + vs_off), n, Ap, Fp, V.S.iters, Gp);
     CLE_LAUNCHED();
