@@ -64,6 +64,14 @@ int cle_set(const cle_buf* dst, float value, void* stream);
 int cle_copy(const cle_buf* src, const cle_buf* dst, void* stream);
 
 /* ---- filters ------------------------------------------------------------ */
+/* Autotuner of the isotropic blur's z chunking (blocks per launch); no counterpart in
+ * the reference.  On by default (CLE_AUTOTUNE=0 in the environment turns it off):
+ * the first blur of at least 2^22 voxels per (half-width, dimensionality, source
+ * type, size class) times the candidate chunk counts on the caller's stream,
+ * synchronising it once, and keeps the fastest.  Results do not depend on it.
+ * on = 0 turns it off and forgets the choices made. */
+int cle_set_autotune(int on);
+
 /* _tier1/_gaussian_blur.py:6-58 + _tier1/_execute_separable_kernel.py:7-75:
  * separable normalised Gaussian, N(s) = floor(8 s) | 1 taps, clamp-to-edge,
  * x then y then z; sigma <= 0 leaves that axis; sigma_z ignored for 2D. */

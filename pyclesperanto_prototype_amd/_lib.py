@@ -67,6 +67,7 @@ SIGNATURES = {
     "cle_label_spots": [_P, _P, _V],
     "cle_masked_voronoi_labeling": [_P, _P, _P, ctypes.POINTER(ctypes.c_int), _V],
     "cle_extend_labeling_via_voronoi": [_P, _P, ctypes.POINTER(ctypes.c_int), _V],
+    "cle_set_autotune": [ctypes.c_int],
     "cle_voronoi_otsu_labeling": [_P, _P, _F, _F, _V],
     "cle_gauss_otsu_labeling": [_P, _P, _F, _V],
     "cle_voronoi_otsu_labeling_batch": [_P, _P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),

@@ -417,6 +417,11 @@ logical_and = binary_and
 binary_intersection = binary_and
 
 
+def set_autotune(on: bool = True):
+    """Switch the blur's z-chunk autotuner (on by default; results never depend on it)."""
+    _lib.call("cle_set_autotune", 1 if on else 0)
+
+
 def release_pool():
     """Free the library's cached scratch memory on the current device (synchronises it)."""
     _lib.call("cle_release_pool")

@@ -19,6 +19,14 @@
 
 #include "common.hpp"
 
+#include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
 namespace cle {
 
 constexpr int MAXC = 400;  // largest half-width (sigma up to 50)
@@ -577,20 +585,112 @@ static int try_fused(const cle_buf* src, const cle_buf* dst, const Taps* taps[3]
     return rc;
 }
 
+// ---------------------------------------------------------------- z-chunk autotuner
+// The isotropic blur splits every column tile's z range into nzc chunks: more chunks
+// fill the chip with more blocks, each chunk re-reads 2C halo slices.  The output is
+// the same for every nzc (each voxel's taps are summed in the same order).  For images
+// of at least 2^22 voxels the first launch of a (half-width, dimensionality, source
+// type, size class) times the candidate chunk counts on the caller's stream and keeps
+// the fastest for the process; smaller images and captured streams use the default
+// (about ISO_BLOCKS blocks).  CLE_AUTOTUNE=0 or cle_set_autotune(0) turns it off.
+static std::atomic<int> g_autotune{-1};  // -1: read CLE_AUTOTUNE on first use
+
+static bool autotune_on() {
+    int v = g_autotune.load();
+    if (v < 0) {
+        const char* e = getenv("CLE_AUTOTUNE");
+        v = (e && e[0] == '0') ? 0 : 1;
+        g_autotune.store(v);
+    }
+    return v != 0;
+}
+
+struct TuneKey {
+    int c, z3, src, dev, lxy, lz;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(c, z3, src, dev, lxy, lz) < std::tie(o.c, o.z3, o.src, o.dev, o.lxy, o.lz);
+    }
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, int64_t> g_tune;  // key -> chosen nzc
+
+static int ilog2(int64_t v) {
+    int l = 0;
+    while (v > 1) { v >>= 1; ++l; }
+    return l;
+}
+
+template <int C, bool Z3, typename LD>
+static void launch_iso_nzc(LD in, float* out, const Geom& g, const IsoW<C>& w, int ntx, int nty, int64_t nzc,
+                           hipStream_t s) {
+    const int zc = (int)((g.D + nzc - 1) / nzc);
+    nzc = (g.D + zc - 1) / zc;
+    k_blur_iso<C, Z3, LD><<<(unsigned)((int64_t)ntx * nty * nzc), ISO_THREADS, 0, s>>>(in, out, g, zc, ntx, nty, w);
+}
+
+template <typename LD> constexpr int ld_code();
+template <> constexpr int ld_code<LdF32>() { return 0; }
+template <> constexpr int ld_code<LdU16>() { return 1; }
+template <> constexpr int ld_code<LdU8>() { return 2; }
+
 template <int C, bool Z3, typename LD>
 static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hipStream_t s) {
     const int ntx = (int)((g.W + ISO_TX - 1) / ISO_TX), nty = (int)((g.H + ISO_TY - 1) / ISO_TY);
     const int64_t nxy = (int64_t)ntx * nty;
     constexpr int CZ = Z3 ? C : 0;
-    int64_t nzc = (ISO_BLOCKS + nxy - 1) / nxy;
     const int64_t max_nzc = g.D / (4 * CZ + 8) > 1 ? g.D / (4 * CZ + 8) : 1;
-    if (nzc > max_nzc) nzc = max_nzc;
-    if (nzc < 1) nzc = 1;
-    const int zc = (int)((g.D + nzc - 1) / nzc);
-    nzc = (g.D + zc - 1) / zc;
+    auto nzc_for = [&](int64_t blocks) {
+        int64_t v = (blocks + nxy - 1) / nxy;
+        return v < 1 ? 1 : (v > max_nzc ? max_nzc : v);
+    };
     IsoW<C> w;
     for (int v = 0; v <= C; ++v) w.w[v] = wts[v];
-    k_blur_iso<C, Z3, LD><<<(unsigned)(nxy * nzc), ISO_THREADS, 0, s>>>(in, out, g, zc, ntx, nty, w);
+    int64_t nzc = nzc_for(ISO_BLOCKS);
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (g.n() >= ((int64_t)1 << 22) && max_nzc > 1 && autotune_on() &&
+        hipStreamIsCapturing(s, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        const TuneKey key{C, Z3 ? 1 : 0, ld_code<LD>(), dev, ilog2(nxy), ilog2(g.D)};
+        bool known = false;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune.find(key);
+            if (it != g_tune.end()) { nzc = it->second < max_nzc ? it->second : max_nzc; known = true; }
+        }
+        if (!known) {
+            std::vector<int64_t> cand;
+            for (int64_t b : {256, 512, 1024, 2048, 4096}) {
+                const int64_t v = nzc_for(b);
+                if (std::find(cand.begin(), cand.end(), v) == cand.end()) cand.push_back(v);
+            }
+            hipEvent_t e0, e1;
+            if (cand.size() > 1 && hipEventCreate(&e0) == hipSuccess) {
+                if (hipEventCreate(&e1) == hipSuccess) {
+                    float best = 1e30f;
+                    for (int64_t v : cand) {
+                        launch_iso_nzc<C, Z3>(in, out, g, w, ntx, nty, v, s);  // warm
+                        for (int rep = 0; rep < 3; ++rep) {                   // best of three
+                            (void)hipEventRecord(e0, s);
+                            launch_iso_nzc<C, Z3>(in, out, g, w, ntx, nty, v, s);
+                            (void)hipEventRecord(e1, s);
+                            float ms = 1e30f;
+                            if (hipEventSynchronize(e1) == hipSuccess &&
+                                hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best) {
+                                best = ms;
+                                nzc = v;
+                            }
+                        }
+                    }
+                    (void)hipEventDestroy(e1);
+                }
+                (void)hipEventDestroy(e0);
+            }
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune[key] = nzc;
+        }
+    }
+    launch_iso_nzc<C, Z3>(in, out, g, w, ntx, nty, nzc, s);
 }
 
 constexpr int ISO_MAXC = 20;
@@ -876,6 +976,15 @@ int blur_impl(const cle_buf* src, const cle_buf* dst, const float sig[3], hipStr
 using namespace cle;
 
 extern "C" {
+
+int cle_set_autotune(int on) {
+    g_autotune.store(on ? 1 : 0);
+    if (!on) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune.clear();
+    }
+    return CLE_OK;
+}
 
 int cle_gaussian_blur(const cle_buf* src, const cle_buf* dst, float sigma_x, float sigma_y, float sigma_z,
                       void* stream) {
