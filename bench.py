@@ -221,6 +221,22 @@ def batch_section(cle, torch, n_images):
     same = sum(int(torch.equal(a.tensor, b.tensor)) for a, b in zip(plain_out, batch_out))
     nv = info["voxels"]
     log(f"batched: plain {plain * 1e3:.1f} ms, batched {batched * 1e3:.1f} ms, identical {same}/{n_images}")
+    # the same batch captured once as a hipGraph and replayed
+    graph_out = [cle.create(t.shape, np.uint32) for t in imgs_t]
+    t0 = time.perf_counter()
+    plan = cle.voronoi_otsu_labeling_batch_plan(imgs, ss, so, destinations=graph_out, graph_pairs=32)
+    torch.cuda.synchronize()
+    build = time.perf_counter() - t0
+    plan.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.run()
+    torch.cuda.synchronize()
+    replay = time.perf_counter() - t0
+    same_g = sum(int(torch.equal(a.tensor, b.tensor)) for a, b in zip(plain_out, graph_out))
+    plan.close()
+    del graph_out
+    log(f"graph: build {build * 1e3:.0f} ms, replay {replay * 1e3:.1f} ms, identical {same_g}/{n_images}")
     del imgs, imgs_t, plain_out, batch_out
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
@@ -231,7 +247,10 @@ def batch_section(cle, torch, n_images):
             "images": n_images, "voxels": nv, "plain_ms": round(plain * 1e3, 1),
             "plain_mvox_s": round(nv / plain / 1e6, 1), "batched_ms": round(batched * 1e3, 1),
             "batched_mvox_s": round(nv / batched / 1e6, 1), "speedup_vs_plain": round(plain / batched, 2),
-            "growth_steps": steps, "identical_to_plain": f"{same}/{n_images}"}
+            "growth_steps": steps, "identical_to_plain": f"{same}/{n_images}",
+            "graph": {"build_ms": round(build * 1e3, 1), "replay_ms": round(replay * 1e3, 1),
+                      "replay_mvox_s": round(nv / replay / 1e6, 1), "speedup_vs_plain": round(plain / replay, 2),
+                      "identical_to_plain": f"{same_g}/{n_images}"}}
 
 
 def serve_section(cle, torch, vol_c2, n_requests, qps):

@@ -176,6 +176,22 @@ int cle_gauss_otsu_labeling(const cle_buf* src, const cle_buf* dst, float outlin
 int cle_voronoi_otsu_labeling_batch(const cle_buf* srcs, const cle_buf* dsts, const float* spot_sigmas,
                                     const float* outline_sigmas, int n_images, int* n_steps, void* stream);
 
+/* Replayable plan of cle_voronoi_otsu_labeling_batch for these exact buffers and
+ * sigmas (no counterpart in the reference: the batched executor as a captured
+ * hipGraph).  Building it runs the batch twice on a private stream (after the work
+ * already queued on `stream`; dsts hold the result), then captures every stage and
+ * graph_pairs pairs of growth steps into one graph; the plan owns the device memory
+ * the graph uses until cle_plan_destroy.  cle_plan_run replays the graph on `stream`
+ * with the sources' current contents and, if some image still grows after
+ * graph_pairs pairs, finishes the growth eagerly; it returns when the labels are in
+ * dsts (it synchronises the stream).  Runs of one plan must not overlap. */
+typedef struct cle_plan cle_plan;
+int cle_voronoi_otsu_labeling_batch_plan(const cle_buf* srcs, const cle_buf* dsts, const float* spot_sigmas,
+                                         const float* outline_sigmas, int n_images, int graph_pairs,
+                                         cle_plan** out_plan, void* stream);
+int cle_plan_run(cle_plan* plan, int* n_steps, void* stream);
+int cle_plan_destroy(cle_plan* plan);
+
 /* ---- statistics --------------------------------------------------------- */
 /* number of columns of the statistics table; column c holds STATISTICS_ENTRY c
  * (_tier9/_statistics_entry.py:3-46): IDENTIFIER, BOUNDING_BOX_X, ... ,

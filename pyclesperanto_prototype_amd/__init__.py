@@ -22,7 +22,8 @@ from ._ops import (STATISTICS_KEYS, add_image_and_scalar, add_images, add_images
                    set_fp32_key_compat, set_non_zero_pixels_to_pixel_index, set_nonzero_pixels_to_pixelindex,
                    statistics_of_background_and_labelled_pixels, statistics_of_labeled_pixels,
                    statistics_of_labelled_pixels, statistics_table, sum_of_all_pixels, threshold_otsu,
-                   voronoi_labeling, voronoi_otsu_labeling, voronoi_otsu_labeling_batch)
+                   voronoi_labeling, voronoi_otsu_labeling, voronoi_otsu_labeling_batch,
+                   voronoi_otsu_labeling_batch_plan, BatchPlan)
 from .io import imread, imread_stack, push_regionprops, push_regionprops_column
 from ._lib import load as _load_library
 

@@ -310,6 +310,67 @@ def voronoi_otsu_labeling_batch(sources, spot_sigmas=2, outline_sigmas=2, destin
     return (destinations, steps.value) if return_steps else destinations
 
 
+class BatchPlan:
+    """A batch of voronoi_otsu_labeling captured as one hipGraph for these exact source
+    and destination buffers and sigmas (cle_voronoi_otsu_labeling_batch_plan).
+    Building it runs the batch (the destinations hold its labels); `run()` replays the
+    graph on the sources' current contents -- write new images into the same source
+    buffers and run again.  The plan owns its device memory until `close()`."""
+
+    def __init__(self, sources, spot_sigmas=2, outline_sigmas=2, destinations=None, graph_pairs: int = 32):
+        srcs = [s if isinstance(s, Array) else push(s) for s in sources]
+        n = len(srcs)
+        if n == 0:
+            raise ValueError("a plan needs at least one image")
+        ss = np.broadcast_to(np.asarray(spot_sigmas, np.float32), (n,)).copy()
+        so = np.broadcast_to(np.asarray(outline_sigmas, np.float32), (n,)).copy()
+        if destinations is None:
+            destinations = [create_labels_like(s) for s in srcs]
+        if len(destinations) != n:
+            raise ValueError(f"{len(destinations)} destinations for {n} sources")
+        self.sources, self.destinations = srcs, list(destinations)  # the graph holds their addresses
+        src_bufs = (_lib.cle_buf * n)(*[s.buf() for s in srcs])
+        dst_bufs = (_lib.cle_buf * n)(*[d.buf() for d in self.destinations])
+        fp = ctypes.POINTER(ctypes.c_float)
+        h = ctypes.c_void_p()
+        _lib.call("cle_voronoi_otsu_labeling_batch_plan", src_bufs, dst_bufs, ss.ctypes.data_as(fp),
+                  so.ctypes.data_as(fp), n, int(graph_pairs), ctypes.byref(h), _S())
+        self._h = h
+        self.steps = None
+
+    def run(self):
+        """Replay: the labels of the sources' current contents, in `destinations`."""
+        if self._h is None:
+            raise ValueError("the plan is closed")
+        steps = ctypes.c_int(0)
+        _lib.call("cle_plan_run", self._h, ctypes.byref(steps), _S())
+        self.steps = steps.value
+        return self.destinations
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            _lib.call("cle_plan_destroy", self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def voronoi_otsu_labeling_batch_plan(sources, spot_sigmas=2, outline_sigmas=2, destinations=None,
+                                     graph_pairs: int = 32) -> BatchPlan:
+    """voronoi_otsu_labeling of a batch as a replayable hipGraph (see BatchPlan)."""
+    return BatchPlan(sources, spot_sigmas, outline_sigmas, destinations, graph_pairs)
+
+
 @plugin_function(categories=['label', 'in assistant'], output_creator=create_labels_like)
 def gauss_otsu_labeling(source: Image, label_image_destination: Image = None, outline_sigma: float = 2) -> Image:
     """_tier9/_gauss_otsu_labeling.py:6-49."""

@@ -63,6 +63,9 @@ inline DBuf dbuf(const cle_buf* b) { return DBuf{b->data, b->dtype}; }
 // released by a call is reused by the next call on the same stream.
 void* scratch_alloc(size_t bytes, hipStream_t s);
 void scratch_free(void* p, hipStream_t s);
+// arena of a private stream handed to a captured graph / freed with it
+int arena_set_owned(hipStream_t s);
+int arena_release_stream(hipStream_t s);
 
 // RAII holder for one scratch allocation on a stream
 struct Scratch {

@@ -94,6 +94,7 @@ struct Arena {
     std::vector<ArenaBlock> stack;
     int cur = 0;      // chunk the bump offset points into
     size_t top = 0;   // bump offset in chunks[cur]
+    bool owned = false;  // held by a captured graph: cle_release_pool leaves it alone
     size_t capacity() const {
         size_t t = 0;
         for (auto& c : chunks) t += c.second;
@@ -161,6 +162,30 @@ void* scratch_alloc(size_t bytes, hipStream_t s) {
     a->stack.push_back({a->cur, a->top, need, false});
     a->top += need;
     return p;
+}
+
+// The arena of (current device, s) becomes owned by a captured graph: its memory
+// stays where the graph's kernels expect it until arena_release_stream(s).
+int arena_set_owned(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    Arena* a = arena_for(s);
+    if (!a) return fail(CLE_E_HIP, "no arena for the stream");
+    a->owned = true;
+    return CLE_OK;
+}
+
+// Frees the arena of (current device, s) after draining s.
+int arena_release_stream(hipStream_t s) {
+    int dev = 0;
+    CLE_HIP(hipGetDevice(&dev));
+    CLE_HIP(hipStreamSynchronize(s));
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    auto it = g_arenas.find({dev, s});
+    if (it == g_arenas.end()) return CLE_OK;
+    for (auto& c : it->second->chunks) CLE_HIP(hipFree(c.first));
+    delete it->second;
+    g_arenas.erase(it);
+    return CLE_OK;
 }
 
 void scratch_free(void* p, hipStream_t s) {
@@ -234,7 +259,7 @@ int cle_release_pool(void) {
     std::lock_guard<std::mutex> lk(g_arena_mu);
     for (auto it = g_arenas.begin(); it != g_arenas.end();) {
         Arena* a = it->second;
-        if (it->first.first != dev || !a->stack.empty()) {
+        if (it->first.first != dev || !a->stack.empty() || a->owned) {
             ++it;
             continue;
         }

@@ -674,23 +674,30 @@ struct VorScratch {
     }
 };
 
-// Runs pairs until every image has converged; P0 holds the initial state (as does P1);
-// pair b reads P[b & 1] and writes P[(b + 1) & 1].  `max_iters` (host, may be null)
-// receives the largest reference iteration count over the images.
-static int grow_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t ntiles, const VorScratch& V,
-                      bool raster, int* max_iters, hipStream_t s) {
+// Launches pairs [b, b + count) and the close of the last one (no host synchronisation).
+static void launch_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t ntiles, const VorLists& L,
+                         const VorState& S, bool raster, int b, int count, hipStream_t s) {
     const unsigned grid = raster ? (unsigned)ntiles : (unsigned)(ntiles < 4096 ? (ntiles > 0 ? ntiles : 1) : 4096);
-    int b = 0;
+    for (int k = 0; k < count; ++k, ++b) {
+        if (raster) k_vor_pair_grid<<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S);
+        else k_vor_pair<<<grid, VB, 0, s>>>(dimgs, nimg, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S);
+    }
+    k_vor_close<<<1, VB, 0, s>>>(nimg, b - 1, S);
+}
+
+// Runs pairs from b0 until every image has converged; P0 holds the initial state (as
+// does P1); pair b reads P[b & 1] and writes P[(b + 1) & 1].  `max_iters` (host, may
+// be null) receives the largest reference iteration count over the images.
+static int grow_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t ntiles, const VorLists& L,
+                      const VorState& S, bool raster, int b0, int* max_iters, hipStream_t s) {
+    int b = b0;
     uint32_t hw[2] = {0, 0};
     for (;;) {
         const int chunk = b == 0 ? 2 : VOR_CHUNK;
-        for (int k = 0; k < chunk; ++k, ++b) {
-            if (raster) k_vor_pair_grid<<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, V.L, V.S);
-            else k_vor_pair<<<grid, VB, 0, s>>>(dimgs, nimg, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, V.L, V.S);
-        }
-        k_vor_close<<<1, VB, 0, s>>>(nimg, b - 1, V.S);
+        launch_pairs(dimgs, nimg, P0, P1, ntiles, L, S, raster, b, chunk, s);
+        b += chunk;
         CLE_LAUNCHED();
-        CLE_HIP(hipMemcpyAsync(hw, V.S.word, sizeof(hw), hipMemcpyDeviceToHost, s));
+        CLE_HIP(hipMemcpyAsync(hw, S.word, sizeof(hw), hipMemcpyDeviceToHost, s));
         CLE_HIP(hipStreamSynchronize(s));
         if (hw[0] == 0) break;
     }
@@ -735,7 +742,7 @@ static int grow_single(VF val, float* P0, float* P1, const Geom& g, int* n_itera
     k_vor_init<<<(unsigned)ntiles, VB, 0, s>>>(val, P0, P1, g, V.L);
     CLE_LAUNCHED();
     int iters = 0;
-    CLE_TRY(grow_pairs(tab.get<BImg>(), 1, P0, P1, ntiles, V, true, &iters, s));
+    CLE_TRY(grow_pairs(tab.get<BImg>(), 1, P0, P1, ntiles, V.L, V.S, true, 0, &iters, s));
     if (n_iterations) *n_iterations = iters;
     *result = vor_result_buffer(iters) ? P1 : P0;
     return CLE_OK;
@@ -892,6 +899,12 @@ struct PinnedStage {
         CLE_HIP(hipHostMalloc(&p, cap));
         return CLE_OK;
     }
+    // hands the buffer to a captured graph, which reads it on every replay
+    void* detach() {
+        void* q = p;
+        p = nullptr;
+        return q;
+    }
     ~PinnedStage() {
         if (p) {
             (void)hipStreamSynchronize(s);
@@ -902,8 +915,28 @@ struct PinnedStage {
     }
 };
 
+// Capture of the batched pipeline into a graph: the growth runs a fixed number of
+// pairs and copies its "still growing" word to pinned host memory instead of checking
+// it; nothing synchronises.  The batch fills in what a replay needs to finish the
+// growth eagerly when the fixed pairs were not enough.
+struct BatchCapture {
+    int graph_pairs = 0;
+    uint32_t* word_host = nullptr;  // pinned uint32[2], written by the graph
+    // filled in by the capture run
+    void* pinned = nullptr;         // table staging the graph copies from
+    const BImg* dimgs = nullptr;
+    const int64_t* dvs = nullptr;
+    int n = 0;
+    int64_t ntiles = 0, vblocks = 0;
+    float* Ap = nullptr;
+    float* Fp = nullptr;
+    const uint8_t* Gp = nullptr;
+    VorLists L{};
+    VorState S{};
+};
+
 static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const float* ss, const float* so, int n,
-                              int* n_steps, hipStream_t s) {
+                              int* n_steps, hipStream_t s, BatchCapture* cap = nullptr) {
     if (n > VOR_MAX_IMAGES) return fail(CLE_E_VALUE, "at most %d images per batch (got %d)", VOR_MAX_IMAGES, n);
     std::vector<BImg> imgs((size_t)n);
     int64_t total = 0, ntiles = 0;
@@ -1008,26 +1041,153 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     CLE_TRY(pin.take(blob.h.size(), s));
     memcpy(pin.p, blob.h.data(), blob.h.size());
     CLE_HIP(hipMemcpyAsync(dblob.p, pin.p, blob.h.size(), hipMemcpyHostToDevice, s));
+    if (cap) cap->pinned = pin.detach();
     const char* dev = dblob.get<char>();
     for (auto& l : L) CLE_TRY(l(dev));
 
     // growth: all images together, each stopping on its own convergence
     const BImg* dimgs = (const BImg*)(dev + imgs_off);
-    int steps = 0;
-    if (ntiles > 0) CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V, n == 1, &steps, s));
-    if (n_steps) *n_steps = steps;
-    k_mask_out_batch<<<(unsigned)vblocks, 256, 0, s>>>(dimgs, (const int64_t*)(dev + vs_off), n, Ap, Fp, V.S.iters, Gp);
+    const int64_t* dvs = (const int64_t*)(dev + vs_off);
+    if (cap) {
+        cap->dimgs = dimgs;
+        cap->dvs = dvs;
+        cap->n = n;
+        cap->ntiles = ntiles;
+        cap->vblocks = vblocks;
+        cap->Ap = Ap;
+        cap->Fp = Fp;
+        cap->Gp = Gp;
+        cap->L = V.L;
+        cap->S = V.S;
+        if (ntiles > 0) {
+            launch_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, n == 1, 0, cap->graph_pairs, s);
+            CLE_LAUNCHED();
+            CLE_HIP(hipMemcpyAsync(cap->word_host, V.S.word, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        }
+    } else {
+        int steps = 0;
+        if (ntiles > 0) CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, n == 1, 0, &steps, s));
+        if (n_steps) *n_steps = steps;
+    }
+    k_mask_out_batch<<<(unsigned)vblocks, 256, 0, s>>>(dimgs, dvs, n, Ap, Fp, V.S.iters, Gp);
     CLE_LAUNCHED();
-    // the pinned tables and the scratch are released once the work has run
-    CLE_HIP(hipStreamSynchronize(s));
+    // the pinned tables and the scratch are released once the work has run (a
+    // capture has recorded the work and keeps both)
+    if (!cap) CLE_HIP(hipStreamSynchronize(s));
     return CLE_OK;
 }
 
 }  // namespace cle
 
+// A replayable plan of one batch (declared opaque in the C ABI).
+struct cle_plan {
+    int dev = 0;
+    hipStream_t ps = nullptr;  // private stream: its arena holds the plan's device memory
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t* word_host = nullptr;
+    cle::BatchCapture cap;
+};
+
 using namespace cle;
 
+static void plan_free(cle_plan* P) {
+    if (!P) return;
+    if (P->exec) (void)hipGraphExecDestroy(P->exec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    if (P->ps) {
+        (void)arena_release_stream(P->ps);
+        (void)hipStreamDestroy(P->ps);
+    }
+    if (P->word_host) (void)hipHostFree(P->word_host);
+    if (P->cap.pinned) (void)hipHostFree(P->cap.pinned);
+    delete P;
+}
+
 extern "C" {
+
+int cle_voronoi_otsu_labeling_batch_plan(const cle_buf* srcs, const cle_buf* dsts, const float* spot_sigmas,
+                                         const float* outline_sigmas, int n_images, int graph_pairs,
+                                         cle_plan** out_plan, void* stream) {
+    if (!out_plan) return fail(CLE_E_VALUE, "null plan pointer");
+    *out_plan = nullptr;
+    if (n_images < 1) return fail(CLE_E_VALUE, "a plan needs at least one image");
+    if (graph_pairs < 1) return fail(CLE_E_VALUE, "graph_pairs must be >= 1");
+    if (!srcs || !dsts || !spot_sigmas || !outline_sigmas) return fail(CLE_E_VALUE, "null batch argument");
+    cle_plan* P = new cle_plan;
+    int rc = CLE_OK;
+    hipEvent_t ready = nullptr;
+    do {
+        if (hipGetDevice(&P->dev) != hipSuccess || hipStreamCreateWithFlags(&P->ps, hipStreamNonBlocking) != hipSuccess ||
+            hipHostMalloc((void**)&P->word_host, 4 * sizeof(uint32_t)) != hipSuccess) {
+            rc = fail(CLE_E_HIP, "plan: stream / pinned allocation failed");
+            break;
+        }
+        // work already queued on the caller's stream (the sources) comes first
+        if (hipEventCreateWithFlags(&ready, hipEventDisableTiming) != hipSuccess ||
+            hipEventRecord(ready, (hipStream_t)stream) != hipSuccess || hipStreamWaitEvent(P->ps, ready, 0) != hipSuccess) {
+            rc = fail(CLE_E_HIP, "plan: stream ordering failed");
+            break;
+        }
+        // two eager runs size the private arena (the second merges its chunks), so the
+        // capture allocates nothing from the driver; their results are the batch's
+        for (int k = 0; k < 2 && rc == CLE_OK; ++k)
+            rc = voronoi_otsu_batch(srcs, dsts, spot_sigmas, outline_sigmas, n_images, nullptr, P->ps);
+        if (rc != CLE_OK) break;
+        if ((rc = arena_set_owned(P->ps)) != CLE_OK) break;
+        P->cap.graph_pairs = graph_pairs;
+        P->cap.word_host = P->word_host;
+        if (hipStreamBeginCapture(P->ps, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            rc = fail(CLE_E_HIP, "plan: stream capture could not start");
+            break;
+        }
+        rc = voronoi_otsu_batch(srcs, dsts, spot_sigmas, outline_sigmas, n_images, nullptr, P->ps, &P->cap);
+        const hipError_t e = hipStreamEndCapture(P->ps, &P->graph);
+        if (rc != CLE_OK) break;
+        if (e != hipSuccess || !P->graph) {
+            rc = fail(CLE_E_HIP, "plan: stream capture failed (%s)", hipGetErrorString(e));
+            break;
+        }
+        if (hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0) != hipSuccess) {
+            rc = fail(CLE_E_HIP, "plan: graph instantiation failed");
+            break;
+        }
+    } while (false);
+    if (ready) (void)hipEventDestroy(ready);
+    if (rc != CLE_OK) {
+        (void)hipGetLastError();
+        plan_free(P);
+        return rc;
+    }
+    *out_plan = P;
+    return CLE_OK;
+}
+
+int cle_plan_run(cle_plan* P, int* n_steps, void* stream) {
+    if (!P || !P->exec) return fail(CLE_E_VALUE, "invalid plan");
+    int dev = 0;
+    CLE_HIP(hipGetDevice(&dev));
+    if (dev != P->dev) return fail(CLE_E_VALUE, "plan was built on device %d, current device is %d", P->dev, dev);
+    hipStream_t s = (hipStream_t)stream;
+    CLE_HIP(hipGraphLaunch(P->exec, s));
+    CLE_HIP(hipStreamSynchronize(s));
+    int steps = (int)P->word_host[1];
+    const BatchCapture& c = P->cap;
+    if (c.ntiles > 0 && P->word_host[0]) {
+        // still growing after the recorded pairs: finish eagerly, then mask again
+        CLE_TRY(grow_pairs(c.dimgs, c.n, c.Ap, c.Fp, c.ntiles, c.L, c.S, c.n == 1, c.graph_pairs, &steps, s));
+        k_mask_out_batch<<<(unsigned)c.vblocks, 256, 0, s>>>(c.dimgs, c.dvs, c.n, c.Ap, c.Fp, c.S.iters, c.Gp);
+        CLE_LAUNCHED();
+        CLE_HIP(hipStreamSynchronize(s));
+    }
+    if (n_steps) *n_steps = steps;
+    return CLE_OK;
+}
+
+int cle_plan_destroy(cle_plan* P) {
+    plan_free(P);
+    return CLE_OK;
+}
 
 int cle_masked_voronoi_labeling(const cle_buf* binary, const cle_buf* mask, const cle_buf* dst, int* n_iterations,
                                 void* stream) {

@@ -62,3 +62,44 @@ def test_batch_large_member_uses_row_histogram(cle):
     for k, (a, b) in enumerate([(2.0, 1.5), (1.5, 1.0)]):
         want = cle.voronoi_otsu_labeling(imgs[k], None, a, b)
         assert torch.equal(out[k].tensor, want.tensor), k
+
+
+def _refill(cle, imgs, shift):
+    """New contents in the same device buffers (rolled images)."""
+    import torch
+    for k, im in enumerate(imgs):
+        a = np.roll(cle.pull(im), shift + k, axis=-1)
+        t = im.tensor
+        if a.dtype == np.uint16:
+            t.view(torch.int16).copy_(torch.from_numpy(np.ascontiguousarray(a).view(np.int16)).to(t.device))
+        else:
+            t.copy_(torch.from_numpy(np.ascontiguousarray(a)).to(t.device))
+
+
+@pytest.mark.parametrize("graph_pairs,single", [(32, False), (1, False), (2, True)])
+def test_batch_plan_replays_equal_eager(cle, graph_pairs, single):
+    # the captured graph replayed three times on new source contents, with eager work
+    # and synchronisations between the replays; every replay equals the eager batch
+    # (graph_pairs = 1 forces the eager tail of the growth after the replay)
+    import torch
+    rng = np.random.default_rng(51)
+    imgs, ss, so = _mixed_batch(rng, cle)
+    if single:
+        imgs, ss, so = imgs[3:4], ss[3:4], so[3:4]
+    plan = cle.voronoi_otsu_labeling_batch_plan(imgs, ss, so, graph_pairs=graph_pairs)
+    try:
+        want = [cle.pull(o) for o in cle.voronoi_otsu_labeling_batch(imgs, ss, so)]
+        for d, w in zip(plan.destinations, want):  # building ran the batch
+            assert np.array_equal(cle.pull(d), w)
+        for rep in range(3):
+            _refill(cle, imgs, rep + 1)
+            torch.cuda.synchronize()
+            want = [cle.pull(o) for o in cle.voronoi_otsu_labeling_batch(imgs, ss, so)]
+            plan.run()
+            assert plan.steps > 0
+            for k, d in enumerate(plan.destinations):
+                assert np.array_equal(cle.pull(d), want[k]), (rep, k, imgs[k].shape)
+    finally:
+        plan.close()
+    with pytest.raises(ValueError):
+        plan.run()
