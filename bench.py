@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB float32 cle_copy first (known bytes, to scale PMC byte counters)")
     ap.add_argument("--batch-images", type=int, default=2000, help="mixed-size batch (0 = skip)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batched calls only (no per-image loop, no graph): for kernel traces of the batched path")
     ap.add_argument("--serve-requests", type=int, default=400, help="serving trace length (0 = skip)")
     ap.add_argument("--serve-qps", type=float, default=300.0)
     ap.add_argument("--ingest-pages", type=int, default=128,
@@ -102,9 +104,8 @@ def main():
         return stats
 
     for _ in range(args.warmup):
-        stats = step()
+        step()
     torch.cuda.synchronize()
-    n_labels = len(stats["label"])
 
     events = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
     torch.cuda.synchronize()
@@ -113,6 +114,7 @@ def main():
         stats = step(events[k])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    n_labels = len(stats["label"])
     ms = elapsed * 1e3 / args.steps
     stage_ms = {s: float(np.mean([events[k][i].elapsed_time(events[k][i + 1]) for k in range(args.steps)]))
                 for i, s in enumerate(stages)}
@@ -142,7 +144,7 @@ def main():
                    "statistics_of_labelled_pixels": nvox * (4 + 2)}
     stage_roof = {k: round(b / (stage_ms[k] / 1e3) / 1e9 / HBM_PEAK_GBS, 3) for k, b in stage_bytes.items()}
 
-    batched = batch_section(cle, torch, args.batch_images) if args.batch_images > 0 else None
+    batched = batch_section(cle, torch, args.batch_images, args.batch_only) if args.batch_images > 0 else None
     serving = serve_section(cle, torch, vol, args.serve_requests, args.serve_qps) if args.serve_requests > 0 else None
 
     ingest = ingest_section(cle, torch, vol_t, args.ingest_pages) if args.ingest_pages > 0 else None
@@ -191,7 +193,7 @@ def measured_traffic():
     return None
 
 
-def batch_section(cle, torch, n_images):
+def batch_section(cle, torch, n_images, batch_only=False):
     """Mixed-size batch (70 % 2D 64^2..1024^2, 30 % 3D 32x128x128..96x512x512, per-image
     sigmas): the plain per-image loop of voronoi_otsu_labeling against one batched call."""
     import numpy as np
@@ -204,14 +206,16 @@ def batch_section(cle, torch, n_images):
     imgs = [cle.asarray(t) for t in imgs_t]
     plain_out = [cle.create(t.shape, np.uint32) for t in imgs_t]
     batch_out = [cle.create(t.shape, np.uint32) for t in imgs_t]
-    for k in range(min(20, n_images)):  # warm-up of the per-image path
-        cle.voronoi_otsu_labeling(imgs[k], plain_out[k], float(ss[k]), float(so[k]))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(n_images):
-        cle.voronoi_otsu_labeling(imgs[k], plain_out[k], float(ss[k]), float(so[k]))
-    torch.cuda.synchronize()
-    plain = time.perf_counter() - t0
+    plain = float("nan")
+    if not batch_only:
+        for k in range(min(20, n_images)):  # warm-up of the per-image path
+            cle.voronoi_otsu_labeling(imgs[k], plain_out[k], float(ss[k]), float(so[k]))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(n_images):
+            cle.voronoi_otsu_labeling(imgs[k], plain_out[k], float(ss[k]), float(so[k]))
+        torch.cuda.synchronize()
+        plain = time.perf_counter() - t0
     cle.voronoi_otsu_labeling_batch(imgs, ss, so, destinations=batch_out)  # warm-up (scratch growth)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -220,7 +224,11 @@ def batch_section(cle, torch, n_images):
     batched = time.perf_counter() - t0
     same = sum(int(torch.equal(a.tensor, b.tensor)) for a, b in zip(plain_out, batch_out))
     nv = info["voxels"]
-    log(f"batched: plain {plain * 1e3:.1f} ms, batched {batched * 1e3:.1f} ms, identical {same}/{n_images}")
+    if not batch_only:
+        log(f"batched: plain {plain * 1e3:.1f} ms, batched {batched * 1e3:.1f} ms, identical {same}/{n_images}")
+    if batch_only:
+        return {"workload": "batched calls only (kernel trace)", "images": n_images, "voxels": nv,
+                "batched_ms": round(batched * 1e3, 1), "growth_steps": steps}
     # the same batch captured once as a hipGraph and replayed
     graph_out = [cle.create(t.shape, np.uint32) for t in imgs_t]
     t0 = time.perf_counter()
