@@ -52,6 +52,8 @@ def parse():
                     help="batched calls only (no per-image loop, no graph): for kernel traces of the batched path")
     ap.add_argument("--serve-requests", type=int, default=400, help="serving trace length (0 = skip)")
     ap.add_argument("--serve-qps", type=float, default=300.0)
+    ap.add_argument("--large", type=int, default=1,
+                    help="also time voronoi_otsu_labeling + statistics of the 512x2048x2048 stack (0 = skip)")
     ap.add_argument("--ingest-pages", type=int, default=128,
                     help="TIFF ingest: pages of the 1024 x 1024 uint16 stack written and read back (0 = skip)")
     return ap.parse_args()
@@ -148,6 +150,7 @@ def main():
     serving = serve_section(cle, torch, vol, args.serve_requests, args.serve_qps) if args.serve_requests > 0 else None
 
     ingest = ingest_section(cle, torch, vol_t, args.ingest_pages) if args.ingest_pages > 0 else None
+    large = large_section(cle, torch) if args.large else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -177,6 +180,7 @@ def main():
         "batched": batched,
         "serving": serving,
         "ingest": ingest,
+        "large": large,
     }
     print(json.dumps(line), flush=True)
 
@@ -315,6 +319,56 @@ def serve_section(cle, torch, vol_c2, n_requests, qps):
                       "p99_ms": round(float(np.percentile(pl, 99)), 2),
                       "mvox_s": round(nv / plain_span / 1e6, 1)},
             "identical_to_plain": f"{same}/{n_requests}"}
+
+
+def large_section(cle, torch, reps=3):
+    """512 x 2048 x 2048 uint16 stack (about 100 k nuclei, semi-axes 3-8, 15 % touching
+    pairs, seed 3) resident in HBM: voronoi_otsu_labeling(spot 2.0, outline 1.5) and
+    statistics_of_labelled_pixels, timed over `reps` runs after one warm-up."""
+    import numpy as np
+    from pyclesperanto_prototype_amd import synthetic
+
+    shape = (512, 2048, 2048)
+    t0 = time.time()
+    vol_t, info = synthetic.nuclei_3d(shape, 100_000, semi=(3.0, 8.0), seed=3, touching_fraction=0.15)
+    torch.cuda.synchronize()
+    log(f"large: {info['nuclei']} nuclei generated in {time.time() - t0:.1f}s")
+    vol = cle.asarray(vol_t)
+    labels = cle.create(shape, np.uint32)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+
+    def run(timed):
+        if timed: ev[0].record()
+        cle.voronoi_otsu_labeling(vol, labels, 2.0, 1.5)
+        if timed: ev[1].record()
+        st = cle.statistics_of_labelled_pixels(vol, labels)
+        if timed: ev[2].record()
+        return st
+
+    run(False)
+    torch.cuda.synchronize()
+    vor, sta, tot = [], [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        st = run(True)
+        torch.cuda.synchronize()
+        tot.append(time.perf_counter() - t0)
+        vor.append(ev[0].elapsed_time(ev[1]))
+        sta.append(ev[1].elapsed_time(ev[2]))
+    nv = int(np.prod(shape))
+    ms = float(np.mean(tot)) * 1e3
+    log(f"large: {ms:.1f} ms per run (voronoi_otsu {np.mean(vor):.1f} ms, statistics {np.mean(sta):.1f} ms), "
+        f"{len(st['label'])} labels")
+    out = {"workload": "voronoi_otsu_labeling(2.0, 1.5) + statistics_of_labelled_pixels of a 512x2048x2048 uint16 "
+                       f"nuclei stack ({info['nuclei']} nuclei, 15% touching pairs, seed 3)",
+           "voxels": nv, "ms": round(ms, 2), "mvox_s": round(nv / (ms / 1e3) / 1e6, 1),
+           "voronoi_otsu_ms": round(float(np.mean(vor)), 2), "statistics_ms": round(float(np.mean(sta)), 2),
+           "labels": len(st["label"])}
+    del vol, labels, vol_t
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    cle.release_pool()
+    return out
 
 
 def ingest_section(cle, torch, vol_t, pages):
