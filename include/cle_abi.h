@@ -65,11 +65,14 @@ int cle_copy(const cle_buf* src, const cle_buf* dst, void* stream);
 
 /* ---- filters ------------------------------------------------------------ */
 /* Autotuner of the isotropic blur's z chunking (blocks per launch); no counterpart in
- * the reference.  On by default (CLE_AUTOTUNE=0 in the environment turns it off):
- * the first blur of at least 2^22 voxels per (half-width, dimensionality, source
- * type, size class) times the candidate chunk counts on the caller's stream,
- * synchronising it once, and keeps the fastest.  Results do not depend on it.
- * on = 0 turns it off and forgets the choices made. */
+ * the reference.  cle_autotune_gaussian_blur runs the blur of these arguments once per
+ * candidate chunking (timed with events on `stream`, which it synchronises) and keeps
+ * the fastest for its (half-width, dimensionality, source type, size class) for the
+ * process; later cle_gaussian_blur calls of that class use it and never time anything.
+ * dst holds the blur afterwards.  Results do not depend on the choice.
+ * cle_set_autotune(0) (or CLE_AUTOTUNE=0) ignores and forgets the choices. */
+int cle_autotune_gaussian_blur(const cle_buf* src, const cle_buf* dst,
+                               float sigma_x, float sigma_y, float sigma_z, void* stream);
 int cle_set_autotune(int on);
 
 /* _tier1/_gaussian_blur.py:6-58 + _tier1/_execute_separable_kernel.py:7-75:

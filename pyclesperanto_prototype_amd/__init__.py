@@ -11,7 +11,7 @@ reached through a C ABI (include/cle_abi.h).
 from ._array import (Array, asarray, create, create_binary_like, create_labels_like, create_like, create_none,
                      create_zyx, pull, pull_zyx, push, push_zyx)
 from ._plugin import Image, is_image, plugin_function
-from ._ops import (STATISTICS_KEYS, add_image_and_scalar, add_images, add_images_weighted, binary_and,
+from ._ops import (STATISTICS_KEYS, add_image_and_scalar, autotune_gaussian_blur, add_images, add_images_weighted, binary_and,
                    binary_intersection, close_index_gaps_in_label_map, connected_components_labeling_box,
                    extend_labeling_via_voronoi,
                    connected_components_labeling_diamond, copy, detect_maxima_box, execute, gauss_otsu_labeling,

@@ -479,8 +479,21 @@ binary_intersection = binary_and
 
 
 def set_autotune(on: bool = True):
-    """Switch the blur's z-chunk autotuner (on by default; results never depend on it)."""
+    """Use (on, the default) or ignore and forget the blur's tuned z chunkings; results
+    never depend on them."""
     _lib.call("cle_set_autotune", 1 if on else 0)
+
+
+def autotune_gaussian_blur(source, destination=None, sigma_x: float = 0, sigma_y: float = 0, sigma_z: float = 0):
+    """Tune the isotropic blur's z chunking for this configuration ahead of time (the
+    call synchronises); later gaussian_blur calls of the same class use the choice.
+    Returns the blurred destination."""
+    source = source if isinstance(source, Array) else push(source)
+    if destination is None:
+        destination = create_like(source)
+    _lib.call("cle_autotune_gaussian_blur", source.buf(), destination.buf(), float(sigma_x), float(sigma_y),
+              float(sigma_z), _S())
+    return destination
 
 
 def release_pool():

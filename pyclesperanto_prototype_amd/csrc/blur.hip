@@ -588,11 +588,12 @@ static int try_fused(const cle_buf* src, const cle_buf* dst, const Taps* taps[3]
 // ---------------------------------------------------------------- z-chunk autotuner
 // The isotropic blur splits every column tile's z range into nzc chunks: more chunks
 // fill the chip with more blocks, each chunk re-reads 2C halo slices.  The output is
-// the same for every nzc (each voxel's taps are summed in the same order).  For images
-// of at least 2^22 voxels the first launch of a (half-width, dimensionality, source
-// type, size class) times the candidate chunk counts on the caller's stream and keeps
-// the fastest for the process; smaller images and captured streams use the default
-// (about ISO_BLOCKS blocks).  CLE_AUTOTUNE=0 or cle_set_autotune(0) turns it off.
+// the same for every nzc (each voxel's taps are summed in the same order).
+// cle_autotune_gaussian_blur times the candidate chunk counts for one configuration
+// (half-width, dimensionality, source type, size class) ahead of time and keeps the
+// fastest for the process; blurs look the choice up and otherwise use the default
+// (about ISO_BLOCKS blocks), so no blur call ever pays for tuning.  CLE_AUTOTUNE=0 or
+// cle_set_autotune(0) ignores (and forgets) the choices.
 static std::atomic<int> g_autotune{-1};  // -1: read CLE_AUTOTUNE on first use
 
 static bool autotune_on() {
@@ -633,6 +634,8 @@ template <> constexpr int ld_code<LdF32>() { return 0; }
 template <> constexpr int ld_code<LdU16>() { return 1; }
 template <> constexpr int ld_code<LdU8>() { return 2; }
 
+static thread_local bool t_tuning = false;  // set by cle_autotune_gaussian_blur around its blur
+
 template <int C, bool Z3, typename LD>
 static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hipStream_t s) {
     const int ntx = (int)((g.W + ISO_TX - 1) / ISO_TX), nty = (int)((g.H + ISO_TY - 1) / ISO_TY);
@@ -647,8 +650,7 @@ static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hip
     for (int v = 0; v <= C; ++v) w.w[v] = wts[v];
     int64_t nzc = nzc_for(ISO_BLOCKS);
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    if (g.n() >= ((int64_t)1 << 22) && max_nzc > 1 && autotune_on() &&
-        hipStreamIsCapturing(s, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone) {
+    if (max_nzc > 1 && autotune_on()) {
         int dev = 0;
         (void)hipGetDevice(&dev);
         const TuneKey key{C, Z3 ? 1 : 0, ld_code<LD>(), dev, ilog2(nxy), ilog2(g.D)};
@@ -658,7 +660,7 @@ static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hip
             auto it = g_tune.find(key);
             if (it != g_tune.end()) { nzc = it->second < max_nzc ? it->second : max_nzc; known = true; }
         }
-        if (!known) {
+        if (!known && t_tuning && hipStreamIsCapturing(s, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone) {
             std::vector<int64_t> cand;
             for (int64_t b : {256, 512, 1024, 2048, 4096}) {
                 const int64_t v = nzc_for(b);
@@ -984,6 +986,16 @@ int cle_set_autotune(int on) {
         g_tune.clear();
     }
     return CLE_OK;
+}
+
+int cle_autotune_gaussian_blur(const cle_buf* src, const cle_buf* dst, float sigma_x, float sigma_y, float sigma_z,
+                               void* stream) {
+    if (!autotune_on()) return CLE_OK;
+    const float sig[3] = {sigma_x, sigma_y, sigma_z};
+    t_tuning = true;
+    const int rc = blur_impl(src, dst, sig, (hipStream_t)stream);
+    t_tuning = false;
+    return rc;
 }
 
 int cle_gaussian_blur(const cle_buf* src, const cle_buf* dst, float sigma_x, float sigma_y, float sigma_z,

@@ -42,6 +42,7 @@
 // L2 in every block.)
 #include "common.hpp"
 
+#include <cstdlib>
 #include <mutex>
 
 namespace cle {
@@ -564,17 +565,20 @@ __global__ __launch_bounds__(VB, 4) void k_vor_pair(const BImg* __restrict__ img
     }
 }
 
-// Pair b in raster mode (one image, one block per tile in raster order, so that the
-// halos of neighbouring tiles are read while still in L2): tile t runs iff
+// Pair b in raster mode (one image, blocks over runs of tpb consecutive tiles in raster
+// order, so that the halos of neighbouring tiles are read while still in L2): tile t runs iff
 // mark[b & 1][t] == b; a tile that changes marks its neighbourhood with b + 1 in the
 // other array (plain stores of one value: no atomics, no list).  Block 0 first closes
 // pair b - 1 as in the list mode.
-__global__ __launch_bounds__(VB, 4) void k_vor_pair_grid(const BImg* __restrict__ imgs,
-                                                         const float* __restrict__ src, float* __restrict__ dst,
-                                                         int b, VorLists L, VorState S) {
+// MULTI = false: one tile per block (tpb = 1; 4 waves per SIMD); true: runs of tpb
+// tiles per block (the loop costs registers: 3 waves per SIMD)
+template <bool MULTI>
+__device__ __forceinline__ void vor_pair_grid(const BImg* __restrict__ imgs, const float* __restrict__ src,
+                                              float* __restrict__ dst, int b, const VorLists& L, const VorState& S,
+                                              int tpb) {
     __shared__ float lds[PairGeo<true>::NR > PairGeo<false>::NR ? PairGeo<true>::NR : PairGeo<false>::NR];
     __shared__ float lds2[PairGeo<true>::NR > PairGeo<false>::NR ? PairGeo<true>::NR : PairGeo<false>::NR];
-    __shared__ int sbits;
+    __shared__ int sbits[2];
     const uint32_t* prev = S.bits + (b + 2) % 3;  // pair b - 1 (one image)
     uint32_t* cur = S.bits + b % 3;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -585,41 +589,70 @@ __global__ __launch_bounds__(VB, 4) void k_vor_pair_grid(const BImg* __restrict_
         }
         S.bits[(b + 1) % 3] = 0;
     }
-    const int64_t t = blockIdx.x;
-    if (L.mark[(int64_t)(b & 1) * L.cap + t] != b) return;
     if (S.iters[0] != 0 || (b > 0 && prev[0] != 3u)) return;  // converged in an earlier pair
-    if (L.settled[t]) return;
     const Geom g = imgs[0].g;
     const int nx = imgs[0].vnx, ny = imgs[0].vny, nz = imgs[0].vnz;
-    const int txy = (int)(t / nx), tx = (int)(t - (int64_t)txy * nx), ty = txy % ny, tz = txy / ny;
-    if (threadIdx.x == 0) sbits = 0;
-    float v[VOR_KV];
-    int bits;
-    if (vor_3d(g)) {
-        const PairGeo<true> R(g, tx, ty, tz);
-        vor_stage_load<true>(src, R, v);
-        vor_stage_store<true>(v, lds);
-        __syncthreads();
-        bits = vor_pair_compute<true>(dst, R, lds, lds2, &sbits);
-    } else {
-        const PairGeo<false> R(g, tx, ty, tz);
-        vor_stage_load<false>(src, R, v);
-        vor_stage_store<false>(v, lds);
-        __syncthreads();
-        bits = vor_pair_compute<false>(dst, R, lds, lds2, &sbits);
-    }
-    if (bits & 3) {
-        if (threadIdx.x == 0) atomicOr(cur, (uint32_t)(bits & 3));
-        const int k = threadIdx.x;
-        if (k < 27) {
-            const int qx = tx + k % 3 - 1, qy = ty + (k / 3) % 3 - 1, qz = tz + k / 9 - 1;
-            if (qx >= 0 && qx < nx && qy >= 0 && qy < ny && qz >= 0 && qz < nz)
-                L.mark[(int64_t)((b + 1) & 1) * L.cap + ((int64_t)qz * ny + qy) * nx + qx] = b + 1;
+    const int64_t ntiles = (int64_t)nx * ny * nz;
+    const int32_t* mark = L.mark + (int64_t)(b & 1) * L.cap;
+    int32_t* mark_next = L.mark + (int64_t)((b + 1) & 1) * L.cap;
+    int par = 0;
+    // tiles [blockIdx.x * tpb, + tpb): consecutive in raster order
+    const int64_t t0 = (int64_t)blockIdx.x * (MULTI ? tpb : 1);
+    const int64_t t1 = MULTI ? (t0 + tpb < ntiles ? t0 + tpb : ntiles) : (t0 + 1 < ntiles ? t0 + 1 : ntiles);
+#pragma unroll 1
+    for (int64_t t = t0; t < t1; ++t) {
+        if (mark[t] != b || L.settled[t]) continue;
+        const int txy = (int)(t / nx), tx = (int)(t - (int64_t)txy * nx), ty = txy % ny, tz = txy / ny;
+        // sbits[par] was last read two tiles ago, before a barrier of the previous tile
+        if (threadIdx.x == 0) sbits[par] = 0;
+        float v[VOR_KV];
+        int bits;
+        if (vor_3d(g)) {
+            const PairGeo<true> R(g, tx, ty, tz);
+            vor_stage_load<true>(src, R, v);
+            vor_stage_store<true>(v, lds);
+            __syncthreads();
+            bits = vor_pair_compute<true>(dst, R, lds, lds2, &sbits[par]);
+        } else {
+            const PairGeo<false> R(g, tx, ty, tz);
+            vor_stage_load<false>(src, R, v);
+            vor_stage_store<false>(v, lds);
+            __syncthreads();
+            bits = vor_pair_compute<false>(dst, R, lds, lds2, &sbits[par]);
         }
-    } else if (!(bits & 4) && threadIdx.x == 0) {
-        L.settled[t] = 1;
+        par ^= 1;
+        if (bits & 3) {
+            if (threadIdx.x == 0) atomicOr(cur, (uint32_t)(bits & 3));
+            const int k = threadIdx.x;
+            if (k < 27) {
+                const int qx = tx + k % 3 - 1, qy = ty + (k / 3) % 3 - 1, qz = tz + k / 9 - 1;
+                if (qx >= 0 && qx < nx && qy >= 0 && qy < ny && qz >= 0 && qz < nz)
+                    mark_next[((int64_t)qz * ny + qy) * nx + qx] = b + 1;
+            }
+        } else if (!(bits & 4) && threadIdx.x == 0) {
+            L.settled[t] = 1;
+        }
+        // lds is rewritten by the next tile only after every thread passed the last
+        // barrier of vor_pair_compute (its lds reads come before it)
     }
 }
+
+__global__ __launch_bounds__(VB, 4) void k_vor_pair_grid(const BImg* __restrict__ imgs,
+                                                         const float* __restrict__ src, float* __restrict__ dst,
+                                                         int b, VorLists L, VorState S) {
+    vor_pair_grid<false>(imgs, src, dst, b, L, S, 1);
+}
+__global__ __launch_bounds__(VB, 3) void k_vor_pair_grid_multi(const BImg* __restrict__ imgs,
+                                                               const float* __restrict__ src,
+                                                               float* __restrict__ dst, int b, VorLists L,
+                                                               VorState S, int tpb) {
+    vor_pair_grid<true>(imgs, src, dst, b, L, S, tpb);
+}
+
+// tiles per block of the raster mode: one for images below 65536 tiles, 16 above (fewer,
+// longer-lived blocks: measured 15.8 -> 12.7 ms for voronoi_otsu_labeling of the
+// 256x1024x1024 stack and 162 -> 112 ms for the 512x2048x2048 one)
+static int raster_tpb(int64_t ntiles) { return ntiles < 65536 ? 1 : 16; }
 
 // Closes the last pair b of a chunk (one block): iteration counts of the images that
 // converged in it, and the "an image still grows" word the host reads.
@@ -677,9 +710,14 @@ struct VorScratch {
 // Launches pairs [b, b + count) and the close of the last one (no host synchronisation).
 static void launch_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t ntiles, const VorLists& L,
                          const VorState& S, bool raster, int b, int count, hipStream_t s) {
-    const unsigned grid = raster ? (unsigned)ntiles : (unsigned)(ntiles < 4096 ? (ntiles > 0 ? ntiles : 1) : 4096);
+    const int tpb = raster_tpb(ntiles);
+    const unsigned grid =
+        raster ? (unsigned)((ntiles + tpb - 1) / tpb) : (unsigned)(ntiles < 4096 ? (ntiles > 0 ? ntiles : 1) : 4096);
     for (int k = 0; k < count; ++k, ++b) {
-        if (raster) k_vor_pair_grid<<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S);
+        if (raster && tpb > 1)
+            k_vor_pair_grid_multi<<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S, tpb);
+        else if (raster)
+            k_vor_pair_grid<<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S);
         else k_vor_pair<<<grid, VB, 0, s>>>(dimgs, nimg, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S);
     }
     k_vor_close<<<1, VB, 0, s>>>(nimg, b - 1, S);

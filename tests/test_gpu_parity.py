@@ -600,10 +600,10 @@ def test_blur_autotuned_chunking_identical(cle):
     rng = np.random.default_rng(21)
     vol = rng.integers(0, 4000, (64, 256, 256)).astype(np.uint16)
     src = cle.asarray(torch.from_numpy(vol.view(np.int16)).cuda().view(torch.uint16))
-    outs = []
-    for on in (False, True, True):  # default, tuning run, cached choice
-        cle.set_autotune(on)
-        outs.append(cle.pull(cle.gaussian_blur(src, sigma_x=2, sigma_y=2, sigma_z=2)))
+    cle.set_autotune(False)
+    outs = [cle.pull(cle.gaussian_blur(src, sigma_x=2, sigma_y=2, sigma_z=2))]  # default chunking
     cle.set_autotune(True)
+    outs.append(cle.pull(cle.autotune_gaussian_blur(src, None, 2, 2, 2)))      # tuning run
+    outs.append(cle.pull(cle.gaussian_blur(src, sigma_x=2, sigma_y=2, sigma_z=2)))  # tuned choice
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
     assert_blur_close(outs[0], O.gaussian_blur(vol.astype(np.float32), 2, 2, 2))
