@@ -222,8 +222,15 @@ __device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t
                                y0 + iy + 1 < g.H ? (ly + 1) * MX_RX : ly * MX_RX};
             const int zo[3] = {d3 ? (sm >= 0 ? sm * MX_RP : -1) : -1, s0 * MX_RP, d3 ? sp * MX_RP : -1};
             const float v = ring[s0 * MX_RP + ly * MX_RX + lx];
-            float lm = v - 1.f;
-            bool self = true;
+            // The visit in order (start lm = v - 1, a strictly larger value takes over,
+            // the flag is whether the last take-over was the centre) in closed form: B /
+            // A = the largest value visited before / after the centre (fmaxf ignores
+            // NaN, which never takes over; NaN = nothing visited).  A before-value takes
+            // over iff B > v - 1, leaving lm1 = max(v - 1, B); the centre takes over iff
+            // v > lm1 (then lm2 = v); with no take-over before it, the flag stays set;
+            // any after-value above lm2 clears it.  Exact for ties, infinities, NaN and
+            // values where v - 1 == v.
+            float bmax = __builtin_nanf(""), amax = __builtin_nanf("");
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) {
                 if (xo[dx] < 0) continue;
@@ -233,14 +240,20 @@ __device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t
 #pragma unroll
                     for (int dz = 0; dz < 3; ++dz) {
                         if (zo[dz] < 0) continue;
+                        const int order = (dx * 3 + dy) * 3 + dz;  // 13 = the centre
+                        if (order == 13) continue;
                         const float w = ring[zo[dz] + yo[dy] + xo[dx]];
-                        if (w > lm) {
-                            lm = w;
-                            self = (dx == 1 && dy == 1 && dz == 1);
-                        }
+                        if (order < 13) bmax = fmaxf(bmax, w);
+                        else amax = fmaxf(amax, w);
                     }
                 }
             }
+            const float lm0 = v - 1.f;
+            const bool took = bmax > lm0;
+            const float lm1 = took ? bmax : lm0;
+            const bool centre = v > lm1;
+            const float lm2 = centre ? v : lm1;
+            const bool self = (centre || !took) && !(amax > lm2);
             out((z * g.H + y0 + iy) * g.W + x0 + ix, self);
         }
         __syncthreads();

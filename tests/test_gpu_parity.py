@@ -607,3 +607,17 @@ def test_blur_autotuned_chunking_identical(cle):
     outs.append(cle.pull(cle.gaussian_blur(src, sigma_x=2, sigma_y=2, sigma_z=2)))  # tuned choice
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
     assert_blur_close(outs[0], O.gaussian_blur(vol.astype(np.float32), 2, 2, 2))
+
+
+@pytest.mark.parametrize("shape", [(70, 130), (20, 37, 70), (1, 18, 66), (17, 16, 64)])
+def test_maxima_special_values(cle, shape):
+    # the closed-form visit rule against the oracle's literal visit order: plateaus
+    # (ties), NaN (never takes over; a NaN centre stays a maximum), +-inf, and values
+    # above 2^24 where v - 1 == v
+    rng = np.random.default_rng(31)
+    a = rng.integers(0, 4, shape).astype(np.float32)
+    special = np.array([np.nan, np.inf, -np.inf, 3e8, 3e8 + 32, -3e8], np.float32)
+    m = rng.random(shape) < 0.08
+    a[m] = rng.choice(special, int(m.sum()))
+    got = cle.pull(cle.detect_maxima_box(a))
+    assert np.array_equal(got, O.detect_maxima_box(a))
