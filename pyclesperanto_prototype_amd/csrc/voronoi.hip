@@ -108,10 +108,10 @@ __device__ __forceinline__ uint64_t lanes_below_v() {
 }
 
 // wavefront 0 of the block queues the tile neighbourhood of `tile` (of image j, whose
-// tiles start at tbase in the per-tile arrays) for pair `blk`.  `skip_settled`: leave
-// out settled tiles (not during init, where the settled flags are still being written)
+// tiles start at tbase in the per-tile arrays) for pair `blk` (the batch init; settled
+// tiles are not left out here: their flags are still being written)
 __device__ __forceinline__ void vor_queue(const Tiles& T, int64_t tile, int j, int64_t tbase, int blk,
-                                          const VorLists& L, bool skip_settled) {
+                                          const VorLists& L) {
     const int lane = threadIdx.x;
     int64_t q = -1;
     if (lane < 27) {
@@ -120,9 +120,8 @@ __device__ __forceinline__ void vor_queue(const Tiles& T, int64_t tile, int j, i
         if (qx >= 0 && qx < T.nx && qy >= 0 && qy < T.ny && qz >= 0 && qz < T.nz)
             q = (qz * T.ny + qy) * T.nx + qx;
     }
-    // a settled tile never changes again: not queued
-    const bool need =
-        q >= 0 && !(skip_settled && L.settled[tbase + q]) && atomicMax(&L.stamp[tbase + q], blk) < blk;
+    // queued once per pair: the stamp records the last pair a tile was queued for
+    const bool need = q >= 0 && atomicMax(&L.stamp[tbase + q], blk) < blk;
     const uint64_t m = __ballot(need);
     uint32_t base = 0;
     if (lane == 0 && m) base = atomicAdd(&L.count[blk % 3], (uint32_t)__popcll(m));
@@ -879,7 +878,7 @@ __global__ __launch_bounds__(VB) void k_vor_init_batch(const BImg* __restrict__ 
     const Tiles T = make_tiles(im.g);
     if (vor_init_tile(MaskedU8{G + im.off, B + im.off}, A + im.off, B + im.off, im.g, T, b - ts[j], L.settled + b) &&
         threadIdx.x < 64) {
-        vor_queue(T, b - ts[j], j, ts[j], 0, L, false);
+        vor_queue(T, b - ts[j], j, ts[j], 0, L);
         // raster-mode marks as well (a one-image batch grows in raster order)
         if (threadIdx.x < 27) {
             const int64_t t = b - ts[j], k = threadIdx.x;
