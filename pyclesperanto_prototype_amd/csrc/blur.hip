@@ -301,7 +301,7 @@ constexpr int ISO_BLOCKS = 512;
 template <int C>
 struct IsoLds {
     static constexpr int AW = ISO_TX + 2 * C, AH = ISO_TY + 2 * C;
-    static constexpr int AWP = (AW + 3) / 4 * 4;  // row pitch of both buffers (even: float2 reads)
+    static constexpr int AWP = AW;                // row pitch of both buffers (even: float2 reads)
     static constexpr int ASZ = AH * AWP;          // one staged slice (with x and y halo)
     static constexpr int BSZ = ISO_TY * AWP;      // one y-blurred slice (with x halo)
 };
@@ -312,7 +312,7 @@ struct IsoLds {
 // too), then along x into registers; the z pass runs over a register ring of the
 // last 2 * CZ + U xy-blurred values of the thread's two columns.  Doing y before x
 // blurs 1 + 2C / ISO_TX of the tile in the first pass instead of 1 + 2C / ISO_TY.
-template <int C, bool Z3, typename LD>
+template <int C, bool Z3, typename LD, int U, int PB>
 __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, const Geom& g, int zc, int ntx, int nty,
                                               const IsoW<C>& w, int64_t bid, float* __restrict__ A,
                                               float* __restrict__ B) {
@@ -320,7 +320,7 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
     constexpr int AWP = IsoLds<C>::AWP, ASZ = IsoLds<C>::ASZ, BSZ = IsoLds<C>::BSZ;
     constexpr int PF = (NA + ISO_THREADS - 1) / ISO_THREADS;
     constexpr int CZ = Z3 ? C : 0;
-    constexpr int U = C <= 10 ? 4 : 2;   // slices per ring shift (register budget); even
+    static_assert(U % 2 == 0, "slices per ring shift: even");
     constexpr int RL = 2 * CZ + U;       // ring length
     constexpr int RY = 4;                // y-pass outputs per task (rows of one column)
     constexpr int YT = AW * (ISO_TY / RY);
@@ -333,7 +333,8 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
     const int row = threadIdx.x / (ISO_TX / 2);
     const int col = (threadIdx.x % (ISO_TX / 2)) * 2;
     const int Hm = (int)g.H - 1, Wm = (int)g.W - 1, Wi = (int)g.W;
-    int off[PF], lds[PF];
+    static_assert(AWP == AW, "staged slice stored at its linear index");
+    int off[PF];
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
         const int idx = threadIdx.x + k * ISO_THREADS;
@@ -342,10 +343,9 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
         gy = gy < 0 ? 0 : (gy > Hm ? Hm : gy);
         gx = gx < 0 ? 0 : (gx > Wm ? Wm : gx);
         off[k] = gy * Wi + gx;
-        lds[k] = ay * AWP + ax;
     }
     using R = typename LD::raw_t;
-    R pa[PF], pb[PF];
+    R pa[PF], pb[PB == 2 ? PF : 1];
     float r0[RL], r1[RL];
 #pragma unroll
     for (int j = 0; j < RL; ++j) { r0[j] = 0.f; r1[j] = 0.f; }
@@ -364,7 +364,7 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
     auto store = [&](const R (&pf)[PF], float* a) {
 #pragma unroll
         for (int m = 0; m < PF; ++m)
-            if (threadIdx.x + m * ISO_THREADS < NA) a[lds[m]] = LD::cvt(pf[m]);
+            if (threadIdx.x + m * ISO_THREADS < NA) a[threadIdx.x + m * ISO_THREADS] = LD::cvt(pf[m]);
     };
     // The x and y passes take the taps in order (w[|j - C|] * in[j], j = 0 .. 2C),
     // one fused multiply-add each, as the values stream in from LDS.
@@ -434,17 +434,30 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
     };
 
     // Software pipeline over the input slices, one barrier per phase: phase p stages
-    // slice p (into A[p & 1]) and prefetches slice p + 2, runs the y pass of slice
-    // p - 1 (A[(p-1) & 1] -> B[(p-1) & 1]) and the x pass of slice p - 2 (from
-    // B[p & 1]).  The buffers a phase writes were last read one phase earlier.
-    load(pa, 0);
-    load(pb, 1);
-    store(pa, A);
-    load(pa, 2);
+    // slice p (into A[p & 1]), runs the y pass of slice p - 1 (A[(p-1) & 1] ->
+    // B[(p-1) & 1]) and the x pass of slice p - 2 (from B[p & 1]).  The buffers a phase
+    // writes were last read one phase earlier.  The loads run PB slices ahead of the
+    // staging: a phase prefetches slice p + 2 (two register buffers pa / pb) or p + 1
+    // (one buffer).
+    if constexpr (PB == 2) {
+        load(pa, 0);
+        load(pb, 1);
+        store(pa, A);
+        load(pa, 2);
+    } else {
+        load(pa, 0);
+        store(pa, A);
+        load(pa, 1);
+    }
     __syncthreads();
     if (N > 1) {
-        store(pb, A + ASZ);
-        load(pb, 3);
+        if constexpr (PB == 2) {
+            store(pb, A + ASZ);
+            load(pb, 3);
+        } else {
+            store(pa, A + ASZ);
+            load(pa, 2);
+        }
     }
     ypass(A, B);
     __syncthreads();
@@ -458,8 +471,13 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
                 float* Bq = B + ((u & 1) ? 0 : BSZ);       // B[(p - 1) & 1]
                 const float* Bk = B + ((u & 1) ? BSZ : 0);  // B[k & 1]
                 if (k + 2 < N) {
-                    if (u & 1) { store(pb, Ap); load(pb, k + 4); }
-                    else { store(pa, Ap); load(pa, k + 4); }
+                    if constexpr (PB == 2) {
+                        if (u & 1) { store(pb, Ap); load(pb, k + 4); }
+                        else { store(pa, Ap); load(pa, k + 4); }
+                    } else {
+                        store(pa, Ap);
+                        load(pa, k + 3);
+                    }
                 }
                 if (k + 1 < N) ypass(Aq, Bq);
                 xpass(Bk, k, u);
@@ -471,12 +489,30 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
     }
 }
 
+// Two register budgets of the same tile code.  k_blur_iso leaves the compiler its
+// choice (about 80 VGPRs at C = 8: one 16-wavefront block per CU), shifts the z ring
+// every U = 4 slices (2 above C = 10) and prefetches two slices ahead; k_blur_iso_o8
+// holds it to 64 VGPRs (eight wavefronts per SIMD: two blocks per CU, each running
+// while the other waits at a barrier) with a ring shift every 2 slices and a one-slice
+// prefetch.  The 64-VGPR form exists for C <= 8 (3D) / 12 (2D); beyond that it spills.
+// Both sum every voxel's taps in the same order: the output does not depend on the
+// choice.
+template <int C, bool Z3> constexpr bool iso_o8_ok() { return C <= (Z3 ? 8 : 12); }
+template <int C> constexpr int iso_u() { return C <= 10 ? 4 : 2; }
+
 template <int C, bool Z3, typename LD>
 __global__ __launch_bounds__(ISO_THREADS) void k_blur_iso(LD in, float* __restrict__ out, Geom g, int zc, int ntx,
                                                         int nty, const IsoW<C> w) {
     __shared__ __align__(16) float A[2 * IsoLds<C>::ASZ];
     __shared__ __align__(16) float B[2 * IsoLds<C>::BSZ];
-    blur_iso_tile<C, Z3>(in, out, g, zc, ntx, nty, w, xcd_tile(blockIdx.x, gridDim.x), A, B);
+    blur_iso_tile<C, Z3, LD, iso_u<C>(), 2>(in, out, g, zc, ntx, nty, w, xcd_tile(blockIdx.x, gridDim.x), A, B);
+}
+template <int C, bool Z3, typename LD>
+__global__ __launch_bounds__(ISO_THREADS, 8) void k_blur_iso_o8(LD in, float* __restrict__ out, Geom g, int zc,
+                                                              int ntx, int nty, const IsoW<C> w) {
+    __shared__ __align__(16) float A[2 * IsoLds<C>::ASZ];
+    __shared__ __align__(16) float B[2 * IsoLds<C>::BSZ];
+    blur_iso_tile<C, Z3, LD, 2, 1>(in, out, g, zc, ntx, nty, w, xcd_tile(blockIdx.x, gridDim.x), A, B);
 }
 
 // Batched form: one grid over the column tiles of many images that share the
@@ -496,15 +532,25 @@ template <> __device__ __forceinline__ LdF32 make_ld<LdF32>(const void* p) { ret
 template <> __device__ __forceinline__ LdU16 make_ld<LdU16>(const void* p) { return LdU16{(const uint16_t*)p}; }
 template <> __device__ __forceinline__ LdU8 make_ld<LdU8>(const void* p) { return LdU8{(const uint8_t*)p}; }
 
-template <int C, bool Z3, typename LD>
-__global__ __launch_bounds__(ISO_THREADS) void k_blur_iso_batch(const IsoJob<C>* __restrict__ jobs,
-                                                              const int64_t* __restrict__ bstart, int njobs) {
+template <int C, bool Z3, typename LD, int U, int PB>
+__device__ __forceinline__ void blur_iso_batch_tile(const IsoJob<C>* __restrict__ jobs,
+                                                    const int64_t* __restrict__ bstart, int njobs) {
     __shared__ __align__(16) float A[2 * IsoLds<C>::ASZ];
     __shared__ __align__(16) float B[2 * IsoLds<C>::BSZ];
     const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
     const int j = find_seg(bstart, njobs, b);
     const IsoJob<C>& jb = jobs[j];
-    blur_iso_tile<C, Z3>(make_ld<LD>(jb.in), jb.out, jb.g, jb.zc, jb.ntx, jb.nty, jb.w, b - bstart[j], A, B);
+    blur_iso_tile<C, Z3, LD, U, PB>(make_ld<LD>(jb.in), jb.out, jb.g, jb.zc, jb.ntx, jb.nty, jb.w, b - bstart[j], A, B);
+}
+template <int C, bool Z3, typename LD>
+__global__ __launch_bounds__(ISO_THREADS) void k_blur_iso_batch(const IsoJob<C>* __restrict__ jobs,
+                                                              const int64_t* __restrict__ bstart, int njobs) {
+    blur_iso_batch_tile<C, Z3, LD, iso_u<C>(), 2>(jobs, bstart, njobs);
+}
+template <int C, bool Z3, typename LD>
+__global__ __launch_bounds__(ISO_THREADS, 8) void k_blur_iso_batch_o8(const IsoJob<C>* __restrict__ jobs,
+                                                                    const int64_t* __restrict__ bstart, int njobs) {
+    blur_iso_batch_tile<C, Z3, LD, 2, 1>(jobs, bstart, njobs);
 }
 
 // ----------------------------------------------------------------- host side
@@ -585,14 +631,16 @@ static int try_fused(const cle_buf* src, const cle_buf* dst, const Taps* taps[3]
     return rc;
 }
 
-// ---------------------------------------------------------------- z-chunk autotuner
+// ---------------------------------------------------------------- autotuner
 // The isotropic blur splits every column tile's z range into nzc chunks: more chunks
-// fill the chip with more blocks, each chunk re-reads 2C halo slices.  The output is
-// the same for every nzc (each voxel's taps are summed in the same order).
-// cle_autotune_gaussian_blur times the candidate chunk counts for one configuration
-// (half-width, dimensionality, source type, size class) ahead of time and keeps the
-// fastest for the process; blurs look the choice up and otherwise use the default
-// (about ISO_BLOCKS blocks), so no blur call ever pays for tuning.  CLE_AUTOTUNE=0 or
+// fill the chip with more blocks, each chunk re-reads 2C halo slices; and it runs
+// under one of two register budgets (k_blur_iso / k_blur_iso_o8).  The output is the
+// same for every choice (each voxel's taps are summed in the same order).
+// cle_autotune_gaussian_blur times the candidate (chunk count, budget) pairs for one
+// configuration (half-width, dimensionality, source type, size class) ahead of time
+// and keeps the fastest for the process; blurs look the choice up and otherwise use
+// the defaults (about ISO_BLOCKS blocks, iso_o8_default), so no blur call ever pays
+// for tuning.  CLE_AUTOTUNE=0 or
 // cle_set_autotune(0) ignores (and forgets) the choices.
 static std::atomic<int> g_autotune{-1};  // -1: read CLE_AUTOTUNE on first use
 
@@ -612,8 +660,12 @@ struct TuneKey {
         return std::tie(c, z3, src, dev, lxy, lz) < std::tie(o.c, o.z3, o.src, o.dev, o.lxy, o.lz);
     }
 };
+struct TuneChoice {
+    int64_t nzc;
+    int o8;  // register budget: 1 = k_blur_iso_o8
+};
 static std::mutex g_tune_mu;
-static std::map<TuneKey, int64_t> g_tune;  // key -> chosen nzc
+static std::map<TuneKey, TuneChoice> g_tune;
 
 static int ilog2(int64_t v) {
     int l = 0;
@@ -621,12 +673,26 @@ static int ilog2(int64_t v) {
     return l;
 }
 
+// Register budget of a configuration when no tuned choice is known (1: k_blur_iso_o8),
+// from MI355X timings in ms, k_blur_iso / k_blur_iso_o8: 3D 256 x 1024 x 1024 u16
+// C = 3 0.406 / 0.413, 4 0.605 / 0.427, 6 0.635 / 0.521, 7 0.657 / 0.568, 8 0.682 /
+// 0.757 (spills); 2D 8192 x 8192 u16 C = 6 0.297 / 0.273, 8 0.324 / 0.278, 10 0.505 /
+// 0.308, 12 0.487 / 0.329.
+static int iso_o8_default(int c, bool z3) { return z3 ? (c >= 4 && c <= 7) : (c <= 12); }
+
 template <int C, bool Z3, typename LD>
 static void launch_iso_nzc(LD in, float* out, const Geom& g, const IsoW<C>& w, int ntx, int nty, int64_t nzc,
-                           hipStream_t s) {
+                           int o8, hipStream_t s) {
     const int zc = (int)((g.D + nzc - 1) / nzc);
     nzc = (g.D + zc - 1) / zc;
-    k_blur_iso<C, Z3, LD><<<(unsigned)((int64_t)ntx * nty * nzc), ISO_THREADS, 0, s>>>(in, out, g, zc, ntx, nty, w);
+    const unsigned nb = (unsigned)((int64_t)ntx * nty * nzc);
+    if constexpr (iso_o8_ok<C, Z3>()) {
+        if (o8) {
+            k_blur_iso_o8<C, Z3, LD><<<nb, ISO_THREADS, 0, s>>>(in, out, g, zc, ntx, nty, w);
+            return;
+        }
+    }
+    k_blur_iso<C, Z3, LD><<<nb, ISO_THREADS, 0, s>>>(in, out, g, zc, ntx, nty, w);
 }
 
 template <typename LD> constexpr int ld_code();
@@ -649,8 +715,9 @@ static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hip
     IsoW<C> w;
     for (int v = 0; v <= C; ++v) w.w[v] = wts[v];
     int64_t nzc = nzc_for(ISO_BLOCKS);
+    int o8 = iso_o8_default(C, Z3);
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    if (max_nzc > 1 && autotune_on()) {
+    if (autotune_on()) {
         int dev = 0;
         (void)hipGetDevice(&dev);
         const TuneKey key{C, Z3 ? 1 : 0, ld_code<LD>(), dev, ilog2(nxy), ilog2(g.D)};
@@ -658,29 +725,37 @@ static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hip
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tune.find(key);
-            if (it != g_tune.end()) { nzc = it->second < max_nzc ? it->second : max_nzc; known = true; }
+            if (it != g_tune.end()) {
+                nzc = it->second.nzc < max_nzc ? it->second.nzc : max_nzc;
+                o8 = it->second.o8;
+                known = true;
+            }
         }
         if (!known && t_tuning && hipStreamIsCapturing(s, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone) {
-            std::vector<int64_t> cand;
-            for (int64_t b : {256, 512, 1024, 2048, 4096}) {
-                const int64_t v = nzc_for(b);
-                if (std::find(cand.begin(), cand.end(), v) == cand.end()) cand.push_back(v);
-            }
+            std::vector<TuneChoice> cand;
+            for (int v8 = 0; v8 <= (iso_o8_ok<C, Z3>() ? 1 : 0); ++v8)
+                for (int64_t b : {256, 512, 1024, 2048, 4096}) {
+                    const int64_t v = nzc_for(b);
+                    bool dup = false;
+                    for (const TuneChoice& t : cand) dup |= t.nzc == v && t.o8 == v8;
+                    if (!dup) cand.push_back({v, v8});
+                }
             hipEvent_t e0, e1;
             if (cand.size() > 1 && hipEventCreate(&e0) == hipSuccess) {
                 if (hipEventCreate(&e1) == hipSuccess) {
                     float best = 1e30f;
-                    for (int64_t v : cand) {
-                        launch_iso_nzc<C, Z3>(in, out, g, w, ntx, nty, v, s);  // warm
-                        for (int rep = 0; rep < 3; ++rep) {                   // best of three
+                    for (const TuneChoice& t : cand) {
+                        launch_iso_nzc<C, Z3>(in, out, g, w, ntx, nty, t.nzc, t.o8, s);  // warm
+                        for (int rep = 0; rep < 3; ++rep) {                             // best of three
                             (void)hipEventRecord(e0, s);
-                            launch_iso_nzc<C, Z3>(in, out, g, w, ntx, nty, v, s);
+                            launch_iso_nzc<C, Z3>(in, out, g, w, ntx, nty, t.nzc, t.o8, s);
                             (void)hipEventRecord(e1, s);
                             float ms = 1e30f;
                             if (hipEventSynchronize(e1) == hipSuccess &&
                                 hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best) {
                                 best = ms;
-                                nzc = v;
+                                nzc = t.nzc;
+                                o8 = t.o8;
                             }
                         }
                     }
@@ -689,10 +764,10 @@ static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hip
                 (void)hipEventDestroy(e0);
             }
             std::lock_guard<std::mutex> lk(g_tune_mu);
-            g_tune[key] = nzc;
+            g_tune[key] = {nzc, o8};
         }
     }
-    launch_iso_nzc<C, Z3>(in, out, g, w, ntx, nty, nzc, s);
+    launch_iso_nzc<C, Z3>(in, out, g, w, ntx, nty, nzc, o8, s);
 }
 
 constexpr int ISO_MAXC = 20;
@@ -890,9 +965,18 @@ static void plan_iso_group(const std::vector<BlurReq>& reqs, const std::vector<i
     bst[idx.size()] = nb;
     const size_t jo = blob.put_vec(jobs), bo = blob.put_vec(bst);
     const int nj = (int)idx.size();
+    const int o8 = iso_o8_default(C, Z3);
     out.push_back([=](const char* dev) -> int {
-        k_blur_iso_batch<C, Z3, LD><<<(unsigned)nb, ISO_THREADS, 0, s>>>((const IsoJob<C>*)(dev + jo),
-                                                                       (const int64_t*)(dev + bo), nj);
+        const IsoJob<C>* dj = (const IsoJob<C>*)(dev + jo);
+        const int64_t* db = (const int64_t*)(dev + bo);
+        bool done = false;
+        if constexpr (iso_o8_ok<C, Z3>()) {
+            if (o8) {
+                k_blur_iso_batch_o8<C, Z3, LD><<<(unsigned)nb, ISO_THREADS, 0, s>>>(dj, db, nj);
+                done = true;
+            }
+        }
+        if (!done) k_blur_iso_batch<C, Z3, LD><<<(unsigned)nb, ISO_THREADS, 0, s>>>(dj, db, nj);
         CLE_LAUNCHED();
         return CLE_OK;
     });

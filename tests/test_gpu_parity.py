@@ -593,20 +593,35 @@ def test_histogram_bin_boundaries(cle, shape):
     assert np.array_equal(cle.pull(cle.histogram(c, num_bins=256)), O.histogram(c, 256))
 
 
-def test_blur_autotuned_chunking_identical(cle):
-    # the z-chunk autotuner only changes how the volume is split into blocks: the
-    # tuned and the default launch give bit-identical output (4.2 M voxels, 3 dtypes)
+@pytest.mark.parametrize("shape,sig", [((64, 256, 256), 2.0), ((64, 256, 256), 1.5), ((1024, 1024), 2.6)])
+def test_blur_autotuned_chunking_identical(cle, shape, sig):
+    # the autotuner only changes how the volume is split into blocks and the register
+    # budget the tile code runs under: the tuned and the default launch give
+    # bit-identical output
     import torch
     rng = np.random.default_rng(21)
-    vol = rng.integers(0, 4000, (64, 256, 256)).astype(np.uint16)
+    vol = rng.integers(0, 4000, shape).astype(np.uint16)
     src = cle.asarray(torch.from_numpy(vol.view(np.int16)).cuda().view(torch.uint16))
     cle.set_autotune(False)
-    outs = [cle.pull(cle.gaussian_blur(src, sigma_x=2, sigma_y=2, sigma_z=2))]  # default chunking
+    outs = [cle.pull(cle.gaussian_blur(src, sigma_x=sig, sigma_y=sig, sigma_z=sig))]  # defaults
     cle.set_autotune(True)
-    outs.append(cle.pull(cle.autotune_gaussian_blur(src, None, 2, 2, 2)))      # tuning run
-    outs.append(cle.pull(cle.gaussian_blur(src, sigma_x=2, sigma_y=2, sigma_z=2)))  # tuned choice
+    outs.append(cle.pull(cle.autotune_gaussian_blur(src, None, sig, sig, sig)))      # tuning run
+    outs.append(cle.pull(cle.gaussian_blur(src, sigma_x=sig, sigma_y=sig, sigma_z=sig)))  # tuned choice
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
-    assert_blur_close(outs[0], O.gaussian_blur(vol.astype(np.float32), 2, 2, 2))
+    assert_blur_close(outs[0], O.gaussian_blur(vol.astype(np.float32), sig, sig, sig))
+
+
+@pytest.mark.parametrize("c", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 16, 20])
+def test_blur_isotropic_half_widths(cle, c):
+    # isotropic blurs of every register budget the half-width selects by default (3D
+    # and 2D, u16 and f32 sources, tiles cut by the image edges) against the oracle
+    sig = (2 * c + 0.5) / 8  # floor(8 sigma) = 2c, made odd: 2c + 1 taps
+    rng = np.random.default_rng(100 + c)
+    for shape in ((37, 70, 130), (150, 260)):
+        a = rng.integers(0, 4000, shape).astype(np.uint16)
+        want = O.gaussian_blur(a.astype(np.float32), sig, sig, sig)
+        assert_blur_close(cle.pull(cle.gaussian_blur(a, None, sig, sig, sig)), want)
+        assert_blur_close(cle.pull(cle.gaussian_blur(a.astype(np.float32), None, sig, sig, sig)), want)
 
 
 @pytest.mark.parametrize("shape", [(70, 130), (20, 37, 70), (1, 18, 66), (17, 16, 64)])
