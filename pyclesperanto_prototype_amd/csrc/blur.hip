@@ -43,6 +43,10 @@ struct Taps {
 struct LdF32 {
     const float* p;
     using raw_t = float;
+    static constexpr int kSize = 4;
+    __device__ __forceinline__ raw_t raw_b(int64_t slice, uint32_t boff) const {
+        return *(const float*)((const char*)(p + slice) + boff);
+    }
     __device__ __forceinline__ float operator()(int64_t i) const { return p[i]; }
     __device__ __forceinline__ raw_t raw(int64_t i) const { return p[i]; }
     __device__ __forceinline__ static float cvt(raw_t r) { return r; }
@@ -50,6 +54,10 @@ struct LdF32 {
 struct LdU16 {
     const uint16_t* p;
     using raw_t = uint32_t;
+    static constexpr int kSize = 2;
+    __device__ __forceinline__ raw_t raw_b(int64_t slice, uint32_t boff) const {
+        return *(const uint16_t*)((const char*)(p + slice) + boff);
+    }
     __device__ __forceinline__ float operator()(int64_t i) const { return (float)p[i]; }
     __device__ __forceinline__ raw_t raw(int64_t i) const { return p[i]; }
     __device__ __forceinline__ static float cvt(raw_t r) { return (float)r; }
@@ -57,6 +65,10 @@ struct LdU16 {
 struct LdU8 {
     const uint8_t* p;
     using raw_t = uint32_t;
+    static constexpr int kSize = 1;
+    __device__ __forceinline__ raw_t raw_b(int64_t slice, uint32_t boff) const {
+        return *(const uint8_t*)((const char*)(p + slice) + boff);
+    }
     __device__ __forceinline__ float operator()(int64_t i) const { return (float)p[i]; }
     __device__ __forceinline__ raw_t raw(int64_t i) const { return p[i]; }
     __device__ __forceinline__ static float cvt(raw_t r) { return (float)r; }
@@ -333,6 +345,9 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
     const int row = threadIdx.x / (ISO_TX / 2);
     const int col = (threadIdx.x % (ISO_TX / 2)) * 2;
     const int Hm = (int)g.H - 1, Wm = (int)g.W - 1, Wi = (int)g.W;
+    // 64-VGPR form: global addresses as a uniform slice base plus a 32-bit byte offset
+    // (no 64-bit per-lane address arithmetic); the other form indexes from the image base
+    const uint32_t obyte = (uint32_t)(((int)y0 + row) * Wi + (int)x0 + col) * 4u;
     static_assert(AWP == AW, "staged slice stored at its linear index");
     int off[PF];
 #pragma unroll
@@ -342,7 +357,7 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
         int gy = (int)y0 - C + ay, gx = (int)x0 - C + ax;
         gy = gy < 0 ? 0 : (gy > Hm ? Hm : gy);
         gx = gx < 0 ? 0 : (gx > Wm ? Wm : gx);
-        off[k] = gy * Wi + gx;
+        off[k] = PB == 1 ? (int)((uint32_t)(gy * Wi + gx) * (uint32_t)LD::kSize) : gy * Wi + gx;
     }
     using R = typename LD::raw_t;
     R pa[PF], pb[PB == 2 ? PF : 1];
@@ -359,7 +374,8 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
         const int64_t base = zz * g.H * g.W;
 #pragma unroll
         for (int m = 0; m < PF; ++m)
-            if (threadIdx.x + m * ISO_THREADS < NA) pf[m] = in.raw(base + off[m]);
+            if (threadIdx.x + m * ISO_THREADS < NA)
+                pf[m] = PB == 1 ? in.raw_b(base, (uint32_t)off[m]) : in.raw(base + off[m]);
     };
     auto store = [&](const R (&pf)[PF], float* a) {
 #pragma unroll
@@ -388,6 +404,9 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
                         const int d = m - j - C;  // tap offset of input row m for output j
                         if (d >= -C && d <= C) acc[j] = fmaf(w.w[d < 0 ? -d : d], av, acc[j]);
                     }
+                    // 64-VGPR form: a scheduling fence every few rows keeps the compiler
+                    // from hoisting all the pass's LDS reads (registers it does not have)
+                    if (PB == 1 && (m % 6) == 5) __builtin_amdgcn_sched_barrier(0);
                 }
 #pragma unroll
                 for (int j = 0; j < RY; ++j) bo[(q * RY + j) * AWP + c] = acc[j];
@@ -407,6 +426,7 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
             if (d1 <= C) v0 = fmaf(w.w[d1 < 0 ? -d1 : d1], f.y, v0);
             if (d0 - 1 >= -C) v1 = fmaf(w.w[d0 - 1 < 0 ? 1 - d0 : d0 - 1], f.x, v1);
             v1 = fmaf(w.w[d1 - 1 < 0 ? 1 - d1 : d1 - 1], f.y, v1);
+            if (PB == 1 && (m % 4) == 3) __builtin_amdgcn_sched_barrier(0);  // as in the y pass
         }
         r0[2 * CZ + u] = v0;
         r1[2 * CZ + u] = v1;
@@ -426,7 +446,8 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
             }
             const int64_t zo = z00 + k - CZ, y = y0 + row, x = x0 + col;
             if (y < g.H) {
-                float* o = out + (zo * g.H + y) * g.W + x;
+                float* o = PB == 1 ? (float*)((char*)(out + zo * g.H * g.W) + obyte)
+                                   : out + (zo * g.H + y) * g.W + x;
                 if (x + 1 < g.W) { o[0] = o0; o[1] = o1; }
                 else if (x < g.W) o[0] = o0;
             }
@@ -675,9 +696,9 @@ static int ilog2(int64_t v) {
 
 // Register budget of a configuration when no tuned choice is known (1: k_blur_iso_o8),
 // from MI355X timings in ms, k_blur_iso / k_blur_iso_o8: 3D 256 x 1024 x 1024 u16
-// C = 3 0.406 / 0.413, 4 0.605 / 0.427, 6 0.635 / 0.521, 7 0.657 / 0.568, 8 0.682 /
-// 0.757 (spills); 2D 8192 x 8192 u16 C = 6 0.297 / 0.273, 8 0.324 / 0.278, 10 0.505 /
-// 0.308, 12 0.487 / 0.329.
+// C = 3 0.406 / 0.413, 4 0.605 / 0.431, 6 0.635 / 0.506, 7 0.657 / 0.551, 8 0.682 /
+// 0.700 (4 VGPRs spilled); 2D 8192 x 8192 u16 C = 6 0.297 / 0.258, 8 0.324 / 0.272,
+// 10 0.505 / 0.288, 12 0.487 / 0.303.
 static int iso_o8_default(int c, bool z3) { return z3 ? (c >= 4 && c <= 7) : (c <= 12); }
 
 template <int C, bool Z3, typename LD>
@@ -772,6 +793,9 @@ static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hip
 
 constexpr int ISO_MAXC = 20;
 
+// the isotropic tile code addresses a slice by 32-bit byte offsets (float32 output)
+static bool iso_slice_ok(const Geom& g) { return g.H * g.W < ((int64_t)1 << 30); }
+
 template <bool Z3, typename LD>
 static int launch_iso(LD in, float* out, const Geom& g, int c, const float* wts, hipStream_t s) {
     switch (c) {
@@ -797,7 +821,7 @@ static int try_iso(const cle_buf* src, const cle_buf* dst, const Taps* taps[3], 
     const bool z3 = g.ndim == 3;
     if (!taps[0] || !taps[1] || (z3 && !taps[2])) return CLE_OK;
     const int c = taps[0]->c;
-    if (c < 1 || c > ISO_MAXC || taps[1]->c != c || (z3 && taps[2]->c != c)) return CLE_OK;
+    if (c < 1 || c > ISO_MAXC || taps[1]->c != c || (z3 && taps[2]->c != c) || !iso_slice_ok(g)) return CLE_OK;
     for (int v = 0; v <= c; ++v)
         if (taps[1]->w[v] != taps[0]->w[v] || (z3 && taps[2]->w[v] != taps[0]->w[v])) return CLE_OK;
     float* out = (float*)dst->data;
@@ -1004,7 +1028,7 @@ int plan_blur_batch(const std::vector<BlurReq>& reqs, Blob& blob, std::vector<La
         const bool typed = r.in_dt == CLE_U16 || r.in_dt == CLE_U8 || r.in_dt == CLE_F32;
         if (r.sigma > 0 && typed) {
             gauss_taps(r.sigma, &t);
-            if (t.c >= 1 && t.c <= ISO_MAXC) {
+            if (t.c >= 1 && t.c <= ISO_MAXC && iso_slice_ok(r.g)) {
                 groups[key(t.c, r.g.ndim == 3, r.in_dt)].push_back((int)i);
                 continue;
             }
