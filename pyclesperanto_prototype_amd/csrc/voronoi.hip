@@ -648,10 +648,12 @@ __global__ __launch_bounds__(VB, 3) void k_vor_pair_grid_multi(const BImg* __res
     vor_pair_grid<true>(imgs, src, dst, b, L, S, tpb);
 }
 
-// tiles per block of the raster mode: one for images below 65536 tiles, 16 above (fewer,
-// longer-lived blocks: measured 15.8 -> 12.7 ms for voronoi_otsu_labeling of the
-// 256x1024x1024 stack and 162 -> 112 ms for the 512x2048x2048 one)
-static int raster_tpb(int64_t ntiles) { return ntiles < 65536 ? 1 : 16; }
+// tiles per block of the raster mode: one for images below 4096 tiles, 16 above (fewer,
+// longer-lived blocks, and a block whose tiles are all inactive costs one dispatch
+// instead of 16: measured 15.8 -> 12.7 ms for voronoi_otsu_labeling of the
+// 256x1024x1024 stack, 162 -> 112 ms for the 512x2048x2048 one, and 1236 -> 931 ms for
+// 600 stacks of 32x128x128 .. 96x512x512 one by one)
+static int raster_tpb(int64_t ntiles) { return ntiles < 4096 ? 1 : 16; }
 
 // Closes the last pair b of a chunk (one block): iteration counts of the images that
 // converged in it, and the "an image still grows" word the host reads.

@@ -181,7 +181,16 @@ def _render_nuclei(shape, centres, axes, rot, values, background, read_noise, se
         lin = p[:, 0]
         for d in range(1, nd):
             lin = lin * shape[d] + p[:, d]
-        clean.view(-1)[lin] = v[:, None].expand(-1, offs.shape[0])[m]
+        vals = v[:, None].expand(-1, offs.shape[0])[m]
+        # overlaps: the later nucleus wins, decided before the write (a device write
+        # with repeated indices would keep an arbitrary one): sort by (voxel, nucleus)
+        # and keep the last entry of every voxel
+        ids = torch.arange(s0, e, device=device)[:, None].expand(-1, offs.shape[0])[m]
+        order = torch.argsort(lin * n + ids)
+        lin, vals = lin[order], vals[order]
+        last = torch.ones_like(lin, dtype=torch.bool)
+        last[:-1] = lin[1:] != lin[:-1]
+        clean.view(-1)[lin[last]] = vals[last]
     gen = torch.Generator(device=device)
     gen.manual_seed(seed)
     noisy = torch.poisson(clean, generator=gen)

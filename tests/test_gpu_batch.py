@@ -39,6 +39,22 @@ def test_batch_equals_single_image_path(cle):
         assert got.max() > 0, k
 
 
+def test_batch_equals_single_multi_tile_raster(cle):
+    # single images of 4096+ growth tiles grow in raster order over runs of 16 tiles per
+    # block; a batch of them grows from work lists: both give the same labels
+    import torch
+    from pyclesperanto_prototype_amd import synthetic
+    imgs = [cle.asarray(synthetic.nuclei_3d(sh, n, seed=60 + k)[0])
+            for k, (sh, n) in enumerate((((64, 256, 256), 300), ((48, 300, 330), 250)))]
+    imgs.append(cle.asarray(synthetic.nuclei_overlapping((2048, 2048), 2e-4, seed=62)[0]))
+    ss, so = [2.0, 1.5, 3.0], [1.5, 1.0, 2.0]
+    out = cle.voronoi_otsu_labeling_batch(imgs, ss, so)
+    for k, im in enumerate(imgs):
+        want = cle.voronoi_otsu_labeling(im, None, ss[k], so[k])
+        assert torch.equal(out[k].tensor, want.tensor), k
+        assert int(want.tensor.view(torch.int32).max()) > 10, k
+
+
 def test_batch_single_member_and_repeat(cle):
     """A batch of one equals the single call; running a batch twice gives the same."""
     rng = np.random.default_rng(51)
