@@ -64,9 +64,9 @@ int cle_set(const cle_buf* dst, float value, void* stream);
 int cle_copy(const cle_buf* src, const cle_buf* dst, void* stream);
 
 /* ---- filters ------------------------------------------------------------ */
-/* Autotuner of the isotropic blur's z chunking (blocks per launch); no counterpart in
- * the reference.  cle_autotune_gaussian_blur runs the blur of these arguments once per
- * candidate chunking (timed with events on `stream`, which it synchronises) and keeps
+/* Autotuner of the isotropic blur's z chunking (blocks per launch) and register budget;
+ * no counterpart in the reference.  cle_autotune_gaussian_blur runs the blur of these
+ * arguments once per candidate (timed with events on `stream`, which it synchronises) and keeps
  * the fastest for its (half-width, dimensionality, source type, size class) for the
  * process; later cle_gaussian_blur calls of that class use it and never time anything.
  * dst holds the blur afterwards.  Results do not depend on the choice.
