@@ -863,10 +863,25 @@ __global__ __launch_bounds__(256) void k_seg_and_batch(const BImg* __restrict__ 
     const BImg& im = imgs[j];
     const int64_t n = im.g.n(), i0 = (b - vs[j]) * BV, i1 = i0 + BV < n ? i0 + BV : n;
     const float t = (float)thr[j];
-    for (int64_t i = im.off + i0 + threadIdx.x; i < im.off + i1; i += 256) {
-        const uint8_t sg = F[i] > t ? 1 : 0;
-        G[i] = sg;
-        S[i] = (S[i] != 0 && sg) ? 1 : 0;
+    // four voxels per thread: an image starts at a multiple of 64 voxels in the batch
+    // buffers (16-byte aligned float4 / 4-byte uchar4) and its slot is padded to 64, so
+    // the last group may run past n inside the padding (written as 0)
+    const float* f = F + im.off;
+    uint8_t* sp = S + im.off;
+    uint8_t* gp = G + im.off;
+    for (int64_t i = i0 + 4 * threadIdx.x; i < i1; i += 4 * 256) {
+        const float4 v = *(const float4*)(f + i);
+        const uchar4 sv = *(const uchar4*)(sp + i);
+        const float vv[4] = {v.x, v.y, v.z, v.w};
+        const uint8_t so[4] = {sv.x, sv.y, sv.z, sv.w};
+        uint8_t go[4], sn[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            go[c] = (i + c < n && vv[c] > t) ? 1 : 0;
+            sn[c] = (so[c] != 0 && go[c]) ? 1 : 0;
+        }
+        *(uchar4*)(gp + i) = make_uchar4(go[0], go[1], go[2], go[3]);
+        *(uchar4*)(sp + i) = make_uchar4(sn[0], sn[1], sn[2], sn[3]);
     }
 }
 
@@ -901,8 +916,32 @@ __global__ __launch_bounds__(256) void k_mask_out_batch(const BImg* __restrict__
     const BImg& im = imgs[j];
     const int64_t n = im.g.n(), i0 = (b - vs[j]) * BV, i1 = i0 + BV < n ? i0 + BV : n;
     const DBuf d{im.dst, im.dst_dt};
-    const float* A = vor_result_buffer(iters[j]) ? P1 : P0;
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) st(d, i, G[im.off + i] ? A[im.off + i] : 0.f);
+    const float* A = (vor_result_buffer(iters[j]) ? P1 : P0) + im.off;
+    const uint8_t* g = G + im.off;
+    if ((im.dst_dt == CLE_U32 || im.dst_dt == CLE_I32) && ((uintptr_t)im.dst & 15) == 0) {
+        // 32-bit labels into a 16-byte aligned destination: four voxels per thread (the
+        // batch buffers are aligned as in k_seg_and_batch), the image's last partial
+        // group element by element
+        uint32_t* o = (uint32_t*)im.dst;
+        const bool u = im.dst_dt == CLE_U32;
+        for (int64_t i = i0 + 4 * threadIdx.x; i < i1; i += 4 * 256) {
+            const float4 a = *(const float4*)(A + i);
+            const uchar4 m = *(const uchar4*)(g + i);
+            const float av[4] = {m.x ? a.x : 0.f, m.y ? a.y : 0.f, m.z ? a.z : 0.f, m.w ? a.w : 0.f};
+            if (i + 4 <= n) {
+                uint32_t ov[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    ov[c] = u ? sat_cast<uint32_t>(av[c], 0.f, 4294967295.f)
+                              : (uint32_t)sat_cast<int32_t>(av[c], -2147483648.f, 2147483647.f);
+                *(uint4*)(o + i) = make_uint4(ov[0], ov[1], ov[2], ov[3]);
+            } else {
+                for (int c = 0; c < 4 && i + c < n; ++c) st(d, i + c, av[c]);
+            }
+        }
+        return;
+    }
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) st(d, i, g[i] ? A[i] : 0.f);
 }
 
 // Pinned host staging of the batch tables.  Buffers are taken from a process-wide
