@@ -103,13 +103,21 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
 
 // Batches: one grid over the blocks of many independent images; image j owns the
 // blocks [start[j], start[j + 1]).  Returns the largest j with start[j] <= b (images
-// without blocks are skipped).
+// without blocks are skipped; start[0] = 0 <= b).  A 64-ary search by the whole
+// wavefront: per round every lane tests one candidate and a ballot keeps the last one
+// not above b, so 2000 images cost 2 dependent loads instead of the 11 of a binary
+// search.  Every lane of the wavefront must call it together (1D blocks of a multiple
+// of 64 threads, uniform control flow); the result is the same in every lane.
 __device__ __forceinline__ int find_seg(const int64_t* start, int n, int64_t b) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= b) lo = mid;
-        else hi = mid - 1;
+    const int lane = (int)(threadIdx.x & 63);
+    int lo = 0, span = n;  // the answer lies in [lo, lo + span)
+    while (span > 1) {
+        const int step = (span + 63) >> 6;
+        const int k = lo + lane * step;
+        const uint64_t m = __ballot(k < lo + span && start[k] <= b);
+        const int nlo = lo + (63 - __clzll((long long)m)) * step;  // lane 0 (k = lo) always holds
+        span = min(step, lo + span - nlo);
+        lo = nlo;
     }
     return lo;
 }

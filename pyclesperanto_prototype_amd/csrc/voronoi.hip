@@ -173,7 +173,32 @@ __device__ __forceinline__ bool vor_init_tile(VF val, float* A, float* B, const 
                                               int64_t tile, uint8_t* settled) {
     const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
     bool pos = false, zero = false;
-    for (int k = 0; k < VPT; ++k) {
+    bool done = false;
+    if constexpr (VF::kVec) {
+        // rows of a multiple of 4 voxels: four consecutive x voxels per thread (tile rows
+        // of 16 / 32 voxels start at multiples of 4; every buffer is 16-byte aligned)
+        static_assert(VB * 4 == 1024, "one group of four voxels per thread and tile");
+        if ((g.W & 3) == 0) {
+            const int e = 4 * threadIdx.x;
+            const int lx = e % T.tx, ly = (e / T.tx) % T.ty, lz = e / (T.tx * T.ty);
+            const int64_t x = tx * T.tx + lx, y = ty * T.ty + ly, z = tz * T.tz + lz;
+            if (lz < T.tz && x < g.W && y < g.H && z < g.D) {
+                const int64_t i = (z * g.H + y) * g.W + x;
+                float v[4];
+                val.vec4(i, v);
+                const float4 f = make_float4(v[0], v[1], v[2], v[3]);
+                *(float4*)(A + i) = f;
+                *(float4*)(B + i) = f;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    pos |= v[c] > 0.f;
+                    zero |= v[c] == 0.f;
+                }
+            }
+            done = true;
+        }
+    }
+    for (int k = 0; k < VPT && !done; ++k) {
         const int li = threadIdx.x + k * VB;
         const int lx = li % T.tx, ly = (li / T.tx) % T.ty, lz = li / (T.tx * T.ty);
         if (lz >= T.tz) continue;
@@ -195,20 +220,33 @@ __device__ __forceinline__ bool vor_init_tile(VF val, float* A, float* B, const 
 // masked growth: (mask - 1) + CCL label, the label held in B (float32 add, as the
 // reference's add_image_and_scalar then add_images)
 struct MaskedAny {
+    static constexpr bool kVec = false;
     DBuf mask;
     const float* B;
     __device__ __forceinline__ float operator()(int64_t i) const { return (ld(mask, i) - 1.f) + B[i]; }
+    __device__ __forceinline__ void vec4(int64_t, float*) const {}
 };
 struct MaskedU8 {
+    static constexpr bool kVec = true;  // batch buffers: 16-byte aligned image slots
     const uint8_t* mask;
     const float* B;
     __device__ __forceinline__ float operator()(int64_t i) const { return ((float)mask[i] - 1.f) + B[i]; }
+    __device__ __forceinline__ void vec4(int64_t i, float* v) const {
+        const uchar4 m = *(const uchar4*)(mask + i);
+        const float4 b = *(const float4*)(B + i);
+        v[0] = ((float)m.x - 1.f) + b.x;
+        v[1] = ((float)m.y - 1.f) + b.y;
+        v[2] = ((float)m.z - 1.f) + b.z;
+        v[3] = ((float)m.w - 1.f) + b.w;
+    }
 };
 // unmasked growth of an existing label image: the labels converted to float32 as the
 // reference's copy into a float32 buffer does
 struct LabelsAny {
+    static constexpr bool kVec = false;
     DBuf src;
     __device__ __forceinline__ float operator()(int64_t i) const { return ld(src, i); }
+    __device__ __forceinline__ void vec4(int64_t, float*) const {}
 };
 
 // single image: init every tile, queue the neighbourhood of tiles holding a positive
@@ -825,7 +863,11 @@ static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, 
     const int64_t n = g.n();
     CLE_TRY(ccl_impl(binary, g, DBuf{B, CLE_F32}, CLE_CONN_BOX, CLE_FP32_KEY_COMPAT, nullptr, s));
     const float* res = nullptr;
-    CLE_TRY(grow_single(MaskedAny{mask, B}, A, B, g, n_iterations, &res, s));
+    // a uint8 mask and 16-byte aligned buffers take the four-voxel init
+    if (mask.dt == CLE_U8 && ((uintptr_t)mask.p & 3) == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0)
+        CLE_TRY(grow_single(MaskedU8{(const uint8_t*)mask.p, B}, A, B, g, n_iterations, &res, s));
+    else
+        CLE_TRY(grow_single(MaskedAny{mask, B}, A, B, g, n_iterations, &res, s));
     k_mask_out<<<vg(n), 256, 0, s>>>(res, mask, dst, n);
     CLE_LAUNCHED();
     return CLE_OK;
