@@ -402,62 +402,79 @@ static int histogram_impl(DBuf src, const Geom& g, int bins, float mn, float mx,
 }
 
 // ----------------------------------------------------------------- Otsu
-// Single block.  Thread 0 runs the cumulative sums in the exact order numpy's
-// cumsum uses (sequential, float64, no contraction); the 255 between-class variances
-// are then formed in parallel and the first maximum (NaN counting as maximum, as
-// numpy.argmax does) is selected.
+// Single block of 256 threads.  The histogram is staged in LDS; thread 0 runs the
+// cumulative sums in the exact order numpy's cumsum uses (sequential, float64, no
+// contraction), thread 64 the reversed ones; the class means (sum / weight, the same
+// operands as a division inside the running loop), the 255 between-class variances and
+// the first maximum (NaN counting as maximum, as numpy.argmax does: the first NaN if
+// there is one, else the first index of the largest value) are formed in parallel.
 struct OtsuLds {
-    double bc[256], w1[256], w2[256], m1[256], m2[256], var[256];
+    double bc[256], hs[256], w1[256], w2[256], m1[256], m2[256];
+    double wv[4];
+    int wi[4];
+    int first_nan;
 };
 
 __device__ __forceinline__ void otsu_body(const float* hist, const float* mm, double* thr, OtsuLds& sm) {
 #pragma clang fp contract(off)
     double* bc = sm.bc;
+    double* hs = sm.hs;
     double* w1 = sm.w1;
     double* w2 = sm.w2;
     double* m1 = sm.m1;
     double* m2 = sm.m2;
-    double* var = sm.var;
     const int t = threadIdx.x;
     const float mnf = mm[0], mxf = mm[1];
     const float rng = mxf - mnf;  // float32 subtraction, as in the reference
     bc[t] = (double)t * (double)rng / 255.0 + (double)mnf;
+    hs[t] = (double)hist[t];
+    if (t == 0) sm.first_nan = 256;
     __syncthreads();
     if (t == 0) {
         double a = 0, b = 0;
         for (int i = 0; i < 256; ++i) {
-            const double c = (double)hist[i];
+            const double c = hs[i];
             a = a + c;
             b = b + c * bc[i];
             w1[i] = a;
-            m1[i] = b / a;
+            m1[i] = b;  // running sum; divided below
         }
     } else if (t == 64) {
         double a = 0, b = 0;
         for (int i = 255; i >= 0; --i) {
-            const double c = (double)hist[i];
+            const double c = hs[i];
             a = a + c;
             b = b + c * bc[i];
             w2[i] = a;
-            m2[i] = b / a;
+            m2[i] = b;
         }
     }
     __syncthreads();
+    m1[t] = m1[t] / w1[t];
+    m2[t] = m2[t] / w2[t];
+    __syncthreads();
+    double v = -1.0;  // never the maximum: every variance is >= 0 or NaN
     if (t < 255) {
         const double d = m1[t] - m2[t + 1];
-        var[t] = (w1[t] * w2[t + 1]) * (d * d);
+        const double vt = (w1[t] * w2[t + 1]) * (d * d);
+        if (vt != vt) atomicMin(&sm.first_nan, t);
+        else v = vt;
     }
+    // largest value, smallest index among equals: within each wavefront, then across
+    int vi = t;
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(v, o);
+        const int oi = __shfl_xor(vi, o);
+        if (ov > v || (ov == v && oi < vi)) { v = ov; vi = oi; }
+    }
+    if ((t & 63) == 0) { sm.wv[t >> 6] = v; sm.wi[t >> 6] = vi; }
     __syncthreads();
     if (t == 0) {
-        int best = 0;
-        double bv = var[0];
-        if (bv == bv) {
-            for (int i = 1; i < 255; ++i) {
-                const double v = var[i];
-                if (v != v) { best = i; break; }
-                if (v > bv) { bv = v; best = i; }
-            }
-        }
+        int best = sm.wi[0];
+        double bv = sm.wv[0];
+        for (int w = 1; w < 4; ++w)
+            if (sm.wv[w] > bv) { bv = sm.wv[w]; best = sm.wi[w]; }  // waves in index order: ties keep the first
+        if (sm.first_nan < 255) best = sm.first_nan;
         thr[0] = bc[best];
     }
 }

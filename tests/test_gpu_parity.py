@@ -192,6 +192,16 @@ def test_otsu_constant_image(cle):
     assert cle.pull(cle.threshold_otsu(a)).sum() == 0
 
 
+@pytest.mark.parametrize("levels", [(0.0, 1.0), (0.0, 5.0, 10.0), (2.0, 2.5, 2.5, 9.0, 9.0, 9.0)])
+def test_otsu_tied_variances(cle, levels):
+    # few distinct values: runs of equal between-class variances; the first maximum wins
+    rng = np.random.default_rng(31)
+    a = rng.choice(np.asarray(levels, np.float32), size=(40, 50)).astype(np.float32)
+    out, t = O.threshold_otsu(a, return_threshold=True)
+    assert cle.otsu_threshold(a) == t
+    assert np.array_equal(cle.pull(cle.threshold_otsu(a)), out)
+
+
 def test_elementwise_dtypes(cle):
     rng = np.random.default_rng(8)
     a = (rng.random((9, 10)) * 600 - 100).astype(np.float32)
