@@ -835,10 +835,41 @@ __global__ void k_mask_out(const float* A, DBuf mask, DBuf dst, int64_t n) {
         st(dst, i, ld(mask, i) != 0.f ? A[i] : 0.f);
 }
 
-// seg = blurred > float32(thr); spots <- spots AND seg (in place)
+// mask-out of 32-bit labels with a uint8 mask: four voxels per thread (A 16-byte, mask
+// 4-byte, dst 16-byte aligned), the last n % 4 voxels one by one
+__global__ void k_mask_out_u8_32(const float* __restrict__ A, const uint8_t* __restrict__ mask, DBuf dst, int64_t n) {
+    const int64_t nv = n / 4;
+    const bool u = dst.dt == CLE_U32;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nv; q += (int64_t)gridDim.x * blockDim.x) {
+        const float4 a = ((const float4*)A)[q];
+        const uchar4 m = ((const uchar4*)mask)[q];
+        const float av[4] = {m.x ? a.x : 0.f, m.y ? a.y : 0.f, m.z ? a.z : 0.f, m.w ? a.w : 0.f};
+        uint32_t ov[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            ov[c] = u ? sat_cast<uint32_t>(av[c], 0.f, 4294967295.f)
+                      : (uint32_t)sat_cast<int32_t>(av[c], -2147483648.f, 2147483647.f);
+        ((uint4*)dst.p)[q] = make_uint4(ov[0], ov[1], ov[2], ov[3]);
+    }
+    for (int64_t i = nv * 4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        st(dst, i, mask[i] ? A[i] : 0.f);
+}
+
+// seg = blurred > float32(thr); spots <- spots AND seg (in place); four voxels per
+// thread (the library's scratch buffers are 256-byte aligned), the tail one by one
 __global__ void k_seg_and(const float* blurred, const double* thr, uint8_t* spots, uint8_t* seg, int64_t n) {
     const float t = (float)thr[0];
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t nv = n / 4;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nv; q += (int64_t)gridDim.x * blockDim.x) {
+        const float4 v = ((const float4*)blurred)[q];
+        const uchar4 sp = ((const uchar4*)spots)[q];
+        const uchar4 g = make_uchar4(v.x > t ? 1 : 0, v.y > t ? 1 : 0, v.z > t ? 1 : 0, v.w > t ? 1 : 0);
+        ((uchar4*)seg)[q] = g;
+        ((uchar4*)spots)[q] = make_uchar4((sp.x != 0 && g.x) ? 1 : 0, (sp.y != 0 && g.y) ? 1 : 0,
+                                          (sp.z != 0 && g.z) ? 1 : 0, (sp.w != 0 && g.w) ? 1 : 0);
+    }
+    for (int64_t i = nv * 4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
         const uint8_t sg = blurred[i] > t ? 1 : 0;
         seg[i] = sg;
         spots[i] = (spots[i] != 0 && sg) ? 1 : 0;
@@ -868,7 +899,11 @@ static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, 
         CLE_TRY(grow_single(MaskedU8{(const uint8_t*)mask.p, B}, A, B, g, n_iterations, &res, s));
     else
         CLE_TRY(grow_single(MaskedAny{mask, B}, A, B, g, n_iterations, &res, s));
-    k_mask_out<<<vg(n), 256, 0, s>>>(res, mask, dst, n);
+    if (mask.dt == CLE_U8 && (dst.dt == CLE_U32 || dst.dt == CLE_I32) && ((uintptr_t)res & 15) == 0 &&
+        ((uintptr_t)mask.p & 3) == 0 && ((uintptr_t)dst.p & 15) == 0)
+        k_mask_out_u8_32<<<vg(n / 4 + 1), 256, 0, s>>>(res, (const uint8_t*)mask.p, dst, n);
+    else
+        k_mask_out<<<vg(n), 256, 0, s>>>(res, mask, dst, n);
     CLE_LAUNCHED();
     return CLE_OK;
 }
@@ -1371,7 +1406,7 @@ int cle_voronoi_otsu_labeling(const cle_buf* src, const cle_buf* dst, float spot
     const float s2[3] = {outline_sigma, outline_sigma, outline_sigma};
     CLE_TRY(blur_impl(src, &bb, s2, s));
     CLE_TRY(otsu_impl(DBuf{blurred.p, CLE_F32}, g, mm, thr, s));
-    k_seg_and<<<vg(n), 256, 0, s>>>(blurred.get<float>(), thr, spots.get<uint8_t>(), seg.get<uint8_t>(), n);
+    k_seg_and<<<vg(n / 4 + 1), 256, 0, s>>>(blurred.get<float>(), thr, spots.get<uint8_t>(), seg.get<uint8_t>(), n);
     CLE_LAUNCHED();
     // masked Voronoi of the seeds inside the segmentation; the blurred buffer is
     // reused as the second growth buffer
