@@ -184,10 +184,13 @@ __device__ __forceinline__ bool nz(const DBuf b, int64_t i) {
 // above the last index clamped).
 // A block of 256 threads handles a 64 x 16 (x, y) tile of MX_ZC consecutive slices,
 // the slices z-1, z, z+1 of the tile (+ 1-voxel x / y halo) held in an LDS ring of 3
-// planes; every read stays inside the image and inside the staged planes.
+// planes, plus one plane of per-(x, y) maxima over those slices shared by the 8
+// neighbouring outputs that read it; every read stays inside the image and inside the
+// staged planes.
 // out(i, is_max) stores voxel i's flag.
 constexpr int MX_TX = 64, MX_TY = 16, MX_ZC = 16;
 constexpr int MX_RX = MX_TX + 2, MX_RY = MX_TY + 2, MX_RP = MX_RX * MX_RY;
+constexpr int MX_LDS = 4 * MX_RP;  // floats of LDS per block: 3 staged planes + column maxima
 
 __host__ __device__ inline int64_t maxima_tiles(const Geom& g) {
     const int64_t nzc = g.ndim == 3 ? (g.D + MX_ZC - 1) / MX_ZC : 1;
@@ -212,50 +215,55 @@ __device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t
     if (z0 > 0) stage(z0 - 1);
     stage(z0);
     if (z0 + 1 < g.D && d3) stage(z0 + 1);
+    float* C = ring + 3 * MX_RP;  // column maxima of the current z
     for (int64_t z = z0; z < z1; ++z) {
         __syncthreads();
-        // ring planes of z - 1 (skipped below 0), z, z + 1 (clamped to the last slice)
+        // ring planes of z - 1 (sm, none below 0), z (s0), z + 1 (sp; none past the
+        // last slice, where the visit reads the centre plane again)
         const int s0 = (int)(z % 3);
-        const int sm = z > 0 ? (s0 + 2) % 3 : -1;
-        const int sp = z + 1 < g.D ? (s0 + 1) % 3 : s0;
+        const int sm = (d3 && z > 0) ? (s0 + 2) % 3 : -1;
+        const int sp = (d3 && z + 1 < g.D) ? (s0 + 1) % 3 : -1;
+        const float* P0 = ring + s0 * MX_RP;
+        // C(x, y) = max over the visited z neighbours at every staged (x, y): positions
+        // before the image's first column / row are never visited (NaN: nothing), those
+        // past its last column / row are the clamped last one (as the visit's clamped
+        // offsets read them)
+        for (int c = tid; c < MX_RP; c += 256) {
+            const int cx = c % MX_RX, cy = c / MX_RX;
+            const int64_t xx = x0 - 1 + cx, yy = y0 - 1 + cy;
+            float m = __builtin_nanf("");
+            if (xx >= 0 && yy >= 0) {
+                const int qx = xx < g.W ? cx : (int)(g.W - x0);
+                const int qy = yy < g.H ? cy : (int)(g.H - y0);
+                const int q = qy * MX_RX + qx;
+                m = P0[q];
+                if (sm >= 0) m = fmaxf(m, ring[sm * MX_RP + q]);
+                if (sp >= 0) m = fmaxf(m, ring[sp * MX_RP + q]);
+            }
+            C[c] = m;
+        }
+        __syncthreads();
 #pragma unroll
         for (int k = 0; k < (MX_TX * MX_TY) / 256; ++k) {
             const int c = tid + k * 256;
             const int ix = c % MX_TX, iy = c / MX_TX;
             if (x0 + ix >= g.W || y0 + iy >= g.H) continue;
-            // staged-plane offsets of x - 1 (skipped below 0), x, x + 1 (clamped), same for y
-            const int lx = ix + 1, ly = iy + 1;
-            const int xo[3] = {x0 + ix > 0 ? lx - 1 : -1, lx, x0 + ix + 1 < g.W ? lx + 1 : lx};
-            const int yo[3] = {y0 + iy > 0 ? (ly - 1) * MX_RX : -1, ly * MX_RX,
-                               y0 + iy + 1 < g.H ? (ly + 1) * MX_RX : ly * MX_RX};
-            const int zo[3] = {d3 ? (sm >= 0 ? sm * MX_RP : -1) : -1, s0 * MX_RP, d3 ? sp * MX_RP : -1};
-            const float v = ring[s0 * MX_RP + ly * MX_RX + lx];
-            // The visit in order (start lm = v - 1, a strictly larger value takes over,
-            // the flag is whether the last take-over was the centre) in closed form: B /
-            // A = the largest value visited before / after the centre (fmaxf ignores
-            // NaN, which never takes over; NaN = nothing visited).  A before-value takes
-            // over iff B > v - 1, leaving lm1 = max(v - 1, B); the centre takes over iff
-            // v > lm1 (then lm2 = v); with no take-over before it, the flag stays set;
+            const int q = (iy + 1) * MX_RX + ix + 1;
+            const float v = P0[q];
+            // The visit in order (x, then y, then z offsets; start lm = v - 1, a strictly
+            // larger value takes over, the flag is whether the last take-over was the
+            // centre) in closed form: B / A = the largest value visited before / after
+            // the centre (fmaxf ignores NaN, which never takes over; NaN = nothing
+            // visited).  Before the centre: the x - 1 columns, (x, y - 1), (x, y, z - 1);
+            // after it: (x, y, z + 1), (x, y + 1), the x + 1 columns.  A before-value
+            // takes over iff B > v - 1, leaving lm1 = max(v - 1, B); the centre takes over
+            // iff v > lm1 (then lm2 = v); with no take-over before it, the flag stays set;
             // any after-value above lm2 clears it.  Exact for ties, infinities, NaN and
             // values where v - 1 == v.
-            float bmax = __builtin_nanf(""), amax = __builtin_nanf("");
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-                if (xo[dx] < 0) continue;
-#pragma unroll
-                for (int dy = 0; dy < 3; ++dy) {
-                    if (yo[dy] < 0) continue;
-#pragma unroll
-                    for (int dz = 0; dz < 3; ++dz) {
-                        if (zo[dz] < 0) continue;
-                        const int order = (dx * 3 + dy) * 3 + dz;  // 13 = the centre
-                        if (order == 13) continue;
-                        const float w = ring[zo[dz] + yo[dy] + xo[dx]];
-                        if (order < 13) bmax = fmaxf(bmax, w);
-                        else amax = fmaxf(amax, w);
-                    }
-                }
-            }
+            float bmax = fmaxf(fmaxf(C[q - MX_RX - 1], C[q - 1]), fmaxf(C[q + MX_RX - 1], C[q - MX_RX]));
+            float amax = fmaxf(fmaxf(C[q + MX_RX], C[q - MX_RX + 1]), fmaxf(C[q + 1], C[q + MX_RX + 1]));
+            if (sm >= 0) bmax = fmaxf(bmax, ring[sm * MX_RP + q]);
+            if (d3) amax = fmaxf(amax, sp >= 0 ? ring[sp * MX_RP + q] : v);
             const float lm0 = v - 1.f;
             const bool took = bmax > lm0;
             const float lm1 = took ? bmax : lm0;

@@ -111,7 +111,7 @@ __global__ void k_onlyzero_max(DBuf s, DBuf flag, DBuf d, Geom g, int diamond) {
 // running maximum starts at v - 1 and a neighbour wins only if strictly greater.
 template <typename LD>
 __global__ __launch_bounds__(256) void k_maxima_t(LD in, DBuf d, Geom g) {
-    __shared__ float ring[3 * MX_RP];
+    __shared__ float ring[MX_LDS];
     maxima_tile(in, g, blockIdx.x, ring, [&](int64_t i, bool m) { st(d, i, m ? 1.f : 0.f); });
 }
 

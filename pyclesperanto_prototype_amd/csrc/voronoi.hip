@@ -923,7 +923,7 @@ struct F32In {
 // one block per maxima tile (64 x 16 x 16 voxels) of one image (ms = tile-prefix table)
 __global__ __launch_bounds__(256) void k_maxima_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ ms,
                                                      int nimg, const float* F, uint8_t* S) {
-    __shared__ float ring[3 * MX_RP];
+    __shared__ float ring[MX_LDS];
     const int64_t b = blockIdx.x;
     const int j = find_seg(ms, nimg, b);
     const BImg& im = imgs[j];
