@@ -325,6 +325,23 @@ def test_voronoi_otsu_parity(cle, shape, ss, so):
     assert np.array_equal(got, O.voronoi_otsu_labeling(img, ss, so))
 
 
+@pytest.mark.parametrize("shape", [(37, 53), (7, 31, 45)])
+def test_voronoi_otsu_destination_types(cle, shape):
+    # uint32 / int32 destinations, aligned (four labels per store, then the n % 4 tail)
+    # and offset by one element (element by element)
+    import torch
+    img = blobs_image(shape, 8, 21)
+    want = O.voronoi_otsu_labeling(img, 2.0, 1.0)
+    n = int(np.prod(shape))
+    for tdt, ndt in ((torch.int32, np.int32), (torch.uint32, np.uint32)):
+        for shift in (0, 1):
+            buf = torch.zeros(n + 1, dtype=tdt, device="cuda")
+            dst = cle.asarray(buf[shift:shift + n].view(shape))
+            cle.voronoi_otsu_labeling(img, dst, spot_sigma=2.0, outline_sigma=1.0)
+            got = cle.pull(dst)
+            assert got.dtype == ndt and np.array_equal(got.astype(np.int64), want.astype(np.int64)), (ndt, shift)
+
+
 def _pipeline_from_blurs(blur_s, blur_o):
     """The oracle's voronoi_otsu stages downstream of the two blurs."""
     spots = O.detect_maxima_box(blur_s, 0, 0, 0, np.float32)
