@@ -322,26 +322,47 @@ __global__ __launch_bounds__(SB) void k_bm_blocksum(const uint32_t* bm, int64_t 
     }
 }
 
-// exclusive scan of the block sums (single block, chunked with a carry); the
-// grand total goes to *total if requested
+// exclusive scan of the block sums (single block, chunks of SB x 8 sums with a
+// carry): every thread adds its 8 consecutive sums, a wavefront scans the thread sums
+// with shuffles, one LDS word per wavefront joins the wavefronts; the grand total goes
+// to *total if requested
 __global__ __launch_bounds__(SB) void k_scan_bsum(uint32_t* bsum, int64_t nb, uint32_t* total) {
-    __shared__ uint32_t sm[SB];
+    constexpr int PT = 8;
+    __shared__ uint32_t wsum[SB / 64];
+    const int lane = (int)(threadIdx.x & 63), wid = (int)(threadIdx.x >> 6);
     uint32_t carry = 0;
-    for (int64_t base = 0; base < nb; base += SB) {
-        const int64_t i = base + threadIdx.x;
-        const uint32_t v = i < nb ? bsum[i] : 0;
-        sm[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < SB; off <<= 1) {
-            const uint32_t t = threadIdx.x >= off ? sm[threadIdx.x - off] : 0;
-            __syncthreads();
-            sm[threadIdx.x] += t;
-            __syncthreads();
+    for (int64_t base = 0; base < nb; base += (int64_t)SB * PT) {
+        const int64_t i0 = base + (int64_t)threadIdx.x * PT;
+        uint32_t v[PT];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < PT; ++k) {
+            v[k] = i0 + k < nb ? bsum[i0 + k] : 0u;
+            sum += v[k];
         }
-        if (i < nb) bsum[i] = carry + sm[threadIdx.x] - v;
-        const uint32_t chunk = sm[SB - 1];
+        uint32_t x = sum;  // inclusive scan over the wavefront
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wid] = x;
         __syncthreads();
+        uint32_t before = 0, chunk = 0;
+#pragma unroll
+        for (int w = 0; w < SB / 64; ++w) {
+            const uint32_t t = wsum[w];
+            before += w < wid ? t : 0u;
+            chunk += t;
+        }
+        uint32_t run = carry + before + (x - sum);
+#pragma unroll
+        for (int k = 0; k < PT; ++k) {
+            if (i0 + k < nb) bsum[i0 + k] = run;
+            run += v[k];
+        }
         carry += chunk;
+        __syncthreads();  // wsum is rewritten by the next chunk
     }
     if (threadIdx.x == 0 && total) *total = carry;
 }
