@@ -253,12 +253,22 @@ __global__ __launch_bounds__(CT_THREADS) void k_ccl_border(FG fg, Geom g, TileGr
     }
 }
 
+// Kernels over the local roots of every tile: one wavefront per tile, RT_TPB tiles per
+// block (most tiles hold a few roots; a block per tile made these launches
+// dispatch-bound); consecutive tiles stay on one XCD.
+constexpr int RT_TPB = CT_THREADS / 64;
+__device__ __forceinline__ int64_t root_tile() {
+    return xcd_tile(blockIdx.x, gridDim.x) * RT_TPB + (threadIdx.x >> 6);
+}
+static unsigned root_grid(int64_t nt) { return (unsigned)((nt + RT_TPB - 1) / RT_TPB); }
+
 template <typename TS>
 __global__ __launch_bounds__(CT_THREADS) void k_ccl_resolve(uint32_t* L, uint32_t* K, const uint32_t* roots,
-                                                            const uint32_t* nroots) {
-    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+                                                            const uint32_t* nroots, int64_t nt) {
+    const int64_t tile = root_tile();
+    if (tile >= nt) return;
     const uint32_t n = nroots[tile];
-    for (uint32_t k = threadIdx.x; k < n; k += CT_THREADS) {
+    for (uint32_t k = threadIdx.x & 63; k < n; k += 64) {
         const uint32_t lr = roots[tile * TS::ROOTS + k];
         const uint32_t r = find_root(L, lr);
         if (r != lr) {
@@ -271,10 +281,11 @@ __global__ __launch_bounds__(CT_THREADS) void k_ccl_resolve(uint32_t* L, uint32_
 template <typename TS>
 __global__ __launch_bounds__(CT_THREADS) void k_ccl_mark_roots(const uint32_t* L, const uint32_t* K,
                                                                const uint32_t* roots, const uint32_t* nroots,
-                                                               int compat, uint32_t* bm) {
-    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+                                                               int compat, uint32_t* bm, int64_t nt) {
+    const int64_t tile = root_tile();
+    if (tile >= nt) return;
     const uint32_t n = nroots[tile];
-    for (uint32_t k = threadIdx.x; k < n; k += CT_THREADS) {
+    for (uint32_t k = threadIdx.x & 63; k < n; k += 64) {
         const uint32_t lr = roots[tile * TS::ROOTS + k];
         if (L[lr] != lr) continue;  // not a global root
         const uint64_t b = (uint64_t)K[lr] + 1;
@@ -426,10 +437,11 @@ struct RankTable {
 template <typename TS>
 __global__ __launch_bounds__(CT_THREADS) void k_ccl_final(const uint32_t* L, const uint32_t* K, const uint32_t* roots,
                                                           const uint32_t* nroots, int compat, const uint32_t* bm,
-                                                          const uint32_t* prefix, uint32_t* F) {
-    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+                                                          const uint32_t* prefix, uint32_t* F, int64_t nt) {
+    const int64_t tile = root_tile();
+    if (tile >= nt) return;
     const uint32_t n = nroots[tile];
-    for (uint32_t k = threadIdx.x; k < n; k += CT_THREADS) {
+    for (uint32_t k = threadIdx.x & 63; k < n; k += 64) {
         const uint32_t lr = roots[tile * TS::ROOTS + k];
         F[lr] = bit_rank(bm, prefix, key_bit(K[L[lr]], compat));
     }
@@ -547,17 +559,17 @@ static int ccl_run(FG fg, const Geom& g, DBuf dst, int connectivity, int options
     uint32_t *l = Ls.get<uint32_t>(), *k = Ks.get<uint32_t>(), *roots = Rs.get<uint32_t>(), *nr = Ns.get<uint32_t>();
     k_ccl_local<TS><<<(unsigned)nt, CT_THREADS, 0, s>>>(fg, g, T, diamond, l, k, roots, nr);
     k_ccl_border<TS><<<(unsigned)nt, CT_THREADS, 0, s>>>(fg, g, T, diamond, l, nr);
-    k_ccl_resolve<TS><<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr);
+    k_ccl_resolve<TS><<<root_grid(nt), CT_THREADS, 0, s>>>(l, k, roots, nr, nt);
     CLE_LAUNCHED();
     // bit positions run to float32(n) (key n rounded up at most to the next float)
     const uint64_t maxbit = compat ? (uint64_t)(float)(uint64_t)n : (uint64_t)n;
     RankTable rt;
     CLE_TRY(rt.init(maxbit + 1, s));
-    k_ccl_mark_roots<TS><<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>());
+    k_ccl_mark_roots<TS><<<root_grid(nt), CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>(), nt);
     CLE_LAUNCHED();
     CLE_TRY(rt.scan(n_labels_dev, s));
-    k_ccl_final<TS><<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>(),
-                                                        rt.prefix.get<uint32_t>(), Fs.get<uint32_t>());
+    k_ccl_final<TS><<<root_grid(nt), CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>(),
+                                                          rt.prefix.get<uint32_t>(), Fs.get<uint32_t>(), nt);
     launch_ccl_write(fg, l, Fs.get<uint32_t>(), n, dst, lg(n / 4 + 1), s);
     CLE_LAUNCHED();
     return CLE_OK;
@@ -915,17 +927,17 @@ static int bccl_run(FG fg, const Geom& g, DBuf dst, int options, uint32_t* n_lab
     uint32_t *l = Ls.get<uint32_t>(), *k = Ks.get<uint32_t>(), *roots = Rs.get<uint32_t>(), *nr = Ns.get<uint32_t>();
     k_bccl_local<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(fg, g, B, T, Ms.get<uint8_t>(), l, k, roots, nr);
     k_bccl_border<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(B, T, Ms.get<uint8_t>(), l, nr);
-    k_ccl_resolve<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr);
+    k_ccl_resolve<BT><<<root_grid(nt), CT_THREADS, 0, s>>>(l, k, roots, nr, nt);
     CLE_LAUNCHED();
     const int64_t n = g.n();
     const uint64_t maxbit = compat ? (uint64_t)(float)(uint64_t)n : (uint64_t)n;
     RankTable rt;
     CLE_TRY(rt.init(maxbit + 1, s));
-    k_ccl_mark_roots<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>());
+    k_ccl_mark_roots<BT><<<root_grid(nt), CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>(), nt);
     CLE_LAUNCHED();
     CLE_TRY(rt.scan(n_labels_dev, s));
-    k_ccl_final<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>(),
-                                                        rt.prefix.get<uint32_t>(), Fs.get<uint32_t>());
+    k_ccl_final<BT><<<root_grid(nt), CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>(),
+                                                          rt.prefix.get<uint32_t>(), Fs.get<uint32_t>(), nt);
     launch_bccl_write<BT::BZ>(Ms.get<uint8_t>(), g, B, l, Fs.get<uint32_t>(), dst, s);
     CLE_LAUNCHED();
     return CLE_OK;
@@ -1091,7 +1103,7 @@ static int plan_ccl_kind(const std::vector<BImg>& imgs, size_t imgs_off, bool wa
         const int64_t* ts = (const int64_t*)(dev + to);
         k_bccl_local_batch<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(im, ts, nimg, S, M, L, K, roots, nroots);
         k_bccl_border_batch<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(im, ts, nimg, M, L, nroots);
-        k_ccl_resolve<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(L, K, roots, nroots);
+        k_ccl_resolve<BT><<<root_grid(nt), CT_THREADS, 0, s>>>(L, K, roots, nroots, nt);
         k_ccl_mark_roots_batch<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(im, ts, nimg, L, K, roots, nroots,
                                                                       rt->bm.get<uint32_t>());
         CLE_LAUNCHED();
