@@ -55,6 +55,28 @@ def test_batch_equals_single_multi_tile_raster(cle):
         assert int(want.tensor.view(torch.int32).max()) > 10, k
 
 
+def test_batch_many_small_images(cle):
+    # 4100 small images (a block finds its image among them in three rounds of the
+    # 64-ary wavefront search; empty-tile images and 1-voxel rows included)
+    import torch
+    rng = np.random.default_rng(63)
+    imgs, ss, so = [], [], []
+    for k in range(4100):
+        h, w = int(rng.integers(1, 48)), int(rng.integers(1, 48))
+        a = np.zeros((h, w), np.float32)
+        for _ in range(int(rng.integers(0, 3))):
+            y, x = int(rng.integers(0, h)), int(rng.integers(0, w))
+            a[max(0, y - 2):y + 3, max(0, x - 2):x + 3] += float(rng.uniform(50, 200))
+        imgs.append(torch.from_numpy(a))
+        ss.append(float(rng.uniform(0.5, 2.0)))
+        so.append(float(rng.uniform(0.5, 1.5)))
+    dev = [cle.asarray(t.cuda()) for t in imgs]
+    out = cle.voronoi_otsu_labeling_batch(dev, ss, so)
+    for k in range(0, 4100, 7):
+        want = cle.voronoi_otsu_labeling(dev[k], None, ss[k], so[k])
+        assert torch.equal(out[k].tensor, want.tensor), k
+
+
 def test_batch_single_member_and_repeat(cle):
     """A batch of one equals the single call; running a batch twice gives the same."""
     rng = np.random.default_rng(51)
