@@ -39,17 +39,21 @@ def _clamp_shift(a3: np.ndarray, axis: int, v: int) -> np.ndarray:
 
 
 def convert_sat(values: np.ndarray, dtype) -> np.ndarray:
-    """Float -> integer store: saturate, then truncate toward zero.
+    """Float -> integer store: saturate, then truncate toward zero; NaN -> 0.
 
     Reference: clij_convert_<t>_sat (pyclesperanto_prototype/_tier0/preamble.cl:505-591);
     pinned by tests/test_create.py:30-60 of the reference (1.5->1, -7.8->0, 2000->255).
+    The reference tests `value > max` against max as a float, which rounds up to 2^31 /
+    2^32 for the 32-bit types, so exactly 2^31 / 2^32 would reach an out-of-range cast
+    whose result the OpenCL standard leaves to the implementation; here (and on the
+    device) every value from the rounded limit on saturates.
     """
     dtype = np.dtype(dtype)
     v = np.asarray(values, dtype=np.float32)
     if dtype.kind == "f":
         return v.astype(dtype)
     info = np.iinfo(dtype)
-    out = np.where(v > np.float32(info.max), np.float64(info.max),
+    out = np.where(v >= np.float32(info.max), np.float64(info.max),
                    np.where(v < np.float32(info.min), np.float64(info.min), np.trunc(v.astype(np.float64))))
     out = np.nan_to_num(out, nan=0.0)
     return out.astype(dtype)
@@ -102,8 +106,11 @@ def mean_box(src: np.ndarray, radius_x: int = 1, radius_y: int = 1, radius_z: in
     """Separable box mean; radius 0 copies that axis.
 
     Reference: pyclesperanto_prototype/_tier1/_mean_box.py:6-54 (kernel size
-    2r+1 per axis through execute_separable_kernel).  Parity unpinned beyond r=0 (the
-    only radius the hot path uses): restated from the clij mean_separable kernel.
+    2r+1 per axis through execute_separable_kernel; the mean_separable kernel itself is
+    absent from the reference tree).  Pinned by tests/test_mean_box.py:5-29 of the
+    reference (radius 1, 1, 0) and by a float64 brute-force window mean with
+    clamp-to-edge reads at several radii (tests/test_oracle_pins.py); the edge rule is
+    the clamp of every separable kernel's reads (_tier0/preamble.cl:33-49).
     """
     ndim = src.ndim
     a = _as3d(np.asarray(src, np.float32))
@@ -454,6 +461,29 @@ def extend_labeling_via_voronoi(labels: np.ndarray, out_dtype=np.uint32, return_
     last = flip if it % 2 == 0 else flop
     out = convert_sat(last, out_dtype)
     return (out, it) if return_iterations else out
+
+
+def dilate_labels(labels: np.ndarray, radius: int = 2, out_dtype=np.uint32) -> np.ndarray:
+    """At most `radius` alternating box (even) / diamond (odd) growth steps on float32
+    buffers, stopping after a step that changes nothing; the last written buffer is
+    copied (saturating) into the destination type.
+
+    Reference: pyclesperanto_prototype/_tier4/_dilate_labels.py:41-61 (exported as
+    extend_labels_with_maximum_radius, _tier4/__init__.py:18).  Pinned by
+    tests/test_dilate_labels.py:4-75 of the reference (2D, and 3D across z).
+    """
+    ndim = labels.ndim
+    flip = np.asarray(labels).astype(np.float32)
+    flop = flip
+    it = 0
+    flag = True
+    while flag and it < radius:
+        if it % 2 == 0:
+            flop, flag = onlyzero_overwrite_maximum(flip, "box", ndim)
+        else:
+            flip, flag = onlyzero_overwrite_maximum(flop, "diamond", ndim)
+        it += 1
+    return convert_sat(flip if it % 2 == 0 else flop, out_dtype)
 
 
 def voronoi_labeling(binary: np.ndarray, out_dtype=np.uint32) -> np.ndarray:

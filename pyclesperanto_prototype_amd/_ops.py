@@ -265,6 +265,32 @@ def extend_labeling_via_voronoi(labeling_source: Image, labeling_destination: Im
     return labeling_destination
 
 
+@plugin_function(categories=['label processing', 'in assistant', 'bia-bob-suggestion'],
+                 output_creator=create_labels_like)
+def dilate_labels(labeling_source: Image, labeling_destination: Image = None, radius: int = 2) -> Image:
+    """_tier4/_dilate_labels.py:12-61: at most `radius` alternating box / diamond
+    growth steps on float32 buffers, stopping after a step that changes nothing."""
+    flip = create_like(labeling_destination)
+    flop = create_like(labeling_destination)
+    flag = create((1, 1, 1))
+    copy(labeling_source, flip)
+    it = 0
+    changed = True
+    while changed and it < radius:
+        set(flag, 0)
+        if it % 2 == 0:
+            onlyzero_overwrite_maximum_box(flip, flag, flop)
+        else:
+            onlyzero_overwrite_maximum_diamond(flop, flag, flip)
+        changed = float(pull(flag).ravel()[0]) > 0
+        it += 1
+    copy(flip if it % 2 == 0 else flop, labeling_destination)
+    return labeling_destination
+
+
+extend_labels_with_maximum_radius = dilate_labels
+
+
 @plugin_function(categories=['label', 'in assistant', 'bia-bob-suggestion'], output_creator=create_labels_like)
 def voronoi_labeling(binary_source: Image, labeling_destination: Image = None) -> Image:
     """_tier5/_voronoi_labeling.py:7-39: CCL box into a float32 image, then

@@ -3,6 +3,7 @@
 // and the stream-ordered scratch pool.
 #pragma once
 
+#include <limits>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -129,12 +130,17 @@ inline unsigned grid_for(size_t n, unsigned block) {
 
 // ----------------------------------------------------------------- typed access
 // Saturating float -> integer conversion, then truncation toward zero
-// (pyclesperanto_prototype/_tier0/preamble.cl:505-591).
-template <typename T> __device__ __forceinline__ T sat_cast(float v, float lo, float hi) {
-    if (v > hi) return (T)hi;
-    if (v < lo) return (T)lo;
+// (pyclesperanto_prototype/_tier0/preamble.cl:505-591).  The type's largest value as a
+// float rounds up to 2^31 / 2^32 for the 32-bit types, so values from there on (where
+// the reference's `value > max` test falls through to an out-of-range cast) saturate
+// too; the limits are returned as integers, never cast from a float.  NaN -> 0.
+template <typename T> __device__ __forceinline__ T sat_cast(float v) {
+    constexpr T tmax = std::numeric_limits<T>::max(), tmin = std::numeric_limits<T>::lowest();
+    constexpr float fhi = (float)tmax, flo = (float)tmin;
+    if (v >= fhi) return tmax;
+    if (v < flo) return tmin;
     if (v != v) return (T)0;
-    return (T)v;
+    return (T)v;  // flo <= v < fhi: inside T's range
 }
 
 __device__ __forceinline__ float ld(const DBuf b, int64_t i) {
@@ -153,12 +159,12 @@ __device__ __forceinline__ float ld(const DBuf b, int64_t i) {
 __device__ __forceinline__ void st(DBuf b, int64_t i, float v) {
     switch (b.dt) {
         case CLE_F32: ((float*)b.p)[i] = v; break;
-        case CLE_U8: ((uint8_t*)b.p)[i] = sat_cast<uint8_t>(v, 0.f, 255.f); break;
-        case CLE_U16: ((uint16_t*)b.p)[i] = sat_cast<uint16_t>(v, 0.f, 65535.f); break;
-        case CLE_U32: ((uint32_t*)b.p)[i] = sat_cast<uint32_t>(v, 0.f, 4294967295.f); break;
-        case CLE_I32: ((int32_t*)b.p)[i] = sat_cast<int32_t>(v, -2147483648.f, 2147483647.f); break;
-        case CLE_I8: ((int8_t*)b.p)[i] = sat_cast<int8_t>(v, -128.f, 127.f); break;
-        case CLE_I16: ((int16_t*)b.p)[i] = sat_cast<int16_t>(v, -32768.f, 32767.f); break;
+        case CLE_U8: ((uint8_t*)b.p)[i] = sat_cast<uint8_t>(v); break;
+        case CLE_U16: ((uint16_t*)b.p)[i] = sat_cast<uint16_t>(v); break;
+        case CLE_U32: ((uint32_t*)b.p)[i] = sat_cast<uint32_t>(v); break;
+        case CLE_I32: ((int32_t*)b.p)[i] = sat_cast<int32_t>(v); break;
+        case CLE_I8: ((int8_t*)b.p)[i] = sat_cast<int8_t>(v); break;
+        case CLE_I16: ((int16_t*)b.p)[i] = sat_cast<int16_t>(v); break;
         default: break;
     }
 }

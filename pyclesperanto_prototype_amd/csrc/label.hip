@@ -450,7 +450,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_ccl_final(const uint32_t* L, con
 // store4 writes voxels i..i+3 with one 16-byte store (i a multiple of 4, p 16-byte aligned)
 struct StLabI32 {
     int32_t* p;
-    __device__ __forceinline__ static int32_t cv(float v) { return v > 2147483647.f ? 2147483647 : (int32_t)v; }
+    __device__ __forceinline__ static int32_t cv(float v) { return sat_cast<int32_t>(v); }
     __device__ __forceinline__ void operator()(int64_t i, float v) const { p[i] = cv(v); }
     __device__ __forceinline__ void store4(int64_t i, float a, float b, float c, float d) const {
         *(int4*)(p + i) = make_int4(cv(a), cv(b), cv(c), cv(d));
@@ -458,7 +458,7 @@ struct StLabI32 {
 };
 struct StLabU32 {
     uint32_t* p;
-    __device__ __forceinline__ static uint32_t cv(float v) { return v > 4294967295.f ? 4294967295u : (uint32_t)v; }
+    __device__ __forceinline__ static uint32_t cv(float v) { return sat_cast<uint32_t>(v); }
     __device__ __forceinline__ void operator()(int64_t i, float v) const { p[i] = cv(v); }
     __device__ __forceinline__ void store4(int64_t i, float a, float b, float c, float d) const {
         *(uint4*)(p + i) = make_uint4(cv(a), cv(b), cv(c), cv(d));

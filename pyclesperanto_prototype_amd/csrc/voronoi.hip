@@ -847,8 +847,8 @@ __global__ void k_mask_out_u8_32(const float* __restrict__ A, const uint8_t* __r
         uint32_t ov[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-            ov[c] = u ? sat_cast<uint32_t>(av[c], 0.f, 4294967295.f)
-                      : (uint32_t)sat_cast<int32_t>(av[c], -2147483648.f, 2147483647.f);
+            ov[c] = u ? sat_cast<uint32_t>(av[c])
+                      : (uint32_t)sat_cast<int32_t>(av[c]);
         ((uint4*)dst.p)[q] = make_uint4(ov[0], ov[1], ov[2], ov[3]);
     }
     for (int64_t i = nv * 4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1009,8 +1009,8 @@ __global__ __launch_bounds__(256) void k_mask_out_batch(const BImg* __restrict__
                 uint32_t ov[4];
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
-                    ov[c] = u ? sat_cast<uint32_t>(av[c], 0.f, 4294967295.f)
-                              : (uint32_t)sat_cast<int32_t>(av[c], -2147483648.f, 2147483647.f);
+                    ov[c] = u ? sat_cast<uint32_t>(av[c])
+                              : (uint32_t)sat_cast<int32_t>(av[c]);
                 *(uint4*)(o + i) = make_uint4(ov[0], ov[1], ov[2], ov[3]);
             } else {
                 for (int c = 0; c < 4 && i + c < n; ++c) st(d, i + c, av[c]);

@@ -38,6 +38,7 @@ class Request:
     spot_sigma: float
     outline_sigma: float
     ready: object = None  # event recorded on the submitting thread's stream
+    client_stream: object = None  # the submitting thread's stream (where the labels will be used)
     arrival: float = 0.0
     admitted: float = 0.0
     finished: float = 0.0
@@ -95,9 +96,10 @@ class SegmentationServer:
         import torch
 
         img = image if isinstance(image, Array) else asarray(image)
+        client = torch.cuda.current_stream()
         ready = torch.cuda.Event()
-        ready.record()  # the worker's stream waits for the image to be written
-        req = Request(img, float(spot_sigma), float(outline_sigma), ready, arrival=time.perf_counter())
+        ready.record(client)  # the worker's stream waits for the image to be written
+        req = Request(img, float(spot_sigma), float(outline_sigma), ready, client, arrival=time.perf_counter())
         if req.nbytes > self.budget:
             req.error = MemoryError(f"request of {req.nbytes} bytes exceeds the server budget {self.budget}")
             req.finished = req.arrival
@@ -171,6 +173,12 @@ class SegmentationServer:
                     for r in batch:
                         stream.wait_event(r.ready)
                     outs = [create(r.image.shape, np.uint32) for r in batch]
+                    # the labels come from the worker stream's pool but are used on the
+                    # client's stream: when the client frees them, the allocator must
+                    # wait for the work the client queued on them before reusing the
+                    # block for another batch
+                    for r, o in zip(batch, outs):
+                        o.tensor.record_stream(r.client_stream)
                     voronoi_otsu_labeling_batch([r.image for r in batch], [r.spot_sigma for r in batch],
                                                 [r.outline_sigma for r in batch], destinations=outs)
                     stream.synchronize()

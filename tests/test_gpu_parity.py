@@ -156,11 +156,44 @@ def test_blur_typed_source(cle, dtype):
     assert_blur_close(cle.pull(dst), O.gaussian_blur(a.astype(np.float32), 2, 2, 2))
 
 
-def test_mean_box_parity(cle):
+@pytest.mark.parametrize("shape,rad", [((7, 20, 30), (1, 2, 1)), ((5, 17, 23), (2, 1, 1)), ((4, 9, 40), (0, 3, 2)),
+                                       ((33, 45), (3, 0, 1)), ((19, 28), (1, 1, 0))])
+def test_mean_box_parity(cle, shape, rad):
     rng = np.random.default_rng(4)
-    a = (rng.random((7, 20, 30)) * 100).astype(np.float32)
-    got = cle.pull(cle.mean_box(a, None, 1, 2, 1))
-    assert_blur_close(got, O.mean_box(a, 1, 2, 1))
+    a = (rng.random(shape) * 100).astype(np.float32)
+    got = cle.pull(cle.mean_box(a, None, *rad))
+    assert_blur_close(got, O.mean_box(a, *rad))
+
+
+def test_mean_box_reference(cle, ref_tests):
+    t = ref_tests["mean_box"]
+    src = cle.push(np.asarray(t["input"]))
+    res = cle.create(src)
+    cle.mean_box(src, res, *t["radius"])
+    assert np.allclose(cle.pull(res), np.asarray(t["expected"]))
+
+
+@pytest.mark.parametrize("key", ["dilate_labels_2d", "dilate_labels_3d"])
+def test_dilate_labels_reference(cle, ref_tests, key):
+    # tests/test_dilate_labels.py of the reference: radius 1 is one box step, and in 3D
+    # it fixes the 27-voxel neighbourhood across z
+    t = ref_tests[key]
+    out = cle.extend_labels_with_maximum_radius(np.asarray(t["input"]), radius=1)
+    assert out.dtype == np.uint32 and np.array_equal(cle.pull(out), np.asarray(t["expected"]))
+    flag = cle.create((1, 1, 1))
+    cle.set(flag, 0)
+    res = cle.onlyzero_overwrite_maximum_box(np.asarray(t["input"]), flag)
+    assert np.array_equal(cle.pull(res), np.asarray(t["expected"])) and cle.pull(flag).ravel()[0] == 1
+
+
+@pytest.mark.parametrize("shape,radius", [((30, 40), 1), ((30, 40), 4), ((9, 20, 25), 2), ((9, 20, 25), 5)])
+def test_dilate_labels_parity(cle, shape, radius):
+    rng = np.random.default_rng(16)
+    lab = np.zeros(shape, np.uint32)
+    idx = rng.choice(lab.size, max(2, lab.size // 120), replace=False)
+    lab.flat[idx] = rng.integers(1, 5000, idx.size)
+    got = cle.pull(cle.dilate_labels(lab, None, radius))
+    assert got.dtype == np.uint32 and np.array_equal(got, O.dilate_labels(lab, radius))
 
 
 @pytest.mark.parametrize("shape", [(31, 45), (6, 17, 23), (1, 9, 9), (3, 5, 300), (1, 1), (700, 3)])
@@ -200,6 +233,20 @@ def test_otsu_tied_variances(cle, levels):
     out, t = O.threshold_otsu(a, return_threshold=True)
     assert cle.otsu_threshold(a) == t
     assert np.array_equal(cle.pull(cle.threshold_otsu(a)), out)
+
+
+def test_saturating_stores_at_the_limits(cle):
+    # float -> integer stores at and beyond the 32-bit limits (2^31 and 2^32 are the
+    # limits rounded to float32), +-inf and NaN, against the oracle's convert_sat
+    v = np.array([2.0 ** 31, 2.0 ** 32, 3e9, -3e9, np.inf, -np.inf, np.nan, 4294967040.0, 2147483520.0,
+                  -2147483648.0, 1.5, -1.5, 255.0, 256.0, -129.0, 65535.5, 65536.0], np.float32)
+    src = cle.push(v.reshape(1, -1))
+    for dt in (np.uint32, np.int32, np.uint16, np.int16, np.uint8, np.int8):
+        d = cle.create(src.shape, dt)
+        cle.copy(src, d)
+        assert np.array_equal(cle.pull(d).ravel(), O.convert_sat(v, dt)), dt
+        cle.add_image_and_scalar(src, d, 0)
+        assert np.array_equal(cle.pull(d).ravel(), O.convert_sat(v, dt)), dt
 
 
 def test_elementwise_dtypes(cle):

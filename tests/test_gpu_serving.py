@@ -40,3 +40,34 @@ def test_server_rejects_oversized_request(cle):
     r = srv.submit(cle.create((64, 64), np.float32), 2.0, 1.0)
     with pytest.raises(MemoryError):
         r.result(timeout=1)
+
+
+def test_server_labels_survive_queued_client_reads(cle):
+    # The client frees its labels while a read of them is still queued on its own
+    # stream (behind other work), then submits again: the next batch must not be given
+    # that memory before the queued read has run.
+    import torch
+    from pyclesperanto_prototype_amd import serving, synthetic
+
+    a = cle.asarray(synthetic.nuclei_overlapping((256, 256), 5e-4, seed=90)[0])
+    b = cle.asarray(synthetic.nuclei_overlapping((256, 256), 5e-4, seed=91)[0])
+    want_a = cle.voronoi_otsu_labeling(a, None, 2.0, 1.0).tensor.clone()
+    want_b = cle.voronoi_otsu_labeling(b, None, 2.0, 1.0).tensor.clone()
+    assert not torch.equal(want_a, want_b)
+    busy = torch.ones(1 << 28, device="cuda")
+    torch.cuda.synchronize()
+    client = torch.cuda.Stream()
+    with serving.SegmentationServer(workers=1) as srv:
+        with torch.cuda.stream(client):
+            r = srv.submit(a, 2.0, 1.0)
+            lab = r.result()
+            r.labels = None
+            for _ in range(100):
+                busy.mul_(1.0)           # client work queued ahead of the read
+            copy = lab.tensor.clone()    # the read of the labels, queued behind it
+            del lab
+            r2 = srv.submit(b, 2.0, 1.0)
+            got_b = r2.result().tensor.clone()
+        client.synchronize()
+    assert torch.equal(copy, want_a)
+    assert torch.equal(got_b, want_b)

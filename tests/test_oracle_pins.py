@@ -77,6 +77,46 @@ def test_sigma_to_kernel_size():
     assert [O.sigma_to_kernel_size(s) for s in (0, 0.5, 1, 1.5, 2, 3.5, 5)] == [1, 5, 9, 13, 17, 29, 41]
 
 
+# ----------------------------------------------------------------- mean_box
+def test_mean_box_reference(ref_tests):
+    t = ref_tests["mean_box"]
+    out = O.mean_box(A(t["input"], np.float32), *t["radius"])
+    assert np.allclose(out, A(t["expected"]))
+
+
+def _brute_mean_box(a, rad):
+    """Window mean with literal loops in float64: per axis the window of 2r+1 reads
+    clamped to the image (the separable kernels' clamp-to-edge reads)."""
+    a3 = a[None] if a.ndim == 2 else a
+    D, H, W = a3.shape
+    rx, ry, rz = rad
+    if a.ndim == 2:
+        rz = 0
+    out = np.zeros(a3.shape)
+    for z in range(D):
+        for y in range(H):
+            for x in range(W):
+                s, n = 0.0, 0
+                for dz in range(-rz, rz + 1):
+                    for dy in range(-ry, ry + 1):
+                        for dx in range(-rx, rx + 1):
+                            s += float(a3[min(max(z + dz, 0), D - 1), min(max(y + dy, 0), H - 1),
+                                          min(max(x + dx, 0), W - 1)])
+                            n += 1
+                out[z, y, x] = s / n
+    return out[0] if a.ndim == 2 else out
+
+
+@pytest.mark.parametrize("shape,rad", [((9, 11), (1, 2, 0)), ((5, 7, 9), (2, 1, 1)), ((4, 6, 3), (0, 3, 2)),
+                                       ((7, 5), (3, 0, 1)), ((3, 4, 8), (1, 1, 1))])
+def test_mean_box_vs_float64_bruteforce(shape, rad):
+    # random values up to the border: the clamp-to-edge window, the 2r+1 divisor and the
+    # radius-0 axes are all visible
+    rng = np.random.default_rng(sum(shape) + sum(rad))
+    a = (rng.random(shape) * 100).astype(np.float32)
+    np.testing.assert_allclose(O.mean_box(a, *rad), _brute_mean_box(a, rad), rtol=1e-5, atol=1e-5)
+
+
 # ----------------------------------------------------------------- maxima
 def test_detect_maxima_reference(ref_tests):
     t = ref_tests["detect_maxima_box"]
@@ -237,6 +277,28 @@ def test_onlyzero_reference(ref_tests, kind):
     t = ref_tests["onlyzero_overwrite_maximum_" + kind]
     out, _ = O.onlyzero_overwrite_maximum(A(t["input"], np.float32), kind)
     assert np.array_equal(out, A(t["expected"]))
+
+
+@pytest.mark.parametrize("key", ["dilate_labels_2d", "dilate_labels_3d"])
+def test_dilate_labels_reference(ref_tests, key):
+    # radius 1 = one box step: the 3D case fixes the 27-voxel box neighbourhood across z
+    t = ref_tests[key]
+    out, changed = O.onlyzero_overwrite_maximum(A(t["input"], np.float32), "box")
+    assert changed and np.array_equal(out, A(t["expected"]))
+    assert np.array_equal(O.dilate_labels(A(t["input"]), radius=1), A(t["expected"]))
+
+
+def test_dilate_labels_radius_bounds_growth():
+    # radius r grows a single seed to the alternating box/diamond ball of r steps; r
+    # beyond convergence changes nothing more
+    a = np.zeros((5, 9, 9), np.uint32)
+    a[2, 4, 4] = 7
+    one = O.dilate_labels(a, 1)
+    assert (one == 7).sum() == 27
+    two = O.dilate_labels(a, 2)
+    # box then diamond: the 3x3x3 cube plus its 6 face layers of 3x3 voxels
+    assert (two == 7).sum() == 27 + 6 * 9
+    assert np.array_equal(O.dilate_labels(a, 50), O.extend_labeling_via_voronoi(a))
 
 
 def test_masked_voronoi_reference(ref_tests):
