@@ -20,7 +20,7 @@ from ._ops import (STATISTICS_KEYS, add_image_and_scalar, autotune_gaussian_blur
                    masked_voronoi_labeling, maximum_of_all_pixels, mean_box, minimum_of_all_pixels, release_pool,
                    nonzero_minimum_box, nonzero_minimum_diamond, onlyzero_overwrite_maximum_box,
                    onlyzero_overwrite_maximum_diamond, otsu_threshold, relabel_sequential, set, set_autotune,
-                   set_fp32_key_compat, set_non_zero_pixels_to_pixel_index, set_nonzero_pixels_to_pixelindex,
+                   set_fp32_key_compat, set_growth_method, set_non_zero_pixels_to_pixel_index, set_nonzero_pixels_to_pixelindex,
                    statistics_of_background_and_labelled_pixels, statistics_of_labeled_pixels,
                    statistics_of_labelled_pixels, statistics_table, sum_of_all_pixels, threshold_otsu,
                    voronoi_labeling, voronoi_otsu_labeling, voronoi_otsu_labeling_batch,

@@ -510,6 +510,12 @@ def set_autotune(on: bool = True):
     _lib.call("cle_set_autotune", 1 if on else 0)
 
 
+def set_growth_method(method: str = "tiles"):
+    """Voronoi growth method (results are identical): "tiles" (default, asynchronous
+    exact tile solves) or "steps" (image-wide pairs of steps)."""
+    _lib.call("cle_set_growth_method", {"tiles": 0, "steps": 1}[method])
+
+
 def autotune_gaussian_blur(source, destination=None, sigma_x: float = 0, sigma_y: float = 0, sigma_z: float = 0):
     """Tune the isotropic blur's z chunking for this configuration ahead of time (the
     call synchronises); later gaussian_blur calls of the same class use the choice.

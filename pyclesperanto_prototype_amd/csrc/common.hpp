@@ -302,6 +302,9 @@ struct BImg {
     int64_t tbase;      // voronoi: first tile of the image
     int vnx, vny, vnz;  // voronoi: tile counts along x, y, z
     int pad2;
+    int64_t wbase;      // tile growth: first tile of the image
+    int wnx, wny, wnz;  // tile growth: tile counts along x, y, z
+    int pad3;
 };
 
 // Host-side staging of a batch's device tables: appended while the stages are

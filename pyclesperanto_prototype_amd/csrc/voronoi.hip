@@ -41,7 +41,9 @@
 // (No grid-wide "last block" ticket: the device-scope fence it needs writes back the
 // L2 in every block.)
 #include "common.hpp"
+#include "growth.hpp"
 
+#include <algorithm>
 #include <cstdlib>
 #include <mutex>
 
@@ -887,12 +889,24 @@ static unsigned vg(int64_t n) {
     return g > 131072 ? 131072 : g;
 }
 
+// Growth method: the asynchronous tile solves (growth.hip), or the step-synchronous
+// pairs of steps below (also the fallback for growths beyond the tile solves' step range).
+static int g_growth_method = 0;  // 0: tile solves, 1: pairs of steps
+
 // labels of `binary` grown inside `mask`, written (masked) to dst.  `B` holds
 // scratch for the CCL labels (float32, n voxels).
-static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, float* A, float* B,
-                               int* n_iterations, hipStream_t s) {
+static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, float* B, int* n_iterations,
+                               hipStream_t s) {
     const int64_t n = g.n();
     CLE_TRY(ccl_impl(binary, g, DBuf{B, CLE_F32}, CLE_CONN_BOX, CLE_FP32_KEY_COMPAT, nullptr, s));
+    if (g_growth_method == 0) {
+        bool over = false;
+        CLE_TRY(grow_masked(mask, B, dst, g, n_iterations, &over, s));
+        if (!over) return CLE_OK;
+    }
+    Scratch a;
+    CLE_TRY(a.alloc((size_t)n * 4, s));
+    float* A = a.get<float>();
     const float* res = nullptr;
     // a uint8 mask and 16-byte aligned buffers take the four-voxel init
     if (mask.dt == CLE_U8 && ((uintptr_t)mask.p & 3) == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0)
@@ -1094,7 +1108,10 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
                               int* n_steps, hipStream_t s, BatchCapture* cap = nullptr) {
     if (n > VOR_MAX_IMAGES) return fail(CLE_E_VALUE, "at most %d images per batch (got %d)", VOR_MAX_IMAGES, n);
     std::vector<BImg> imgs((size_t)n);
-    int64_t total = 0, ntiles = 0;
+    int64_t total = 0, ntiles = 0, wtiles = 0;
+    // the growth: tile solves, or pairs of steps (a captured graph runs a fixed number
+    // of pairs without host round trips; also the fallback)
+    const bool pairs = cap != nullptr || g_growth_method != 0;
     for (int j = 0; j < n; ++j) {
         Geom g, gd;
         CLE_TRY(check_buf(&srcs[j], "source", &g));
@@ -1113,6 +1130,11 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
         im.tbase = ntiles;
         CLE_TRY(vor_set_tiles(im));
         ntiles += g.n() > 0 ? make_tiles(g).count() : 0;
+        grow_tiling(im);
+        if (im.wnx > 65535 || im.wny > 65535 || im.wnz > 65535)
+            return fail(CLE_E_VALUE, "image %d too large for the growth tiling", j);
+        im.wbase = wtiles;
+        wtiles += grow_tile_count(im);
     }
     if (total == 0) return CLE_OK;
     CclBatchSizes csz;
@@ -1121,7 +1143,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     // buffer A), S (uint8 seeds), G (uint8 segmentation)
     Scratch F, A, S, G, aux;
     CLE_TRY(F.alloc((size_t)total * 4, s));
-    CLE_TRY(A.alloc((size_t)total * 4, s));
+    if (pairs) CLE_TRY(A.alloc((size_t)total * 4, s));
     CLE_TRY(S.alloc((size_t)total, s));
     CLE_TRY(G.alloc((size_t)total, s));
     const size_t otsu_words = (size_t)n * (2 + 256 + 256);
@@ -1130,7 +1152,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     double* thr = (double*)(aux.get<char>() + (size_t)n * 8);
     uint32_t* ou = (uint32_t*)(aux.get<char>() + (size_t)n * 16);
     VorScratch V;
-    CLE_TRY(V.alloc(ntiles, n, s));
+    if (pairs) CLE_TRY(V.alloc(ntiles, n, s));
     const VorLists VL = V.L;
     float* Fp = F.get<float>();
     float* Ap = A.get<float>();
@@ -1154,6 +1176,10 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     }
     mst[n] = nmt;
     const size_t ms_off = blob.put_vec(mst);
+    std::vector<int64_t> wst((size_t)n + 1);
+    for (int j = 0; j < n; ++j) wst[j] = imgs[j].wbase;
+    wst[n] = wtiles;
+    const size_t ws_off = blob.put_vec(wst);
 
     std::vector<Launch> L;
     std::vector<std::shared_ptr<void>> keep;
@@ -1181,7 +1207,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     });
     // seed labels (float32, into F), then A = F = (segmentation - 1) + labels
     CLE_TRY(plan_ccl_batch(imgs, imgs_off, csz, Sp, Fp, blob, L, keep, s));
-    if (ntiles > 0)
+    if (ntiles > 0 && pairs)
         L.push_back([=](const char* dev) -> int {
             k_vor_init_batch<<<(unsigned)ntiles, VB, 0, s>>>((const BImg*)(dev + imgs_off),
                                                              (const int64_t*)(dev + ts_off), n, Gp, Ap, Fp, VL);
@@ -1219,9 +1245,28 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
             CLE_LAUNCHED();
             CLE_HIP(hipMemcpyAsync(cap->word_host, V.S.word, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         }
-    } else {
+    } else if (pairs) {
         int steps = 0;
         if (ntiles > 0) CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, n == 1, 0, &steps, s));
+        if (n_steps) *n_steps = steps;
+    } else {
+        // tile solves: the labels go straight to the destinations
+        bool over = false;
+        std::vector<int> its((size_t)n, 1);
+        CLE_TRY(grow_batch(dimgs, (const int64_t*)(dev + ws_off), n, wtiles, Gp, Fp, its.data(), &over, s));
+        if (!over) {
+            if (n_steps) *n_steps = *std::max_element(its.begin(), its.end());
+            CLE_HIP(hipStreamSynchronize(s));
+            return CLE_OK;
+        }
+        // a growth beyond the tile solves' step range: pairs of steps
+        CLE_TRY(A.alloc((size_t)total * 4, s));
+        CLE_TRY(V.alloc(ntiles, n, s));
+        Ap = A.get<float>();
+        k_vor_init_batch<<<(unsigned)ntiles, VB, 0, s>>>(dimgs, (const int64_t*)(dev + ts_off), n, Gp, Ap, Fp, V.L);
+        CLE_LAUNCHED();
+        int steps = 0;
+        CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, n == 1, 0, &steps, s));
         if (n_steps) *n_steps = steps;
     }
     k_mask_out_batch<<<(unsigned)vblocks, 256, 0, s>>>(dimgs, dvs, n, Ap, Fp, V.S.iters, Gp);
@@ -1344,6 +1389,12 @@ int cle_plan_destroy(cle_plan* P) {
     return CLE_OK;
 }
 
+int cle_set_growth_method(int method) {
+    if (method != 0 && method != 1) return fail(CLE_E_VALUE, "growth method must be 0 or 1 (got %d)", method);
+    g_growth_method = method;
+    return CLE_OK;
+}
+
 int cle_masked_voronoi_labeling(const cle_buf* binary, const cle_buf* mask, const cle_buf* dst, int* n_iterations,
                                 void* stream) {
     Geom g, gm, gd;
@@ -1355,10 +1406,9 @@ int cle_masked_voronoi_labeling(const cle_buf* binary, const cle_buf* mask, cons
     hipStream_t s = (hipStream_t)stream;
     const int64_t n = g.n();
     if (!n) return CLE_OK;
-    Scratch a, b;
-    CLE_TRY(a.alloc((size_t)n * 4, s));
+    Scratch b;
     CLE_TRY(b.alloc((size_t)n * 4, s));
-    return masked_voronoi_core(dbuf(binary), dbuf(mask), dbuf(dst), g, a.get<float>(), b.get<float>(), n_iterations, s);
+    return masked_voronoi_core(dbuf(binary), dbuf(mask), dbuf(dst), g, b.get<float>(), n_iterations, s);
 }
 
 int cle_extend_labeling_via_voronoi(const cle_buf* src, const cle_buf* dst, int* n_iterations, void* stream) {
@@ -1369,6 +1419,11 @@ int cle_extend_labeling_via_voronoi(const cle_buf* src, const cle_buf* dst, int*
     hipStream_t s = (hipStream_t)stream;
     const int64_t n = g.n();
     if (!n) return CLE_OK;
+    if (g_growth_method == 0) {
+        bool over = false;
+        CLE_TRY(grow_labels(dbuf(src), dbuf(dst), g, n_iterations, &over, s));
+        if (!over) return CLE_OK;
+    }
     Scratch a, b;
     CLE_TRY(a.alloc((size_t)n * 4, s));
     CLE_TRY(b.alloc((size_t)n * 4, s));
@@ -1388,7 +1443,7 @@ int cle_voronoi_otsu_labeling(const cle_buf* src, const cle_buf* dst, float spot
     hipStream_t s = (hipStream_t)stream;
     const int64_t n = g.n();
     if (!n) return CLE_OK;
-    Scratch blurred, spots, seg, a, aux;
+    Scratch blurred, spots, seg, aux;
     CLE_TRY(blurred.alloc((size_t)n * 4, s));
     CLE_TRY(spots.alloc((size_t)n, s));
     CLE_TRY(seg.alloc((size_t)n, s));
@@ -1408,11 +1463,10 @@ int cle_voronoi_otsu_labeling(const cle_buf* src, const cle_buf* dst, float spot
     CLE_TRY(otsu_impl(DBuf{blurred.p, CLE_F32}, g, mm, thr, s));
     k_seg_and<<<vg(n / 4 + 1), 256, 0, s>>>(blurred.get<float>(), thr, spots.get<uint8_t>(), seg.get<uint8_t>(), n);
     CLE_LAUNCHED();
-    // masked Voronoi of the seeds inside the segmentation; the blurred buffer is
-    // reused as the second growth buffer
-    CLE_TRY(a.alloc((size_t)n * 4, s));
-    return masked_voronoi_core(DBuf{spots.p, CLE_U8}, DBuf{seg.p, CLE_U8}, dbuf(dst), g, a.get<float>(),
-                               blurred.get<float>(), nullptr, s);
+    // masked Voronoi of the seeds inside the segmentation; the seed labels go to the
+    // blurred buffer
+    return masked_voronoi_core(DBuf{spots.p, CLE_U8}, DBuf{seg.p, CLE_U8}, dbuf(dst), g, blurred.get<float>(), nullptr,
+                               s);
 }
 
 int cle_voronoi_otsu_labeling_batch(const cle_buf* srcs, const cle_buf* dsts, const float* spot_sigmas,
