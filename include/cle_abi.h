@@ -152,10 +152,9 @@ int cle_relabel_sequential(const cle_buf* src, const cle_buf* dst, void* stream)
  * 1.. in raster (z, y, x) order */
 int cle_label_spots(const cle_buf* src, const cle_buf* dst, void* stream);
 /* _tier5/_masked_voronoi_labeling.py:7-76: seeds labelled by CCL box, grown inside
- * the mask by alternating box / diamond steps until no change.  The result is computed
- * by exact tile solves that synchronise a few times per call (see
- * cle_set_growth_method).  If n_iterations is not NULL the reference's iteration count
- * is stored there (host). */
+ * the mask by alternating box / diamond steps until no change.  Synchronises once per
+ * batch of growth steps (see cle_set_growth_method).  If n_iterations is not NULL the
+ * reference's iteration count is stored there (host). */
 int cle_masked_voronoi_labeling(const cle_buf* binary, const cle_buf* mask, const cle_buf* dst,
                                 int* n_iterations, void* stream);
 /* _tier4/_extend_labeling_via_voronoi.py:12-61: the labels (converted to float32)
@@ -163,9 +162,9 @@ int cle_masked_voronoi_labeling(const cle_buf* binary, const cle_buf* mask, cons
  * then stored in dst's type.  No mask.  n_iterations as above. */
 int cle_extend_labeling_via_voronoi(const cle_buf* src, const cle_buf* dst, int* n_iterations, void* stream);
 /* Growth method of the Voronoi ops (no counterpart in the reference; results do not
- * depend on it): 0 (default) asynchronous exact tile solves, 1 image-wide pairs of
- * steps (what the tile solves fall back to for growths of 254 steps or more, and what
- * a captured batch plan records). */
+ * depend on it): 1 (default) image-wide pairs of steps, 0 asynchronous exact tile solves
+ * (which fall back to the pairs for growths of 254 steps or more; a captured batch plan
+ * always records the pairs). */
 int cle_set_growth_method(int method);
 /* _tier9/_voronoi_otsu_labeling.py:11-73: blur(spot) -> maxima -> blur(outline) ->
  * Otsu -> AND -> masked Voronoi -> mask.  Synchronises only inside the Voronoi

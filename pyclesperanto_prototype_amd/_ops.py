@@ -510,9 +510,9 @@ def set_autotune(on: bool = True):
     _lib.call("cle_set_autotune", 1 if on else 0)
 
 
-def set_growth_method(method: str = "tiles"):
-    """Voronoi growth method (results are identical): "tiles" (default, asynchronous
-    exact tile solves) or "steps" (image-wide pairs of steps)."""
+def set_growth_method(method: str = "steps"):
+    """Voronoi growth method (results are identical): "steps" (default, image-wide
+    pairs of steps) or "tiles" (asynchronous exact tile solves)."""
     _lib.call("cle_set_growth_method", {"tiles": 0, "steps": 1}[method])
 
 

@@ -28,8 +28,9 @@ int grow_masked(DBuf mask, const float* lab, DBuf dst, const Geom& g, int* iters
 // unmasked growth of a label image (extend_labeling_via_voronoi) into dst
 int grow_labels(DBuf src, DBuf dst, const Geom& g, int* iters, bool* overflow, hipStream_t s);
 // batch: images of `dimgs` (device table with wbase / wn* set, packed at .off, output
-// to .dst), tile prefix `dts` (nimg + 1 entries); v0 = (seg - 1) + lab
-int grow_batch(const BImg* dimgs, const int64_t* dts, int nimg, int64_t ntiles, const uint8_t* seg, const float* lab,
-               int* iters, bool* overflow, hipStream_t s);
+// to .dst), tile prefix `dts` (nimg + 1 entries), kinds: bit 0 some image is 2D, bit 1
+// some image is 3D; v0 = (seg - 1) + lab
+int grow_batch(const BImg* dimgs, const int64_t* dts, int nimg, int64_t ntiles, int kinds, const uint8_t* seg,
+               const float* lab, int* iters, bool* overflow, hipStream_t s);
 
 }  // namespace cle

@@ -889,9 +889,10 @@ static unsigned vg(int64_t n) {
     return g > 131072 ? 131072 : g;
 }
 
-// Growth method: the asynchronous tile solves (growth.hip), or the step-synchronous
-// pairs of steps below (also the fallback for growths beyond the tile solves' step range).
-static int g_growth_method = 0;  // 0: tile solves, 1: pairs of steps
+// Growth method: the step-synchronous pairs of steps below (default: the faster on the
+// measured workloads), or the asynchronous tile solves of growth.hip (same results; a
+// growth beyond their step range falls back to the pairs).
+static int g_growth_method = 1;  // 0: tile solves, 1: pairs of steps
 
 // labels of `binary` grown inside `mask`, written (masked) to dst.  `B` holds
 // scratch for the CCL labels (float32, n voxels).
@@ -1253,7 +1254,9 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
         // tile solves: the labels go straight to the destinations
         bool over = false;
         std::vector<int> its((size_t)n, 1);
-        CLE_TRY(grow_batch(dimgs, (const int64_t*)(dev + ws_off), n, wtiles, Gp, Fp, its.data(), &over, s));
+        int kinds = 0;
+        for (const BImg& im : imgs) kinds |= im.g.n() == 0 ? 0 : ((im.g.ndim == 3 && im.g.D > 1) ? 2 : 1);
+        CLE_TRY(grow_batch(dimgs, (const int64_t*)(dev + ws_off), n, wtiles, kinds, Gp, Fp, its.data(), &over, s));
         if (!over) {
             if (n_steps) *n_steps = *std::max_element(its.begin(), its.end());
             CLE_HIP(hipStreamSynchronize(s));

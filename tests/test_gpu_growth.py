@@ -1,5 +1,5 @@
-"""The Voronoi growth's two methods against the oracle and each other: the asynchronous
-tile solves (default) and the image-wide pairs of steps.  Cases: masks and seeds that
+"""The Voronoi growth's two methods against the oracle and each other: the image-wide
+pairs of steps (default) and the asynchronous tile solves.  Cases: masks and seeds that
 cross tile borders in every direction, the reference's stop rule (a growth that could
 only continue diagonally after an image-wide step without change), growths beyond the
 tile solves' step range (the fallback), and the iteration count."""
@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture
 def methods(cle):
     yield ("tiles", "steps")
-    cle.set_growth_method("tiles")
+    cle.set_growth_method("steps")
 
 
 def _masked(cle, seeds, mask, method):
@@ -29,7 +29,7 @@ def _masked(cle, seeds, mask, method):
         _lib.call("cle_masked_voronoi_labeling", b.buf(), m.buf(), out.buf(), ctypes.byref(it), cle._ops._S())
         return cle.pull(out), it.value
     finally:
-        cle.set_growth_method("tiles")
+        cle.set_growth_method("steps")
 
 
 # tile shapes: 62 x 6 x 6 (3D), 62 x 62 (2D): sizes around and across their borders
@@ -117,7 +117,7 @@ def test_extend_labeling_methods(cle, methods, shape):
         try:
             got = cle.pull(cle.extend_labeling_via_voronoi(lab, cle.create(shape, np.float32)))
         finally:
-            cle.set_growth_method("tiles")
+            cle.set_growth_method("steps")
         assert np.array_equal(got, want), m
 
 
@@ -132,7 +132,7 @@ def test_batch_methods_identical(cle, methods):
         try:
             outs[m], outs[m + "_steps"] = cle.voronoi_otsu_labeling_batch(imgs, ss, so, return_steps=True)
         finally:
-            cle.set_growth_method("tiles")
+            cle.set_growth_method("steps")
     assert outs["tiles_steps"] == outs["steps_steps"]
     for a, b in zip(outs["tiles"], outs["steps"]):
         assert torch.equal(a.tensor, b.tensor)
