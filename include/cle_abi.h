@@ -166,6 +166,12 @@ int cle_extend_labeling_via_voronoi(const cle_buf* src, const cle_buf* dst, int*
  * (which fall back to the pairs for growths of 254 steps or more; a captured batch plan
  * always records the pairs). */
 int cle_set_growth_method(int method);
+/* Counters of the pair-of-steps growth launches made by the calls that synchronise on
+ * them (no counterpart in the reference; process-wide): out[0] launches, out[1] growth
+ * tiles processed (1024 voxels each: 16 x 8 x 8 in 3D, 32 x 32 in 2D), out[2]
+ * nanoseconds of GPU time of those launches (events around each batch of launches).
+ * reset != 0 sets them to zero after reading. */
+int cle_growth_counters(unsigned long long* out, int reset);
 /* _tier9/_voronoi_otsu_labeling.py:11-73: blur(spot) -> maxima -> blur(outline) ->
  * Otsu -> AND -> masked Voronoi -> mask.  Synchronises only inside the Voronoi
  * convergence test. */

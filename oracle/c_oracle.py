@@ -38,6 +38,12 @@ def lib():
     return _lib
 
 
+def set_threads(n):
+    """OpenMP threads of the C restatement's parallel loops."""
+    lib()
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+
+
 def _dims(a):
     return (1,) + a.shape if a.ndim == 2 else a.shape
 

@@ -16,7 +16,7 @@ from ._ops import (STATISTICS_KEYS, add_image_and_scalar, autotune_gaussian_blur
                    extend_labeling_via_voronoi,
                    connected_components_labeling_diamond, copy, detect_maxima_box, dilate_labels, execute,
                    extend_labels_with_maximum_radius, gauss_otsu_labeling,
-                   gaussian_blur, greater_constant, histogram, label, label_spots, logical_and, mask,
+                   gaussian_blur, greater_constant, growth_counters, histogram, label, label_spots, logical_and, mask,
                    masked_voronoi_labeling, maximum_of_all_pixels, mean_box, minimum_of_all_pixels, release_pool,
                    nonzero_minimum_box, nonzero_minimum_diamond, onlyzero_overwrite_maximum_box,
                    onlyzero_overwrite_maximum_diamond, otsu_threshold, relabel_sequential, set, set_autotune,

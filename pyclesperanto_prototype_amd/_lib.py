@@ -69,6 +69,7 @@ SIGNATURES = {
     "cle_extend_labeling_via_voronoi": [_P, _P, ctypes.POINTER(ctypes.c_int), _V],
     "cle_set_autotune": [ctypes.c_int],
     "cle_set_growth_method": [ctypes.c_int],
+    "cle_growth_counters": [ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int],
     "cle_autotune_gaussian_blur": [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _V],
     "cle_voronoi_otsu_labeling_batch_plan": [_P, _P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                              ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), _V],

@@ -516,6 +516,14 @@ def set_growth_method(method: str = "steps"):
     _lib.call("cle_set_growth_method", {"tiles": 0, "steps": 1}[method])
 
 
+def growth_counters(reset: bool = False) -> dict:
+    """Pair-of-steps growth launches since the last reset: launches, tiles processed
+    (1024 voxels each) and their GPU milliseconds."""
+    out = (ctypes.c_ulonglong * 3)()
+    _lib.call("cle_growth_counters", out, 1 if reset else 0)
+    return {"launches": int(out[0]), "tiles": int(out[1]), "ms": out[2] / 1e6}
+
+
 def autotune_gaussian_blur(source, destination=None, sigma_x: float = 0, sigma_y: float = 0, sigma_z: float = 0):
     """Tune the isotropic blur's z chunking for this configuration ahead of time (the
     call synchronises); later gaussian_blur calls of the same class use the choice.

@@ -44,6 +44,7 @@
 #include "growth.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <mutex>
 
@@ -101,8 +102,14 @@ struct VorLists {
 struct VorState {
     uint32_t* bits;    // 3 x nimg: per pair b % 3 and image, OR of the change bits (1 box, 2 diamond)
     int32_t* iters;    // per image: reference iteration count once converged, else 0
-    uint32_t* word;    // [0]: an image still grows after the last pair; [1]: max iteration count
+    uint32_t* word;    // [0]: an image still grows after the last pair; [1]: max iteration count;
+                       // [2..9]: tiles processed by the pair launches (blocks spread over 8)
 };
+
+// the tiles a block processed, added to one of 8 counters (block % 8: one per XCD)
+__device__ __forceinline__ void vor_count_tiles(const VorState& S, uint32_t n) {
+    if (threadIdx.x == 0 && n) atomicAdd(&S.word[2 + (blockIdx.x & 7)], n);
+}
 
 __device__ __forceinline__ uint64_t lanes_below_v() {
     const int lane = threadIdx.x & 63;
@@ -557,7 +564,9 @@ __global__ __launch_bounds__(VB, 4) void k_vor_pair(const BImg* __restrict__ img
     const VorItem* items = L.items + (int64_t)(b % 3) * L.cap;
     PairItem it;
     int par = 0;
+    uint32_t done = 0;
     for (uint32_t e = blockIdx.x; vor_next_item(e, n, items, prev, b, L, S, it); e += gridDim.x) {
+        ++done;
         const Geom g = imgs[it.j].g;
         const int64_t off = imgs[it.j].off;
         // sbits[par] was last read two tiles ago, before a barrier of the previous tile
@@ -602,6 +611,7 @@ __global__ __launch_bounds__(VB, 4) void k_vor_pair(const BImg* __restrict__ img
         __syncthreads();
         vor_flush(pend, np, imgs, b + 1, L);
     }
+    vor_count_tiles(S, done);
 }
 
 // Pair b in raster mode (one image, blocks over runs of tpb consecutive tiles in raster
@@ -635,12 +645,14 @@ __device__ __forceinline__ void vor_pair_grid(const BImg* __restrict__ imgs, con
     const int32_t* mark = L.mark + (int64_t)(b & 1) * L.cap;
     int32_t* mark_next = L.mark + (int64_t)((b + 1) & 1) * L.cap;
     int par = 0;
+    uint32_t done = 0;
     // tiles [blockIdx.x * tpb, + tpb): consecutive in raster order
     const int64_t t0 = (int64_t)blockIdx.x * (MULTI ? tpb : 1);
     const int64_t t1 = MULTI ? (t0 + tpb < ntiles ? t0 + tpb : ntiles) : (t0 + 1 < ntiles ? t0 + 1 : ntiles);
 #pragma unroll 1
     for (int64_t t = t0; t < t1; ++t) {
         if (mark[t] != b || L.settled[t]) continue;
+        ++done;
         const int txy = (int)(t / nx), tx = (int)(t - (int64_t)txy * nx), ty = txy % ny, tz = txy / ny;
         // sbits[par] was last read two tiles ago, before a barrier of the previous tile
         if (threadIdx.x == 0) sbits[par] = 0;
@@ -674,6 +686,7 @@ __device__ __forceinline__ void vor_pair_grid(const BImg* __restrict__ imgs, con
         // lds is rewritten by the next tile only after every thread passed the last
         // barrier of vor_pair_compute (its lds reads come before it)
     }
+    vor_count_tiles(S, done);
 }
 
 __global__ __launch_bounds__(VB, 4) void k_vor_pair_grid(const BImg* __restrict__ imgs,
@@ -767,19 +780,44 @@ static void launch_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int6
 // Runs pairs from b0 until every image has converged; P0 holds the initial state (as
 // does P1); pair b reads P[b & 1] and writes P[(b + 1) & 1].  `max_iters` (host, may
 // be null) receives the largest reference iteration count over the images.
+// Counters of the pair launches run by grow_pairs (process-wide): launches, tiles
+// processed, and their GPU time (events around each chunk of launches).
+static std::atomic<unsigned long long> g_pair_launches{0}, g_pair_tiles{0}, g_pair_ns{0};
+
 static int grow_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t ntiles, const VorLists& L,
                       const VorState& S, bool raster, int b0, int* max_iters, hipStream_t s) {
     int b = b0;
-    uint32_t hw[2] = {0, 0};
+    uint32_t hw[10] = {0};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    const bool timed = hipEventCreate(&ev[0]) == hipSuccess && hipEventCreate(&ev[1]) == hipSuccess;
+    double ms_total = 0;
+    int launches = 0;
+    int rc = CLE_OK;
     for (;;) {
         const int chunk = b == 0 ? 2 : VOR_CHUNK;
+        if (timed) (void)hipEventRecord(ev[0], s);
         launch_pairs(dimgs, nimg, P0, P1, ntiles, L, S, raster, b, chunk, s);
+        if (timed) (void)hipEventRecord(ev[1], s);
         b += chunk;
-        CLE_LAUNCHED();
-        CLE_HIP(hipMemcpyAsync(hw, S.word, sizeof(hw), hipMemcpyDeviceToHost, s));
-        CLE_HIP(hipStreamSynchronize(s));
+        launches += chunk;
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(hw, S.word, sizeof(hw), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            rc = fail(CLE_E_HIP, "Voronoi growth launch failed");
+            break;
+        }
+        float ms = 0.f;
+        if (timed && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) ms_total += ms;
         if (hw[0] == 0) break;
     }
+    if (ev[0]) (void)hipEventDestroy(ev[0]);
+    if (ev[1]) (void)hipEventDestroy(ev[1]);
+    CLE_TRY(rc);
+    unsigned long long tiles = 0;
+    for (int k = 2; k < 10; ++k) tiles += hw[k];
+    g_pair_launches += (unsigned long long)launches;
+    g_pair_tiles += tiles;
+    g_pair_ns += (unsigned long long)(ms_total * 1e6);
     if (max_iters) *max_iters = (int)hw[1];
     return CLE_OK;
 }
@@ -1389,6 +1427,20 @@ int cle_plan_run(cle_plan* P, int* n_steps, void* stream) {
 
 int cle_plan_destroy(cle_plan* P) {
     plan_free(P);
+    return CLE_OK;
+}
+
+int cle_growth_counters(unsigned long long* out, int reset) {
+    if (out) {
+        out[0] = g_pair_launches.load();
+        out[1] = g_pair_tiles.load();
+        out[2] = g_pair_ns.load();
+    }
+    if (reset) {
+        g_pair_launches = 0;
+        g_pair_tiles = 0;
+        g_pair_ns = 0;
+    }
     return CLE_OK;
 }
 
