@@ -177,6 +177,18 @@ int cle_growth_counters(unsigned long long* out, int reset);
  * convergence test. */
 int cle_voronoi_otsu_labeling(const cle_buf* src, const cle_buf* dst,
                               float spot_sigma, float outline_sigma, void* stream);
+/* cle_voronoi_otsu_labeling without any allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory -- code under stream capture, tight
+ * serving loops, harnesses that forbid it.  Same kernels and result; its scratch is a
+ * fixed device array of CLE_NOALLOC_SCRATCH_BYTES per GPU (held from the library's
+ * load until the process ends), which bounds the image to about 6 million voxels
+ * (roughly 11 bytes of scratch per voxel); a larger image fails with CLE_E_VALUE
+ * before anything runs.  All calls on one GPU share that array: calls that overlap on
+ * one GPU (other threads, other streams) race on it.  The Voronoi growth always runs
+ * as pairs of image-wide steps here (cle_set_growth_method does not apply). */
+#define CLE_NOALLOC_SCRATCH_BYTES (64u << 20)
+int cle_voronoi_otsu_labeling_noalloc(const cle_buf* src, const cle_buf* dst,
+                                      float spot_sigma, float outline_sigma, void* stream);
 /* _tier9/_gauss_otsu_labeling.py:6-49: blur -> Otsu -> CCL box */
 int cle_gauss_otsu_labeling(const cle_buf* src, const cle_buf* dst, float outline_sigma, void* stream);
 /* Batched voronoi_otsu_labeling (_tier9/_voronoi_otsu_labeling.py:11-73 applied to

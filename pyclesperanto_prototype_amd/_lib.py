@@ -76,6 +76,7 @@ SIGNATURES = {
     "cle_plan_run": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _V],
     "cle_plan_destroy": [ctypes.c_void_p],
     "cle_voronoi_otsu_labeling": [_P, _P, _F, _F, _V],
+    "cle_voronoi_otsu_labeling_noalloc": [_P, _P, _F, _F, _V],
     "cle_gauss_otsu_labeling": [_P, _P, _F, _V],
     "cle_voronoi_otsu_labeling_batch": [_P, _P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                         ctypes.c_int, ctypes.POINTER(ctypes.c_int), _V],

@@ -68,6 +68,20 @@ void scratch_free(void* p, hipStream_t s);
 int arena_set_owned(hipStream_t s);
 int arena_release_stream(hipStream_t s);
 
+// Fixed scratch region: while a FixedScratch lives, the scratch allocations of the
+// calling thread are bumps in the caller's device region [base, base + cap) (released
+// last-in first-out) instead of arena chunks, so no driver allocation runs; a request
+// beyond the region fails with CLE_E_OOM.
+struct FixedScratch {
+    FixedScratch(char* base, size_t cap);
+    ~FixedScratch();
+    FixedScratch(const FixedScratch&) = delete;
+    FixedScratch& operator=(const FixedScratch&) = delete;
+};
+bool scratch_fixed();  // a FixedScratch is active on this thread
+// bytes one scratch allocation of `bytes` takes (every allocation is 256-byte aligned)
+inline size_t scratch_round(size_t bytes) { return ((bytes ? bytes : 4) + 255) / 256 * 256; }
+
 // RAII holder for one scratch allocation on a stream
 struct Scratch {
     void* p = nullptr;
@@ -80,7 +94,9 @@ struct Scratch {
     int alloc(size_t bytes, hipStream_t st) {
         s = st;
         p = scratch_alloc(bytes ? bytes : 4, st);
-        if (!p) return fail(CLE_E_OOM, "scratch allocation of %zu bytes failed", bytes);
+        if (!p)
+            return scratch_fixed() ? fail(CLE_E_OOM, "the fixed scratch region cannot hold %zu more bytes", bytes)
+                                   : fail(CLE_E_OOM, "scratch allocation of %zu bytes failed", bytes);
         return CLE_OK;
     }
 };
@@ -370,7 +386,13 @@ void ccl_batch_layout(std::vector<BImg>& imgs, CclBatchSizes* sz);
 int plan_ccl_batch(const std::vector<BImg>& imgs, size_t imgs_off, const CclBatchSizes& sz, const uint8_t* S,
                    float* F, Blob& blob, std::vector<Launch>& out, std::vector<std::shared_ptr<void>>& keep,
                    hipStream_t s);
-int voronoi_grow_impl(float* a, float* b, const Geom& g, int* n_iterations, int* final_in_a,
-                      hipStream_t s);
+// single-image voronoi_otsu_labeling (the body of cle_voronoi_otsu_labeling)
+int voronoi_otsu_single(const cle_buf* src, const cle_buf* dst, float spot_sigma, float outline_sigma,
+                        hipStream_t s);
+// Upper bounds of the scratch bytes (sums of scratch_round of every allocation alive at
+// once) that the stages take for an image of geometry g
+size_t ccl_box_scratch_bytes(const Geom& g, int options);
+size_t otsu_scratch_bytes(const Geom& g);
+size_t voronoi_otsu_scratch_bytes(const Geom& g);
 
 }  // namespace cle

@@ -415,9 +415,16 @@ __device__ __forceinline__ uint32_t bit_rank(const uint32_t* bm, const uint32_t*
 struct RankTable {
     Scratch bm, prefix, bsum;
     int64_t nw = 0, nb = 0;
+    static int64_t words(uint64_t nbits) {
+        const int64_t w = (int64_t)((nbits + 31) / 32);
+        return w < 1 ? 1 : w;
+    }
+    static size_t bytes(uint64_t nbits) {
+        const int64_t w = words(nbits);
+        return 2 * scratch_round((size_t)w * 4) + scratch_round((size_t)((w + SWORDS - 1) / SWORDS) * 4);
+    }
     int init(uint64_t nbits, hipStream_t s) {
-        nw = (int64_t)((nbits + 31) / 32);
-        if (nw < 1) nw = 1;
+        nw = words(nbits);
         nb = (nw + SWORDS - 1) / SWORDS;
         CLE_TRY(bm.alloc((size_t)nw * 4, s));
         CLE_TRY(prefix.alloc((size_t)nw * 4, s));
@@ -910,13 +917,33 @@ static void launch_bccl_write(const uint8_t* M, const Geom& g, const BGrid& B, c
         launch_bccl_write_v<BZ, false>(M, g, B, L, F, dst, grid, s);
 }
 
+// the block grid, tile grid and key range of the box CCL of one image, and the scratch
+// bytes its buffers take
+template <typename BT>
+struct BcclSizes {
+    BGrid B;
+    TileGrid T;
+    int64_t nb, nt;
+    uint64_t maxbit;
+    BcclSizes(const Geom& g, int compat)
+        : B{(g.D + BT::BZ - 1) / BT::BZ, (g.H + 1) / 2, (g.W + 1) / 2},
+          T{(B.D + BT::TZ - 1) / BT::TZ, (B.H + BT::TY - 1) / BT::TY, (B.W + BT::TX - 1) / BT::TX},
+          nb(B.D * B.H * B.W), nt(T.count()),
+          maxbit(compat ? (uint64_t)(float)(uint64_t)g.n() : (uint64_t)g.n()) {}
+    size_t bytes() const {
+        return scratch_round((size_t)nb) + 3 * scratch_round((size_t)nb * 4) +
+               scratch_round((size_t)nt * BT::ROOTS * 4) + scratch_round((size_t)nt * 4) + RankTable::bytes(maxbit + 1);
+    }
+};
+
 template <typename BT, typename FG>
 static int bccl_run(FG fg, const Geom& g, DBuf dst, int options, uint32_t* n_labels_dev, hipStream_t s) {
     const int compat = (options & CLE_FP32_KEY_COMPAT) ? 1 : 0;
-    const BGrid B{(g.D + BT::BZ - 1) / BT::BZ, (g.H + 1) / 2, (g.W + 1) / 2};
-    const int64_t nb = B.D * B.H * B.W;
-    const TileGrid T{(B.D + BT::TZ - 1) / BT::TZ, (B.H + BT::TY - 1) / BT::TY, (B.W + BT::TX - 1) / BT::TX};
-    const int64_t nt = T.count();
+    const BcclSizes<BT> Z(g, compat);
+    const BGrid B = Z.B;
+    const int64_t nb = Z.nb;
+    const TileGrid T = Z.T;
+    const int64_t nt = Z.nt;
     Scratch Ms, Ls, Ks, Rs, Ns, Fs;
     CLE_TRY(Ms.alloc((size_t)nb, s));
     CLE_TRY(Ls.alloc((size_t)nb * 4, s));
@@ -929,10 +956,8 @@ static int bccl_run(FG fg, const Geom& g, DBuf dst, int options, uint32_t* n_lab
     k_bccl_border<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(B, T, Ms.get<uint8_t>(), l, nr);
     k_ccl_resolve<BT><<<root_grid(nt), CT_THREADS, 0, s>>>(l, k, roots, nr, nt);
     CLE_LAUNCHED();
-    const int64_t n = g.n();
-    const uint64_t maxbit = compat ? (uint64_t)(float)(uint64_t)n : (uint64_t)n;
     RankTable rt;
-    CLE_TRY(rt.init(maxbit + 1, s));
+    CLE_TRY(rt.init(Z.maxbit + 1, s));
     k_ccl_mark_roots<BT><<<root_grid(nt), CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>(), nt);
     CLE_LAUNCHED();
     CLE_TRY(rt.scan(n_labels_dev, s));
@@ -1170,6 +1195,12 @@ int ccl_impl(DBuf binary, const Geom& g, DBuf dst, int connectivity, int options
                        : bccl_run<BlkTile2D>(fg, g, dst, options, n_labels_dev, s);
     return three_d ? ccl_run<Tile3D>(fg, g, dst, connectivity, options, n_labels_dev, s)
                    : ccl_run<Tile2D>(fg, g, dst, connectivity, options, n_labels_dev, s);
+}
+
+size_t ccl_box_scratch_bytes(const Geom& g, int options) {
+    const int compat = (options & CLE_FP32_KEY_COMPAT) ? 1 : 0;
+    const bool three_d = g.ndim == 3 && g.D > 1;
+    return three_d ? BcclSizes<BlkTile3D>(g, compat).bytes() : BcclSizes<BlkTile2D>(g, compat).bytes();
 }
 
 // ----------------------------------------------------------------- relabel / spots

@@ -505,6 +505,11 @@ __global__ void k_threshold(DBuf s, DBuf d, const double* thr, int64_t n) {
         st(d, i, ld(s, i) > t ? 1.f : 0.f);
 }
 
+// minmax_impl's 8 bytes are released before the histograms are taken
+size_t otsu_scratch_bytes(const Geom& g) {
+    return scratch_round(256 * sizeof(float)) + scratch_round((size_t)g.H * 256 * sizeof(uint32_t));
+}
+
 int otsu_impl(DBuf src, const Geom& g, float* minmax_dev, double* thr_dev, hipStream_t s) {
     CLE_TRY(minmax_impl(src, g.n(), minmax_dev, s));
     Scratch hist;

@@ -129,7 +129,57 @@ bool arena_merge(Arena* a) {
 }
 }  // namespace
 
+// ----------------------------------------------------------------- fixed region
+namespace {
+struct FixedRegion {
+    char* base = nullptr;
+    size_t cap = 0, top = 0;
+    int n = 0;  // live allocations
+    struct Blk {
+        size_t off;
+        bool freed;
+    } blk[64];
+};
+thread_local FixedRegion t_fixed;
+}  // namespace
+
+FixedScratch::FixedScratch(char* base, size_t cap) {
+    t_fixed.base = base;
+    t_fixed.cap = cap;
+    t_fixed.top = 0;
+    t_fixed.n = 0;
+}
+
+FixedScratch::~FixedScratch() {
+    t_fixed.base = nullptr;
+    t_fixed.cap = t_fixed.top = 0;
+    t_fixed.n = 0;
+}
+
+bool scratch_fixed() { return t_fixed.base != nullptr; }
+
+static void* fixed_alloc(size_t bytes) {
+    FixedRegion& f = t_fixed;
+    const size_t need = scratch_round(bytes);
+    if (f.n == (int)(sizeof(f.blk) / sizeof(f.blk[0])) || need > f.cap - f.top) return nullptr;
+    f.blk[f.n++] = {f.top, false};
+    char* p = f.base + f.top;
+    f.top += need;
+    return p;
+}
+
+static void fixed_free(void* p) {
+    FixedRegion& f = t_fixed;
+    for (int i = f.n; i-- > 0;)
+        if (f.base + f.blk[i].off == (char*)p) {
+            f.blk[i].freed = true;
+            break;
+        }
+    while (f.n > 0 && f.blk[f.n - 1].freed) f.top = f.blk[--f.n].off;
+}
+
 void* scratch_alloc(size_t bytes, hipStream_t s) {
+    if (t_fixed.base) return fixed_alloc(bytes);
     std::lock_guard<std::mutex> lk(g_arena_mu);
     Arena* a = arena_for(s);
     if (!a) return nullptr;
@@ -190,6 +240,10 @@ int arena_release_stream(hipStream_t s) {
 
 void scratch_free(void* p, hipStream_t s) {
     if (!p) return;
+    if (t_fixed.base && (char*)p >= t_fixed.base && (char*)p < t_fixed.base + t_fixed.cap) {
+        fixed_free(p);
+        return;
+    }
     std::lock_guard<std::mutex> lk(g_arena_mu);
     Arena* a = arena_for(s);
     if (!a) return;

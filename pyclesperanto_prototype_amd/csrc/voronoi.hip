@@ -736,10 +736,13 @@ struct VorScratch {
     Scratch mem;
     VorLists L;
     VorState S;
+    static size_t bytes(int64_t ntiles, int nimg) {
+        const int64_t cap = ntiles > 0 ? ntiles : 1;
+        return (size_t)cap * 3 * 16 + (size_t)cap * 12 + 64 + (size_t)nimg * 16 + (size_t)cap;
+    }
     int alloc(int64_t ntiles, int nimg, hipStream_t s) {
         const int64_t cap = ntiles > 0 ? ntiles : 1;
-        const size_t bytes = (size_t)cap * 3 * 16 + (size_t)cap * 12 + 64 + (size_t)nimg * 16 + (size_t)cap;
-        CLE_TRY(mem.alloc(bytes, s));
+        CLE_TRY(mem.alloc(bytes(ntiles, nimg), s));
         char* p = mem.get<char>();
         L.items = (VorItem*)p;
         p += (size_t)cap * 3 * 16;
@@ -789,7 +792,9 @@ static int grow_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t
     int b = b0;
     uint32_t hw[10] = {0};
     hipEvent_t ev[2] = {nullptr, nullptr};
-    const bool timed = hipEventCreate(&ev[0]) == hipSuccess && hipEventCreate(&ev[1]) == hipSuccess;
+    // (no events under a fixed scratch region: those calls create nothing)
+    const bool timed =
+        !scratch_fixed() && hipEventCreate(&ev[0]) == hipSuccess && hipEventCreate(&ev[1]) == hipSuccess;
     double ms_total = 0;
     int launches = 0;
     int rc = CLE_OK;
@@ -938,7 +943,8 @@ static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, 
                                hipStream_t s) {
     const int64_t n = g.n();
     CLE_TRY(ccl_impl(binary, g, DBuf{B, CLE_F32}, CLE_CONN_BOX, CLE_FP32_KEY_COMPAT, nullptr, s));
-    if (g_growth_method == 0) {
+    // (the tile solves' stores are not part of the fixed-region bound: pairs there)
+    if (g_growth_method == 0 && !scratch_fixed()) {
         bool over = false;
         CLE_TRY(grow_masked(mask, B, dst, g, n_iterations, &over, s));
         if (!over) return CLE_OK;
@@ -959,6 +965,54 @@ static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, 
         k_mask_out<<<vg(n), 256, 0, s>>>(res, mask, dst, n);
     CLE_LAUNCHED();
     return CLE_OK;
+}
+
+// Scratch bound of voronoi_otsu_single: the blurred image, spots, segmentation and
+// threshold words live through the call; on top of them, at most one of the blur's
+// float32 temporary (its per-axis fallback), the Otsu histograms, the box CCL of the
+// spots, and the growth's second state buffer with its tile lists.
+size_t voronoi_otsu_scratch_bytes(const Geom& g) {
+    const size_t n = (size_t)g.n();
+    const size_t held = scratch_round(n * 4) + 2 * scratch_round(n) + scratch_round(64);
+    const Tiles T = make_tiles(g);
+    const size_t grow =
+        scratch_round(n * 4) + scratch_round(VorScratch::bytes(T.count(), 1)) + scratch_round(sizeof(BImg));
+    size_t stage = scratch_round(n * 4);
+    const size_t others[3] = {otsu_scratch_bytes(g), ccl_box_scratch_bytes(g, CLE_FP32_KEY_COMPAT), grow};
+    for (size_t v : others) stage = v > stage ? v : stage;
+    return held + stage;
+}
+
+int voronoi_otsu_single(const cle_buf* src, const cle_buf* dst, float spot_sigma, float outline_sigma,
+                        hipStream_t s) {
+    Geom g;
+    CLE_TRY(check_buf(src, "source", &g));
+    const int64_t n = g.n();
+    if (!n) return CLE_OK;
+    Scratch blurred, spots, seg, aux;
+    CLE_TRY(blurred.alloc((size_t)n * 4, s));
+    CLE_TRY(spots.alloc((size_t)n, s));
+    CLE_TRY(seg.alloc((size_t)n, s));
+    CLE_TRY(aux.alloc(64, s));
+    float* mm = aux.get<float>();
+    double* thr = (double*)(aux.get<char>() + 16);
+    cle_buf bb = *src;
+    bb.data = blurred.p;
+    bb.dtype = CLE_F32;
+    // spots = detect_maxima_box(gaussian_blur(src, spot_sigma))
+    const float s1[3] = {spot_sigma, spot_sigma, spot_sigma};
+    CLE_TRY(blur_impl(src, &bb, s1, s));
+    CLE_TRY(maxima_impl(DBuf{blurred.p, CLE_F32}, DBuf{spots.p, CLE_U8}, g, s));
+    // segmentation = threshold_otsu(gaussian_blur(src, outline_sigma)); binary = spots AND segmentation
+    const float s2[3] = {outline_sigma, outline_sigma, outline_sigma};
+    CLE_TRY(blur_impl(src, &bb, s2, s));
+    CLE_TRY(otsu_impl(DBuf{blurred.p, CLE_F32}, g, mm, thr, s));
+    k_seg_and<<<vg(n / 4 + 1), 256, 0, s>>>(blurred.get<float>(), thr, spots.get<uint8_t>(), seg.get<uint8_t>(), n);
+    CLE_LAUNCHED();
+    // masked Voronoi of the seeds inside the segmentation; the seed labels go to the
+    // blurred buffer
+    return masked_voronoi_core(DBuf{spots.p, CLE_U8}, DBuf{seg.p, CLE_U8}, dbuf(dst), g, blurred.get<float>(), nullptr,
+                               s);
 }
 
 // ----------------------------------------------------------------- batched pipeline
@@ -1495,33 +1549,7 @@ int cle_voronoi_otsu_labeling(const cle_buf* src, const cle_buf* dst, float spot
     CLE_TRY(check_buf(src, "source", &g));
     CLE_TRY(check_buf(dst, "label_image_destination", &gd));
     CLE_TRY(check_same_shape(g, gd, "source", "label_image_destination"));
-    hipStream_t s = (hipStream_t)stream;
-    const int64_t n = g.n();
-    if (!n) return CLE_OK;
-    Scratch blurred, spots, seg, aux;
-    CLE_TRY(blurred.alloc((size_t)n * 4, s));
-    CLE_TRY(spots.alloc((size_t)n, s));
-    CLE_TRY(seg.alloc((size_t)n, s));
-    CLE_TRY(aux.alloc(64, s));
-    float* mm = aux.get<float>();
-    double* thr = (double*)(aux.get<char>() + 16);
-    cle_buf bb = *src;
-    bb.data = blurred.p;
-    bb.dtype = CLE_F32;
-    // spots = detect_maxima_box(gaussian_blur(src, spot_sigma))
-    const float s1[3] = {spot_sigma, spot_sigma, spot_sigma};
-    CLE_TRY(blur_impl(src, &bb, s1, s));
-    CLE_TRY(maxima_impl(DBuf{blurred.p, CLE_F32}, DBuf{spots.p, CLE_U8}, g, s));
-    // segmentation = threshold_otsu(gaussian_blur(src, outline_sigma)); binary = spots AND segmentation
-    const float s2[3] = {outline_sigma, outline_sigma, outline_sigma};
-    CLE_TRY(blur_impl(src, &bb, s2, s));
-    CLE_TRY(otsu_impl(DBuf{blurred.p, CLE_F32}, g, mm, thr, s));
-    k_seg_and<<<vg(n / 4 + 1), 256, 0, s>>>(blurred.get<float>(), thr, spots.get<uint8_t>(), seg.get<uint8_t>(), n);
-    CLE_LAUNCHED();
-    // masked Voronoi of the seeds inside the segmentation; the seed labels go to the
-    // blurred buffer
-    return masked_voronoi_core(DBuf{spots.p, CLE_U8}, DBuf{seg.p, CLE_U8}, dbuf(dst), g, blurred.get<float>(), nullptr,
-                               s);
+    return voronoi_otsu_single(src, dst, spot_sigma, outline_sigma, (hipStream_t)stream);
 }
 
 int cle_voronoi_otsu_labeling_batch(const cle_buf* srcs, const cle_buf* dsts, const float* spot_sigmas,
