@@ -580,7 +580,8 @@ static void gauss_taps(float sigma, Taps* t) {
     if (n % 2 == 0) n += 1;
     const int c = (n - 1) / 2;
     t->c = c;
-    std::vector<double> h(c + 1);
+    if (c > MAXC) return;  // refused by the caller; the taps would not fit
+    double h[MAXC + 1];
     double sum = 0;
     const double s = (double)sigma;
     for (int v = 0; v <= c; ++v) {

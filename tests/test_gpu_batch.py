@@ -1,6 +1,7 @@
 """Batched voronoi_otsu_labeling: every image of a mixed batch (2D and 3D, several
 sizes, source types and sigmas) must equal the single-image device path bit for bit,
-and small members must equal the oracle's pipeline fed with the same device blurs."""
+and a captured plan's replays must equal the eager batch.  The members against the
+oracle are in test_gpu_oracle_large.py (test_batch_members_vs_oracle)."""
 import numpy as np
 import pytest
 
