@@ -102,13 +102,15 @@ struct VorLists {
 struct VorState {
     uint32_t* bits;    // 3 x nimg: per pair b % 3 and image, OR of the change bits (1 box, 2 diamond)
     int32_t* iters;    // per image: reference iteration count once converged, else 0
-    uint32_t* word;    // [0]: an image still grows after the last pair; [1]: max iteration count;
-                       // [2..9]: tiles processed by the pair launches (blocks spread over 8)
+    uint32_t* word;    // [0]: an image still grows after the last pair; [1]: max iteration count
+    uint32_t* ctr;     // VOR_NCTR counters of the tiles the pair launches processed, 16 words apart
 };
 
-// the tiles a block processed, added to one of 8 counters (block % 8: one per XCD)
+// Tiles processed, counted per block into one of VOR_NCTR counters a cache line apart
+// (block % VOR_NCTR): atomics from many blocks on one word serialise.
+constexpr int VOR_NCTR = 64;
 __device__ __forceinline__ void vor_count_tiles(const VorState& S, uint32_t n) {
-    if (threadIdx.x == 0 && n) atomicAdd(&S.word[2 + (blockIdx.x & 7)], n);
+    if (threadIdx.x == 0 && n) atomicAdd(&S.ctr[(blockIdx.x % VOR_NCTR) * 16], n);
 }
 
 __device__ __forceinline__ uint64_t lanes_below_v() {
@@ -565,6 +567,7 @@ __global__ __launch_bounds__(VB, 4) void k_vor_pair(const BImg* __restrict__ img
     PairItem it;
     int par = 0;
     uint32_t done = 0;
+    int accj = -1, acc = 0;
     for (uint32_t e = blockIdx.x; vor_next_item(e, n, items, prev, b, L, S, it); e += gridDim.x) {
         ++done;
         const Geom g = imgs[it.j].g;
@@ -590,10 +593,14 @@ __global__ __launch_bounds__(VB, 4) void k_vor_pair(const BImg* __restrict__ img
         }
         par ^= 1;
         if (bits & 3) {
-            if (threadIdx.x == 0) {
-                atomicOr(&cur[it.j], (uint32_t)(bits & 3));
-                pend[np] = vor_pack(it.j, it.tx, it.ty, it.tz);
+            // change bits of consecutive tiles of one image: one atomic per run
+            if (it.j != accj) {
+                if (threadIdx.x == 0 && acc) atomicOr(&cur[accj], (uint32_t)acc);
+                accj = it.j;
+                acc = 0;
             }
+            acc |= bits & 3;
+            if (threadIdx.x == 0) pend[np] = vor_pack(it.j, it.tx, it.ty, it.tz);
             ++np;
         } else if (!(bits & 4) && threadIdx.x == 0) {
             L.settled[it.gt] = 1;
@@ -611,6 +618,7 @@ __global__ __launch_bounds__(VB, 4) void k_vor_pair(const BImg* __restrict__ img
         __syncthreads();
         vor_flush(pend, np, imgs, b + 1, L);
     }
+    if (threadIdx.x == 0 && acc) atomicOr(&cur[accj], (uint32_t)acc);
     vor_count_tiles(S, done);
 }
 
@@ -619,9 +627,7 @@ __global__ __launch_bounds__(VB, 4) void k_vor_pair(const BImg* __restrict__ img
 // mark[b & 1][t] == b; a tile that changes marks its neighbourhood with b + 1 in the
 // other array (plain stores of one value: no atomics, no list).  Block 0 first closes
 // pair b - 1 as in the list mode.
-// MULTI = false: one tile per block (tpb = 1; 4 waves per SIMD); true: runs of tpb
-// tiles per block (the loop costs registers: 3 waves per SIMD)
-template <bool MULTI>
+// One tile per block (images below 4096 tiles; larger ones take k_vor_pair_run).
 __device__ __forceinline__ void vor_pair_grid(const BImg* __restrict__ imgs, const float* __restrict__ src,
                                               float* __restrict__ dst, int b, const VorLists& L, const VorState& S,
                                               int tpb) {
@@ -647,8 +653,8 @@ __device__ __forceinline__ void vor_pair_grid(const BImg* __restrict__ imgs, con
     int par = 0;
     uint32_t done = 0;
     // tiles [blockIdx.x * tpb, + tpb): consecutive in raster order
-    const int64_t t0 = (int64_t)blockIdx.x * (MULTI ? tpb : 1);
-    const int64_t t1 = MULTI ? (t0 + tpb < ntiles ? t0 + tpb : ntiles) : (t0 + 1 < ntiles ? t0 + 1 : ntiles);
+    const int64_t t0 = (int64_t)blockIdx.x * tpb;
+    const int64_t t1 = t0 + tpb < ntiles ? t0 + tpb : ntiles;
 #pragma unroll 1
     for (int64_t t = t0; t < t1; ++t) {
         if (mark[t] != b || L.settled[t]) continue;
@@ -692,13 +698,100 @@ __device__ __forceinline__ void vor_pair_grid(const BImg* __restrict__ imgs, con
 __global__ __launch_bounds__(VB, 4) void k_vor_pair_grid(const BImg* __restrict__ imgs,
                                                          const float* __restrict__ src, float* __restrict__ dst,
                                                          int b, VorLists L, VorState S) {
-    vor_pair_grid<false>(imgs, src, dst, b, L, S, 1);
+    vor_pair_grid(imgs, src, dst, b, L, S, 1);
 }
-__global__ __launch_bounds__(VB, 3) void k_vor_pair_grid_multi(const BImg* __restrict__ imgs,
-                                                               const float* __restrict__ src,
-                                                               float* __restrict__ dst, int b, VorLists L,
-                                                               VorState S, int tpb) {
-    vor_pair_grid<true>(imgs, src, dst, b, L, S, tpb);
+
+// Pair b in raster mode over runs of tpb (<= 64) consecutive tiles of one image (2D or
+// 3D by D3).  The block reads the marks and settled flags of its run once (one load
+// per lane, a ballot), then works through the active tiles software-pipelined: the
+// region of the next active tile is loaded into registers while the current tile is
+// computed from LDS.  Same marks, flags and results as vor_pair_grid (the loads read
+// src, which no block writes in this launch; only the block that processes a tile
+// sets its settled flag, after it read it).
+#ifndef CLE_VOR_RUN_WAVES
+#define CLE_VOR_RUN_WAVES 5  // waves per SIMD of the 3D form: 96 VGPRs without spills (6 spills)
+#endif
+template <bool D3>
+__global__ __launch_bounds__(VB, D3 ? CLE_VOR_RUN_WAVES : 4) void k_vor_pair_run(const BImg* __restrict__ imgs, const float* __restrict__ src,
+                                                        float* __restrict__ dst, int b, VorLists L, VorState S,
+                                                        int tpb) {
+    using P = PairGeo<D3>;
+    __shared__ float lds[P::NR];
+    __shared__ float lds2[P::NR];
+    __shared__ int sbits[2];
+    const uint32_t* prev = S.bits + (b + 2) % 3;  // pair b - 1 (one image)
+    uint32_t* cur = S.bits + b % 3;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (b > 0 && S.iters[0] == 0 && prev[0] != 3u) {
+            const int it = 2 * (b - 1) + ((prev[0] & 1u) ? 2 : 1);
+            S.iters[0] = it;
+            atomicMax((int*)&S.word[1], it);
+        }
+        S.bits[(b + 1) % 3] = 0;
+    }
+    if (S.iters[0] != 0 || (b > 0 && prev[0] != 3u)) return;  // converged in an earlier pair
+    const Geom g = imgs[0].g;
+    const int nx = imgs[0].vnx, ny = imgs[0].vny, nz = imgs[0].vnz;
+    const int64_t ntiles = (int64_t)nx * ny * nz;
+    const int32_t* mark = L.mark + (int64_t)(b & 1) * L.cap;
+    int32_t* mark_next = L.mark + (int64_t)((b + 1) & 1) * L.cap;
+    const int64_t t0 = (int64_t)blockIdx.x * tpb;
+    const int nrun = (int)(t0 + tpb < ntiles ? tpb : ntiles - t0);
+    // active tiles of the run: bit k for tile t0 + k (every wave builds the same mask)
+    uint64_t act;
+    {
+        const int lane = threadIdx.x & 63;
+        bool a = false;
+        if (lane < nrun) a = mark[t0 + lane] == b && !L.settled[t0 + lane];
+        act = __ballot(a);
+    }
+    if (!act) return;
+    float v[P::KV];
+    int64_t t = t0 + __builtin_ctzll(act);
+    act &= act - 1;
+    {
+        const int txy = (int)(t / nx), tx = (int)(t - (int64_t)txy * nx);
+        vor_stage_load<D3>(src, P(g, tx, txy % ny, txy / ny), v);
+    }
+    int par = 0;
+    uint32_t done = 0;
+    int acc = 0;  // change bits of the block's tiles: one atomic per block
+#pragma unroll 1
+    for (;;) {
+        ++done;
+        const int txy = (int)(t / nx), tx = (int)(t - (int64_t)txy * nx), ty = txy % ny, tz = txy / ny;
+        const P R(g, tx, ty, tz);
+        // sbits[par] was last read two tiles ago, before a barrier of the previous tile
+        if (threadIdx.x == 0) sbits[par] = 0;
+        vor_stage_store<D3>(v, lds);
+        __syncthreads();
+        const bool more = act != 0;
+        const int64_t tn = more ? t0 + __builtin_ctzll(act) : t;
+        act &= act - 1;
+        if (more) {
+            const int nxy = (int)(tn / nx), nxx = (int)(tn - (int64_t)nxy * nx);
+            vor_stage_load<D3>(src, P(g, nxx, nxy % ny, nxy / ny), v);
+        }
+        const int bits = vor_pair_compute<D3>(dst, R, lds, lds2, &sbits[par]);
+        par ^= 1;
+        if (bits & 3) {
+            acc |= bits & 3;
+            const int k = threadIdx.x;
+            if (k < 27) {
+                const int qx = tx + k % 3 - 1, qy = ty + (k / 3) % 3 - 1, qz = tz + k / 9 - 1;
+                if (qx >= 0 && qx < nx && qy >= 0 && qy < ny && qz >= 0 && qz < nz)
+                    mark_next[((int64_t)qz * ny + qy) * nx + qx] = b + 1;
+            }
+        } else if (!(bits & 4) && threadIdx.x == 0) {
+            L.settled[t] = 1;
+        }
+        // lds is rewritten by the next tile only after every thread passed the last
+        // barrier of vor_pair_compute (its lds reads come before it)
+        if (!more) break;
+        t = tn;
+    }
+    if (threadIdx.x == 0 && acc) atomicOr(cur, (uint32_t)acc);
+    vor_count_tiles(S, done);
 }
 
 // tiles per block of the raster mode: one for images below 4096 tiles, 16 above (fewer,
@@ -738,7 +831,7 @@ struct VorScratch {
     VorState S;
     static size_t bytes(int64_t ntiles, int nimg) {
         const int64_t cap = ntiles > 0 ? ntiles : 1;
-        return (size_t)cap * 3 * 16 + (size_t)cap * 12 + 64 + (size_t)nimg * 16 + (size_t)cap;
+        return (size_t)cap * 3 * 16 + (size_t)cap * 12 + 64 + (size_t)VOR_NCTR * 64 + (size_t)nimg * 16 + (size_t)cap;
     }
     int alloc(int64_t ntiles, int nimg, hipStream_t s) {
         const int64_t cap = ntiles > 0 ? ntiles : 1;
@@ -752,6 +845,8 @@ struct VorScratch {
         L.count = (uint32_t*)p;        // count[0..2], word[0..1]: 5 words, padded to 16
         S.word = L.count + 3;
         p += 64;
+        S.ctr = (uint32_t*)p;          // VOR_NCTR x 16 words
+        p += (size_t)VOR_NCTR * 64;
         S.bits = (uint32_t*)p;         // 3 x nimg
         S.iters = (int32_t*)(S.bits + (size_t)3 * nimg);
         p += (size_t)nimg * 16;
@@ -759,20 +854,27 @@ struct VorScratch {
         L.cap = cap;
         // stamps -1, counters / words / per-image state 0 (contiguous from count)
         CLE_TRY(fill_u32((uint32_t*)L.stamp, 0xFFFFFFFFu, (size_t)cap * 3, s));  // stamp and mark: -1
-        CLE_TRY(fill_u32(L.count, 0u, 16 + (size_t)nimg * 4, s));
+        CLE_TRY(fill_u32(L.count, 0u, 16 + (size_t)VOR_NCTR * 16 + (size_t)nimg * 4, s));
         return CLE_OK;
     }
 };
 
+// Growth modes: work lists (a batch), raster over the tiles of one 2D / 3D image.
+enum { VOR_LISTS = 0, VOR_RASTER_2D = 2, VOR_RASTER_3D = 3 };
+inline int vor_mode_single(const Geom& g) { return vor_3d(g) ? VOR_RASTER_3D : VOR_RASTER_2D; }
+
 // Launches pairs [b, b + count) and the close of the last one (no host synchronisation).
 static void launch_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t ntiles, const VorLists& L,
-                         const VorState& S, bool raster, int b, int count, hipStream_t s) {
+                         const VorState& S, int mode, int b, int count, hipStream_t s) {
+    const bool raster = mode != VOR_LISTS;
     const int tpb = raster_tpb(ntiles);
     const unsigned grid =
         raster ? (unsigned)((ntiles + tpb - 1) / tpb) : (unsigned)(ntiles < 4096 ? (ntiles > 0 ? ntiles : 1) : 4096);
     for (int k = 0; k < count; ++k, ++b) {
-        if (raster && tpb > 1)
-            k_vor_pair_grid_multi<<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S, tpb);
+        if (raster && tpb > 1 && mode == VOR_RASTER_3D)
+            k_vor_pair_run<true><<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S, tpb);
+        else if (raster && tpb > 1)
+            k_vor_pair_run<false><<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S, tpb);
         else if (raster)
             k_vor_pair_grid<<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S);
         else k_vor_pair<<<grid, VB, 0, s>>>(dimgs, nimg, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S);
@@ -788,9 +890,9 @@ static void launch_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int6
 static std::atomic<unsigned long long> g_pair_launches{0}, g_pair_tiles{0}, g_pair_ns{0};
 
 static int grow_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t ntiles, const VorLists& L,
-                      const VorState& S, bool raster, int b0, int* max_iters, hipStream_t s) {
+                      const VorState& S, int mode, int b0, int* max_iters, hipStream_t s) {
     int b = b0;
-    uint32_t hw[10] = {0};
+    uint32_t hw[2] = {0, 0};
     hipEvent_t ev[2] = {nullptr, nullptr};
     // (no events under a fixed scratch region: those calls create nothing)
     const bool timed =
@@ -801,7 +903,7 @@ static int grow_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t
     for (;;) {
         const int chunk = b == 0 ? 2 : VOR_CHUNK;
         if (timed) (void)hipEventRecord(ev[0], s);
-        launch_pairs(dimgs, nimg, P0, P1, ntiles, L, S, raster, b, chunk, s);
+        launch_pairs(dimgs, nimg, P0, P1, ntiles, L, S, mode, b, chunk, s);
         if (timed) (void)hipEventRecord(ev[1], s);
         b += chunk;
         launches += chunk;
@@ -818,8 +920,11 @@ static int grow_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t
     if (ev[0]) (void)hipEventDestroy(ev[0]);
     if (ev[1]) (void)hipEventDestroy(ev[1]);
     CLE_TRY(rc);
+    uint32_t hc[VOR_NCTR * 16];
+    CLE_HIP(hipMemcpyAsync(hc, S.ctr, sizeof(hc), hipMemcpyDeviceToHost, s));
+    CLE_HIP(hipStreamSynchronize(s));
     unsigned long long tiles = 0;
-    for (int k = 2; k < 10; ++k) tiles += hw[k];
+    for (int k = 0; k < VOR_NCTR; ++k) tiles += hc[k * 16];
     g_pair_launches += (unsigned long long)launches;
     g_pair_tiles += tiles;
     g_pair_ns += (unsigned long long)(ms_total * 1e6);
@@ -864,7 +969,7 @@ static int grow_single(VF val, float* P0, float* P1, const Geom& g, int* n_itera
     k_vor_init<<<(unsigned)ntiles, VB, 0, s>>>(val, P0, P1, g, V.L);
     CLE_LAUNCHED();
     int iters = 0;
-    CLE_TRY(grow_pairs(tab.get<BImg>(), 1, P0, P1, ntiles, V.L, V.S, true, 0, &iters, s));
+    CLE_TRY(grow_pairs(tab.get<BImg>(), 1, P0, P1, ntiles, V.L, V.S, vor_mode_single(g), 0, &iters, s));
     if (n_iterations) *n_iterations = iters;
     *result = vor_result_buffer(iters) ? P1 : P0;
     return CLE_OK;
@@ -1189,6 +1294,7 @@ struct BatchCapture {
     const BImg* dimgs = nullptr;
     const int64_t* dvs = nullptr;
     int n = 0;
+    int mode = 0;  // growth mode (VOR_LISTS, or raster for a one-image batch)
     int64_t ntiles = 0, vblocks = 0;
     float* Ap = nullptr;
     float* Fp = nullptr;
@@ -1230,6 +1336,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
         wtiles += grow_tile_count(im);
     }
     if (total == 0) return CLE_OK;
+    const int mode = n == 1 ? vor_mode_single(imgs[0].g) : VOR_LISTS;
     CclBatchSizes csz;
     ccl_batch_layout(imgs, &csz);
     // packed work images: F (float32: blurs, CCL labels, growth buffer B), A (growth
@@ -1326,6 +1433,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
         cap->dimgs = dimgs;
         cap->dvs = dvs;
         cap->n = n;
+        cap->mode = mode;
         cap->ntiles = ntiles;
         cap->vblocks = vblocks;
         cap->Ap = Ap;
@@ -1334,13 +1442,13 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
         cap->L = V.L;
         cap->S = V.S;
         if (ntiles > 0) {
-            launch_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, n == 1, 0, cap->graph_pairs, s);
+            launch_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, mode, 0, cap->graph_pairs, s);
             CLE_LAUNCHED();
             CLE_HIP(hipMemcpyAsync(cap->word_host, V.S.word, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         }
     } else if (pairs) {
         int steps = 0;
-        if (ntiles > 0) CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, n == 1, 0, &steps, s));
+        if (ntiles > 0) CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, mode, 0, &steps, s));
         if (n_steps) *n_steps = steps;
     } else {
         // tile solves: the labels go straight to the destinations
@@ -1361,7 +1469,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
         k_vor_init_batch<<<(unsigned)ntiles, VB, 0, s>>>(dimgs, (const int64_t*)(dev + ts_off), n, Gp, Ap, Fp, V.L);
         CLE_LAUNCHED();
         int steps = 0;
-        CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, n == 1, 0, &steps, s));
+        CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, mode, 0, &steps, s));
         if (n_steps) *n_steps = steps;
     }
     k_mask_out_batch<<<(unsigned)vblocks, 256, 0, s>>>(dimgs, dvs, n, Ap, Fp, V.S.iters, Gp);
@@ -1470,7 +1578,7 @@ int cle_plan_run(cle_plan* P, int* n_steps, void* stream) {
     const BatchCapture& c = P->cap;
     if (c.ntiles > 0 && P->word_host[0]) {
         // still growing after the recorded pairs: finish eagerly, then mask again
-        CLE_TRY(grow_pairs(c.dimgs, c.n, c.Ap, c.Fp, c.ntiles, c.L, c.S, c.n == 1, c.graph_pairs, &steps, s));
+        CLE_TRY(grow_pairs(c.dimgs, c.n, c.Ap, c.Fp, c.ntiles, c.L, c.S, c.mode, c.graph_pairs, &steps, s));
         k_mask_out_batch<<<(unsigned)c.vblocks, 256, 0, s>>>(c.dimgs, c.dvs, c.n, c.Ap, c.Fp, c.S.iters, c.Gp);
         CLE_LAUNCHED();
         CLE_HIP(hipStreamSynchronize(s));
