@@ -93,7 +93,8 @@ struct VorLists {
     VorItem* items;    // 3 lists of cap entries
     uint32_t* count;   // [0..2]: list sizes
     int32_t* stamp;    // cap: last pair the tile was queued for
-    int32_t* mark;     // 2 x cap (raster mode): mark[b & 1][t] == b iff tile t is active in pair b
+    int32_t* mark;     // 2 x cap (raster mode): mark[b & 1][t] == b iff tile t changed in pair b
+                       // (tiles with a seed: mark[1][t] == -1, "changed" before pair 0)
     uint8_t* settled;  // cap
     int64_t cap;       // tiles of all images
 };
@@ -260,18 +261,13 @@ struct LabelsAny {
     __device__ __forceinline__ void vec4(int64_t, float*) const {}
 };
 
-// single image: init every tile, queue the neighbourhood of tiles holding a positive
-// value for pair 0
+// single image: init every tile; a tile holding a positive value counts as changed
+// before pair 0 (its neighbourhood runs in pair 0).  Consecutive tiles go to one XCD.
 template <typename VF>
 __global__ __launch_bounds__(VB) void k_vor_init(VF val, float* A, float* B, Geom g, VorLists L) {
     const Tiles T = make_tiles(g);
-    if (vor_init_tile(val, A, B, g, T, blockIdx.x, L.settled + blockIdx.x) && threadIdx.x < 27) {
-        const int64_t t = blockIdx.x, k = threadIdx.x;
-        const int64_t tx = t % T.nx, ty = (t / T.nx) % T.ny, tz = t / (T.nx * T.ny);
-        const int64_t qx = tx + k % 3 - 1, qy = ty + (k / 3) % 3 - 1, qz = tz + k / 9 - 1;
-        if (qx >= 0 && qx < T.nx && qy >= 0 && qy < T.ny && qz >= 0 && qz < T.nz)
-            L.mark[(qz * T.ny + qy) * T.nx + qx] = 0;  // active in pair 0
-    }
+    const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+    if (vor_init_tile(val, A, B, g, T, t, L.settled + t) && threadIdx.x == 0) L.mark[L.cap + t] = -1;
 }
 
 // The staged region of a tile: the tile and a GH-voxel halo (z halo only in 3D).
@@ -622,15 +618,13 @@ __global__ __launch_bounds__(VB, 4) void k_vor_pair(const BImg* __restrict__ img
     vor_count_tiles(S, done);
 }
 
-// Pair b in raster mode (one image, blocks over runs of tpb consecutive tiles in raster
-// order, so that the halos of neighbouring tiles are read while still in L2): tile t runs iff
-// mark[b & 1][t] == b; a tile that changes marks its neighbourhood with b + 1 in the
-// other array (plain stores of one value: no atomics, no list).  Block 0 first closes
+// Pair b in raster mode for an image below 4096 tiles, one tile per block: tile t runs
+// iff a tile of its 3 x 3 x 3 neighbourhood changed in pair b - 1 (its stamp in
+// mark[(b - 1) & 1] is b - 1) and t is not settled; a tile that changes stamps
+// mark[b & 1][t] = b (one plain store; the neighbours gather).  Block 0 first closes
 // pair b - 1 as in the list mode.
-// One tile per block (images below 4096 tiles; larger ones take k_vor_pair_run).
 __device__ __forceinline__ void vor_pair_grid(const BImg* __restrict__ imgs, const float* __restrict__ src,
-                                              float* __restrict__ dst, int b, const VorLists& L, const VorState& S,
-                                              int tpb) {
+                                              float* __restrict__ dst, int b, const VorLists& L, const VorState& S) {
     __shared__ float lds[PairGeo<true>::NR > PairGeo<false>::NR ? PairGeo<true>::NR : PairGeo<false>::NR];
     __shared__ float lds2[PairGeo<true>::NR > PairGeo<false>::NR ? PairGeo<true>::NR : PairGeo<false>::NR];
     __shared__ int sbits[2];
@@ -647,74 +641,80 @@ __device__ __forceinline__ void vor_pair_grid(const BImg* __restrict__ imgs, con
     if (S.iters[0] != 0 || (b > 0 && prev[0] != 3u)) return;  // converged in an earlier pair
     const Geom g = imgs[0].g;
     const int nx = imgs[0].vnx, ny = imgs[0].vny, nz = imgs[0].vnz;
-    const int64_t ntiles = (int64_t)nx * ny * nz;
-    const int32_t* mark = L.mark + (int64_t)(b & 1) * L.cap;
-    int32_t* mark_next = L.mark + (int64_t)((b + 1) & 1) * L.cap;
-    int par = 0;
-    uint32_t done = 0;
-    // tiles [blockIdx.x * tpb, + tpb): consecutive in raster order
-    const int64_t t0 = (int64_t)blockIdx.x * tpb;
-    const int64_t t1 = t0 + tpb < ntiles ? t0 + tpb : ntiles;
-#pragma unroll 1
-    for (int64_t t = t0; t < t1; ++t) {
-        if (mark[t] != b || L.settled[t]) continue;
-        ++done;
-        const int txy = (int)(t / nx), tx = (int)(t - (int64_t)txy * nx), ty = txy % ny, tz = txy / ny;
-        // sbits[par] was last read two tiles ago, before a barrier of the previous tile
-        if (threadIdx.x == 0) sbits[par] = 0;
-        float v[VOR_KV];
-        int bits;
-        if (vor_3d(g)) {
-            const PairGeo<true> R(g, tx, ty, tz);
-            vor_stage_load<true>(src, R, v);
-            vor_stage_store<true>(v, lds);
-            __syncthreads();
-            bits = vor_pair_compute<true>(dst, R, lds, lds2, &sbits[par]);
-        } else {
-            const PairGeo<false> R(g, tx, ty, tz);
-            vor_stage_load<false>(src, R, v);
-            vor_stage_store<false>(v, lds);
-            __syncthreads();
-            bits = vor_pair_compute<false>(dst, R, lds, lds2, &sbits[par]);
+    const int64_t t = blockIdx.x;
+    const int txy = (int)(t / nx), tx = (int)(t - (int64_t)txy * nx), ty = txy % ny, tz = txy / ny;
+    const int32_t* chg_prev = L.mark + (int64_t)((b - 1) & 1) * L.cap;
+    bool c = false;
+    {
+        const int k = threadIdx.x & 63;
+        if (k < 27) {
+            const int qx = tx + k % 3 - 1, qy = ty + (k / 3) % 3 - 1, qz = tz + k / 9 - 1;
+            if (qx >= 0 && qx < nx && qy >= 0 && qy < ny && qz >= 0 && qz < nz)
+                c = chg_prev[((int64_t)qz * ny + qy) * nx + qx] == b - 1;
         }
-        par ^= 1;
-        if (bits & 3) {
-            if (threadIdx.x == 0) atomicOr(cur, (uint32_t)(bits & 3));
-            const int k = threadIdx.x;
-            if (k < 27) {
-                const int qx = tx + k % 3 - 1, qy = ty + (k / 3) % 3 - 1, qz = tz + k / 9 - 1;
-                if (qx >= 0 && qx < nx && qy >= 0 && qy < ny && qz >= 0 && qz < nz)
-                    mark_next[((int64_t)qz * ny + qy) * nx + qx] = b + 1;
-            }
-        } else if (!(bits & 4) && threadIdx.x == 0) {
-            L.settled[t] = 1;
-        }
-        // lds is rewritten by the next tile only after every thread passed the last
-        // barrier of vor_pair_compute (its lds reads come before it)
     }
-    vor_count_tiles(S, done);
+    if (!__ballot(c) || L.settled[t]) return;
+    if (threadIdx.x == 0) sbits[0] = 0;
+    float v[VOR_KV];
+    int bits;
+    if (vor_3d(g)) {
+        const PairGeo<true> R(g, tx, ty, tz);
+        vor_stage_load<true>(src, R, v);
+        vor_stage_store<true>(v, lds);
+        __syncthreads();
+        bits = vor_pair_compute<true>(dst, R, lds, lds2, &sbits[0]);
+    } else {
+        const PairGeo<false> R(g, tx, ty, tz);
+        vor_stage_load<false>(src, R, v);
+        vor_stage_store<false>(v, lds);
+        __syncthreads();
+        bits = vor_pair_compute<false>(dst, R, lds, lds2, &sbits[0]);
+    }
+    if (bits & 3) {
+        if (threadIdx.x == 0) {
+            atomicOr(cur, (uint32_t)(bits & 3));
+            L.mark[(int64_t)(b & 1) * L.cap + t] = b;
+        }
+    } else if (!(bits & 4) && threadIdx.x == 0) {
+        L.settled[t] = 1;
+    }
+    vor_count_tiles(S, 1);
 }
 
 __global__ __launch_bounds__(VB, 4) void k_vor_pair_grid(const BImg* __restrict__ imgs,
                                                          const float* __restrict__ src, float* __restrict__ dst,
                                                          int b, VorLists L, VorState S) {
-    vor_pair_grid(imgs, src, dst, b, L, S, 1);
+    vor_pair_grid(imgs, src, dst, b, L, S);
 }
 
-// Pair b in raster mode over runs of tpb (<= 64) consecutive tiles of one image (2D or
-// 3D by D3).  The block reads the marks and settled flags of its run once (one load
-// per lane, a ballot), then works through the active tiles software-pipelined: the
-// region of the next active tile is loaded into registers while the current tile is
-// computed from LDS.  Same marks, flags and results as vor_pair_grid (the loads read
-// src, which no block writes in this launch; only the block that processes a tile
-// sets its settled flag, after it read it).
+// Pair b in raster mode over runs: a run is up to VOR_RUN consecutive tiles of one
+// tile row (x) of one image (2D or 3D by D3); block b takes run xcd_tile(b), so the
+// runs of one XCD form a contiguous slab of rows, processed in raster order (the
+// halo rows a run shares with the runs above and below are read while in that XCD's
+// L2).  The block gathers, with three loads per lane and ballots, the change stamps of
+// the 3 x 3 rows of VOR_RUN + 2 tiles around its run and the settled flags of the
+// run: tile t runs iff a neighbour changed in pair b - 1 and t is not settled.  It
+// then works through the active tiles software-pipelined: the region of the next one
+// is loaded into registers while the current one is computed from LDS.  A tile that
+// changes stamps mark[b & 1][t] = b (one plain store).  The loads read src, which no
+// block writes in this launch; only the block that processes a tile sets its settled
+// flag, after it read it.
+constexpr int VOR_RUN = 16;
+
+// blocks of k_vor_pair_run for an image: runs of VOR_RUN tiles per tile row
+inline int64_t vor_run_blocks(const Geom& g) {
+    const Tiles T = make_tiles(g);
+    return T.nz * T.ny * ((T.nx + VOR_RUN - 1) / VOR_RUN);
+}
+
 #ifndef CLE_VOR_RUN_WAVES
 #define CLE_VOR_RUN_WAVES 5  // waves per SIMD of the 3D form: 96 VGPRs without spills (6 spills)
 #endif
 template <bool D3>
-__global__ __launch_bounds__(VB, D3 ? CLE_VOR_RUN_WAVES : 4) void k_vor_pair_run(const BImg* __restrict__ imgs, const float* __restrict__ src,
-                                                        float* __restrict__ dst, int b, VorLists L, VorState S,
-                                                        int tpb) {
+__global__ __launch_bounds__(VB, D3 ? CLE_VOR_RUN_WAVES : 4) void k_vor_pair_run(const BImg* __restrict__ imgs,
+                                                                               const float* __restrict__ src,
+                                                                               float* __restrict__ dst, int b,
+                                                                               VorLists L, VorState S) {
     using P = PairGeo<D3>;
     __shared__ float lds[P::NR];
     __shared__ float lds2[P::NR];
@@ -732,74 +732,86 @@ __global__ __launch_bounds__(VB, D3 ? CLE_VOR_RUN_WAVES : 4) void k_vor_pair_run
     if (S.iters[0] != 0 || (b > 0 && prev[0] != 3u)) return;  // converged in an earlier pair
     const Geom g = imgs[0].g;
     const int nx = imgs[0].vnx, ny = imgs[0].vny, nz = imgs[0].vnz;
-    const int64_t ntiles = (int64_t)nx * ny * nz;
-    const int32_t* mark = L.mark + (int64_t)(b & 1) * L.cap;
-    int32_t* mark_next = L.mark + (int64_t)((b + 1) & 1) * L.cap;
-    const int64_t t0 = (int64_t)blockIdx.x * tpb;
-    const int nrun = (int)(t0 + tpb < ntiles ? tpb : ntiles - t0);
+    const int segs = (nx + VOR_RUN - 1) / VOR_RUN;
+    const int64_t r = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t row = r / segs;
+    const int xs = (int)(r - row * segs) * VOR_RUN;
+    const int ty = (int)(row % ny), tz = (int)(row / ny);
+    const int nrun = nx - xs < VOR_RUN ? nx - xs : VOR_RUN;
+    const int64_t t0 = row * nx + xs;
+    const int32_t* chg_prev = L.mark + (int64_t)((b - 1) & 1) * L.cap;
     // active tiles of the run: bit k for tile t0 + k (every wave builds the same mask)
     uint64_t act;
     {
+        constexpr int RW = VOR_RUN + 2;  // stamps per neighbour row: x from xs - 1
         const int lane = threadIdx.x & 63;
-        bool a = false;
-        if (lane < nrun) a = mark[t0 + lane] == b && !L.settled[t0 + lane];
-        act = __ballot(a);
+        uint64_t w[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int idx = lane + 64 * k;  // row idx / RW (dz major, then dy), column idx % RW
+            const int rr = idx / RW, cx = idx - rr * RW;
+            const int qx = xs - 1 + cx, qy = ty + rr % 3 - 1, qz = tz + rr / 3 - 1;
+            bool c = false;
+            if (idx < 9 * RW && cx < nrun + 2 && qx >= 0 && qx < nx && qy >= 0 && qy < ny && qz >= 0 && qz < nz)
+                c = chg_prev[((int64_t)qz * ny + qy) * nx + qx] == b - 1;
+            w[k] = __ballot(c);
+        }
+        // OR of the 9 rows: bit c = some neighbour row changed at x = xs - 1 + c
+        uint32_t m = 0;
+#pragma unroll
+        for (int rr = 0; rr < 9; ++rr) {
+            const int o = rr * RW, wi = o >> 6, sh = o & 63;
+            uint64_t bitsr = w[wi] >> sh;
+            if (sh + RW > 64) bitsr |= w[wi + 1] << (64 - sh);
+            m |= (uint32_t)(bitsr & ((1u << RW) - 1));
+        }
+        const bool settled = lane < nrun && L.settled[t0 + lane] != 0;
+        const uint64_t st = __ballot(settled);
+        act = (uint64_t)((m | (m >> 1) | (m >> 2)) & ((1u << nrun) - 1)) & ~st;
     }
     if (!act) return;
     float v[P::KV];
-    int64_t t = t0 + __builtin_ctzll(act);
+    int tk = __builtin_ctzll(act);  // run index of the current tile
     act &= act - 1;
-    {
-        const int txy = (int)(t / nx), tx = (int)(t - (int64_t)txy * nx);
-        vor_stage_load<D3>(src, P(g, tx, txy % ny, txy / ny), v);
-    }
+    vor_stage_load<D3>(src, P(g, xs + tk, ty, tz), v);
     int par = 0;
     uint32_t done = 0;
     int acc = 0;  // change bits of the block's tiles: one atomic per block
 #pragma unroll 1
     for (;;) {
         ++done;
-        const int txy = (int)(t / nx), tx = (int)(t - (int64_t)txy * nx), ty = txy % ny, tz = txy / ny;
+        const int tx = xs + tk;
         const P R(g, tx, ty, tz);
         // sbits[par] was last read two tiles ago, before a barrier of the previous tile
         if (threadIdx.x == 0) sbits[par] = 0;
         vor_stage_store<D3>(v, lds);
         __syncthreads();
         const bool more = act != 0;
-        const int64_t tn = more ? t0 + __builtin_ctzll(act) : t;
+        const int tn = more ? __builtin_ctzll(act) : tk;
         act &= act - 1;
-        if (more) {
-            const int nxy = (int)(tn / nx), nxx = (int)(tn - (int64_t)nxy * nx);
-            vor_stage_load<D3>(src, P(g, nxx, nxy % ny, nxy / ny), v);
-        }
+        if (more) vor_stage_load<D3>(src, P(g, xs + tn, ty, tz), v);
         const int bits = vor_pair_compute<D3>(dst, R, lds, lds2, &sbits[par]);
         par ^= 1;
         if (bits & 3) {
             acc |= bits & 3;
-            const int k = threadIdx.x;
-            if (k < 27) {
-                const int qx = tx + k % 3 - 1, qy = ty + (k / 3) % 3 - 1, qz = tz + k / 9 - 1;
-                if (qx >= 0 && qx < nx && qy >= 0 && qy < ny && qz >= 0 && qz < nz)
-                    mark_next[((int64_t)qz * ny + qy) * nx + qx] = b + 1;
-            }
+            if (threadIdx.x == 0) L.mark[(int64_t)(b & 1) * L.cap + t0 + tk] = b;
         } else if (!(bits & 4) && threadIdx.x == 0) {
-            L.settled[t] = 1;
+            L.settled[t0 + tk] = 1;
         }
         // lds is rewritten by the next tile only after every thread passed the last
         // barrier of vor_pair_compute (its lds reads come before it)
         if (!more) break;
-        t = tn;
+        tk = tn;
     }
     if (threadIdx.x == 0 && acc) atomicOr(cur, (uint32_t)acc);
     vor_count_tiles(S, done);
 }
 
-// tiles per block of the raster mode: one for images below 4096 tiles, 16 above (fewer,
-// longer-lived blocks, and a block whose tiles are all inactive costs one dispatch
-// instead of 16: measured 15.8 -> 12.7 ms for voronoi_otsu_labeling of the
-// 256x1024x1024 stack, 162 -> 112 ms for the 512x2048x2048 one, and 1236 -> 931 ms for
-// 600 stacks of 32x128x128 .. 96x512x512 one by one)
-static int raster_tpb(int64_t ntiles) { return ntiles < 4096 ? 1 : 16; }
+// raster mode: one tile per block for images below VOR_RUN_MIN_TILES tiles, runs above
+// (fewer, longer-lived blocks, and a block whose tiles are all inactive costs one
+// dispatch instead of 16: measured 15.8 -> 12.7 ms for voronoi_otsu_labeling of the
+// 256x1024x1024 stack, 162 -> 112 ms for the 512x2048x2048 one)
+constexpr int64_t VOR_RUN_MIN_TILES = 4096;
 
 // Closes the last pair b of a chunk (one block): iteration counts of the images that
 // converged in it, and the "an image still grows" word the host reads.
@@ -852,8 +864,10 @@ struct VorScratch {
         p += (size_t)nimg * 16;
         L.settled = (uint8_t*)p;
         L.cap = cap;
-        // stamps -1, counters / words / per-image state 0 (contiguous from count)
-        CLE_TRY(fill_u32((uint32_t*)L.stamp, 0xFFFFFFFFu, (size_t)cap * 3, s));  // stamp and mark: -1
+        // stamps -1, change stamps -2 (no pair), counters / words / per-image state 0
+        // (contiguous from count)
+        CLE_TRY(fill_u32((uint32_t*)L.stamp, 0xFFFFFFFFu, (size_t)cap, s));
+        CLE_TRY(fill_u32((uint32_t*)L.mark, 0xFFFFFFFEu, (size_t)cap * 2, s));
         CLE_TRY(fill_u32(L.count, 0u, 16 + (size_t)VOR_NCTR * 16 + (size_t)nimg * 4, s));
         return CLE_OK;
     }
@@ -865,16 +879,17 @@ inline int vor_mode_single(const Geom& g) { return vor_3d(g) ? VOR_RASTER_3D : V
 
 // Launches pairs [b, b + count) and the close of the last one (no host synchronisation).
 static void launch_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t ntiles, const VorLists& L,
-                         const VorState& S, int mode, int b, int count, hipStream_t s) {
+                         const VorState& S, int mode, int64_t rblocks, int b, int count, hipStream_t s) {
     const bool raster = mode != VOR_LISTS;
-    const int tpb = raster_tpb(ntiles);
-    const unsigned grid =
-        raster ? (unsigned)((ntiles + tpb - 1) / tpb) : (unsigned)(ntiles < 4096 ? (ntiles > 0 ? ntiles : 1) : 4096);
+    const bool runs = raster && ntiles >= VOR_RUN_MIN_TILES;
+    const unsigned grid = runs     ? (unsigned)rblocks
+                          : raster ? (unsigned)ntiles
+                                   : (unsigned)(ntiles < 4096 ? (ntiles > 0 ? ntiles : 1) : 4096);
     for (int k = 0; k < count; ++k, ++b) {
-        if (raster && tpb > 1 && mode == VOR_RASTER_3D)
-            k_vor_pair_run<true><<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S, tpb);
-        else if (raster && tpb > 1)
-            k_vor_pair_run<false><<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S, tpb);
+        if (runs && mode == VOR_RASTER_3D)
+            k_vor_pair_run<true><<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S);
+        else if (runs)
+            k_vor_pair_run<false><<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S);
         else if (raster)
             k_vor_pair_grid<<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S);
         else k_vor_pair<<<grid, VB, 0, s>>>(dimgs, nimg, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S);
@@ -890,7 +905,7 @@ static void launch_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int6
 static std::atomic<unsigned long long> g_pair_launches{0}, g_pair_tiles{0}, g_pair_ns{0};
 
 static int grow_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t ntiles, const VorLists& L,
-                      const VorState& S, int mode, int b0, int* max_iters, hipStream_t s) {
+                      const VorState& S, int mode, int64_t rblocks, int b0, int* max_iters, hipStream_t s) {
     int b = b0;
     uint32_t hw[2] = {0, 0};
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -903,7 +918,7 @@ static int grow_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t
     for (;;) {
         const int chunk = b == 0 ? 2 : VOR_CHUNK;
         if (timed) (void)hipEventRecord(ev[0], s);
-        launch_pairs(dimgs, nimg, P0, P1, ntiles, L, S, mode, b, chunk, s);
+        launch_pairs(dimgs, nimg, P0, P1, ntiles, L, S, mode, rblocks, b, chunk, s);
         if (timed) (void)hipEventRecord(ev[1], s);
         b += chunk;
         launches += chunk;
@@ -969,7 +984,8 @@ static int grow_single(VF val, float* P0, float* P1, const Geom& g, int* n_itera
     k_vor_init<<<(unsigned)ntiles, VB, 0, s>>>(val, P0, P1, g, V.L);
     CLE_LAUNCHED();
     int iters = 0;
-    CLE_TRY(grow_pairs(tab.get<BImg>(), 1, P0, P1, ntiles, V.L, V.S, vor_mode_single(g), 0, &iters, s));
+    CLE_TRY(grow_pairs(tab.get<BImg>(), 1, P0, P1, ntiles, V.L, V.S, vor_mode_single(g), vor_run_blocks(g), 0, &iters,
+                       s));
     if (n_iterations) *n_iterations = iters;
     *result = vor_result_buffer(iters) ? P1 : P0;
     return CLE_OK;
@@ -1185,14 +1201,8 @@ __global__ __launch_bounds__(VB) void k_vor_init_batch(const BImg* __restrict__ 
     if (vor_init_tile(MaskedU8{G + im.off, B + im.off}, A + im.off, B + im.off, im.g, T, b - ts[j], L.settled + b) &&
         threadIdx.x < 64) {
         vor_queue(T, b - ts[j], j, ts[j], 0, L);
-        // raster-mode marks as well (a one-image batch grows in raster order)
-        if (threadIdx.x < 27) {
-            const int64_t t = b - ts[j], k = threadIdx.x;
-            const int64_t tx = t % T.nx, ty = (t / T.nx) % T.ny, tz = t / (T.nx * T.ny);
-            const int64_t qx = tx + k % 3 - 1, qy = ty + (k / 3) % 3 - 1, qz = tz + k / 9 - 1;
-            if (qx >= 0 && qx < T.nx && qy >= 0 && qy < T.ny && qz >= 0 && qz < T.nz)
-                L.mark[ts[j] + (qz * T.ny + qy) * T.nx + qx] = 0;
-        }
+        // the raster-mode stamp as well (a one-image batch grows in raster order)
+        if (threadIdx.x == 0) L.mark[L.cap + b] = -1;
     }
 }
 
@@ -1295,6 +1305,7 @@ struct BatchCapture {
     const int64_t* dvs = nullptr;
     int n = 0;
     int mode = 0;  // growth mode (VOR_LISTS, or raster for a one-image batch)
+    int64_t rblocks = 0;  // raster runs of a one-image batch
     int64_t ntiles = 0, vblocks = 0;
     float* Ap = nullptr;
     float* Fp = nullptr;
@@ -1337,6 +1348,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     }
     if (total == 0) return CLE_OK;
     const int mode = n == 1 ? vor_mode_single(imgs[0].g) : VOR_LISTS;
+    const int64_t rblocks = n == 1 ? vor_run_blocks(imgs[0].g) : 0;
     CclBatchSizes csz;
     ccl_batch_layout(imgs, &csz);
     // packed work images: F (float32: blurs, CCL labels, growth buffer B), A (growth
@@ -1434,6 +1446,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
         cap->dvs = dvs;
         cap->n = n;
         cap->mode = mode;
+        cap->rblocks = rblocks;
         cap->ntiles = ntiles;
         cap->vblocks = vblocks;
         cap->Ap = Ap;
@@ -1442,13 +1455,13 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
         cap->L = V.L;
         cap->S = V.S;
         if (ntiles > 0) {
-            launch_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, mode, 0, cap->graph_pairs, s);
+            launch_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, mode, rblocks, 0, cap->graph_pairs, s);
             CLE_LAUNCHED();
             CLE_HIP(hipMemcpyAsync(cap->word_host, V.S.word, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         }
     } else if (pairs) {
         int steps = 0;
-        if (ntiles > 0) CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, mode, 0, &steps, s));
+        if (ntiles > 0) CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, mode, rblocks, 0, &steps, s));
         if (n_steps) *n_steps = steps;
     } else {
         // tile solves: the labels go straight to the destinations
@@ -1469,7 +1482,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
         k_vor_init_batch<<<(unsigned)ntiles, VB, 0, s>>>(dimgs, (const int64_t*)(dev + ts_off), n, Gp, Ap, Fp, V.L);
         CLE_LAUNCHED();
         int steps = 0;
-        CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, mode, 0, &steps, s));
+        CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, mode, rblocks, 0, &steps, s));
         if (n_steps) *n_steps = steps;
     }
     k_mask_out_batch<<<(unsigned)vblocks, 256, 0, s>>>(dimgs, dvs, n, Ap, Fp, V.S.iters, Gp);
@@ -1578,7 +1591,7 @@ int cle_plan_run(cle_plan* P, int* n_steps, void* stream) {
     const BatchCapture& c = P->cap;
     if (c.ntiles > 0 && P->word_host[0]) {
         // still growing after the recorded pairs: finish eagerly, then mask again
-        CLE_TRY(grow_pairs(c.dimgs, c.n, c.Ap, c.Fp, c.ntiles, c.L, c.S, c.mode, c.graph_pairs, &steps, s));
+        CLE_TRY(grow_pairs(c.dimgs, c.n, c.Ap, c.Fp, c.ntiles, c.L, c.S, c.mode, c.rblocks, c.graph_pairs, &steps, s));
         k_mask_out_batch<<<(unsigned)c.vblocks, 256, 0, s>>>(c.dimgs, c.dvs, c.n, c.Ap, c.Fp, c.S.iters, c.Gp);
         CLE_LAUNCHED();
         CLE_HIP(hipStreamSynchronize(s));
