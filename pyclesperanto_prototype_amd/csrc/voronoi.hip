@@ -286,6 +286,7 @@ struct PairGeo {
     int64_t plane;
     int W;
     int xlo, xhi, ylo, yhi, zlo, zhi;  // the part inside the image, region coordinates [lo, hi)
+    bool interior;                     // the whole region lies inside the image
     __device__ __forceinline__ PairGeo(const Geom& g, int tx, int ty, int tz) {
         const int64_t x0 = (int64_t)tx * TX - GH, y0 = (int64_t)ty * TY - GH, z0 = (int64_t)tz * TZ - HZ;
         xlo = x0 < 0 ? (int)-x0 : 0;
@@ -297,6 +298,7 @@ struct PairGeo {
         plane = g.H * g.W;
         W = (int)g.W;
         obase = (z0 * g.H + y0) * g.W + x0;
+        interior = xlo == 0 && ylo == 0 && zlo == 0 && xhi == RX && yhi == RY && zhi == RZ;
     }
 };
 constexpr int VOR_KV = PairGeo<true>::KV > PairGeo<false>::KV ? PairGeo<true>::KV : PairGeo<false>::KV;
@@ -306,6 +308,20 @@ template <bool D3>
 __device__ __forceinline__ void vor_stage_load(const float* __restrict__ src, const PairGeo<D3>& R, float* v) {
     using P = PairGeo<D3>;
     const int tid = threadIdx.x;
+    if (R.interior) {  // no bounds to test
+        if (D3) {
+            const float* c = src + R.obase + ((tid / P::RX) * R.W + tid % P::RX);
+#pragma unroll
+            for (int rz = 0; rz < P::KV; ++rz) v[rz] = tid < P::RP ? c[rz * R.plane] : 0.f;
+        } else {
+#pragma unroll
+            for (int k = 0; k < P::KV; ++k) {
+                const int cc = tid + k * VB;
+                v[k] = cc < P::NR ? src[R.obase + ((cc / P::RX) * R.W + cc % P::RX)] : 0.f;
+            }
+        }
+        return;
+    }
     if (D3) {
         const int rx = tid % P::RX, ry = tid / P::RX;
         const bool col = tid < P::RP && rx >= R.xlo && rx < R.xhi && ry >= R.ylo && ry < R.yhi;
@@ -339,6 +355,139 @@ __device__ __forceinline__ void vor_stage_store(const float* v, float* lds) {
     }
 }
 
+// vor_pair_compute for a tile whose region lies inside the image: the same steps with
+// no clamping, so every LDS offset is a constant.
+template <bool D3>
+__device__ __forceinline__ int vor_pair_interior(float* __restrict__ dst, const PairGeo<D3>& R, const float* lds,
+                                                 float* lds2, int* sbits) {
+    using P = PairGeo<D3>;
+    constexpr int TX = P::TX, TY = P::TY, TZ = P::TZ, RX = P::RX, RY = P::RY, RZ = P::RZ, RP = P::RP, HZ = P::HZ;
+    const int tid = threadIdx.x;
+    {
+        bool zero = false;
+#pragma unroll
+        for (int k = 0; k < (TX * TY * TZ) / VB; ++k) {
+            const int c = tid + k * VB;
+            const int rx = c % TX + GH, ry = (c / TX) % TY + GH, rz = c / (TX * TY) + HZ;
+            zero |= lds[(rz * RY + ry) * RX + rx] == 0.f;
+        }
+        if (!__syncthreads_or(zero ? 1 : 0)) {
+#pragma unroll
+            for (int k = 0; k < (TX * TY * TZ) / VB; ++k) {
+                const int c = tid + k * VB;
+                const int rx = c % TX + GH, ry = (c / TX) % TY + GH, rz = c / (TX * TY) + HZ;
+                dst[R.obase + rz * R.plane + (ry * R.W + rx)] = lds[(rz * RY + ry) * RX + rx];
+            }
+            __syncthreads();
+            return 0;
+        }
+    }
+    int bits = 0;
+    if (D3) {
+        constexpr int SX = TX + 2, SY = TY + 2;
+        if (tid < SX * SY) {
+            const int rx = tid % SX + 1, ry = tid / SX + 1;
+            const int c0 = ry * RX + rx;
+            const float* p = lds + c0;
+            bool anyzero = false;
+#pragma unroll
+            for (int rz = 1; rz < RZ - 1; ++rz) anyzero |= p[rz * RP] == 0.f;
+            if (!anyzero) {
+#pragma unroll
+                for (int rz = 1; rz < RZ - 1; ++rz) lds2[rz * RP + c0] = p[rz * RP];
+            } else {
+                auto m9 = [&](int rz) {
+                    const float* q = p + rz * RP;
+                    float m = fmaxf(fmaxf(q[-RX - 1], q[-RX]), q[-RX + 1]);
+                    m = fmaxf(m, fmaxf(fmaxf(q[-1], q[0]), q[1]));
+                    return fmaxf(m, fmaxf(fmaxf(q[RX - 1], q[RX]), q[RX + 1]));
+                };
+                const bool inner_xy = rx >= GH && rx < TX + GH && ry >= GH && ry < TY + GH;
+                float mprev = m9(0), mcur = m9(1);
+#pragma unroll
+                for (int rz = 1; rz < RZ - 1; ++rz) {
+                    const float mnext = m9(rz + 1);
+                    const float v = p[rz * RP];
+                    float out = v;
+                    if (v == 0.f) {
+                        out = fmaxf(0.f, fmaxf(fmaxf(mprev, mcur), mnext));
+                        if (inner_xy && rz >= GH && rz < TZ + GH && out != 0.f) bits |= 1;
+                    }
+                    lds2[rz * RP + c0] = out;
+                    mprev = mcur;
+                    mcur = mnext;
+                }
+            }
+        }
+        __syncthreads();
+        {
+            const int col = tid % (TX * TY), half = tid / (TX * TY);
+            const int rx = col % TX + GH, ry = col / TX + GH;
+            const int c0 = ry * RX + rx;
+            float* dcol = dst + R.obase + (ry * R.W + rx);
+#pragma unroll
+            for (int k = 0; k < TZ / 2; ++k) {
+                const int rz = GH + half * (TZ / 2) + k;
+                const float* q = lds2 + rz * RP + c0;
+                const float v = q[0];
+                float out = v;
+                if (v == 0.f) {
+                    float best = fmaxf(0.f, q[-1]);
+                    best = fmaxf(best, q[1]);
+                    best = fmaxf(best, q[-RX]);
+                    best = fmaxf(best, q[RX]);
+                    best = fmaxf(best, q[-RP]);
+                    best = fmaxf(best, q[RP]);
+                    out = best;
+                    bits |= out != 0.f ? 2 : 4;
+                }
+                dcol[rz * R.plane] = out;
+            }
+        }
+    } else {
+        constexpr int SX = TX + 2, SY = TY + 2, NS = SX * SY;
+#pragma unroll 2
+        for (int c = tid; c < NS; c += VB) {
+            const int rx = c % SX + 1, ry = c / SX + 1;
+            const int rc = ry * RX + rx;
+            const float* q = lds + rc;
+            const float v = q[0];
+            float out = v;
+            if (v == 0.f) {
+                float best = fmaxf(0.f, fmaxf(fmaxf(q[-RX - 1], q[-RX]), q[-RX + 1]));
+                best = fmaxf(best, fmaxf(q[-1], q[1]));
+                best = fmaxf(best, fmaxf(fmaxf(q[RX - 1], q[RX]), q[RX + 1]));
+                out = best;
+                const bool inner = rx >= GH && rx < TX + GH && ry >= GH && ry < TY + GH;
+                if (inner && out != 0.f) bits |= 1;
+            }
+            lds2[rc] = out;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < (TX * TY) / VB; ++k) {
+            const int c = tid + k * VB;
+            const int rx = c % TX + GH, ry = c / TX + GH;
+            const int rc = ry * RX + rx;
+            const float* q = lds2 + rc;
+            const float v = q[0];
+            float out = v;
+            if (v == 0.f) {
+                float best = fmaxf(0.f, q[-1]);
+                best = fmaxf(best, q[1]);
+                best = fmaxf(best, q[-RX]);
+                best = fmaxf(best, q[RX]);
+                out = best;
+                bits |= out != 0.f ? 2 : 4;
+            }
+            dst[R.obase + (ry * R.W + rx)] = out;
+        }
+    }
+    if (bits) atomicOr(sbits, bits);
+    __syncthreads();
+    return *sbits;
+}
+
 // One pair of steps (box, then diamond) on a staged tile: lds holds the region, lds2
 // receives the box step's result, the diamond step is written to dst.  Returns, in
 // every thread, bit 0: a tile voxel changed in the box step, bit 1: in the diamond
@@ -347,6 +496,7 @@ __device__ __forceinline__ void vor_stage_store(const float* v, float* lds) {
 template <bool D3>
 __device__ __forceinline__ int vor_pair_compute(float* __restrict__ dst, const PairGeo<D3>& R, const float* lds,
                                                 float* lds2, int* sbits) {
+    if (R.interior) return vor_pair_interior<D3>(dst, R, lds, lds2, sbits);
     using P = PairGeo<D3>;
     constexpr int TX = P::TX, TY = P::TY, TZ = P::TZ, RX = P::RX, RZ = P::RZ, RP = P::RP;
     const int tid = threadIdx.x;
@@ -534,7 +684,7 @@ __device__ __forceinline__ bool vor_next_item(uint32_t& e, uint32_t n, const Vor
 // list and the change bits that pair b + 1 will fill (both last used by pair b - 2 /
 // b - 1, untouched during pair b).  Changed tiles are collected in LDS and their
 // neighbourhoods queued by the whole block at once.
-__global__ __launch_bounds__(VB, 4) void k_vor_pair(const BImg* __restrict__ imgs, int nimg,
+__global__ __launch_bounds__(VB, 3) void k_vor_pair(const BImg* __restrict__ imgs, int nimg,
                                                     const float* __restrict__ src, float* __restrict__ dst, int b,
                                                     VorLists L, VorState S) {
     __shared__ float lds[PairGeo<true>::NR > PairGeo<false>::NR ? PairGeo<true>::NR : PairGeo<false>::NR];
@@ -708,7 +858,7 @@ inline int64_t vor_run_blocks(const Geom& g) {
 }
 
 #ifndef CLE_VOR_RUN_WAVES
-#define CLE_VOR_RUN_WAVES 5  // waves per SIMD of the 3D form: 96 VGPRs without spills (6 spills)
+#define CLE_VOR_RUN_WAVES 4  // waves per SIMD of the 3D form (128 VGPRs)
 #endif
 template <bool D3>
 __global__ __launch_bounds__(VB, D3 ? CLE_VOR_RUN_WAVES : 4) void k_vor_pair_run(const BImg* __restrict__ imgs,
