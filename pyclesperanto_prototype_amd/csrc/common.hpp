@@ -226,19 +226,41 @@ __device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t
     const int64_t x0 = (t % ntx) * MX_TX, y0 = ((t / ntx) % nty) * MX_TY, zc = t / (ntx * nty);
     const bool d3 = g.ndim == 3;
     const int64_t z0 = d3 ? zc * MX_ZC : 0, z1 = d3 ? (z0 + MX_ZC < g.D ? z0 + MX_ZC : g.D) : 1;
-    // plane zz -> ring slot zz % 3 (voxels inside the image only)
-    auto stage = [&](int64_t zz) {
-        float* pl = ring + (zz % 3) * MX_RP;
-        for (int c = tid; c < MX_RP; c += 256) {
+    // the staged x / y halo lies inside the image: no bounds to test, no clamping
+    const bool inner = x0 >= 1 && y0 >= 1 && x0 + MX_TX < g.W && y0 + MX_TY < g.H;
+    // plane zz -> registers (voxels inside the image only), then -> ring slot zz % 3;
+    // the loads of plane z + 2 are in flight while slice z is computed
+    constexpr int NPT = (MX_RP + 255) / 256;
+    float r[NPT];
+    auto load = [&](int64_t zz) {
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int c = tid + k * 256;
             const int64_t xx = x0 - 1 + c % MX_RX, yy = y0 - 1 + c / MX_RX;
-            if (xx >= 0 && xx < g.W && yy >= 0 && yy < g.H) pl[c] = in((zz * g.H + yy) * g.W + xx);
+            const bool ok = c < MX_RP && (inner || (xx >= 0 && xx < g.W && yy >= 0 && yy < g.H));
+            r[k] = ok ? in((zz * g.H + yy) * g.W + xx) : 0.f;
         }
     };
-    if (z0 > 0) stage(z0 - 1);
-    stage(z0);
-    if (z0 + 1 < g.D && d3) stage(z0 + 1);
+    auto store = [&](int64_t zz) {
+        float* pl = ring + (zz % 3) * MX_RP;
+#pragma unroll
+        for (int k = 0; k < NPT; ++k)
+            if (tid + k * 256 < MX_RP) pl[tid + k * 256] = r[k];
+    };
+    if (z0 > 0) {
+        load(z0 - 1);
+        store(z0 - 1);
+    }
+    load(z0);
+    store(z0);
+    if (z0 + 1 < g.D && d3) {
+        load(z0 + 1);
+        store(z0 + 1);
+    }
     float* C = ring + 3 * MX_RP;  // column maxima of the current z
     for (int64_t z = z0; z < z1; ++z) {
+        const bool pre = d3 && z + 2 < z1 + 1 && z + 2 < g.D;
+        if (pre) load(z + 2);
         __syncthreads();
         // ring planes of z - 1 (sm, none below 0), z (s0), z + 1 (sp; none past the
         // last slice, where the visit reads the centre plane again)
@@ -250,19 +272,28 @@ __device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t
         // before the image's first column / row are never visited (NaN: nothing), those
         // past its last column / row are the clamped last one (as the visit's clamped
         // offsets read them)
-        for (int c = tid; c < MX_RP; c += 256) {
-            const int cx = c % MX_RX, cy = c / MX_RX;
-            const int64_t xx = x0 - 1 + cx, yy = y0 - 1 + cy;
-            float m = __builtin_nanf("");
-            if (xx >= 0 && yy >= 0) {
-                const int qx = xx < g.W ? cx : (int)(g.W - x0);
-                const int qy = yy < g.H ? cy : (int)(g.H - y0);
-                const int q = qy * MX_RX + qx;
-                m = P0[q];
-                if (sm >= 0) m = fmaxf(m, ring[sm * MX_RP + q]);
-                if (sp >= 0) m = fmaxf(m, ring[sp * MX_RP + q]);
+        if (inner) {
+            for (int c = tid; c < MX_RP; c += 256) {
+                float m = P0[c];
+                if (sm >= 0) m = fmaxf(m, ring[sm * MX_RP + c]);
+                if (sp >= 0) m = fmaxf(m, ring[sp * MX_RP + c]);
+                C[c] = m;
             }
-            C[c] = m;
+        } else {
+            for (int c = tid; c < MX_RP; c += 256) {
+                const int cx = c % MX_RX, cy = c / MX_RX;
+                const int64_t xx = x0 - 1 + cx, yy = y0 - 1 + cy;
+                float m = __builtin_nanf("");
+                if (xx >= 0 && yy >= 0) {
+                    const int qx = xx < g.W ? cx : (int)(g.W - x0);
+                    const int qy = yy < g.H ? cy : (int)(g.H - y0);
+                    const int q = qy * MX_RX + qx;
+                    m = P0[q];
+                    if (sm >= 0) m = fmaxf(m, ring[sm * MX_RP + q]);
+                    if (sp >= 0) m = fmaxf(m, ring[sp * MX_RP + q]);
+                }
+                C[c] = m;
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -295,7 +326,7 @@ __device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t
             out((z * g.H + y0 + iy) * g.W + x0 + ix, self);
         }
         __syncthreads();
-        if (d3 && z + 2 < z1 + 1 && z + 2 < g.D) stage(z + 2);
+        if (pre) store(z + 2);
     }
 }
 

@@ -112,7 +112,8 @@ __global__ void k_onlyzero_max(DBuf s, DBuf flag, DBuf d, Geom g, int diamond) {
 template <typename LD>
 __global__ __launch_bounds__(256) void k_maxima_t(LD in, DBuf d, Geom g) {
     __shared__ float ring[MX_LDS];
-    maxima_tile(in, g, blockIdx.x, ring, [&](int64_t i, bool m) { st(d, i, m ? 1.f : 0.f); });
+    // consecutive tiles (which share halo lines) on one XCD
+    maxima_tile(in, g, xcd_tile(blockIdx.x, gridDim.x), ring, [&](int64_t i, bool m) { st(d, i, m ? 1.f : 0.f); });
 }
 
 struct LdAnyE {
