@@ -181,9 +181,9 @@ int cle_voronoi_otsu_labeling(const cle_buf* src, const cle_buf* dst,
  * for callers that cannot allocate device memory -- code under stream capture, tight
  * serving loops, harnesses that forbid it.  Same kernels and result; its scratch is a
  * fixed device array of CLE_NOALLOC_SCRATCH_BYTES per GPU (held from the library's
- * load until the process ends), which bounds the image to about 6 million voxels
- * (roughly 11 bytes of scratch per voxel); a larger image fails with CLE_E_VALUE
- * before anything runs.  All calls on one GPU share that array: calls that overlap on
+ * load until the process ends), which bounds the image to about 4.5 million voxels in
+ * 2D and 5.5 million in 3D (roughly 15 / 12 bytes of scratch per voxel); a larger
+ * image fails with CLE_E_VALUE before anything runs.  All calls on one GPU share that array: calls that overlap on
  * one GPU (other threads, other streams) race on it.  The Voronoi growth always runs
  * as pairs of image-wide steps here (cle_set_growth_method does not apply). */
 #define CLE_NOALLOC_SCRATCH_BYTES (64u << 20)

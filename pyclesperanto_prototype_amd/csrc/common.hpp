@@ -417,6 +417,20 @@ void ccl_batch_layout(std::vector<BImg>& imgs, CclBatchSizes* sz);
 int plan_ccl_batch(const std::vector<BImg>& imgs, size_t imgs_off, const CclBatchSizes& sz, const uint8_t* S,
                    float* F, Blob& blob, std::vector<Launch>& out, std::vector<std::shared_ptr<void>>& keep,
                    hipStream_t s);
+// Box CCL of `binary` whose write pass initialises the masked Voronoi growth instead of
+// storing labels: v = (mask - 1) + label (float32, the reference's add order) into A
+// and B, and seed[tile] = -1 for every growth tile (tx x ty x tz voxels, nx x ny tiles
+// per slice / row) holding a positive v.  The mask is uint8; A, B 16-byte and the mask
+// 4-byte aligned.
+struct GrowInitOut {
+    float* A;
+    float* B;
+    const uint8_t* mask;
+    int32_t* seed;
+    int tx, ty, tz;
+    int64_t nx, ny;
+};
+int ccl_box_growth_init(DBuf binary, const Geom& g, const GrowInitOut& out, hipStream_t s);
 // single-image voronoi_otsu_labeling (the body of cle_voronoi_otsu_labeling)
 int voronoi_otsu_single(const cle_buf* src, const cle_buf* dst, float spot_sigma, float outline_sigma,
                         hipStream_t s);

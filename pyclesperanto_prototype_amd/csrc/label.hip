@@ -456,6 +456,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_ccl_final(const uint32_t* L, con
 
 // store4 writes voxels i..i+3 with one 16-byte store (i a multiple of 4, p 16-byte aligned)
 struct StLabI32 {
+    static constexpr bool kCoords = false;
     int32_t* p;
     __device__ __forceinline__ static int32_t cv(float v) { return sat_cast<int32_t>(v); }
     __device__ __forceinline__ void operator()(int64_t i, float v) const { p[i] = cv(v); }
@@ -464,6 +465,7 @@ struct StLabI32 {
     }
 };
 struct StLabU32 {
+    static constexpr bool kCoords = false;
     uint32_t* p;
     __device__ __forceinline__ static uint32_t cv(float v) { return sat_cast<uint32_t>(v); }
     __device__ __forceinline__ void operator()(int64_t i, float v) const { p[i] = cv(v); }
@@ -472,6 +474,7 @@ struct StLabU32 {
     }
 };
 struct StLabF32 {
+    static constexpr bool kCoords = false;
     float* p;
     __device__ __forceinline__ void operator()(int64_t i, float v) const { p[i] = v; }
     __device__ __forceinline__ void store4(int64_t i, float a, float b, float c, float d) const {
@@ -479,6 +482,7 @@ struct StLabF32 {
     }
 };
 struct StLabAny {
+    static constexpr bool kCoords = false;
     DBuf b;
     __device__ __forceinline__ void operator()(int64_t i, float v) const { st(b, i, v); }
     __device__ __forceinline__ void store4(int64_t i, float a, float b_, float c, float d) const {
@@ -486,6 +490,31 @@ struct StLabAny {
         st(b, i + 1, b_);
         st(b, i + 2, c);
         st(b, i + 3, d);
+    }
+};
+// growth init instead of labels (GrowInitOut); put / put4 take the voxel coordinates
+struct StGrowInit {
+    static constexpr bool kCoords = true;
+    GrowInitOut o;
+    __device__ __forceinline__ void seed(int64_t x, int64_t y, int64_t z) const {
+        o.seed[((z / o.tz) * o.ny + y / o.ty) * o.nx + x / o.tx] = -1;
+    }
+    __device__ __forceinline__ void put(int64_t i, int64_t x, int64_t y, int64_t z, float v) const {
+        const float w = ((float)o.mask[i] - 1.f) + v;
+        o.A[i] = w;
+        o.B[i] = w;
+        if (w > 0.f) seed(x, y, z);
+    }
+    // voxels x .. x + 3 (x a multiple of 4: one growth tile, whose x extent is a
+    // multiple of 4)
+    __device__ __forceinline__ void put4(int64_t i, int64_t x, int64_t y, int64_t z, float a, float b, float c,
+                                         float d) const {
+        const uchar4 m = *(const uchar4*)(o.mask + i);
+        const float4 w = make_float4(((float)m.x - 1.f) + a, ((float)m.y - 1.f) + b, ((float)m.z - 1.f) + c,
+                                     ((float)m.w - 1.f) + d);
+        *(float4*)(o.A + i) = w;
+        *(float4*)(o.B + i) = w;
+        if (w.x > 0.f || w.y > 0.f || w.z > 0.f || w.w > 0.f) seed(x, y, z);
     }
 };
 
@@ -882,7 +911,17 @@ __global__ void k_bccl_write(const uint8_t* __restrict__ M, Geom g, BGrid B, con
                     const float v0 = (m2 >> sh) & 1u ? f0 : 0.f, v1 = (m2 >> (sh + 1)) & 1u ? f0 : 0.f;
                     const float v2 = (m2 >> (sh + 8)) & 1u ? f1 : 0.f, v3 = (m2 >> (sh + 9)) & 1u ? f1 : 0.f;
                     const int64_t i = ((z0 + dz) * g.H + y0 + dy) * g.W + x;
-                    if constexpr (VEC) {
+                    if constexpr (ST::kCoords) {
+                        const int64_t yy = y0 + dy, zz = z0 + dz;
+                        if constexpr (VEC) {
+                            out.put4(i, x, yy, zz, v0, v1, v2, v3);
+                        } else {
+                            out.put(i, x, yy, zz, v0);
+                            if (x + 1 < g.W) out.put(i + 1, x + 1, yy, zz, v1);
+                            if (x + 2 < g.W) out.put(i + 2, x + 2, yy, zz, v2);
+                            if (x + 3 < g.W) out.put(i + 3, x + 3, yy, zz, v3);
+                        }
+                    } else if constexpr (VEC) {
                         out.store4(i, v0, v1, v2, v3);
                     } else {
                         out(i, v0);
@@ -936,8 +975,10 @@ struct BcclSizes {
     }
 };
 
+// gi: write the growth init (GrowInitOut) instead of labels into dst
 template <typename BT, typename FG>
-static int bccl_run(FG fg, const Geom& g, DBuf dst, int options, uint32_t* n_labels_dev, hipStream_t s) {
+static int bccl_run(FG fg, const Geom& g, DBuf dst, int options, uint32_t* n_labels_dev, hipStream_t s,
+                    const GrowInitOut* gi = nullptr) {
     const int compat = (options & CLE_FP32_KEY_COMPAT) ? 1 : 0;
     const BcclSizes<BT> Z(g, compat);
     const BGrid B = Z.B;
@@ -963,7 +1004,16 @@ static int bccl_run(FG fg, const Geom& g, DBuf dst, int options, uint32_t* n_lab
     CLE_TRY(rt.scan(n_labels_dev, s));
     k_ccl_final<BT><<<root_grid(nt), CT_THREADS, 0, s>>>(l, k, roots, nr, compat, rt.bm.get<uint32_t>(),
                                                           rt.prefix.get<uint32_t>(), Fs.get<uint32_t>(), nt);
-    launch_bccl_write<BT::BZ>(Ms.get<uint8_t>(), g, B, l, Fs.get<uint32_t>(), dst, s);
+    if (gi) {
+        const int64_t brows = B.D * B.H;
+        const unsigned grid = (unsigned)(brows < 262144 ? brows : 262144);
+        if ((g.W & 3) == 0)
+            k_bccl_write<BT::BZ, true><<<grid, 256, 0, s>>>(Ms.get<uint8_t>(), g, B, l, Fs.get<uint32_t>(), StGrowInit{*gi});
+        else
+            k_bccl_write<BT::BZ, false><<<grid, 256, 0, s>>>(Ms.get<uint8_t>(), g, B, l, Fs.get<uint32_t>(), StGrowInit{*gi});
+    } else {
+        launch_bccl_write<BT::BZ>(Ms.get<uint8_t>(), g, B, l, Fs.get<uint32_t>(), dst, s);
+    }
     CLE_LAUNCHED();
     return CLE_OK;
 }
@@ -1195,6 +1245,22 @@ int ccl_impl(DBuf binary, const Geom& g, DBuf dst, int connectivity, int options
                        : bccl_run<BlkTile2D>(fg, g, dst, options, n_labels_dev, s);
     return three_d ? ccl_run<Tile3D>(fg, g, dst, connectivity, options, n_labels_dev, s)
                    : ccl_run<Tile2D>(fg, g, dst, connectivity, options, n_labels_dev, s);
+}
+
+int ccl_box_growth_init(DBuf binary, const Geom& g, const GrowInitOut& out, hipStream_t s) {
+    if (g.n() == 0) return CLE_OK;
+    if (((uintptr_t)out.A & 15) || ((uintptr_t)out.B & 15) || ((uintptr_t)out.mask & 3))
+        return fail(CLE_E_INTERNAL, "growth init: misaligned buffers");
+    const bool three_d = g.ndim == 3 && g.D > 1;
+    const DBuf none{nullptr, CLE_F32};
+    if (binary.dt == CLE_U8 || binary.dt == CLE_I8) {
+        FgU8 fg{(const uint8_t*)binary.p};
+        return three_d ? bccl_run<BlkTile3D>(fg, g, none, CLE_FP32_KEY_COMPAT, nullptr, s, &out)
+                       : bccl_run<BlkTile2D>(fg, g, none, CLE_FP32_KEY_COMPAT, nullptr, s, &out);
+    }
+    FgAny fg{binary};
+    return three_d ? bccl_run<BlkTile3D>(fg, g, none, CLE_FP32_KEY_COMPAT, nullptr, s, &out)
+                   : bccl_run<BlkTile2D>(fg, g, none, CLE_FP32_KEY_COMPAT, nullptr, s, &out);
 }
 
 size_t ccl_box_scratch_bytes(const Geom& g, int options) {

@@ -54,8 +54,8 @@ int cle_voronoi_otsu_labeling_noalloc(const cle_buf* src, const cle_buf* dst, fl
     if (need > kScratchBytes)
         return fail(CLE_E_VALUE,
                     "voronoi_otsu_labeling_noalloc: a (%lld, %lld, %lld) image needs %zu bytes of scratch, more than "
-                    "the fixed %zu (about %zu voxels at most)",
-                    (long long)g.D, (long long)g.H, (long long)g.W, need, kScratchBytes, kScratchBytes / 11);
+                    "the fixed %zu (roughly 12 to 15 bytes per voxel)",
+                    (long long)g.D, (long long)g.H, (long long)g.W, need, kScratchBytes);
     void* base = nullptr;
     CLE_TRY(scratch_address(&base));
     FixedScratch fixed((char*)base, kScratchBytes);

@@ -993,7 +993,7 @@ struct VorScratch {
     VorState S;
     static size_t bytes(int64_t ntiles, int nimg) {
         const int64_t cap = ntiles > 0 ? ntiles : 1;
-        return (size_t)cap * 3 * 16 + (size_t)cap * 12 + 64 + (size_t)VOR_NCTR * 64 + (size_t)nimg * 16 + (size_t)cap;
+        return (size_t)cap * 3 * 16 + (size_t)cap * 12 + 64 + (size_t)VOR_NCTR * 64 + (size_t)nimg * 16 + (size_t)cap + 4;
     }
     int alloc(int64_t ntiles, int nimg, hipStream_t s) {
         const int64_t cap = ntiles > 0 ? ntiles : 1;
@@ -1019,6 +1019,7 @@ struct VorScratch {
         CLE_TRY(fill_u32((uint32_t*)L.stamp, 0xFFFFFFFFu, (size_t)cap, s));
         CLE_TRY(fill_u32((uint32_t*)L.mark, 0xFFFFFFFEu, (size_t)cap * 2, s));
         CLE_TRY(fill_u32(L.count, 0u, 16 + (size_t)VOR_NCTR * 16 + (size_t)nimg * 4, s));
+        CLE_TRY(fill_u32((uint32_t*)L.settled, 0u, ((size_t)cap + 3) / 4, s));  // nothing settled yet
         return CLE_OK;
     }
 };
@@ -1210,12 +1211,78 @@ static int g_growth_method = 1;  // 0: tile solves, 1: pairs of steps
 
 // labels of `binary` grown inside `mask`, written (masked) to dst.  `B` holds
 // scratch for the CCL labels (float32, n voxels).
+// settled[t] = 1 iff growth tile t holds no mask voxel (so no zero of the initial
+// state: outside the mask it is -1); one wavefront per tile, consecutive tiles on one
+// XCD.  Tiles with a mask voxel stay 0 (the fill of VorScratch).
+__global__ __launch_bounds__(256) void k_vor_settle(const uint8_t* __restrict__ mask, Geom g, Tiles T,
+                                                   uint8_t* settled) {
+    const int64_t ntiles = T.count();
+    const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    if (t >= ntiles) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t tx = t % T.nx, ty = (t / T.nx) % T.ny, tz = t / (T.nx * T.ny);
+    const int64_t x0 = tx * T.tx, y0 = ty * T.ty, z0 = tz * T.tz;
+    bool any = false;
+    if ((g.W & 15) == 0 && ((uintptr_t)mask & 15) == 0 && x0 + T.tx <= g.W) {
+        // 16-byte pieces of the tile's rows: lane -> (row, piece)
+        const int pieces = T.tx / 16, rows = T.ty * T.tz;
+        for (int k = lane; k < rows * pieces; k += 64) {
+            const int r = k / pieces, pc = k - r * pieces;
+            const int64_t y = y0 + r % T.ty, z = z0 + r / T.ty;
+            if (y < g.H && z < g.D) {
+                const uint4 m = *(const uint4*)(mask + (z * g.H + y) * g.W + x0 + 16 * pc);
+                any |= (m.x | m.y | m.z | m.w) != 0u;
+            }
+        }
+    } else {
+        for (int k = lane; k < T.tx * T.ty * T.tz; k += 64) {
+            const int64_t x = x0 + k % T.tx, y = y0 + (k / T.tx) % T.ty, z = z0 + k / (T.tx * T.ty);
+            if (x < g.W && y < g.H && z < g.D) any |= mask[(z * g.H + y) * g.W + x] != 0;
+        }
+    }
+    if (!__ballot(any) && lane == 0) settled[t] = 1;
+}
+
+// Masked growth whose initial state comes straight from the box CCL of the seeds
+// (ccl_box_growth_init writes v = (mask - 1) + label into A and B and stamps the
+// seed tiles): no separate init pass.  Returns the buffer holding the result.
+static int grow_masked_fused(DBuf binary, const uint8_t* mask, float* A, float* B, const Geom& g,
+                             int* n_iterations, const float** result, hipStream_t s) {
+    const Tiles T = make_tiles(g);
+    const int64_t ntiles = T.count();
+    BImg im;
+    memset(&im, 0, sizeof(im));
+    im.g = g;
+    CLE_TRY(vor_set_tiles(im));
+    VorScratch V;
+    CLE_TRY(V.alloc(ntiles, 1, s));
+    Scratch tab;
+    CLE_TRY(tab.alloc(sizeof(BImg), s));
+    k_put_img<<<1, 1, 0, s>>>(tab.get<BImg>(), im);
+    CLE_LAUNCHED();
+    k_vor_settle<<<(unsigned)((ntiles + 3) / 4), 256, 0, s>>>(mask, g, T, V.L.settled);
+    CLE_LAUNCHED();
+    const GrowInitOut o{A, B, mask, V.L.mark + V.L.cap, T.tx, T.ty, T.tz, T.nx, T.ny};
+    CLE_TRY(ccl_box_growth_init(binary, g, o, s));
+    int iters = 0;
+    CLE_TRY(grow_pairs(tab.get<BImg>(), 1, A, B, ntiles, V.L, V.S, vor_mode_single(g), vor_run_blocks(g), 0, &iters,
+                       s));
+    if (n_iterations) *n_iterations = iters;
+    *result = vor_result_buffer(iters) ? B : A;
+    return CLE_OK;
+}
+
+// masked_voronoi_labeling (_tier5/_masked_voronoi_labeling.py:44-74): label the seeds (box
+// CCL), grow inside the mask, mask the result into dst.  B: a float32 work buffer of
+// the image's size.
 static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, float* B, int* n_iterations,
                                hipStream_t s) {
     const int64_t n = g.n();
-    CLE_TRY(ccl_impl(binary, g, DBuf{B, CLE_F32}, CLE_CONN_BOX, CLE_FP32_KEY_COMPAT, nullptr, s));
+    bool labelled = false;  // B holds the CCL labels
     // (the tile solves' stores are not part of the fixed-region bound: pairs there)
     if (g_growth_method == 0 && !scratch_fixed()) {
+        CLE_TRY(ccl_impl(binary, g, DBuf{B, CLE_F32}, CLE_CONN_BOX, CLE_FP32_KEY_COMPAT, nullptr, s));
+        labelled = true;
         bool over = false;
         CLE_TRY(grow_masked(mask, B, dst, g, n_iterations, &over, s));
         if (!over) return CLE_OK;
@@ -1224,11 +1291,18 @@ static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, 
     CLE_TRY(a.alloc((size_t)n * 4, s));
     float* A = a.get<float>();
     const float* res = nullptr;
-    // a uint8 mask and 16-byte aligned buffers take the four-voxel init
-    if (mask.dt == CLE_U8 && ((uintptr_t)mask.p & 3) == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0)
-        CLE_TRY(grow_single(MaskedU8{(const uint8_t*)mask.p, B}, A, B, g, n_iterations, &res, s));
-    else
-        CLE_TRY(grow_single(MaskedAny{mask, B}, A, B, g, n_iterations, &res, s));
+    const bool u8mask = mask.dt == CLE_U8 && ((uintptr_t)mask.p & 3) == 0 && ((uintptr_t)A & 15) == 0 &&
+                        ((uintptr_t)B & 15) == 0;
+    if (u8mask && !labelled) {
+        CLE_TRY(grow_masked_fused(binary, (const uint8_t*)mask.p, A, B, g, n_iterations, &res, s));
+    } else {
+        if (!labelled) CLE_TRY(ccl_impl(binary, g, DBuf{B, CLE_F32}, CLE_CONN_BOX, CLE_FP32_KEY_COMPAT, nullptr, s));
+        // a uint8 mask and 16-byte aligned buffers take the four-voxel init
+        if (u8mask)
+            CLE_TRY(grow_single(MaskedU8{(const uint8_t*)mask.p, B}, A, B, g, n_iterations, &res, s));
+        else
+            CLE_TRY(grow_single(MaskedAny{mask, B}, A, B, g, n_iterations, &res, s));
+    }
     if (mask.dt == CLE_U8 && (dst.dt == CLE_U32 || dst.dt == CLE_I32) && ((uintptr_t)res & 15) == 0 &&
         ((uintptr_t)mask.p & 3) == 0 && ((uintptr_t)dst.p & 15) == 0)
         k_mask_out_u8_32<<<vg(n / 4 + 1), 256, 0, s>>>(res, (const uint8_t*)mask.p, dst, n);
@@ -1240,16 +1314,17 @@ static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, 
 
 // Scratch bound of voronoi_otsu_single: the blurred image, spots, segmentation and
 // threshold words live through the call; on top of them, at most one of the blur's
-// float32 temporary (its per-axis fallback), the Otsu histograms, the box CCL of the
-// spots, and the growth's second state buffer with its tile lists.
+// float32 temporary (its per-axis fallback), the Otsu histograms, and the growth's
+// second state buffer with its tile lists together with the box CCL of the spots.
 size_t voronoi_otsu_scratch_bytes(const Geom& g) {
     const size_t n = (size_t)g.n();
     const size_t held = scratch_round(n * 4) + 2 * scratch_round(n) + scratch_round(64);
     const Tiles T = make_tiles(g);
-    const size_t grow =
-        scratch_round(n * 4) + scratch_round(VorScratch::bytes(T.count(), 1)) + scratch_round(sizeof(BImg));
+    // the growth buffers and tile lists are alive while the CCL of the seeds runs
+    const size_t grow = scratch_round(n * 4) + scratch_round(VorScratch::bytes(T.count(), 1)) +
+                        scratch_round(sizeof(BImg)) + ccl_box_scratch_bytes(g, CLE_FP32_KEY_COMPAT);
     size_t stage = scratch_round(n * 4);
-    const size_t others[3] = {otsu_scratch_bytes(g), ccl_box_scratch_bytes(g, CLE_FP32_KEY_COMPAT), grow};
+    const size_t others[2] = {otsu_scratch_bytes(g), grow};
     for (size_t v : others) stage = v > stage ? v : stage;
     return held + stage;
 }

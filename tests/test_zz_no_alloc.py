@@ -68,11 +68,11 @@ def test_noalloc_rejects_an_image_beyond_its_scratch(cle):
 
 def test_noalloc_near_its_limit_and_the_oracle(cle):
     from pyclesperanto_prototype_amd import _lib, synthetic
-    # 2400 x 2400 (5.76 M voxels) still fits the fixed scratch; labels as the allocating call
-    img = synthetic.blobs_2d((2400, 2400), n_spots=900, seed=8)
+    # 2100 x 2100 (4.41 M voxels) still fits the fixed scratch; labels as the allocating call
+    img = synthetic.blobs_2d((2100, 2100), n_spots=700, seed=8)
     src = cle.push(img)
     out, ref = cle.create(img.shape, np.uint32), cle.create(img.shape, np.uint32)
-    print("noalloc call (2400, 2400)", flush=True)
+    print("noalloc call (2100, 2100)", flush=True)
     assert _noalloc(cle, src, out, 2.0, 2.0) == 0, _lib.last_error()
     _plain(cle, src, ref, 2.0, 2.0)
     assert np.array_equal(cle.pull(out), cle.pull(ref))
