@@ -348,7 +348,7 @@ struct BImg {
     int64_t boff;       // CCL: first 2x2x2 block of the image
     int64_t tbase;      // voronoi: first tile of the image
     int vnx, vny, vnz;  // voronoi: tile counts along x, y, z
-    int pad2;
+    int vrun;           // voronoi: tiles per run of the growth launches
     int64_t wbase;      // tile growth: first tile of the image
     int wnx, wny, wnz;  // tile growth: tile counts along x, y, z
     int pad3;

@@ -76,26 +76,15 @@ __host__ __device__ inline Tiles make_tiles(const Geom& g) {
     return t;
 }
 
-// A work-list entry: image j (< 2^16) and tile coordinates (each < 2^16) packed as
-// j << 48 | tz << 32 | ty << 16 | tx, and the tile's index in the per-tile arrays.
-struct VorItem {
-    uint64_t c;
-    int64_t gt;
-};
-__host__ __device__ inline uint64_t vor_pack(int j, int64_t tx, int64_t ty, int64_t tz) {
-    return ((uint64_t)j << 48) | ((uint64_t)tz << 32) | ((uint64_t)ty << 16) | (uint64_t)tx;
-}
 constexpr int64_t VOR_MAX_TILES_AXIS = 65535;
 constexpr int VOR_MAX_IMAGES = 65535;
 
-// Work lists of the growth (one set for a single image or a whole batch).
+// Per-tile state of the growth (one set for a single image or a whole batch; image j's
+// tiles start at its tbase).
 struct VorLists {
-    VorItem* items;    // 3 lists of cap entries
-    uint32_t* count;   // [0..2]: list sizes
-    int32_t* stamp;    // cap: last pair the tile was queued for
-    int32_t* mark;     // 2 x cap (raster mode): mark[b & 1][t] == b iff tile t changed in pair b
-                       // (tiles with a seed: mark[1][t] == -1, "changed" before pair 0)
-    uint8_t* settled;  // cap
+    int32_t* mark;     // 2 x cap: mark[b & 1][t] == b iff tile t changed in pair b (tiles
+                       // with a seed: mark[1][t] == -1, "changed" before pair 0)
+    uint8_t* settled;  // cap: the tile holds no zero and never changes again
     int64_t cap;       // tiles of all images
 };
 
@@ -112,69 +101,6 @@ struct VorState {
 constexpr int VOR_NCTR = 64;
 __device__ __forceinline__ void vor_count_tiles(const VorState& S, uint32_t n) {
     if (threadIdx.x == 0 && n) atomicAdd(&S.ctr[(blockIdx.x % VOR_NCTR) * 16], n);
-}
-
-__device__ __forceinline__ uint64_t lanes_below_v() {
-    const int lane = threadIdx.x & 63;
-    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
-}
-
-// wavefront 0 of the block queues the tile neighbourhood of `tile` (of image j, whose
-// tiles start at tbase in the per-tile arrays) for pair `blk` (the batch init; settled
-// tiles are not left out here: their flags are still being written)
-__device__ __forceinline__ void vor_queue(const Tiles& T, int64_t tile, int j, int64_t tbase, int blk,
-                                          const VorLists& L) {
-    const int lane = threadIdx.x;
-    int64_t q = -1;
-    if (lane < 27) {
-        const int64_t tx = tile % T.nx, ty = (tile / T.nx) % T.ny, tz = tile / (T.nx * T.ny);
-        const int64_t qx = tx + lane % 3 - 1, qy = ty + (lane / 3) % 3 - 1, qz = tz + lane / 9 - 1;
-        if (qx >= 0 && qx < T.nx && qy >= 0 && qy < T.ny && qz >= 0 && qz < T.nz)
-            q = (qz * T.ny + qy) * T.nx + qx;
-    }
-    // queued once per pair: the stamp records the last pair a tile was queued for
-    const bool need = q >= 0 && atomicMax(&L.stamp[tbase + q], blk) < blk;
-    const uint64_t m = __ballot(need);
-    uint32_t base = 0;
-    if (lane == 0 && m) base = atomicAdd(&L.count[blk % 3], (uint32_t)__popcll(m));
-    base = __shfl(base, 0);
-    if (need) {
-        const int64_t qx = q % T.nx, qy = (q / T.nx) % T.ny, qz = q / (T.nx * T.ny);
-        L.items[(int64_t)(blk % 3) * L.cap + base + __popcll(m & lanes_below_v())] =
-            VorItem{vor_pack(j, qx, qy, qz), tbase + q};
-    }
-}
-
-constexpr int VOR_PEND = 64;  // changed tiles a block collects before queueing their neighbourhoods
-
-// All threads of the block queue the tile neighbourhoods of the np changed tiles in
-// `pend` (packed image / tile coordinates) for pair `blk`, leaving out settled tiles.
-__device__ __forceinline__ void vor_flush(const uint64_t* pend, int np, const BImg* __restrict__ imgs, int blk,
-                                          const VorLists& L) {
-    const int total = np * 27;
-    for (int base0 = 0; base0 < total; base0 += VB) {
-        const int idx = base0 + (int)threadIdx.x;
-        bool need = false;
-        VorItem item{0, 0};
-        if (idx < total) {
-            const uint64_t pe = pend[idx / 27];
-            const int k = idx % 27;
-            const int j = (int)(pe >> 48);
-            const int nx = imgs[j].vnx, ny = imgs[j].vny, nz = imgs[j].vnz;
-            const int qx = (int)(pe & 0xFFFF) + k % 3 - 1, qy = (int)((pe >> 16) & 0xFFFF) + (k / 3) % 3 - 1;
-            const int qz = (int)((pe >> 32) & 0xFFFF) + k / 9 - 1;
-            if (qx >= 0 && qx < nx && qy >= 0 && qy < ny && qz >= 0 && qz < nz) {
-                const int64_t gq = imgs[j].tbase + ((int64_t)qz * ny + qy) * nx + qx;
-                need = !L.settled[gq] && atomicMax(&L.stamp[gq], blk) < blk;
-                item = VorItem{vor_pack(j, qx, qy, qz), gq};
-            }
-        }
-        const uint64_t m = __ballot(need);
-        uint32_t b0 = 0;
-        if ((threadIdx.x & 63) == 0 && m) b0 = atomicAdd(&L.count[blk % 3], (uint32_t)__popcll(m));
-        b0 = __shfl(b0, 0);
-        if (need) L.items[(int64_t)(blk % 3) * L.cap + b0 + __popcll(m & lanes_below_v())] = item;
-    }
 }
 
 // A = B = val(i) on one tile; the tile is settled when it holds no zero; returns, in
@@ -652,46 +578,52 @@ __device__ __forceinline__ int vor_pair_compute(float* __restrict__ dst, const P
     return *sbits;
 }
 
-// A list entry to process.
-struct PairItem {
-    int j, tx, ty, tz;
-    int64_t gt;
-};
+// Pair b over runs: a run is up to vrun (<= VOR_RUN) consecutive tiles of one tile row
+// (x) of one image, 2D or 3D by D3.  Run r of a launch belongs to image find_seg(rs, r) (rs:
+// run prefix over the images, an image of the other kind contributing none; null for
+// one image); block b takes run xcd_tile(b), so the runs of one XCD form a contiguous
+// slab of rows, processed in raster order (the halo rows a run shares with the runs
+// above and below are read while in that XCD's L2).  The block gathers, with three
+// loads per lane and ballots, the change stamps of the 3 x 3 rows of VOR_RUN + 2 tiles
+// around its run and the settled flags of the run: tile t runs iff a neighbour
+// changed in pair b - 1 and t is not settled.  It then works through the active tiles
+// software-pipelined: the region of the next one is loaded into registers while the
+// current one is computed from LDS.  A tile that changes stamps mark[b & 1][t] = b
+// (one plain store).  The loads read src, which no block writes in this launch; only
+// the block that processes a tile sets its settled flag, after it read it.  Block 0
+// first closes pair b - 1: images whose pair b - 1 was not "both changed" get their
+// iteration count; it also empties the change bits pair b + 1 will fill (both
+// idempotent: the 2D and the 3D launch of a pair both do it).
+constexpr int VOR_RUN = 16;
 
-// the next entry at or after e (stride gridDim.x) that is not skipped: its image
-// converged in an earlier pair (iteration count set, or pair b - 1 not "both
-// changed"), or the tile is settled.  One round trip after the entry itself.
-// Uniform over the block.
-__device__ __forceinline__ bool vor_next_item(uint32_t& e, uint32_t n, const VorItem* items, const uint32_t* prev,
-                                              int b, const VorLists& L, const VorState& S, PairItem& it) {
-    for (; e < n; e += gridDim.x) {
-        const VorItem v = items[e];
-        it.j = (int)(v.c >> 48);
-        it.tz = (int)((v.c >> 32) & 0xFFFF);
-        it.ty = (int)((v.c >> 16) & 0xFFFF);
-        it.tx = (int)(v.c & 0xFFFF);
-        it.gt = v.gt;
-        const bool conv = S.iters[it.j] != 0 || (b > 0 && prev[it.j] != 3u);
-        const bool settled = L.settled[it.gt] != 0;
-        if (!conv && !settled) return true;
-    }
-    return false;
+// Tiles per run: VOR_RUN in a batch (images enough to fill the chip) and for a single
+// image from 4096 tiles on; one for a smaller single image, where a few blocks working
+// through runs one tile after another would leave the chip idle (its growth is
+// latency-bound).  Measured: 2000 mixed images 387 ms with VOR_RUN for all, 438 ms
+// with one tile per run below 4096 tiles.
+inline int vor_run_length(const Geom& g, bool batch) {
+    return batch || make_tiles(g).count() >= 4096 ? VOR_RUN : 1;
+}
+// runs of an image of run length rl: pieces of rl tiles of every tile row
+inline int64_t vor_run_blocks(const Geom& g, int rl) {
+    if (g.n() == 0) return 0;
+    const Tiles T = make_tiles(g);
+    return T.nz * T.ny * ((T.nx + rl - 1) / rl);
 }
 
-// Pair b (steps 2b, 2b + 1) over list b % 3, src -> dst, for the images of `imgs`
-// (one image: the single-image path).  Block 0 first closes pair b - 1: images whose
-// pair b - 1 was not "both changed" get their iteration count; it also empties the
-// list and the change bits that pair b + 1 will fill (both last used by pair b - 2 /
-// b - 1, untouched during pair b).  Changed tiles are collected in LDS and their
-// neighbourhoods queued by the whole block at once.
-__global__ __launch_bounds__(VB, 3) void k_vor_pair(const BImg* __restrict__ imgs, int nimg,
-                                                    const float* __restrict__ src, float* __restrict__ dst, int b,
-                                                    VorLists L, VorState S) {
-    __shared__ float lds[PairGeo<true>::NR > PairGeo<false>::NR ? PairGeo<true>::NR : PairGeo<false>::NR];
-    __shared__ float lds2[PairGeo<true>::NR > PairGeo<false>::NR ? PairGeo<true>::NR : PairGeo<false>::NR];
+#ifndef CLE_VOR_RUN_WAVES
+#define CLE_VOR_RUN_WAVES 4  // waves per SIMD of the 3D form (128 VGPRs)
+#endif
+template <bool D3>
+__global__ __launch_bounds__(VB, D3 ? CLE_VOR_RUN_WAVES : 4) void k_vor_pair_run(const BImg* __restrict__ imgs,
+                                                                               const int64_t* __restrict__ rs,
+                                                                               int nimg, const float* __restrict__ src,
+                                                                               float* __restrict__ dst, int b,
+                                                                               VorLists L, VorState S) {
+    using P = PairGeo<D3>;
+    __shared__ float lds[P::NR];
+    __shared__ float lds2[P::NR];
     __shared__ int sbits[2];
-    __shared__ uint64_t pend[VOR_PEND];  // changed tiles not yet queued
-    int np = 0;
     const uint32_t* prev = S.bits + (size_t)((b + 2) % 3) * nimg;  // pair b - 1
     uint32_t* cur = S.bits + (size_t)(b % 3) * nimg;
     if (blockIdx.x == 0) {
@@ -706,190 +638,25 @@ __global__ __launch_bounds__(VB, 3) void k_vor_pair(const BImg* __restrict__ img
             next[j] = 0;
         }
         if (mx) atomicMax((int*)&S.word[1], mx);
-        if (threadIdx.x == 0) L.count[(b + 2) % 3] = 0;
     }
-    const uint32_t n = L.count[b % 3];
-    const VorItem* items = L.items + (int64_t)(b % 3) * L.cap;
-    PairItem it;
-    int par = 0;
-    uint32_t done = 0;
-    int accj = -1, acc = 0;
-    for (uint32_t e = blockIdx.x; vor_next_item(e, n, items, prev, b, L, S, it); e += gridDim.x) {
-        ++done;
-        const Geom g = imgs[it.j].g;
-        const int64_t off = imgs[it.j].off;
-        // sbits[par] was last read two tiles ago, before a barrier of the previous tile
-        if (threadIdx.x == 0) sbits[par] = 0;
-        int bits;
-        {
-            float v[VOR_KV];
-            if (vor_3d(g)) {
-                const PairGeo<true> R(g, it.tx, it.ty, it.tz);
-                vor_stage_load<true>(src + off, R, v);
-                vor_stage_store<true>(v, lds);
-                __syncthreads();
-                bits = vor_pair_compute<true>(dst + off, R, lds, lds2, &sbits[par]);
-            } else {
-                const PairGeo<false> R(g, it.tx, it.ty, it.tz);
-                vor_stage_load<false>(src + off, R, v);
-                vor_stage_store<false>(v, lds);
-                __syncthreads();
-                bits = vor_pair_compute<false>(dst + off, R, lds, lds2, &sbits[par]);
-            }
-        }
-        par ^= 1;
-        if (bits & 3) {
-            // change bits of consecutive tiles of one image: one atomic per run
-            if (it.j != accj) {
-                if (threadIdx.x == 0 && acc) atomicOr(&cur[accj], (uint32_t)acc);
-                accj = it.j;
-                acc = 0;
-            }
-            acc |= bits & 3;
-            if (threadIdx.x == 0) pend[np] = vor_pack(it.j, it.tx, it.ty, it.tz);
-            ++np;
-        } else if (!(bits & 4) && threadIdx.x == 0) {
-            L.settled[it.gt] = 1;
-        }
-        if (np == VOR_PEND) {
-            __syncthreads();
-            vor_flush(pend, np, imgs, b + 1, L);
-            __syncthreads();
-            np = 0;
-        }
-        // lds is rewritten by the next tile only after every thread passed the
-        // last barrier of vor_pair_compute (lds reads come before it)
-    }
-    if (np) {
-        __syncthreads();
-        vor_flush(pend, np, imgs, b + 1, L);
-    }
-    if (threadIdx.x == 0 && acc) atomicOr(&cur[accj], (uint32_t)acc);
-    vor_count_tiles(S, done);
-}
-
-// Pair b in raster mode for an image below 4096 tiles, one tile per block: tile t runs
-// iff a tile of its 3 x 3 x 3 neighbourhood changed in pair b - 1 (its stamp in
-// mark[(b - 1) & 1] is b - 1) and t is not settled; a tile that changes stamps
-// mark[b & 1][t] = b (one plain store; the neighbours gather).  Block 0 first closes
-// pair b - 1 as in the list mode.
-__device__ __forceinline__ void vor_pair_grid(const BImg* __restrict__ imgs, const float* __restrict__ src,
-                                              float* __restrict__ dst, int b, const VorLists& L, const VorState& S) {
-    __shared__ float lds[PairGeo<true>::NR > PairGeo<false>::NR ? PairGeo<true>::NR : PairGeo<false>::NR];
-    __shared__ float lds2[PairGeo<true>::NR > PairGeo<false>::NR ? PairGeo<true>::NR : PairGeo<false>::NR];
-    __shared__ int sbits[2];
-    const uint32_t* prev = S.bits + (b + 2) % 3;  // pair b - 1 (one image)
-    uint32_t* cur = S.bits + b % 3;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (b > 0 && S.iters[0] == 0 && prev[0] != 3u) {
-            const int it = 2 * (b - 1) + ((prev[0] & 1u) ? 2 : 1);
-            S.iters[0] = it;
-            atomicMax((int*)&S.word[1], it);
-        }
-        S.bits[(b + 1) % 3] = 0;
-    }
-    if (S.iters[0] != 0 || (b > 0 && prev[0] != 3u)) return;  // converged in an earlier pair
-    const Geom g = imgs[0].g;
-    const int nx = imgs[0].vnx, ny = imgs[0].vny, nz = imgs[0].vnz;
-    const int64_t t = blockIdx.x;
-    const int txy = (int)(t / nx), tx = (int)(t - (int64_t)txy * nx), ty = txy % ny, tz = txy / ny;
-    const int32_t* chg_prev = L.mark + (int64_t)((b - 1) & 1) * L.cap;
-    bool c = false;
-    {
-        const int k = threadIdx.x & 63;
-        if (k < 27) {
-            const int qx = tx + k % 3 - 1, qy = ty + (k / 3) % 3 - 1, qz = tz + k / 9 - 1;
-            if (qx >= 0 && qx < nx && qy >= 0 && qy < ny && qz >= 0 && qz < nz)
-                c = chg_prev[((int64_t)qz * ny + qy) * nx + qx] == b - 1;
-        }
-    }
-    if (!__ballot(c) || L.settled[t]) return;
-    if (threadIdx.x == 0) sbits[0] = 0;
-    float v[VOR_KV];
-    int bits;
-    if (vor_3d(g)) {
-        const PairGeo<true> R(g, tx, ty, tz);
-        vor_stage_load<true>(src, R, v);
-        vor_stage_store<true>(v, lds);
-        __syncthreads();
-        bits = vor_pair_compute<true>(dst, R, lds, lds2, &sbits[0]);
-    } else {
-        const PairGeo<false> R(g, tx, ty, tz);
-        vor_stage_load<false>(src, R, v);
-        vor_stage_store<false>(v, lds);
-        __syncthreads();
-        bits = vor_pair_compute<false>(dst, R, lds, lds2, &sbits[0]);
-    }
-    if (bits & 3) {
-        if (threadIdx.x == 0) {
-            atomicOr(cur, (uint32_t)(bits & 3));
-            L.mark[(int64_t)(b & 1) * L.cap + t] = b;
-        }
-    } else if (!(bits & 4) && threadIdx.x == 0) {
-        L.settled[t] = 1;
-    }
-    vor_count_tiles(S, 1);
-}
-
-__global__ __launch_bounds__(VB, 4) void k_vor_pair_grid(const BImg* __restrict__ imgs,
-                                                         const float* __restrict__ src, float* __restrict__ dst,
-                                                         int b, VorLists L, VorState S) {
-    vor_pair_grid(imgs, src, dst, b, L, S);
-}
-
-// Pair b in raster mode over runs: a run is up to VOR_RUN consecutive tiles of one
-// tile row (x) of one image (2D or 3D by D3); block b takes run xcd_tile(b), so the
-// runs of one XCD form a contiguous slab of rows, processed in raster order (the
-// halo rows a run shares with the runs above and below are read while in that XCD's
-// L2).  The block gathers, with three loads per lane and ballots, the change stamps of
-// the 3 x 3 rows of VOR_RUN + 2 tiles around its run and the settled flags of the
-// run: tile t runs iff a neighbour changed in pair b - 1 and t is not settled.  It
-// then works through the active tiles software-pipelined: the region of the next one
-// is loaded into registers while the current one is computed from LDS.  A tile that
-// changes stamps mark[b & 1][t] = b (one plain store).  The loads read src, which no
-// block writes in this launch; only the block that processes a tile sets its settled
-// flag, after it read it.
-constexpr int VOR_RUN = 16;
-
-// blocks of k_vor_pair_run for an image: runs of VOR_RUN tiles per tile row
-inline int64_t vor_run_blocks(const Geom& g) {
-    const Tiles T = make_tiles(g);
-    return T.nz * T.ny * ((T.nx + VOR_RUN - 1) / VOR_RUN);
-}
-
-#ifndef CLE_VOR_RUN_WAVES
-#define CLE_VOR_RUN_WAVES 4  // waves per SIMD of the 3D form (128 VGPRs)
-#endif
-template <bool D3>
-__global__ __launch_bounds__(VB, D3 ? CLE_VOR_RUN_WAVES : 4) void k_vor_pair_run(const BImg* __restrict__ imgs,
-                                                                               const float* __restrict__ src,
-                                                                               float* __restrict__ dst, int b,
-                                                                               VorLists L, VorState S) {
-    using P = PairGeo<D3>;
-    __shared__ float lds[P::NR];
-    __shared__ float lds2[P::NR];
-    __shared__ int sbits[2];
-    const uint32_t* prev = S.bits + (b + 2) % 3;  // pair b - 1 (one image)
-    uint32_t* cur = S.bits + b % 3;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (b > 0 && S.iters[0] == 0 && prev[0] != 3u) {
-            const int it = 2 * (b - 1) + ((prev[0] & 1u) ? 2 : 1);
-            S.iters[0] = it;
-            atomicMax((int*)&S.word[1], it);
-        }
-        S.bits[(b + 1) % 3] = 0;
-    }
-    if (S.iters[0] != 0 || (b > 0 && prev[0] != 3u)) return;  // converged in an earlier pair
-    const Geom g = imgs[0].g;
-    const int nx = imgs[0].vnx, ny = imgs[0].vny, nz = imgs[0].vnz;
-    const int segs = (nx + VOR_RUN - 1) / VOR_RUN;
-    const int64_t r = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t rg = xcd_tile(blockIdx.x, gridDim.x);
+    const int j = rs ? find_seg(rs, nimg, rg) : 0;
+    if (S.iters[j] != 0 || (b > 0 && prev[j] != 3u)) return;  // converged in an earlier pair
+    const BImg& im = imgs[j];
+    const Geom g = im.g;
+    const int nx = im.vnx, ny = im.vny, nz = im.vnz;
+    const int64_t tb = im.tbase;
+    src += im.off;
+    dst += im.off;
+    const int rl = im.vrun;  // tiles per run (<= VOR_RUN)
+    const int segs = (nx + rl - 1) / rl;
+    const int64_t r = rs ? rg - rs[j] : rg;
     const int64_t row = r / segs;
-    const int xs = (int)(r - row * segs) * VOR_RUN;
+    const int xs = (int)(r - row * segs) * rl;
     const int ty = (int)(row % ny), tz = (int)(row / ny);
-    const int nrun = nx - xs < VOR_RUN ? nx - xs : VOR_RUN;
-    const int64_t t0 = row * nx + xs;
-    const int32_t* chg_prev = L.mark + (int64_t)((b - 1) & 1) * L.cap;
+    const int nrun = nx - xs < rl ? nx - xs : rl;
+    const int64_t t0 = tb + row * nx + xs;  // first tile of the run in the per-tile arrays
+    const int32_t* chg_prev = L.mark + (int64_t)((b - 1) & 1) * L.cap + tb;
     // active tiles of the run: bit k for tile t0 + k (every wave builds the same mask)
     uint64_t act;
     {
@@ -953,15 +720,9 @@ __global__ __launch_bounds__(VB, D3 ? CLE_VOR_RUN_WAVES : 4) void k_vor_pair_run
         if (!more) break;
         tk = tn;
     }
-    if (threadIdx.x == 0 && acc) atomicOr(cur, (uint32_t)acc);
+    if (threadIdx.x == 0 && acc) atomicOr(&cur[j], (uint32_t)acc);
     vor_count_tiles(S, done);
 }
-
-// raster mode: one tile per block for images below VOR_RUN_MIN_TILES tiles, runs above
-// (fewer, longer-lived blocks, and a block whose tiles are all inactive costs one
-// dispatch instead of 16: measured 15.8 -> 12.7 ms for voronoi_otsu_labeling of the
-// 256x1024x1024 stack, 162 -> 112 ms for the 512x2048x2048 one)
-constexpr int64_t VOR_RUN_MIN_TILES = 4096;
 
 // Closes the last pair b of a chunk (one block): iteration counts of the images that
 // converged in it, and the "an image still grows" word the host reads.
@@ -993,19 +754,15 @@ struct VorScratch {
     VorState S;
     static size_t bytes(int64_t ntiles, int nimg) {
         const int64_t cap = ntiles > 0 ? ntiles : 1;
-        return (size_t)cap * 3 * 16 + (size_t)cap * 12 + 64 + (size_t)VOR_NCTR * 64 + (size_t)nimg * 16 + (size_t)cap + 4;
+        return (size_t)cap * 8 + 64 + (size_t)VOR_NCTR * 64 + (size_t)nimg * 16 + (size_t)cap + 4;
     }
     int alloc(int64_t ntiles, int nimg, hipStream_t s) {
         const int64_t cap = ntiles > 0 ? ntiles : 1;
         CLE_TRY(mem.alloc(bytes(ntiles, nimg), s));
         char* p = mem.get<char>();
-        L.items = (VorItem*)p;
-        p += (size_t)cap * 3 * 16;
-        L.stamp = (int32_t*)p;
-        L.mark = L.stamp + cap;
-        p += (size_t)cap * 12;
-        L.count = (uint32_t*)p;        // count[0..2], word[0..1]: 5 words, padded to 16
-        S.word = L.count + 3;
+        L.mark = (int32_t*)p;          // 2 x cap
+        p += (size_t)cap * 8;
+        S.word = (uint32_t*)p;         // word[0..1], padded to 16 words
         p += 64;
         S.ctr = (uint32_t*)p;          // VOR_NCTR x 16 words
         p += (size_t)VOR_NCTR * 64;
@@ -1014,36 +771,37 @@ struct VorScratch {
         p += (size_t)nimg * 16;
         L.settled = (uint8_t*)p;
         L.cap = cap;
-        // stamps -1, change stamps -2 (no pair), counters / words / per-image state 0
-        // (contiguous from count)
-        CLE_TRY(fill_u32((uint32_t*)L.stamp, 0xFFFFFFFFu, (size_t)cap, s));
+        // change stamps -2 (no pair); words, counters and per-image state 0 (contiguous
+        // from word); nothing settled yet
         CLE_TRY(fill_u32((uint32_t*)L.mark, 0xFFFFFFFEu, (size_t)cap * 2, s));
-        CLE_TRY(fill_u32(L.count, 0u, 16 + (size_t)VOR_NCTR * 16 + (size_t)nimg * 4, s));
-        CLE_TRY(fill_u32((uint32_t*)L.settled, 0u, ((size_t)cap + 3) / 4, s));  // nothing settled yet
+        CLE_TRY(fill_u32(S.word, 0u, 16 + (size_t)VOR_NCTR * 16 + (size_t)nimg * 4, s));
+        CLE_TRY(fill_u32((uint32_t*)L.settled, 0u, ((size_t)cap + 3) / 4, s));
         return CLE_OK;
     }
 };
 
-// Growth modes: work lists (a batch), raster over the tiles of one 2D / 3D image.
-enum { VOR_LISTS = 0, VOR_RASTER_2D = 2, VOR_RASTER_3D = 3 };
-inline int vor_mode_single(const Geom& g) { return vor_3d(g) ? VOR_RASTER_3D : VOR_RASTER_2D; }
+// The runs of a growth: per kind (0: 2D images, 1: 3D images) the device run-prefix
+// table over the images (null for a single image) and the number of runs.
+struct VorRuns {
+    const int64_t* rs[2];
+    int64_t blocks[2];
+};
+inline VorRuns vor_runs_single(const Geom& g) {
+    VorRuns R{{nullptr, nullptr}, {0, 0}};
+    R.blocks[vor_3d(g) ? 1 : 0] = vor_run_blocks(g, vor_run_length(g, false));
+    return R;
+}
 
 // Launches pairs [b, b + count) and the close of the last one (no host synchronisation).
-static void launch_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t ntiles, const VorLists& L,
-                         const VorState& S, int mode, int64_t rblocks, int b, int count, hipStream_t s) {
-    const bool raster = mode != VOR_LISTS;
-    const bool runs = raster && ntiles >= VOR_RUN_MIN_TILES;
-    const unsigned grid = runs     ? (unsigned)rblocks
-                          : raster ? (unsigned)ntiles
-                                   : (unsigned)(ntiles < 4096 ? (ntiles > 0 ? ntiles : 1) : 4096);
+static void launch_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, const VorLists& L, const VorState& S,
+                         const VorRuns& R, int b, int count, hipStream_t s) {
     for (int k = 0; k < count; ++k, ++b) {
-        if (runs && mode == VOR_RASTER_3D)
-            k_vor_pair_run<true><<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S);
-        else if (runs)
-            k_vor_pair_run<false><<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S);
-        else if (raster)
-            k_vor_pair_grid<<<grid, VB, 0, s>>>(dimgs, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S);
-        else k_vor_pair<<<grid, VB, 0, s>>>(dimgs, nimg, (b & 1) ? P1 : P0, (b & 1) ? P0 : P1, b, L, S);
+        const float* src = (b & 1) ? P1 : P0;
+        float* dst = (b & 1) ? P0 : P1;
+        if (R.blocks[0] > 0)
+            k_vor_pair_run<false><<<(unsigned)R.blocks[0], VB, 0, s>>>(dimgs, R.rs[0], nimg, src, dst, b, L, S);
+        if (R.blocks[1] > 0)
+            k_vor_pair_run<true><<<(unsigned)R.blocks[1], VB, 0, s>>>(dimgs, R.rs[1], nimg, src, dst, b, L, S);
     }
     k_vor_close<<<1, VB, 0, s>>>(nimg, b - 1, S);
 }
@@ -1055,8 +813,8 @@ static void launch_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int6
 // processed, and their GPU time (events around each chunk of launches).
 static std::atomic<unsigned long long> g_pair_launches{0}, g_pair_tiles{0}, g_pair_ns{0};
 
-static int grow_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t ntiles, const VorLists& L,
-                      const VorState& S, int mode, int64_t rblocks, int b0, int* max_iters, hipStream_t s) {
+static int grow_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, const VorLists& L, const VorState& S,
+                      const VorRuns& R, int b0, int* max_iters, hipStream_t s) {
     int b = b0;
     uint32_t hw[2] = {0, 0};
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -1069,7 +827,7 @@ static int grow_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, int64_t
     for (;;) {
         const int chunk = b == 0 ? 2 : VOR_CHUNK;
         if (timed) (void)hipEventRecord(ev[0], s);
-        launch_pairs(dimgs, nimg, P0, P1, ntiles, L, S, mode, rblocks, b, chunk, s);
+        launch_pairs(dimgs, nimg, P0, P1, L, S, R, b, chunk, s);
         if (timed) (void)hipEventRecord(ev[1], s);
         b += chunk;
         launches += chunk;
@@ -1103,14 +861,15 @@ __host__ __device__ inline int vor_result_buffer(int iters) { return (((iters - 
 
 __global__ void k_put_img(BImg* d, BImg im) { *d = im; }
 
-// growth tiling of an image into its table entry; fails for tile counts beyond the
-// packed list-entry fields
-static int vor_set_tiles(BImg& im) {
+// growth tiling of an image into its table entry (tile counts, run length); fails for
+// more than VOR_MAX_TILES_AXIS tiles along an axis
+static int vor_set_tiles(BImg& im, bool batch) {
     const Tiles T = make_tiles(im.g);
     if (T.nx > VOR_MAX_TILES_AXIS || T.ny > VOR_MAX_TILES_AXIS || T.nz > VOR_MAX_TILES_AXIS)
         return fail(CLE_E_VALUE, "image too large for the Voronoi growth tiling (%lld x %lld x %lld tiles, at most %lld "
                     "per axis)", (long long)T.nx, (long long)T.ny, (long long)T.nz, (long long)VOR_MAX_TILES_AXIS);
     im.vnx = (int)T.nx;
+    im.vrun = vor_run_length(im.g, batch);
     im.vny = (int)T.ny;
     im.vnz = (int)T.nz;
     return CLE_OK;
@@ -1126,7 +885,7 @@ static int grow_single(VF val, float* P0, float* P1, const Geom& g, int* n_itera
     BImg im;
     memset(&im, 0, sizeof(im));
     im.g = g;
-    CLE_TRY(vor_set_tiles(im));
+    CLE_TRY(vor_set_tiles(im, false));
     VorScratch V;
     CLE_TRY(V.alloc(ntiles, 1, s));
     Scratch tab;
@@ -1135,8 +894,7 @@ static int grow_single(VF val, float* P0, float* P1, const Geom& g, int* n_itera
     k_vor_init<<<(unsigned)ntiles, VB, 0, s>>>(val, P0, P1, g, V.L);
     CLE_LAUNCHED();
     int iters = 0;
-    CLE_TRY(grow_pairs(tab.get<BImg>(), 1, P0, P1, ntiles, V.L, V.S, vor_mode_single(g), vor_run_blocks(g), 0, &iters,
-                       s));
+    CLE_TRY(grow_pairs(tab.get<BImg>(), 1, P0, P1, V.L, V.S, vor_runs_single(g), 0, &iters, s));
     if (n_iterations) *n_iterations = iters;
     *result = vor_result_buffer(iters) ? P1 : P0;
     return CLE_OK;
@@ -1253,7 +1011,7 @@ static int grow_masked_fused(DBuf binary, const uint8_t* mask, float* A, float* 
     BImg im;
     memset(&im, 0, sizeof(im));
     im.g = g;
-    CLE_TRY(vor_set_tiles(im));
+    CLE_TRY(vor_set_tiles(im, false));
     VorScratch V;
     CLE_TRY(V.alloc(ntiles, 1, s));
     Scratch tab;
@@ -1265,8 +1023,7 @@ static int grow_masked_fused(DBuf binary, const uint8_t* mask, float* A, float* 
     const GrowInitOut o{A, B, mask, V.L.mark + V.L.cap, T.tx, T.ty, T.tz, T.nx, T.ny};
     CLE_TRY(ccl_box_growth_init(binary, g, o, s));
     int iters = 0;
-    CLE_TRY(grow_pairs(tab.get<BImg>(), 1, A, B, ntiles, V.L, V.S, vor_mode_single(g), vor_run_blocks(g), 0, &iters,
-                       s));
+    CLE_TRY(grow_pairs(tab.get<BImg>(), 1, A, B, V.L, V.S, vor_runs_single(g), 0, &iters, s));
     if (n_iterations) *n_iterations = iters;
     *result = vor_result_buffer(iters) ? B : A;
     return CLE_OK;
@@ -1423,12 +1180,10 @@ __global__ __launch_bounds__(VB) void k_vor_init_batch(const BImg* __restrict__ 
     const int j = find_seg(ts, nimg, b);
     const BImg& im = imgs[j];
     const Tiles T = make_tiles(im.g);
+    // a tile holding a positive value counts as changed before pair 0
     if (vor_init_tile(MaskedU8{G + im.off, B + im.off}, A + im.off, B + im.off, im.g, T, b - ts[j], L.settled + b) &&
-        threadIdx.x < 64) {
-        vor_queue(T, b - ts[j], j, ts[j], 0, L);
-        // the raster-mode stamp as well (a one-image batch grows in raster order)
-        if (threadIdx.x == 0) L.mark[L.cap + b] = -1;
-    }
+        threadIdx.x == 0)
+        L.mark[L.cap + b] = -1;
 }
 
 // dst = G ? result : 0, the result in the growth buffer of the image's last pair
@@ -1529,8 +1284,7 @@ struct BatchCapture {
     const BImg* dimgs = nullptr;
     const int64_t* dvs = nullptr;
     int n = 0;
-    int mode = 0;  // growth mode (VOR_LISTS, or raster for a one-image batch)
-    int64_t rblocks = 0;  // raster runs of a one-image batch
+    VorRuns runs{};  // runs of the growth launches
     int64_t ntiles = 0, vblocks = 0;
     float* Ap = nullptr;
     float* Fp = nullptr;
@@ -1563,7 +1317,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
         im.dst_dt = dsts[j].dtype;
         im.big = g.n() >= ((int64_t)1 << 24) ? 1 : 0;
         im.tbase = ntiles;
-        CLE_TRY(vor_set_tiles(im));
+        CLE_TRY(vor_set_tiles(im, true));
         ntiles += g.n() > 0 ? make_tiles(g).count() : 0;
         grow_tiling(im);
         if (im.wnx > 65535 || im.wny > 65535 || im.wnz > 65535)
@@ -1572,8 +1326,8 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
         wtiles += grow_tile_count(im);
     }
     if (total == 0) return CLE_OK;
-    const int mode = n == 1 ? vor_mode_single(imgs[0].g) : VOR_LISTS;
-    const int64_t rblocks = n == 1 ? vor_run_blocks(imgs[0].g) : 0;
+    VorRuns runs{};
+
     CclBatchSizes csz;
     ccl_batch_layout(imgs, &csz);
     // packed work images: F (float32: blurs, CCL labels, growth buffer B), A (growth
@@ -1617,6 +1371,18 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     for (int j = 0; j < n; ++j) wst[j] = imgs[j].wbase;
     wst[n] = wtiles;
     const size_t ws_off = blob.put_vec(wst);
+    // run prefixes of the growth launches: 2D images, 3D images
+    std::vector<int64_t> rst[2] = {std::vector<int64_t>((size_t)n + 1), std::vector<int64_t>((size_t)n + 1)};
+    int64_t nrun[2] = {0, 0};
+    for (int j = 0; j < n; ++j) {
+        const int kind = vor_3d(imgs[j].g) ? 1 : 0;
+        rst[0][j] = nrun[0];
+        rst[1][j] = nrun[1];
+        nrun[kind] += vor_run_blocks(imgs[j].g, imgs[j].vrun);
+    }
+    rst[0][n] = nrun[0];
+    rst[1][n] = nrun[1];
+    const size_t rs_off[2] = {blob.put_vec(rst[0]), blob.put_vec(rst[1])};
 
     std::vector<Launch> L;
     std::vector<std::shared_ptr<void>> keep;
@@ -1665,13 +1431,16 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
 
     // growth: all images together, each stopping on its own convergence
     const BImg* dimgs = (const BImg*)(dev + imgs_off);
+    for (int k = 0; k < 2; ++k) {
+        runs.rs[k] = (const int64_t*)(dev + rs_off[k]);
+        runs.blocks[k] = nrun[k];
+    }
     const int64_t* dvs = (const int64_t*)(dev + vs_off);
     if (cap) {
         cap->dimgs = dimgs;
         cap->dvs = dvs;
         cap->n = n;
-        cap->mode = mode;
-        cap->rblocks = rblocks;
+        cap->runs = runs;
         cap->ntiles = ntiles;
         cap->vblocks = vblocks;
         cap->Ap = Ap;
@@ -1680,13 +1449,13 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
         cap->L = V.L;
         cap->S = V.S;
         if (ntiles > 0) {
-            launch_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, mode, rblocks, 0, cap->graph_pairs, s);
+            launch_pairs(dimgs, n, Ap, Fp, V.L, V.S, runs, 0, cap->graph_pairs, s);
             CLE_LAUNCHED();
             CLE_HIP(hipMemcpyAsync(cap->word_host, V.S.word, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         }
     } else if (pairs) {
         int steps = 0;
-        if (ntiles > 0) CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, mode, rblocks, 0, &steps, s));
+        if (ntiles > 0) CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, V.L, V.S, runs, 0, &steps, s));
         if (n_steps) *n_steps = steps;
     } else {
         // tile solves: the labels go straight to the destinations
@@ -1707,7 +1476,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
         k_vor_init_batch<<<(unsigned)ntiles, VB, 0, s>>>(dimgs, (const int64_t*)(dev + ts_off), n, Gp, Ap, Fp, V.L);
         CLE_LAUNCHED();
         int steps = 0;
-        CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, ntiles, V.L, V.S, mode, rblocks, 0, &steps, s));
+        CLE_TRY(grow_pairs(dimgs, n, Ap, Fp, V.L, V.S, runs, 0, &steps, s));
         if (n_steps) *n_steps = steps;
     }
     k_mask_out_batch<<<(unsigned)vblocks, 256, 0, s>>>(dimgs, dvs, n, Ap, Fp, V.S.iters, Gp);
@@ -1816,7 +1585,7 @@ int cle_plan_run(cle_plan* P, int* n_steps, void* stream) {
     const BatchCapture& c = P->cap;
     if (c.ntiles > 0 && P->word_host[0]) {
         // still growing after the recorded pairs: finish eagerly, then mask again
-        CLE_TRY(grow_pairs(c.dimgs, c.n, c.Ap, c.Fp, c.ntiles, c.L, c.S, c.mode, c.rblocks, c.graph_pairs, &steps, s));
+        CLE_TRY(grow_pairs(c.dimgs, c.n, c.Ap, c.Fp, c.L, c.S, c.runs, c.graph_pairs, &steps, s));
         k_mask_out_batch<<<(unsigned)c.vblocks, 256, 0, s>>>(c.dimgs, c.dvs, c.n, c.Ap, c.Fp, c.S.iters, c.Gp);
         CLE_LAUNCHED();
         CLE_HIP(hipStreamSynchronize(s));
