@@ -59,7 +59,7 @@ def parse():
     ap.add_argument("--batch-only", action="store_true",
                     help="batched calls only (no per-image loop, no graph): for kernel traces of the batched path")
     ap.add_argument("--serve-requests", type=int, default=400)
-    ap.add_argument("--serve-qps", default="300,1500,6000,30000")
+    ap.add_argument("--serve-qps", default="300,700,1500,6000,30000")
     ap.add_argument("--c1-calls", type=int, default=200)
     ap.add_argument("--ingest-pages", type=int, default=128)
     return ap.parse_args()
@@ -447,11 +447,12 @@ def serve_section(cle, torch, n_requests, qps_list):
         same = sum(int(torch.equal(r.result().tensor, o.tensor)) for r, o in zip(served, plain_out))
         pl = np.array(plain_lat) * 1e3
         lat = serving.latency_summary(served)
-        pt = {"offered_qps": qps, "served": {**lat, "mvox_s": round(nv / span / 1e6, 1), "batches": len(batch_sizes),
+        pt = {"offered_qps": qps, "served": {**lat, "mvox_s": round(nv / span / 1e6, 1),
+                                             "req_s": round(n_requests / span, 1), "batches": len(batch_sizes),
                                              "mean_batch": round(float(np.mean(batch_sizes)), 1)},
               "plain": {"p50_ms": round(float(np.percentile(pl, 50)), 2),
                         "p99_ms": round(float(np.percentile(pl, 99)), 2),
-                        "mvox_s": round(nv / plain_span / 1e6, 1)},
+                        "mvox_s": round(nv / plain_span / 1e6, 1), "req_s": round(n_requests / plain_span, 1)},
               "identical_to_plain": f"{same}/{n_requests}"}
         pt["served_over_plain_mvox"] = round(pt["served"]["mvox_s"] / pt["plain"]["mvox_s"], 2)
         log(f"serving @ {qps:g} req/s: {pt}")

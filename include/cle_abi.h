@@ -56,6 +56,9 @@ const char* cle_last_error(void);
 /* free the scratch arenas of the current device (synchronises the device; call it
    while no other call on that device is in flight) */
 int cle_release_pool(void);
+/* free the scratch arena of (current device, stream) after draining that stream, for a
+   stream that is about to be destroyed (no counterpart in the reference) */
+int cle_release_stream(void* stream);
 
 /* ---- fills and copies -------------------------------------------------- */
 /* _tier1/_set.py:5-33: every element = value (converted to the dst type) */

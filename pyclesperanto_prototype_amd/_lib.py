@@ -43,6 +43,7 @@ SIGNATURES = {
     "cle_version": [],
     "cle_last_error": [],
     "cle_release_pool": [],
+    "cle_release_stream": [_V],
     "cle_set": [_P, _F, _V],
     "cle_copy": [_P, _P, _V],
     "cle_gaussian_blur": [_P, _P, _F, _F, _F, _V],

@@ -324,6 +324,8 @@ int cle_release_pool(void) {
     return CLE_OK;
 }
 
+int cle_release_stream(void* stream) { return arena_release_stream((hipStream_t)stream); }
+
 int cle_set(const cle_buf* dst, float value, void* stream) {
     Geom g;
     CLE_TRY(check_buf(dst, "dst", &g));
