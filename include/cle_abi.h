@@ -22,6 +22,7 @@
 #ifndef CLE_ABI_H
 #define CLE_ABI_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -43,10 +44,14 @@ enum {
 };
 
 typedef struct cle_buf {
-    void*   data;      /* device pointer */
-    int32_t dtype;     /* CLE_U8 ... */
-    int32_t ndim;      /* 1, 2 or 3 */
-    int64_t shape[3];  /* outermost first: (z, y, x) / (y, x) / (n) */
+    void*   data;        /* device pointer */
+    int32_t dtype;       /* CLE_U8 ... */
+    int32_t ndim;        /* 1, 2 or 3 */
+    int64_t shape[3];    /* outermost first: (z, y, x) / (y, x) / (n) */
+    int64_t strides[3];  /* element strides, outermost first; all 0 = C-contiguous.  The
+                            kernels take C-contiguous images: strides that differ from the
+                            C-contiguous ones (an axis of extent 1 may have any) give
+                            CLE_E_SHAPE. */
 } cle_buf;
 
 /* ---- library ---------------------------------------------------------- */
@@ -59,6 +64,16 @@ int cle_release_pool(void);
 /* free the scratch arena of (current device, stream) after draining that stream, for a
    stream that is about to be destroyed (no counterpart in the reference) */
 int cle_release_stream(void* stream);
+
+/* ---- device memory for callers without a device-array library ----------- */
+/* _tier0/_opencl_backend.py:35-38 (empty) / _tier0/_create.py: *out = `bytes` of device
+ * memory on the current device (NULL for 0 bytes); free it with cle_free. */
+int cle_alloc(size_t bytes, void** out);
+int cle_free(void* p);
+/* _tier0/_push.py / _tier0/_pull.py (from_array / get): copy `bytes` between host memory
+ * and device memory in stream order; both return after the copy finished. */
+int cle_upload(void* dst_device, const void* src_host, size_t bytes, void* stream);
+int cle_download(void* dst_host, const void* src_device, size_t bytes, void* stream);
 
 /* ---- fills and copies -------------------------------------------------- */
 /* _tier1/_set.py:5-33: every element = value (converted to the dst type) */

@@ -115,8 +115,10 @@ class Array:
         if len(shp) > 3:
             raise ValueError(f"images have at most 3 dimensions, got {len(shp)}")
         b.ndim = len(shp)
+        st = self._t.stride() if self.ndim else (1,)
         for i, s in enumerate(shp):
             b.shape[i] = s
+            b.strides[i] = st[i]
         return b
 
     # ---------------------------------------------------------------- transfer

@@ -30,7 +30,7 @@ CLE_STAT_COLUMNS = 36
 
 class cle_buf(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("dtype", ctypes.c_int32), ("ndim", ctypes.c_int32),
-                ("shape", ctypes.c_int64 * 3)]
+                ("shape", ctypes.c_int64 * 3), ("strides", ctypes.c_int64 * 3)]
 
 
 _P = ctypes.POINTER(cle_buf)
@@ -44,6 +44,10 @@ SIGNATURES = {
     "cle_last_error": [],
     "cle_release_pool": [],
     "cle_release_stream": [_V],
+    "cle_alloc": [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)],
+    "cle_free": [_V],
+    "cle_upload": [_V, _V, ctypes.c_size_t, _V],
+    "cle_download": [_V, _V, ctypes.c_size_t, _V],
     "cle_set": [_P, _F, _V],
     "cle_copy": [_P, _P, _V],
     "cle_gaussian_blur": [_P, _P, _F, _F, _F, _V],

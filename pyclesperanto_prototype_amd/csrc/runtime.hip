@@ -57,6 +57,16 @@ int check_buf(const cle_buf* b, const char* name, Geom* g) {
     }
     g->D = s[0]; g->H = s[1]; g->W = s[2];
     g->ndim = b->ndim;
+    if (b->strides[0] || b->strides[1] || b->strides[2]) {
+        int64_t expect = 1;  // C-contiguous element stride of axis i
+        for (int i = b->ndim - 1; i >= 0; --i) {
+            if (b->shape[i] > 1 && b->strides[i] != expect)
+                return fail(CLE_E_SHAPE, "%s: strides (%lld, %lld, %lld) are not C-contiguous; the kernels take "
+                            "contiguous images", name, (long long)b->strides[0], (long long)b->strides[1],
+                            (long long)b->strides[2]);
+            expect *= b->shape[i];
+        }
+    }
     if (g->n() > 0 && !b->data) return fail(CLE_E_VALUE, "%s: null data pointer", name);
     if (g->n() >= (int64_t)0xFFFFFFFFLL)
         return fail(CLE_E_SHAPE, "%s: %lld voxels exceed the 2^32-1 limit", name, (long long)g->n());
@@ -325,6 +335,43 @@ int cle_release_pool(void) {
 }
 
 int cle_release_stream(void* stream) { return arena_release_stream((hipStream_t)stream); }
+
+int cle_alloc(size_t bytes, void** out) {
+    if (!out) return fail(CLE_E_VALUE, "cle_alloc: null output pointer");
+    *out = nullptr;
+    if (bytes == 0) return CLE_OK;
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(CLE_E_OOM, "cle_alloc of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
+    }
+    *out = p;
+    return CLE_OK;
+}
+
+int cle_free(void* p) {
+    if (p) CLE_HIP(hipFree(p));
+    return CLE_OK;
+}
+
+int cle_upload(void* dst_device, const void* src_host, size_t bytes, void* stream) {
+    if (!bytes) return CLE_OK;
+    if (!dst_device || !src_host) return fail(CLE_E_VALUE, "cle_upload: null pointer");
+    const hipStream_t s = (hipStream_t)stream;
+    CLE_HIP(hipMemcpyAsync(dst_device, src_host, bytes, hipMemcpyHostToDevice, s));
+    CLE_HIP(hipStreamSynchronize(s));
+    return CLE_OK;
+}
+
+int cle_download(void* dst_host, const void* src_device, size_t bytes, void* stream) {
+    if (!bytes) return CLE_OK;
+    if (!dst_host || !src_device) return fail(CLE_E_VALUE, "cle_download: null pointer");
+    const hipStream_t s = (hipStream_t)stream;
+    CLE_HIP(hipMemcpyAsync(dst_host, src_device, bytes, hipMemcpyDeviceToHost, s));
+    CLE_HIP(hipStreamSynchronize(s));
+    return CLE_OK;
+}
 
 int cle_set(const cle_buf* dst, float value, void* stream) {
     Geom g;

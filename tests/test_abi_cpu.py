@@ -104,3 +104,28 @@ def test_array_refuses_host_memory():
         cle.Array(torch.zeros(4))
     with pytest.raises(ValueError):
         cle.Array(torch.from_dlpack(np.zeros(3, np.float32)))
+
+
+def test_strided_descriptors_are_refused_on_the_host():
+    # a non-C-contiguous stride fails in host validation (CLE_E_SHAPE), before any HIP call;
+    # zero strides and the C-contiguous ones are accepted there
+    from pyclesperanto_prototype_amd import _lib
+    src, dst = _buf(_lib.CLE_F32, (4, 6)), _buf(_lib.CLE_F32, (4, 6))
+    src.strides[0], src.strides[1] = 12, 2
+    with pytest.raises(ValueError, match="contiguous"):
+        _lib.call("cle_gaussian_blur", src, dst, 1.0, 1.0, 0.0, None)
+    src.strides[0], src.strides[1] = 6, 1
+    dst.shape[1] = 5  # fails later, at the shape comparison: the strides passed
+    with pytest.raises(ValueError, match="differ in shape"):
+        _lib.call("cle_gaussian_blur", src, dst, 1.0, 1.0, 0.0, None)
+
+
+def test_alloc_of_zero_bytes_and_null_output():
+    import ctypes
+    from pyclesperanto_prototype_amd import _lib
+    p = ctypes.c_void_p(123)
+    _lib.call("cle_alloc", 0, ctypes.byref(p))
+    assert p.value is None
+    _lib.call("cle_free", None)
+    with pytest.raises(ValueError):
+        _lib.call("cle_alloc", 16, None)
