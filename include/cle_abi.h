@@ -64,6 +64,11 @@ int cle_release_pool(void);
 /* free the scratch arena of (current device, stream) after draining that stream, for a
    stream that is about to be destroyed (no counterpart in the reference) */
 int cle_release_stream(void* stream);
+/* size the scratch arena of (current device, stream) as one chunk of at least `bytes`
+   (drains that stream when it has to reallocate), so that later calls on the stream
+   whose scratch fits make no driver allocation; for long-lived worker streams such as
+   a server's (no counterpart in the reference) */
+int cle_reserve_stream(void* stream, size_t bytes);
 
 /* ---- device memory for callers without a device-array library ----------- */
 /* _tier0/_opencl_backend.py:35-38 (empty) / _tier0/_create.py: *out = `bytes` of device

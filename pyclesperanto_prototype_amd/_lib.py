@@ -44,6 +44,7 @@ SIGNATURES = {
     "cle_last_error": [],
     "cle_release_pool": [],
     "cle_release_stream": [_V],
+    "cle_reserve_stream": [_V, ctypes.c_size_t],
     "cle_alloc": [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)],
     "cle_free": [_V],
     "cle_upload": [_V, _V, ctypes.c_size_t, _V],

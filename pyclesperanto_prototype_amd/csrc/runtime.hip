@@ -99,6 +99,7 @@ struct ArenaBlock {
 };
 
 struct Arena {
+    std::mutex mu;  // the arena's own state; g_arena_mu guards only the map
     hipStream_t s = nullptr;
     std::vector<std::pair<char*, size_t>> chunks;
     std::vector<ArenaBlock> stack;
@@ -115,10 +116,13 @@ struct Arena {
 std::mutex g_arena_mu;
 std::map<std::pair<int, hipStream_t>, Arena*> g_arenas;
 
-// caller holds g_arena_mu
+// The arena of (current device, s), created on first use.  Arenas are deleted only by
+// cle_release_pool / arena_release_stream, which the caller runs while no call on that
+// stream is in flight, so the pointer stays valid for the calling thread.
 Arena* arena_for(hipStream_t s) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(g_arena_mu);
     auto& a = g_arenas[{dev, s}];
     if (!a) {
         a = new Arena;
@@ -127,11 +131,17 @@ Arena* arena_for(hipStream_t s) {
     return a;
 }
 
-bool arena_merge(Arena* a) {
-    const size_t total = a->capacity();
+// One chunk of at least `want` bytes in place of the arena's chunks (caller holds a->mu,
+// the arena is idle).  Drains the arena's stream only: other streams' arenas and calls
+// are not held up.
+bool arena_merge(Arena* a, size_t want) {
+    size_t total = a->capacity();
+    if (total < want) total = want;
     if (hipStreamSynchronize(a->s) != hipSuccess) return false;
     for (auto& c : a->chunks) (void)hipFree(c.first);
     a->chunks.clear();
+    a->cur = 0;
+    a->top = 0;
     void* p = nullptr;
     if (hipMalloc(&p, total) != hipSuccess) return false;
     a->chunks.push_back({(char*)p, total});
@@ -190,12 +200,12 @@ static void fixed_free(void* p) {
 
 void* scratch_alloc(size_t bytes, hipStream_t s) {
     if (t_fixed.base) return fixed_alloc(bytes);
-    std::lock_guard<std::mutex> lk(g_arena_mu);
     Arena* a = arena_for(s);
     if (!a) return nullptr;
+    std::lock_guard<std::mutex> lk(a->mu);
     const size_t need = (bytes + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN;
     if (a->stack.empty()) {
-        if (a->chunks.size() > 1 && !arena_merge(a)) {
+        if (a->chunks.size() > 1 && !arena_merge(a, 0)) {
             (void)hipGetLastError();
             return nullptr;
         }
@@ -227,9 +237,9 @@ void* scratch_alloc(size_t bytes, hipStream_t s) {
 // The arena of (current device, s) becomes owned by a captured graph: its memory
 // stays where the graph's kernels expect it until arena_release_stream(s).
 int arena_set_owned(hipStream_t s) {
-    std::lock_guard<std::mutex> lk(g_arena_mu);
     Arena* a = arena_for(s);
     if (!a) return fail(CLE_E_HIP, "no arena for the stream");
+    std::lock_guard<std::mutex> lk(a->mu);
     a->owned = true;
     return CLE_OK;
 }
@@ -239,13 +249,20 @@ int arena_release_stream(hipStream_t s) {
     int dev = 0;
     CLE_HIP(hipGetDevice(&dev));
     CLE_HIP(hipStreamSynchronize(s));
-    std::lock_guard<std::mutex> lk(g_arena_mu);
-    auto it = g_arenas.find({dev, s});
-    if (it == g_arenas.end()) return CLE_OK;
-    for (auto& c : it->second->chunks) CLE_HIP(hipFree(c.first));
-    delete it->second;
-    g_arenas.erase(it);
-    return CLE_OK;
+    Arena* a = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_arena_mu);
+        auto it = g_arenas.find({dev, s});
+        if (it == g_arenas.end()) return CLE_OK;
+        a = it->second;
+        g_arenas.erase(it);
+    }
+    // out of the map: the frees hold up no other stream's lookups
+    int rc = CLE_OK;
+    for (auto& c : a->chunks)
+        if (hipFree(c.first) != hipSuccess && rc == CLE_OK) rc = fail(CLE_E_HIP, "hipFree of a scratch chunk failed");
+    delete a;
+    return rc;
 }
 
 void scratch_free(void* p, hipStream_t s) {
@@ -254,9 +271,9 @@ void scratch_free(void* p, hipStream_t s) {
         fixed_free(p);
         return;
     }
-    std::lock_guard<std::mutex> lk(g_arena_mu);
     Arena* a = arena_for(s);
     if (!a) return;
+    std::lock_guard<std::mutex> lk(a->mu);
     for (size_t i = a->stack.size(); i-- > 0;) {
         ArenaBlock& b = a->stack[i];
         if (a->chunks[b.chunk].first + b.off == (char*)p) {
@@ -323,11 +340,13 @@ int cle_release_pool(void) {
     std::lock_guard<std::mutex> lk(g_arena_mu);
     for (auto it = g_arenas.begin(); it != g_arenas.end();) {
         Arena* a = it->second;
+        std::unique_lock<std::mutex> al(a->mu);
         if (it->first.first != dev || !a->stack.empty() || a->owned) {
             ++it;
             continue;
         }
         for (auto& c : a->chunks) CLE_HIP(hipFree(c.first));
+        al.unlock();
         delete a;
         it = g_arenas.erase(it);
     }
@@ -335,6 +354,19 @@ int cle_release_pool(void) {
 }
 
 int cle_release_stream(void* stream) { return arena_release_stream((hipStream_t)stream); }
+
+int cle_reserve_stream(void* stream, size_t bytes) {
+    Arena* a = arena_for((hipStream_t)stream);
+    if (!a) return fail(CLE_E_HIP, "cle_reserve_stream: no current device");
+    std::lock_guard<std::mutex> lk(a->mu);
+    if (!a->stack.empty()) return fail(CLE_E_VALUE, "cle_reserve_stream: a call on this stream holds scratch");
+    if (a->chunks.size() == 1 && a->chunks[0].second >= bytes) return CLE_OK;
+    if (!arena_merge(a, bytes)) {
+        const hipError_t e = hipGetLastError();
+        return fail(CLE_E_OOM, "cle_reserve_stream of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
+    }
+    return CLE_OK;
+}
 
 int cle_alloc(size_t bytes, void** out) {
     if (!out) return fail(CLE_E_VALUE, "cle_alloc: null output pointer");

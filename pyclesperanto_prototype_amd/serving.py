@@ -3,13 +3,16 @@
 Requests (an image in device memory plus its spot / outline sigmas) arrive at any
 time.  Worker threads, each with its own HIP stream, take the pending requests that
 fit the memory budget (in arrival order, up to a voxel cap per batch) as one batch
-and run voronoi_otsu_labeling_batch on it; the
-request's intermediates live in device memory for the whole pipeline and only the
-label image is handed back.  While one batch is on the GPU the next worker admits
-the requests that arrived meanwhile, so batches overlap on the device.  Admission is
-by memory: a request is admitted when the bytes of the requests in flight plus its
-own estimate stay within the budget (by default 80 % of the HBM free when the
-server starts).
+and run voronoi_otsu_labeling_batch on it (a batch of one request runs the
+single-image pipeline); the request's intermediates live in device memory for the
+whole pipeline and only the label image is handed back.  While one batch is on the
+GPU the next worker admits the requests that arrived meanwhile, so batches overlap on
+the device.  Admission is by memory: a request is admitted when the bytes of the
+requests in flight plus its own estimate stay within the budget (by default 80 % of
+the HBM free when the server starts, less the workers' scratch reservations).  The
+label images are allocated on the submitting thread's stream, and each worker's
+scratch arena is sized once when it starts, so a running server makes no device
+allocation of its own.
 
 The batched pipeline gives every image the result it gets alone
 (cle_voronoi_otsu_labeling_batch), so a request's labels do not depend on the
@@ -27,11 +30,23 @@ import numpy as np
 
 from . import _lib
 from ._array import Array, asarray, create
-from ._ops import voronoi_otsu_labeling_batch
+from ._ops import voronoi_otsu_labeling, voronoi_otsu_labeling_batch
 
 # device bytes per voxel of one request while its batch runs: the uint32 labels, the
 # batch's float32 / uint8 work images (10 B), the 2x2x2-block CCL arrays and lists
 BYTES_PER_VOXEL = 24
+# Library scratch of a batch: at most 16 B per voxel (float32 blur / growth buffers,
+# uint8 seeds and segmentation, CCL block arrays, tile lists) plus tables.  Each
+# worker's stream arena is sized for a batch of max_batch_voxels when the worker starts
+# and grown (once) before a larger single request, so a running server makes no driver
+# allocation: a hipFree / hipMalloc while other workers' work is in flight stalled every
+# worker for up to a second (HIP runtime trace of a saturated run).
+SCRATCH_BYTES_PER_VOXEL = 16
+SCRATCH_SLACK = 64 << 20
+
+
+def scratch_bytes(voxels: int) -> int:
+    return int(voxels) * SCRATCH_BYTES_PER_VOXEL + SCRATCH_SLACK
 
 
 @dataclass
@@ -41,10 +56,10 @@ class Request:
     outline_sigma: float
     ready: object = None  # event recorded on the submitting thread's stream
     client_stream: object = None  # the submitting thread's stream (where the labels will be used)
+    labels: Array = None  # created on the client's stream at submit time
     arrival: float = 0.0
     admitted: float = 0.0
     finished: float = 0.0
-    labels: Array = None
     error: BaseException = None
     done: threading.Event = field(default_factory=threading.Event)
 
@@ -87,9 +102,10 @@ class SegmentationServer:
         # Measured, same trace: 2^25 voxels 20.3 Gvoxels/s (mean batch 6.9), 2^28 24.1
         # (mean batch 10.0).
         self.max_batch_voxels = max_batch_voxels
+        self.reserve_bytes = scratch_bytes(max_batch_voxels)
         if memory_budget is None:
             free, _ = torch.cuda.mem_get_info()
-            memory_budget = int(free * 0.8)
+            memory_budget = int(free * 0.8) - workers * self.reserve_bytes
         self.budget = int(memory_budget)
         self._pending = deque()
         self._lock = threading.Lock()
@@ -105,14 +121,20 @@ class SegmentationServer:
 
         img = image if isinstance(image, Array) else asarray(image)
         client = torch.cuda.current_stream()
-        ready = torch.cuda.Event()
-        ready.record(client)  # the worker's stream waits for the image to be written
-        req = Request(img, float(spot_sigma), float(outline_sigma), ready, client, arrival=time.perf_counter())
+        req = Request(img, float(spot_sigma), float(outline_sigma), client_stream=client,
+                      arrival=time.perf_counter())
         if req.nbytes > self.budget:
             req.error = MemoryError(f"request of {req.nbytes} bytes exceeds the server budget {self.budget}")
             req.finished = req.arrival
             req.done.set()
             return req
+        # The labels come from the client stream's pool, where the client will use and
+        # free them (memory the client freed earlier is reused here, whichever worker
+        # runs the request); the worker's stream waits for the image and for the
+        # client's earlier work on that memory.
+        req.labels = create(img.shape, np.uint32)
+        req.ready = torch.cuda.Event()
+        req.ready.record(client)
         with self._cv:
             self._pending.append(req)
             self._cv.notify()
@@ -173,6 +195,9 @@ class SegmentationServer:
     def _serve(self, stream):
         import torch
 
+        # best effort: without the reservation the arena grows on first use
+        reserved = self.reserve_bytes
+        _lib.call_rc("cle_reserve_stream", ctypes.c_void_p(stream.cuda_stream), ctypes.c_size_t(reserved))
         while True:
             with self._cv:
                 while not self._stop and not self._pending:
@@ -185,19 +210,24 @@ class SegmentationServer:
                     continue
                 self._inflight += nbytes
             t0 = time.perf_counter()
+            need = scratch_bytes(sum(int(r.image.size) for r in batch))
+            if need > reserved:  # a request above the batch cap: grow while this stream is idle
+                if _lib.call_rc("cle_reserve_stream", ctypes.c_void_p(stream.cuda_stream), ctypes.c_size_t(need)) == 0:
+                    reserved = need
             try:
                 with torch.cuda.stream(stream):
-                    for r in batch:
-                        stream.wait_event(r.ready)
-                    outs = [create(r.image.shape, np.uint32) for r in batch]
-                    # the labels come from the worker stream's pool but are used on the
-                    # client's stream: when the client frees them, the allocator must
-                    # wait for the work the client queued on them before reusing the
-                    # block for another batch
+                    outs = [r.labels for r in batch]
                     for r, o in zip(batch, outs):
-                        o.tensor.record_stream(r.client_stream)
-                    voronoi_otsu_labeling_batch([r.image for r in batch], [r.spot_sigma for r in batch],
-                                                [r.outline_sigma for r in batch], destinations=outs)
+                        stream.wait_event(r.ready)
+                        # written on this stream: a later reuse of the block on the
+                        # client's stream waits for this stream's work
+                        o.tensor.record_stream(stream)
+                    if len(batch) == 1:  # alone: the single-image pipeline
+                        r = batch[0]
+                        voronoi_otsu_labeling(r.image, outs[0], r.spot_sigma, r.outline_sigma)
+                    else:
+                        voronoi_otsu_labeling_batch([r.image for r in batch], [r.spot_sigma for r in batch],
+                                                    [r.outline_sigma for r in batch], destinations=outs)
                     stream.synchronize()
                 err = None
             except BaseException as e:  # hand the failure to every request of the batch
