@@ -204,15 +204,18 @@ __device__ __forceinline__ bool nz(const DBuf b, int64_t i) {
 // takes over; the voxel is a maximum if the last take-over was its own value
 // (_tier1/detect_maxima_{2d,3d}_x.cl semantics; coordinates below 0 are skipped,
 // above the last index clamped).
-// A block of 256 threads handles a 64 x 16 (x, y) tile of MX_ZC consecutive slices,
-// the slices z-1, z, z+1 of the tile (+ 1-voxel x / y halo) held in an LDS ring of 3
-// planes, plus one plane of per-(x, y) maxima over those slices shared by the 8
-// neighbouring outputs that read it; every read stays inside the image and inside the
-// staged planes.
+// A block of 256 threads handles a 64 x 16 (x, y) tile of MX_ZC consecutive slices.
+// Every thread owns up to MX_NPT columns (x, y) of the tile plus its 1-voxel x / y
+// halo and keeps their slices z - 1, z, z + 1 (and the prefetched z + 2) in registers,
+// so the z direction never touches LDS.  Per slice: C(x, y) = maximum over the visited
+// z neighbours (registers) goes to LDS; Y(x, y) = max of C over y - 1 .. y + 1 (two LDS
+// reads); each output reads Y at x -/+ 1 and C at y -/+ 1 (four LDS reads).  C and Y
+// are double-buffered by slice parity: two barriers per slice.
 // out(i, is_max) stores voxel i's flag.
 constexpr int MX_TX = 64, MX_TY = 16, MX_ZC = 16;
 constexpr int MX_RX = MX_TX + 2, MX_RY = MX_TY + 2, MX_RP = MX_RX * MX_RY;
-constexpr int MX_LDS = 4 * MX_RP;  // floats of LDS per block: 3 staged planes + column maxima
+constexpr int MX_NPT = (MX_RP + 255) / 256;  // columns per thread
+constexpr int MX_LDS = 4 * MX_RP;            // floats of LDS per block: C and Y, two slices each
 
 __host__ __device__ inline int64_t maxima_tiles(const Geom& g) {
     const int64_t nzc = g.ndim == 3 ? (g.D + MX_ZC - 1) / MX_ZC : 1;
@@ -220,89 +223,63 @@ __host__ __device__ inline int64_t maxima_tiles(const Geom& g) {
 }
 
 template <typename LD, typename OUT>
-__device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t t, float* ring, const OUT& out) {
+__device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t t, float* lds, const OUT& out) {
     const int tid = threadIdx.x;
     const int64_t ntx = (g.W + MX_TX - 1) / MX_TX, nty = (g.H + MX_TY - 1) / MX_TY;
     const int64_t x0 = (t % ntx) * MX_TX, y0 = ((t / ntx) % nty) * MX_TY, zc = t / (ntx * nty);
     const bool d3 = g.ndim == 3;
     const int64_t z0 = d3 ? zc * MX_ZC : 0, z1 = d3 ? (z0 + MX_ZC < g.D ? z0 + MX_ZC : g.D) : 1;
-    // the staged x / y halo lies inside the image: no bounds to test, no clamping
-    const bool inner = x0 >= 1 && y0 >= 1 && x0 + MX_TX < g.W && y0 + MX_TY < g.H;
-    // plane zz -> registers (voxels inside the image only), then -> ring slot zz % 3;
-    // the loads of plane z + 2 are in flight while slice z is computed
-    constexpr int NPT = (MX_RP + 255) / 256;
-    float r[NPT];
-    auto load = [&](int64_t zz) {
+    const int64_t plane = g.H * g.W;
+    const float qnan = __builtin_nanf("");
+    // column k of this thread: region cell c = tid + 256 k.  Positions before the
+    // image's first column / row are never visited (NaN: nothing); those past its last
+    // column / row read the clamped last one, as the visit's clamped offsets do.
+    int64_t off[MX_NPT];
+    bool live[MX_NPT];
 #pragma unroll
-        for (int k = 0; k < NPT; ++k) {
-            const int c = tid + k * 256;
-            const int64_t xx = x0 - 1 + c % MX_RX, yy = y0 - 1 + c / MX_RX;
-            const bool ok = c < MX_RP && (inner || (xx >= 0 && xx < g.W && yy >= 0 && yy < g.H));
-            r[k] = ok ? in((zz * g.H + yy) * g.W + xx) : 0.f;
-        }
-    };
-    auto store = [&](int64_t zz) {
-        float* pl = ring + (zz % 3) * MX_RP;
+    for (int k = 0; k < MX_NPT; ++k) {
+        const int c = tid + k * 256;
+        const int64_t xx = x0 - 1 + c % MX_RX, yy = y0 - 1 + c / MX_RX;
+        live[k] = c < MX_RP && xx >= 0 && yy >= 0;
+        off[k] = (yy < g.H ? yy : g.H - 1) * g.W + (xx < g.W ? xx : g.W - 1);
+    }
+    float pm[MX_NPT], p0[MX_NPT], pp[MX_NPT], nx[MX_NPT];
+    auto load = [&](float* r, int64_t zz) {
 #pragma unroll
-        for (int k = 0; k < NPT; ++k)
-            if (tid + k * 256 < MX_RP) pl[tid + k * 256] = r[k];
+        for (int k = 0; k < MX_NPT; ++k) r[k] = live[k] ? in(zz * plane + off[k]) : qnan;
     };
-    if (z0 > 0) {
-        load(z0 - 1);
-        store(z0 - 1);
-    }
-    load(z0);
-    store(z0);
-    if (z0 + 1 < g.D && d3) {
-        load(z0 + 1);
-        store(z0 + 1);
-    }
-    float* C = ring + 3 * MX_RP;  // column maxima of the current z
+    if (d3 && z0 > 0) load(pm, z0 - 1);
+    load(p0, z0);
+    if (d3 && z0 + 1 < g.D) load(pp, z0 + 1);
     for (int64_t z = z0; z < z1; ++z) {
-        const bool pre = d3 && z + 2 < z1 + 1 && z + 2 < g.D;
-        if (pre) load(z + 2);
-        __syncthreads();
-        // ring planes of z - 1 (sm, none below 0), z (s0), z + 1 (sp; none past the
-        // last slice, where the visit reads the centre plane again)
-        const int s0 = (int)(z % 3);
-        const int sm = (d3 && z > 0) ? (s0 + 2) % 3 : -1;
-        const int sp = (d3 && z + 1 < g.D) ? (s0 + 1) % 3 : -1;
-        const float* P0 = ring + s0 * MX_RP;
-        // C(x, y) = max over the visited z neighbours at every staged (x, y): positions
-        // before the image's first column / row are never visited (NaN: nothing), those
-        // past its last column / row are the clamped last one (as the visit's clamped
-        // offsets read them)
-        if (inner) {
-            for (int c = tid; c < MX_RP; c += 256) {
-                float m = P0[c];
-                if (sm >= 0) m = fmaxf(m, ring[sm * MX_RP + c]);
-                if (sp >= 0) m = fmaxf(m, ring[sp * MX_RP + c]);
-                C[c] = m;
-            }
-        } else {
-            for (int c = tid; c < MX_RP; c += 256) {
-                const int cx = c % MX_RX, cy = c / MX_RX;
-                const int64_t xx = x0 - 1 + cx, yy = y0 - 1 + cy;
-                float m = __builtin_nanf("");
-                if (xx >= 0 && yy >= 0) {
-                    const int qx = xx < g.W ? cx : (int)(g.W - x0);
-                    const int qy = yy < g.H ? cy : (int)(g.H - y0);
-                    const int q = qy * MX_RX + qx;
-                    m = P0[q];
-                    if (sm >= 0) m = fmaxf(m, ring[sm * MX_RP + q]);
-                    if (sp >= 0) m = fmaxf(m, ring[sp * MX_RP + q]);
-                }
-                C[c] = m;
-            }
+        const bool below = d3 && z > 0, above = d3 && z + 1 < g.D;
+        const bool pre = d3 && z + 2 < g.D && z + 1 < z1;
+        if (pre) load(nx, z + 2);
+        float* Cl = lds + (int)(z & 1) * (2 * MX_RP);
+        float* Yl = Cl + MX_RP;
+        float cm[MX_NPT];
+#pragma unroll
+        for (int k = 0; k < MX_NPT; ++k) {
+            float m = p0[k];
+            if (below) m = fmaxf(m, pm[k]);
+            if (above) m = fmaxf(m, pp[k]);
+            cm[k] = m;
+            if (tid + k * 256 < MX_RP) Cl[tid + k * 256] = m;
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < (MX_TX * MX_TY) / 256; ++k) {
-            const int c = tid + k * 256;
-            const int ix = c % MX_TX, iy = c / MX_TX;
-            if (x0 + ix >= g.W || y0 + iy >= g.H) continue;
-            const int q = (iy + 1) * MX_RX + ix + 1;
-            const float v = P0[q];
+        for (int k = 0; k < MX_NPT; ++k) {
+            const int c = tid + k * 256, cy = c / MX_RX;
+            if (c < MX_RP && cy >= 1 && cy <= MX_TY) Yl[c] = fmaxf(fmaxf(Cl[c - MX_RX], cm[k]), Cl[c + MX_RX]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < MX_NPT; ++k) {
+            const int c = tid + k * 256, cx = c % MX_RX, cy = c / MX_RX;
+            if (c >= MX_RP || cx < 1 || cx > MX_TX || cy < 1 || cy > MX_TY) continue;
+            const int64_t xx = x0 + cx - 1, yy = y0 + cy - 1;
+            if (xx >= g.W || yy >= g.H) continue;
+            const float v = p0[k];
             // The visit in order (x, then y, then z offsets; start lm = v - 1, a strictly
             // larger value takes over, the flag is whether the last take-over was the
             // centre) in closed form: B / A = the largest value visited before / after
@@ -313,20 +290,24 @@ __device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t
             // iff v > lm1 (then lm2 = v); with no take-over before it, the flag stays set;
             // any after-value above lm2 clears it.  Exact for ties, infinities, NaN and
             // values where v - 1 == v.
-            float bmax = fmaxf(fmaxf(C[q - MX_RX - 1], C[q - 1]), fmaxf(C[q + MX_RX - 1], C[q - MX_RX]));
-            float amax = fmaxf(fmaxf(C[q + MX_RX], C[q - MX_RX + 1]), fmaxf(C[q + 1], C[q + MX_RX + 1]));
-            if (sm >= 0) bmax = fmaxf(bmax, ring[sm * MX_RP + q]);
-            if (d3) amax = fmaxf(amax, sp >= 0 ? ring[sp * MX_RP + q] : v);
+            float bmax = fmaxf(Yl[c - 1], Cl[c - MX_RX]);
+            float amax = fmaxf(Yl[c + 1], Cl[c + MX_RX]);
+            if (below) bmax = fmaxf(bmax, pm[k]);
+            if (d3) amax = fmaxf(amax, above ? pp[k] : v);
             const float lm0 = v - 1.f;
             const bool took = bmax > lm0;
             const float lm1 = took ? bmax : lm0;
             const bool centre = v > lm1;
             const float lm2 = centre ? v : lm1;
             const bool self = (centre || !took) && !(amax > lm2);
-            out((z * g.H + y0 + iy) * g.W + x0 + ix, self);
+            out((z * g.H + yy) * g.W + xx, self);
         }
-        __syncthreads();
-        if (pre) store(z + 2);
+#pragma unroll
+        for (int k = 0; k < MX_NPT; ++k) {
+            pm[k] = p0[k];
+            p0[k] = pp[k];
+            pp[k] = nx[k];
+        }
     }
 }
 
