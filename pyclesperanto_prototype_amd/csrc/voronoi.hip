@@ -611,25 +611,21 @@ inline int64_t vor_run_blocks(const Geom& g, int rl) {
     return T.nz * T.ny * ((T.nx + rl - 1) / rl);
 }
 
-#ifndef CLE_VOR_RUN_WAVES
-#define CLE_VOR_RUN_WAVES 4  // waves per SIMD of the 3D form (128 VGPRs)
-#endif
-template <bool D3>
-__global__ __launch_bounds__(VB, D3 ? CLE_VOR_RUN_WAVES : 4) void k_vor_pair_run(const BImg* __restrict__ imgs,
-                                                                               const int64_t* __restrict__ rs,
-                                                                               int nimg, const float* __restrict__ src,
-                                                                               float* __restrict__ dst, int b,
-                                                                               VorLists L, VorState S) {
-    using P = PairGeo<D3>;
-    __shared__ float lds[P::NR];
-    __shared__ float lds2[P::NR];
-    __shared__ int sbits[2];
+// The start of a pair launch's block: block 0 closes pair b - 1 (see above); the block's
+// run (image j, tile row, first tile) and the bit mask of its active tiles.  Every
+// wave computes the same mask; returns false when the run has nothing to do.
+struct VorRun {
+    int j, xs, ty, tz, nrun;
+    int64_t t0;    // first tile of the run in the per-tile arrays
+    uint64_t act;  // bit k: tile xs + k runs in this pair
+};
+__device__ __forceinline__ bool vor_run_begin(const BImg* __restrict__ imgs, const int64_t* __restrict__ rs, int nimg,
+                                              int b, const VorLists& L, const VorState& S, VorRun& R) {
     const uint32_t* prev = S.bits + (size_t)((b + 2) % 3) * nimg;  // pair b - 1
-    uint32_t* cur = S.bits + (size_t)(b % 3) * nimg;
     if (blockIdx.x == 0) {
         uint32_t* next = S.bits + (size_t)((b + 1) % 3) * nimg;
         int mx = 0;
-        for (int j = threadIdx.x; j < nimg; j += VB) {
+        for (int j = threadIdx.x; j < nimg; j += blockDim.x) {
             if (b > 0 && S.iters[j] == 0 && prev[j] != 3u) {
                 const int it = 2 * (b - 1) + ((prev[j] & 1u) ? 2 : 1);
                 S.iters[j] = it;
@@ -641,13 +637,10 @@ __global__ __launch_bounds__(VB, D3 ? CLE_VOR_RUN_WAVES : 4) void k_vor_pair_run
     }
     const int64_t rg = xcd_tile(blockIdx.x, gridDim.x);
     const int j = rs ? find_seg(rs, nimg, rg) : 0;
-    if (S.iters[j] != 0 || (b > 0 && prev[j] != 3u)) return;  // converged in an earlier pair
+    if (S.iters[j] != 0 || (b > 0 && prev[j] != 3u)) return false;  // converged in an earlier pair
     const BImg& im = imgs[j];
-    const Geom g = im.g;
     const int nx = im.vnx, ny = im.vny, nz = im.vnz;
     const int64_t tb = im.tbase;
-    src += im.off;
-    dst += im.off;
     const int rl = im.vrun;  // tiles per run (<= VOR_RUN)
     const int segs = (nx + rl - 1) / rl;
     const int64_t r = rs ? rg - rs[j] : rg;
@@ -655,9 +648,9 @@ __global__ __launch_bounds__(VB, D3 ? CLE_VOR_RUN_WAVES : 4) void k_vor_pair_run
     const int xs = (int)(r - row * segs) * rl;
     const int ty = (int)(row % ny), tz = (int)(row / ny);
     const int nrun = nx - xs < rl ? nx - xs : rl;
-    const int64_t t0 = tb + row * nx + xs;  // first tile of the run in the per-tile arrays
+    const int64_t t0 = tb + row * nx + xs;
     const int32_t* chg_prev = L.mark + (int64_t)((b - 1) & 1) * L.cap + tb;
-    // active tiles of the run: bit k for tile t0 + k (every wave builds the same mask)
+    // active tiles of the run: bit k for tile t0 + k
     uint64_t act;
     {
         constexpr int RW = VOR_RUN + 2;  // stamps per neighbour row: x from xs - 1
@@ -686,7 +679,33 @@ __global__ __launch_bounds__(VB, D3 ? CLE_VOR_RUN_WAVES : 4) void k_vor_pair_run
         const uint64_t st = __ballot(settled);
         act = (uint64_t)((m | (m >> 1) | (m >> 2)) & ((1u << nrun) - 1)) & ~st;
     }
-    if (!act) return;
+    R = VorRun{j, xs, ty, tz, nrun, t0, act};
+    return act != 0;
+}
+
+// Tiles staged in LDS: the 2D form, and the 3D form for images with planes of
+// VOR_WIDE_PLANE voxels or more (sparse large stacks: measured on 256 x 1024 x 1024 and
+// 512 x 2048 x 2048 nuclei stacks, the register-column form below takes 5-25 % longer
+// there, while it takes 24 % less on the mixed batch's small 3D images).
+constexpr int64_t VOR_WIDE_PLANE = (int64_t)1 << 20;
+template <bool D3>
+__global__ __launch_bounds__(VB, 4) void k_vor_pair_run(const BImg* __restrict__ imgs, const int64_t* __restrict__ rs,
+                                                      int nimg, const float* __restrict__ src, float* __restrict__ dst,
+                                                      int b, VorLists L, VorState S) {
+    using P = PairGeo<D3>;
+    __shared__ float lds[P::NR];
+    __shared__ float lds2[P::NR];
+    __shared__ int sbits[2];
+    VorRun run;
+    if (!vor_run_begin(imgs, rs, nimg, b, L, S, run)) return;
+    uint32_t* cur = S.bits + (size_t)(b % 3) * nimg;
+    const int j = run.j, xs = run.xs, ty = run.ty, tz = run.tz;
+    const int64_t t0 = run.t0;
+    uint64_t act = run.act;
+    const BImg& im = imgs[j];
+    const Geom g = im.g;
+    src += im.off;
+    dst += im.off;
     float v[P::KV];
     int tk = __builtin_ctzll(act);  // run index of the current tile
     act &= act - 1;
@@ -721,6 +740,181 @@ __global__ __launch_bounds__(VB, D3 ? CLE_VOR_RUN_WAVES : 4) void k_vor_pair_run
         tk = tn;
     }
     if (threadIdx.x == 0 && acc) atomicOr(&cur[j], (uint32_t)acc);
+    vor_count_tiles(S, done);
+}
+
+// The 3D form on register columns.  One thread per (x, y) column of the staged region
+// (the 16 x 8 x 8 tile and a 2-voxel halo: 20 x 12 columns of 12 voxels) holds the
+// column in registers; region voxels outside the image hold the nearest image voxel
+// (clamped loads), so the maxima over the reference's clamped neighbourhoods are plain
+// maxima over the region.  The box step (tile + 1-voxel ring) is taken as separable
+// maxima: over z in registers (C), over y through LDS (Y), over x through LDS; the
+// diamond step (tile) reads its z neighbours from registers and its x / y neighbours
+// from LDS.  Maxima are exact, so the order of the separable passes does not change a
+// value.  Columns without a zero voxel skip the work whose result they do not need
+// (sparse masks leave most columns of an active tile without one): the box maximum of
+// a voxel is taken only where the voxel is zero, Y only next to such columns, the
+// diamond only in them.  Four barriers per tile; the next tile's columns load while
+// this one computes.
+#ifndef CLE_VOR3_BLOCKS
+#define CLE_VOR3_BLOCKS 6  // blocks per CU the register budget allows (80 VGPRs)
+#endif
+// (the register budget matters: at 8 blocks per CU registers spill; 4 and 5 blocks
+// were measured 41 % and 21 % slower on the mixed batch)
+constexpr int V3_RX = 20, V3_RY = 12, V3_RZ = 12;
+constexpr int V3_COLS = V3_RX * V3_RY;  // 240 columns
+constexpr int V3_NB = V3_RZ - 2;        // box planes: region z 1 .. 10
+
+__global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BImg* __restrict__ imgs,
+                                                                     const int64_t* __restrict__ rs, int nimg,
+                                                                     const float* __restrict__ src,
+                                                                     float* __restrict__ dst, int b, VorLists L,
+                                                                     VorState S) {
+    static_assert(VB >= V3_COLS, "one thread per region column");
+    __shared__ float Cl[V3_NB * V3_COLS];  // C of every column, then the box result
+    __shared__ float Yl[V3_NB * V3_COLS];
+    __shared__ uint8_t Fl[V3_COLS];  // the column holds a zero among the box planes
+    __shared__ int sbits[2];
+    VorRun run;
+    if (!vor_run_begin(imgs, rs, nimg, b, L, S, run)) return;
+    uint32_t* cur = S.bits + (size_t)(b % 3) * nimg;
+    const BImg& im = imgs[run.j];
+    const Geom g = im.g;
+    src += im.off;
+    dst += im.off;
+    const int tid = threadIdx.x;
+    const bool col = tid < V3_COLS;
+    const int rx = tid % V3_RX, ry = tid / V3_RX;
+    const int64_t plane = g.H * g.W;
+    const int64_t y0 = (int64_t)run.ty * 8 - 2, z0 = (int64_t)run.tz * 8 - 2;
+    const int64_t yr = y0 + ry;
+    const int64_t yy = yr < 0 ? 0 : (yr >= g.H ? g.H - 1 : yr);
+    int64_t zoff[V3_RZ];
+#pragma unroll
+    for (int rz = 0; rz < V3_RZ; ++rz) {
+        const int64_t z = z0 + rz;
+        zoff[rz] = (z < 0 ? 0 : (z >= g.D ? g.D - 1 : z)) * plane;
+    }
+    const bool ybox = col && ry >= 1 && ry <= V3_RY - 2;
+    const bool ytile = col && ry >= 2 && ry < V3_RY - 2 && yr < g.H;
+    const bool xbox = rx >= 1 && rx <= V3_RX - 2;
+    auto load = [&](int tx, float* v) {
+        const int64_t xr = (int64_t)tx * 16 - 2 + rx;
+        const float* cp = src + yy * g.W + (xr < 0 ? 0 : (xr >= g.W ? g.W - 1 : xr));
+#pragma unroll
+        for (int rz = 0; rz < V3_RZ; ++rz) v[rz] = col ? cp[zoff[rz]] : 0.f;
+    };
+    uint64_t act = run.act;
+    int tk = __builtin_ctzll(act);
+    act &= act - 1;
+    float v[V3_RZ], vn[V3_RZ];
+    load(run.xs + tk, v);
+    int par = 0;
+    uint32_t done = 0;
+    int acc = 0;
+#pragma unroll 1
+    for (;;) {
+        ++done;
+        const int tx = run.xs + tk;
+        const int64_t xr = (int64_t)tx * 16 - 2 + rx;
+        // tile voxel columns inside the image
+        const bool tcol = ytile && rx >= 2 && rx < V3_RX - 2 && xr < g.W;
+        if (tid == 0) sbits[par] = 0;  // last read two tiles ago, before a barrier
+        // C: maximum over z - 1 .. z + 1 for the box planes; zero test of the tile
+        float c[V3_NB];
+        bool zero = false, hz = false;
+#pragma unroll
+        for (int q = 0; q < V3_NB; ++q) {
+            c[q] = fmaxf(fmaxf(v[q], v[q + 1]), v[q + 2]);
+            if (col) Cl[q * V3_COLS + tid] = c[q];
+            hz |= v[q + 1] == 0.f;
+            if (q >= 1 && q <= 8) zero |= tcol && z0 + q + 1 < g.D && v[q + 1] == 0.f;
+        }
+        if (col) Fl[tid] = hz ? 1 : 0;
+        const bool more = act != 0;
+        const int tn = more ? __builtin_ctzll(act) : tk;
+        act &= act - 1;
+        if (more) load(run.xs + tn, vn);
+        const int64_t obase = yr * g.W + xr;  // output offset of the column at z = 0
+        if (!__syncthreads_or(zero ? 1 : 0)) {
+            // no zero voxel in the tile: the output is the input
+            if (tcol) {
+#pragma unroll
+                for (int rz = 2; rz < V3_RZ - 2; ++rz)
+                    if (z0 + rz < g.D) dst[obase + (z0 + rz) * plane] = v[rz];
+            }
+        } else {
+            int bits = 0;
+            // Y: maximum of C over y - 1 .. y + 1 (box rows)
+            float y[V3_NB];
+            // needed where this column or an x neighbour holds a zero (rows 1 .. 10: the
+            // neighbours' flags lie in the region)
+            const bool ny = ybox && (hz || Fl[tid - 1] || Fl[tid + 1]);
+#pragma unroll
+            for (int q = 0; q < V3_NB; ++q) {
+                y[q] = c[q];
+                if (ny) {
+                    y[q] = fmaxf(fmaxf(Cl[q * V3_COLS + tid - V3_RX], c[q]), Cl[q * V3_COLS + tid + V3_RX]);
+                    Yl[q * V3_COLS + tid] = y[q];
+                }
+            }
+            __syncthreads();
+            // box value of the ring + tile: maximum of Y over x - 1 .. x + 1; zero voxels
+            // take max(0, box maximum)
+            float bx[V3_NB];
+#pragma unroll
+            for (int q = 0; q < V3_NB; ++q) {
+                const float vz = v[q + 1];
+                bx[q] = vz;
+                if (ybox && xbox) {
+                    if (hz) {  // non-zero voxels keep their value
+                        const float m = fmaxf(fmaxf(Yl[q * V3_COLS + tid - 1], y[q]), Yl[q * V3_COLS + tid + 1]);
+                        const bool z = vz == 0.f;
+                        bx[q] = z ? fmaxf(0.f, m) : vz;
+                        if (q >= 1 && q <= 8) bits |= (tcol && z0 + q + 1 < g.D && z && bx[q] != 0.f) ? 1 : 0;
+                    }
+                    Cl[q * V3_COLS + tid] = bx[q];
+                }
+            }
+            __syncthreads();
+            // diamond on the tile: z neighbours in registers, x / y neighbours from LDS
+            if (tcol && hz) {
+#pragma unroll
+                for (int q = 1; q <= 8; ++q) {
+                    const float* p = Cl + q * V3_COLS + tid;
+                    const float nb = fmaxf(fmaxf(fmaxf(p[-1], p[1]), fmaxf(p[-V3_RX], p[V3_RX])),
+                                           fmaxf(bx[q - 1], bx[q + 1]));
+                    const bool z = bx[q] == 0.f;
+                    const float out = z ? fmaxf(0.f, nb) : bx[q];
+                    if (z0 + q + 1 < g.D) {
+                        dst[obase + (z0 + q + 1) * plane] = out;
+                        bits |= z ? (out != 0.f ? 2 : 4) : 0;
+                    }
+                }
+            } else if (tcol) {  // no zero voxel in the column: box and diamond keep it
+#pragma unroll
+                for (int q = 1; q <= 8; ++q)
+                    if (z0 + q + 1 < g.D) dst[obase + (z0 + q + 1) * plane] = v[q + 1];
+            }
+            if (bits) atomicOr(&sbits[par], bits);
+        }
+        // every thread is past its LDS reads of this tile before the next tile's C
+        // writes; the block's change bits
+        __syncthreads();
+        const int tb = sbits[par];
+        par ^= 1;
+        if (tb & 3) {
+            acc |= tb & 3;
+            if (tid == 0) L.mark[(int64_t)(b & 1) * L.cap + run.t0 + tk] = b;
+        } else if (!(tb & 4) && tid == 0) {
+            L.settled[run.t0 + tk] = 1;
+        }
+        if (!more) break;
+        tk = tn;
+#pragma unroll
+        for (int rz = 0; rz < V3_RZ; ++rz) v[rz] = vn[rz];
+    }
+    if (tid == 0 && acc) atomicOr(&cur[run.j], (uint32_t)acc);
     vor_count_tiles(S, done);
 }
 
@@ -785,9 +979,10 @@ struct VorScratch {
 struct VorRuns {
     const int64_t* rs[2];
     int64_t blocks[2];
+    bool wide3;  // 3D launches take the LDS-staged form (planes of VOR_WIDE_PLANE+ voxels)
 };
 inline VorRuns vor_runs_single(const Geom& g) {
-    VorRuns R{{nullptr, nullptr}, {0, 0}};
+    VorRuns R{{nullptr, nullptr}, {0, 0}, g.H * g.W >= VOR_WIDE_PLANE};
     R.blocks[vor_3d(g) ? 1 : 0] = vor_run_blocks(g, vor_run_length(g, false));
     return R;
 }
@@ -800,8 +995,10 @@ static void launch_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, cons
         float* dst = (b & 1) ? P0 : P1;
         if (R.blocks[0] > 0)
             k_vor_pair_run<false><<<(unsigned)R.blocks[0], VB, 0, s>>>(dimgs, R.rs[0], nimg, src, dst, b, L, S);
-        if (R.blocks[1] > 0)
+        if (R.blocks[1] > 0 && R.wide3)
             k_vor_pair_run<true><<<(unsigned)R.blocks[1], VB, 0, s>>>(dimgs, R.rs[1], nimg, src, dst, b, L, S);
+        else if (R.blocks[1] > 0)
+            k_vor_pair_run3<<<(unsigned)R.blocks[1], VB, 0, s>>>(dimgs, R.rs[1], nimg, src, dst, b, L, S);
     }
     k_vor_close<<<1, VB, 0, s>>>(nimg, b - 1, S);
 }
@@ -1434,6 +1631,16 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     for (int k = 0; k < 2; ++k) {
         runs.rs[k] = (const int64_t*)(dev + rs_off[k]);
         runs.blocks[k] = nrun[k];
+    }
+    {
+        // the LDS-staged 3D form when most 3D voxels lie in images of wide planes
+        int64_t v3 = 0, vw = 0;
+        for (const BImg& im : imgs)
+            if (vor_3d(im.g)) {
+                v3 += im.g.n();
+                if (im.g.H * im.g.W >= VOR_WIDE_PLANE) vw += im.g.n();
+            }
+        runs.wide3 = 2 * vw > v3;
     }
     const int64_t* dvs = (const int64_t*)(dev + vs_off);
     if (cap) {

@@ -22,7 +22,9 @@ def test_server_results_equal_alone(cle):
             img = cle.asarray((blobs_image(sh, 8, 600 + k).astype(np.float32) * 30).astype(np.uint16))
         reqs.append((img, float(rng.uniform(1, 4)), float(rng.uniform(0.5, 2))))
     alone = [cle.voronoi_otsu_labeling(im, None, a, b) for im, a, b in reqs]
-    arrivals = serving.poisson_trace(len(reqs), qps=2000.0, seed=61)  # bursts: several per batch
+    # one burst (every request within a few microseconds): requests queue behind the
+    # first batch, so later ones are batched together whatever the kernels' speed
+    arrivals = serving.poisson_trace(len(reqs), qps=1e6, seed=61)
     with serving.SegmentationServer(workers=2) as srv:
         done = serving.replay(srv, reqs, arrivals)
         sizes = [b[0] for b in srv.batches]
