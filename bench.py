@@ -364,6 +364,17 @@ def batch_section(cle, torch, n_images, batch_only=False):
     cle.voronoi_otsu_labeling_batch(imgs, ss, so, destinations=batch_out)  # warm-up (scratch growth)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    cle.voronoi_otsu_labeling_batch(imgs, ss, so, destinations=batch_out)  # default launch shapes
+    torch.cuda.synchronize()
+    batched_default = time.perf_counter() - t0
+    # the batched blur's launch shapes tuned for this batch's groups, ahead of the timed call
+    t0 = time.perf_counter()
+    cle.autotune_voronoi_otsu_labeling_batch(imgs, ss, so, destinations=batch_out)
+    torch.cuda.synchronize()
+    tune = time.perf_counter() - t0
+    cle.voronoi_otsu_labeling_batch(imgs, ss, so, destinations=batch_out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     _, steps = cle.voronoi_otsu_labeling_batch(imgs, ss, so, destinations=batch_out, return_steps=True)
     torch.cuda.synchronize()
     batched = time.perf_counter() - t0
@@ -372,7 +383,8 @@ def batch_section(cle, torch, n_images, batch_only=False):
     if batch_only:
         return {"workload": "batched calls only (kernel trace)", "images": n_images, "voxels": nv,
                 "batched_ms": round(batched * 1e3, 1), "growth_steps": steps}
-    log(f"batched: plain {plain * 1e3:.1f} ms, batched {batched * 1e3:.1f} ms, identical {same}/{n_images}")
+    log(f"batched: plain {plain * 1e3:.1f} ms, batched {batched * 1e3:.1f} ms (default launch shapes "
+        f"{batched_default * 1e3:.1f} ms, tuning {tune * 1e3:.0f} ms), identical {same}/{n_images}")
     # the same batch captured once as a hipGraph and replayed
     graph_out = [cle.create(t.shape, np.uint32) for t in imgs_t]
     t0 = time.perf_counter()
@@ -399,6 +411,7 @@ def batch_section(cle, torch, n_images, batch_only=False):
             "images": n_images, "voxels": nv, "plain_ms": round(plain * 1e3, 1),
             "plain_mvox_s": round(nv / plain / 1e6, 1), "batched_ms": round(batched * 1e3, 1),
             "batched_mvox_s": round(nv / batched / 1e6, 1), "speedup_vs_plain": round(plain / batched, 2),
+            "blur_autotune": {"default_shapes_ms": round(batched_default * 1e3, 1), "tuning_ms": round(tune * 1e3)},
             "growth_steps": steps, "identical_to_plain": f"{same}/{n_images}",
             "graph": {"build_ms": round(build * 1e3, 1), "replay_ms": round(replay * 1e3, 1),
                       "replay_mvox_s": round(nv / replay / 1e6, 1), "speedup_vs_plain": round(plain / replay, 2),

@@ -97,6 +97,16 @@ int cle_copy(const cle_buf* src, const cle_buf* dst, void* stream);
 int cle_autotune_gaussian_blur(const cle_buf* src, const cle_buf* dst,
                                float sigma_x, float sigma_y, float sigma_z, void* stream);
 int cle_set_autotune(int on);
+/* _tier1/_gaussian_blur.py:6-58 for many images at once (the batched pipeline's blur
+ * stage): dsts[i] (float32, srcs[i]'s shape, 2D or 3D) = gaussian_blur(srcs[i],
+ * sigmas[i] on every axis).  Isotropic kernels up to 41 taps run as one launch per
+ * (half-width, dimensionality, source type) group, the rest image by image.  Returns
+ * after the work has finished.  The autotune form also times every group's launch
+ * shapes (see cle_autotune_voronoi_otsu_labeling_batch); results are the same. */
+int cle_gaussian_blur_batch(const cle_buf* srcs, const cle_buf* dsts, const float* sigmas, int n_images,
+                            void* stream);
+int cle_autotune_gaussian_blur_batch(const cle_buf* srcs, const cle_buf* dsts, const float* sigmas, int n_images,
+                                     void* stream);
 
 /* _tier1/_gaussian_blur.py:6-58 + _tier1/_execute_separable_kernel.py:7-75:
  * separable normalised Gaussian, N(s) = floor(8 s) | 1 taps, clamp-to-edge,
@@ -224,6 +234,15 @@ int cle_gauss_otsu_labeling(const cle_buf* src, const cle_buf* dst, float outlin
  * image needed.  Returns after the work has finished (it synchronises the stream). */
 int cle_voronoi_otsu_labeling_batch(const cle_buf* srcs, const cle_buf* dsts, const float* spot_sigmas,
                                     const float* outline_sigmas, int n_images, int* n_steps, void* stream);
+/* Autotuner of the batched blur (no counterpart in the reference): runs
+ * cle_voronoi_otsu_labeling_batch on these arguments, timing for every blur group
+ * (images of one half-width, dimensionality and source type) the candidate launch
+ * shapes -- tile height 32 or 16 rows, register budget, blocks the z chunking aims at --
+ * and keeps the fastest per (half-width, dimensionality, source type, log2 voxels,
+ * log2 images) for the process; later batches of that class use it and time nothing.
+ * dsts hold the labels afterwards.  Results do not depend on the choices. */
+int cle_autotune_voronoi_otsu_labeling_batch(const cle_buf* srcs, const cle_buf* dsts, const float* spot_sigmas,
+                                             const float* outline_sigmas, int n_images, void* stream);
 
 /* Replayable plan of cle_voronoi_otsu_labeling_batch for these exact buffers and
  * sigmas (no counterpart in the reference: the batched executor as a captured

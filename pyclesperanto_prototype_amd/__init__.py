@@ -24,7 +24,8 @@ from ._ops import (STATISTICS_KEYS, add_image_and_scalar, autotune_gaussian_blur
                    statistics_of_background_and_labelled_pixels, statistics_of_labeled_pixels,
                    statistics_of_labelled_pixels, statistics_table, sum_of_all_pixels, threshold_otsu,
                    voronoi_labeling, voronoi_otsu_labeling, voronoi_otsu_labeling_batch,
-                   voronoi_otsu_labeling_batch_plan, BatchPlan)
+                   voronoi_otsu_labeling_batch_plan, BatchPlan, autotune_voronoi_otsu_labeling_batch,
+                   gaussian_blur_batch)
 from .io import imread, imread_stack, push_regionprops, push_regionprops_column
 from ._lib import load as _load_library
 

@@ -77,6 +77,8 @@ SIGNATURES = {
     "cle_set_growth_method": [ctypes.c_int],
     "cle_growth_counters": [ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int],
     "cle_autotune_gaussian_blur": [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _V],
+    "cle_gaussian_blur_batch": [_P, _P, ctypes.POINTER(ctypes.c_float), ctypes.c_int, _V],
+    "cle_autotune_gaussian_blur_batch": [_P, _P, ctypes.POINTER(ctypes.c_float), ctypes.c_int, _V],
     "cle_voronoi_otsu_labeling_batch_plan": [_P, _P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                              ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), _V],
     "cle_plan_run": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _V],
@@ -86,6 +88,8 @@ SIGNATURES = {
     "cle_gauss_otsu_labeling": [_P, _P, _F, _V],
     "cle_voronoi_otsu_labeling_batch": [_P, _P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                         ctypes.c_int, ctypes.POINTER(ctypes.c_int), _V],
+    "cle_autotune_voronoi_otsu_labeling_batch": [_P, _P, ctypes.POINTER(ctypes.c_float),
+                                                 ctypes.POINTER(ctypes.c_float), ctypes.c_int, _V],
     "cle_statistics_of_labelled_pixels": [_P, _P, ctypes.c_int64, _P, ctypes.POINTER(ctypes.c_int64), _V],
 }
 _STR = {"cle_version", "cle_last_error"}

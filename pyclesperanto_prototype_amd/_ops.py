@@ -336,6 +336,46 @@ def voronoi_otsu_labeling_batch(sources, spot_sigmas=2, outline_sigmas=2, destin
     return (destinations, steps.value) if return_steps else destinations
 
 
+def gaussian_blur_batch(sources, sigmas=2, destinations=None, autotune: bool = False):
+    """gaussian_blur (_tier1/_gaussian_blur.py:6-58) of many 2D / 3D images at once, the
+    same sigma on every axis of an image (one value for all or one per image), into
+    float32 images -- the batched pipeline's blur stage.  autotune=True also times the
+    launch shapes of every group (ahead of later batches of the same classes)."""
+    srcs = [s if isinstance(s, Array) else push(s) for s in sources]
+    n = len(srcs)
+    sg = np.broadcast_to(np.asarray(sigmas, np.float32), (n,)).copy()
+    if destinations is None:
+        destinations = [create_like(s) for s in srcs]
+    if len(destinations) != n:
+        raise ValueError(f"{len(destinations)} destinations for {n} sources")
+    src_bufs = (_lib.cle_buf * max(n, 1))(*[s.buf() for s in srcs])
+    dst_bufs = (_lib.cle_buf * max(n, 1))(*[d.buf() for d in destinations])
+    fp = ctypes.POINTER(ctypes.c_float)
+    _lib.call("cle_autotune_gaussian_blur_batch" if autotune else "cle_gaussian_blur_batch", src_bufs, dst_bufs,
+              sg.ctypes.data_as(fp), n, _S())
+    return destinations
+
+
+def autotune_voronoi_otsu_labeling_batch(sources, spot_sigmas=2, outline_sigmas=2, destinations=None):
+    """Tune the batched blur's launch shapes (tile height, register budget, z chunking)
+    for every blur group of this batch ahead of time; later batches of the same
+    classes use the choices.  Runs the batch (synchronising) and returns its labels."""
+    srcs = [s if isinstance(s, Array) else push(s) for s in sources]
+    n = len(srcs)
+    ss = np.broadcast_to(np.asarray(spot_sigmas, np.float32), (n,)).copy()
+    so = np.broadcast_to(np.asarray(outline_sigmas, np.float32), (n,)).copy()
+    if destinations is None:
+        destinations = [create_labels_like(s) for s in srcs]
+    if len(destinations) != n:
+        raise ValueError(f"{len(destinations)} destinations for {n} sources")
+    src_bufs = (_lib.cle_buf * max(n, 1))(*[s.buf() for s in srcs])
+    dst_bufs = (_lib.cle_buf * max(n, 1))(*[d.buf() for d in destinations])
+    fp = ctypes.POINTER(ctypes.c_float)
+    _lib.call("cle_autotune_voronoi_otsu_labeling_batch", src_bufs, dst_bufs, ss.ctypes.data_as(fp),
+              so.ctypes.data_as(fp), n, _S())
+    return destinations
+
+
 class BatchPlan:
     """A batch of voronoi_otsu_labeling captured as one hipGraph for these exact source
     and destination buffers and sigmas (cle_voronoi_otsu_labeling_batch_plan).

@@ -23,6 +23,7 @@
 #include <atomic>
 #include <cstdlib>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <tuple>
 #include <vector>
@@ -304,18 +305,19 @@ struct IsoW {
     float w[C + 1];
 };
 
-// Isotropic tile: ISO_TY x ISO_TX outputs per slice, two adjacent columns per thread.
+// Isotropic tile: TY x ISO_TX outputs per slice (TY = ISO_TY, or 16 for the batched
+// kernels when the autotuner picks it), two adjacent columns per thread.
 // A launch aims at ISO_BLOCKS blocks (two rounds of one block per CU): fewer, longer
 // z chunks re-read less of the z halo.
 constexpr int ISO_TX = 64, ISO_TY = 32, ISO_THREADS = ISO_TX * ISO_TY / 2;
 constexpr int ISO_BLOCKS = 512;
 
-template <int C>
+template <int C, int TY = ISO_TY>
 struct IsoLds {
-    static constexpr int AW = ISO_TX + 2 * C, AH = ISO_TY + 2 * C;
+    static constexpr int AW = ISO_TX + 2 * C, AH = TY + 2 * C;
     static constexpr int AWP = AW;                // row pitch of both buffers (even: float2 reads)
     static constexpr int ASZ = AH * AWP;          // one staged slice (with x and y halo)
-    static constexpr int BSZ = ISO_TY * AWP;      // one y-blurred slice (with x halo)
+    static constexpr int BSZ = TY * AWP;          // one y-blurred slice (with x halo)
 };
 
 // One column tile (bid = tile index inside the image) of the isotropic blur; A and B
@@ -324,22 +326,23 @@ struct IsoLds {
 // too), then along x into registers; the z pass runs over a register ring of the
 // last 2 * CZ + U xy-blurred values of the thread's two columns.  Doing y before x
 // blurs 1 + 2C / ISO_TX of the tile in the first pass instead of 1 + 2C / ISO_TY.
-template <int C, bool Z3, typename LD, int U, int PB>
+template <int C, bool Z3, typename LD, int U, int PB, int TY = ISO_TY>
 __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, const Geom& g, int zc, int ntx, int nty,
                                               const IsoW<C>& w, int64_t bid, float* __restrict__ A,
                                               float* __restrict__ B) {
-    constexpr int AW = IsoLds<C>::AW, AH = IsoLds<C>::AH, NA = AW * AH;
-    constexpr int AWP = IsoLds<C>::AWP, ASZ = IsoLds<C>::ASZ, BSZ = IsoLds<C>::BSZ;
-    constexpr int PF = (NA + ISO_THREADS - 1) / ISO_THREADS;
+    constexpr int NT = ISO_TX * TY / 2;  // threads
+    constexpr int AW = IsoLds<C, TY>::AW, AH = IsoLds<C, TY>::AH, NA = AW * AH;
+    constexpr int AWP = IsoLds<C, TY>::AWP, ASZ = IsoLds<C, TY>::ASZ, BSZ = IsoLds<C, TY>::BSZ;
+    constexpr int PF = (NA + NT - 1) / NT;
     constexpr int CZ = Z3 ? C : 0;
     static_assert(U % 2 == 0, "slices per ring shift: even");
     constexpr int RL = 2 * CZ + U;       // ring length
     constexpr int RY = 4;                // y-pass outputs per task (rows of one column)
-    constexpr int YT = AW * (ISO_TY / RY);
-    constexpr int YR = (YT + ISO_THREADS - 1) / ISO_THREADS;
+    constexpr int YT = AW * (TY / RY);
+    constexpr int YR = (YT + NT - 1) / NT;
     const int64_t tx = bid % ntx, rest = bid / ntx;
     const int64_t ty = rest % nty, tzc = rest / nty;
-    const int64_t x0 = tx * ISO_TX, y0 = ty * ISO_TY;
+    const int64_t x0 = tx * ISO_TX, y0 = ty * TY;
     const int64_t zb = tzc * zc;
     const int64_t ze = zb + zc < g.D ? zb + zc : g.D;
     const int row = threadIdx.x / (ISO_TX / 2);
@@ -352,7 +355,7 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
     int off[PF];
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
-        const int idx = threadIdx.x + k * ISO_THREADS;
+        const int idx = threadIdx.x + k * NT;
         const int ay = idx / AW, ax = idx - ay * AW;
         int gy = (int)y0 - C + ay, gx = (int)x0 - C + ax;
         gy = gy < 0 ? 0 : (gy > Hm ? Hm : gy);
@@ -374,13 +377,13 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
         const int64_t base = zz * g.H * g.W;
 #pragma unroll
         for (int m = 0; m < PF; ++m)
-            if (threadIdx.x + m * ISO_THREADS < NA)
+            if (threadIdx.x + m * NT < NA)
                 pf[m] = PB == 1 ? in.raw_b(base, (uint32_t)off[m]) : in.raw(base + off[m]);
     };
     auto store = [&](const R (&pf)[PF], float* a) {
 #pragma unroll
         for (int m = 0; m < PF; ++m)
-            if (threadIdx.x + m * ISO_THREADS < NA) a[threadIdx.x + m * ISO_THREADS] = LD::cvt(pf[m]);
+            if (threadIdx.x + m * NT < NA) a[threadIdx.x + m * NT] = LD::cvt(pf[m]);
     };
     // The x and y passes take the taps in order (w[|j - C|] * in[j], j = 0 .. 2C),
     // one fused multiply-add each, as the values stream in from LDS.
@@ -389,7 +392,7 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
     auto ypass = [&](const float* __restrict__ a, float* __restrict__ bo) {
 #pragma unroll
         for (int k = 0; k < YR; ++k) {
-            const int t = threadIdx.x + k * ISO_THREADS;
+            const int t = threadIdx.x + k * NT;
             if (t < YT) {
                 const int q = t / AW, c = t - q * AW;
                 const float* ap = a + q * RY * AWP + c;
@@ -553,20 +556,21 @@ template <> __device__ __forceinline__ LdF32 make_ld<LdF32>(const void* p) { ret
 template <> __device__ __forceinline__ LdU16 make_ld<LdU16>(const void* p) { return LdU16{(const uint16_t*)p}; }
 template <> __device__ __forceinline__ LdU8 make_ld<LdU8>(const void* p) { return LdU8{(const uint8_t*)p}; }
 
-template <int C, bool Z3, typename LD, int U, int PB>
+template <int C, bool Z3, typename LD, int U, int PB, int TY = ISO_TY>
 __device__ __forceinline__ void blur_iso_batch_tile(const IsoJob<C>* __restrict__ jobs,
                                                     const int64_t* __restrict__ bstart, int njobs) {
-    __shared__ __align__(16) float A[2 * IsoLds<C>::ASZ];
-    __shared__ __align__(16) float B[2 * IsoLds<C>::BSZ];
+    __shared__ __align__(16) float A[2 * IsoLds<C, TY>::ASZ];
+    __shared__ __align__(16) float B[2 * IsoLds<C, TY>::BSZ];
     const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
     const int j = find_seg(bstart, njobs, b);
     const IsoJob<C>& jb = jobs[j];
-    blur_iso_tile<C, Z3, LD, U, PB>(make_ld<LD>(jb.in), jb.out, jb.g, jb.zc, jb.ntx, jb.nty, jb.w, b - bstart[j], A, B);
+    blur_iso_tile<C, Z3, LD, U, PB, TY>(make_ld<LD>(jb.in), jb.out, jb.g, jb.zc, jb.ntx, jb.nty, jb.w, b - bstart[j], A,
+                                        B);
 }
-template <int C, bool Z3, typename LD>
-__global__ __launch_bounds__(ISO_THREADS) void k_blur_iso_batch(const IsoJob<C>* __restrict__ jobs,
-                                                              const int64_t* __restrict__ bstart, int njobs) {
-    blur_iso_batch_tile<C, Z3, LD, iso_u<C>(), 2>(jobs, bstart, njobs);
+template <int C, bool Z3, typename LD, int TY = ISO_TY>
+__global__ __launch_bounds__(ISO_TX * TY / 2) void k_blur_iso_batch(const IsoJob<C>* __restrict__ jobs,
+                                                                  const int64_t* __restrict__ bstart, int njobs) {
+    blur_iso_batch_tile<C, Z3, LD, iso_u<C>(), 2, TY>(jobs, bstart, njobs);
 }
 template <int C, bool Z3, typename LD>
 __global__ __launch_bounds__(ISO_THREADS, 8) void k_blur_iso_batch_o8(const IsoJob<C>* __restrict__ jobs,
@@ -957,12 +961,32 @@ static int separable_impl(const cle_buf* src, const cle_buf* dst, const Taps* ta
 // The isotropic requests of a batch are grouped by (half-width, dimensionality, source
 // type); each group is one launch of k_blur_iso_batch over the column tiles of all
 // its images.  The rest (sigma <= 0, wider kernels, other types) run blur_impl.
-template <int C, bool Z3, typename LD>
-static void plan_iso_group(const std::vector<BlurReq>& reqs, const std::vector<int>& idx, Blob& blob,
-                           std::vector<Launch>& out, hipStream_t s) {
+// A group's launch shape: the tile height (32 or 16 rows, 1024 or 512 threads), the
+// register budget (TY = 32 only) and the number of blocks its z chunking aims at.  The
+// default is 32 rows, the single-image budget rule and 256 blocks; the autotuner
+// (cle_autotune_voronoi_otsu_labeling_batch) times the candidates of every group of
+// one batch and keeps the fastest per (half-width, dimensionality, source type, log2
+// voxels, log2 images) for the process.  Every shape sums each voxel's taps in the
+// same order: the output does not depend on it.
+struct BTuneKey {
+    int c, z3, src, dev, lvox, limg;
+    bool operator<(const BTuneKey& o) const {
+        return std::tie(c, z3, src, dev, lvox, limg) < std::tie(o.c, o.z3, o.src, o.dev, o.lvox, o.limg);
+    }
+};
+struct BTuneChoice {
+    int ty, o8;
+    int64_t target;  // blocks the z chunking aims at
+};
+static std::map<BTuneKey, BTuneChoice> g_btune;  // guarded by g_tune_mu
+
+// job table and block prefix of a group for one launch shape; returns the block count
+template <int C, bool Z3>
+static int64_t iso_group_tables(const std::vector<BlurReq>& reqs, const std::vector<int>& idx, int ty, int64_t target,
+                                std::vector<IsoJob<C>>& jobs, std::vector<int64_t>& bst) {
     constexpr int CZ = Z3 ? C : 0;
-    std::vector<IsoJob<C>> jobs(idx.size());
-    std::vector<int64_t> bst(idx.size() + 1);
+    jobs.resize(idx.size());
+    bst.resize(idx.size() + 1);
     int64_t nb = 0;
     for (size_t k = 0; k < idx.size(); ++k) {
         const BlurReq& r = reqs[idx[k]];
@@ -972,10 +996,10 @@ static void plan_iso_group(const std::vector<BlurReq>& reqs, const std::vector<i
         jb.out = r.out;
         jb.g = g;
         jb.ntx = (int)((g.W + ISO_TX - 1) / ISO_TX);
-        jb.nty = (int)((g.H + ISO_TY - 1) / ISO_TY);
+        jb.nty = (int)((g.H + ty - 1) / ty);
         const int64_t nxy = (int64_t)jb.ntx * jb.nty;
-        // the batch fills the chip: z chunks only for images with few column tiles
-        int64_t nzc = (256 + nxy - 1) / nxy;
+        // z chunks only for images with few column tiles
+        int64_t nzc = (target + nxy - 1) / nxy;
         const int64_t max_nzc = g.D / (4 * CZ + 8) > 1 ? g.D / (4 * CZ + 8) : 1;
         if (nzc > max_nzc) nzc = max_nzc;
         if (nzc < 1) nzc = 1;
@@ -988,20 +1012,114 @@ static void plan_iso_group(const std::vector<BlurReq>& reqs, const std::vector<i
         nb += nxy * nzc;
     }
     bst[idx.size()] = nb;
+    return nb;
+}
+
+template <int C, bool Z3, typename LD>
+static void launch_iso_group(const IsoJob<C>* dj, const int64_t* db, int nj, int64_t nb, const BTuneChoice& ch,
+                             hipStream_t s) {
+    if (ch.ty == 16) {
+        k_blur_iso_batch<C, Z3, LD, 16><<<(unsigned)nb, ISO_TX * 16 / 2, 0, s>>>(dj, db, nj);
+        return;
+    }
+    if constexpr (iso_o8_ok<C, Z3>()) {
+        if (ch.o8) {
+            k_blur_iso_batch_o8<C, Z3, LD><<<(unsigned)nb, ISO_THREADS, 0, s>>>(dj, db, nj);
+            return;
+        }
+    }
+    k_blur_iso_batch<C, Z3, LD><<<(unsigned)nb, ISO_THREADS, 0, s>>>(dj, db, nj);
+}
+
+// Times the candidate shapes of one group (its tables uploaded for each) and returns
+// the fastest; runs only inside an explicit tuning call.
+template <int C, bool Z3, typename LD>
+static BTuneChoice tune_iso_group(const std::vector<BlurReq>& reqs, const std::vector<int>& idx,
+                                  const BTuneChoice& dflt, hipStream_t s) {
+    BTuneChoice best = dflt;
+    float best_ms = 1e30f;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return best;
+    }
+    for (int ty : {32, 16})
+        for (int o8 = 0; o8 <= (ty == 32 && iso_o8_ok<C, Z3>() ? 1 : 0); ++o8)
+            for (int64_t target : {256, 1024, 4096}) {
+                const BTuneChoice ch{ty, o8, target};
+                std::vector<IsoJob<C>> jobs;
+                std::vector<int64_t> bst;
+                const int64_t nb = iso_group_tables<C, Z3>(reqs, idx, ty, target, jobs, bst);
+                Scratch dt;
+                const size_t jb = jobs.size() * sizeof(IsoJob<C>), bb = bst.size() * sizeof(int64_t);
+                if (dt.alloc(jb + 256 + bb, s) != CLE_OK) continue;
+                char* d = dt.get<char>();
+                if (hipMemcpyAsync(d, jobs.data(), jb, hipMemcpyHostToDevice, s) != hipSuccess ||
+                    hipMemcpyAsync(d + ((jb + 255) & ~(size_t)255), bst.data(), bb, hipMemcpyHostToDevice, s) !=
+                        hipSuccess ||
+                    hipStreamSynchronize(s) != hipSuccess)
+                    continue;
+                const IsoJob<C>* dj = (const IsoJob<C>*)d;
+                const int64_t* dbs = (const int64_t*)(d + ((jb + 255) & ~(size_t)255));
+                launch_iso_group<C, Z3, LD>(dj, dbs, (int)idx.size(), nb, ch, s);  // warm
+                for (int rep = 0; rep < 3; ++rep) {
+                    (void)hipEventRecord(e0, s);
+                    launch_iso_group<C, Z3, LD>(dj, dbs, (int)idx.size(), nb, ch, s);
+                    (void)hipEventRecord(e1, s);
+                    float ms = 1e30f;
+                    if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
+                        ms < best_ms) {
+                        best_ms = ms;
+                        best = ch;
+                    }
+                }
+                (void)hipStreamSynchronize(s);  // the tables stay alive until the launches ran
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
+template <int C, bool Z3, typename LD>
+static void plan_iso_group(const std::vector<BlurReq>& reqs, const std::vector<int>& idx, Blob& blob,
+                           std::vector<Launch>& out, hipStream_t s) {
+    BTuneChoice ch{32, iso_o8_default(C, Z3), 256};
+    bool tune = false;
+    BTuneKey key{};
+    if (autotune_on()) {
+        int64_t vox = 0;
+        for (int i : idx) vox += reqs[i].g.n();
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        key = BTuneKey{C, Z3 ? 1 : 0, ld_code<LD>(), dev, ilog2(vox), ilog2((int64_t)idx.size())};
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_btune.find(key);
+        if (it != g_btune.end()) ch = it->second;
+        else tune = t_tuning;
+    }
+    std::vector<IsoJob<C>> jobs;
+    std::vector<int64_t> bst;
+    const int64_t nb = iso_group_tables<C, Z3>(reqs, idx, ch.ty, ch.target, jobs, bst);
     const size_t jo = blob.put_vec(jobs), bo = blob.put_vec(bst);
     const int nj = (int)idx.size();
-    const int o8 = iso_o8_default(C, Z3);
+    // the group's requests travel with the launch only when it tunes
+    std::shared_ptr<std::vector<BlurReq>> treq;
+    if (tune) {
+        treq = std::make_shared<std::vector<BlurReq>>();
+        for (int i : idx) treq->push_back(reqs[i]);
+    }
     out.push_back([=](const char* dev) -> int {
-        const IsoJob<C>* dj = (const IsoJob<C>*)(dev + jo);
-        const int64_t* db = (const int64_t*)(dev + bo);
-        bool done = false;
-        if constexpr (iso_o8_ok<C, Z3>()) {
-            if (o8) {
-                k_blur_iso_batch_o8<C, Z3, LD><<<(unsigned)nb, ISO_THREADS, 0, s>>>(dj, db, nj);
-                done = true;
+        if (treq) {
+            hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(s, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone) {
+                std::vector<int> all(treq->size());
+                for (size_t k = 0; k < all.size(); ++k) all[k] = (int)k;
+                const BTuneChoice best = tune_iso_group<C, Z3, LD>(*treq, all, ch, s);
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                g_btune[key] = best;
             }
         }
-        if (!done) k_blur_iso_batch<C, Z3, LD><<<(unsigned)nb, ISO_THREADS, 0, s>>>(dj, db, nj);
+        launch_iso_group<C, Z3, LD>((const IsoJob<C>*)(dev + jo), (const int64_t*)(dev + bo), nj, nb, ch, s);
         CLE_LAUNCHED();
         return CLE_OK;
     });
@@ -1082,6 +1200,41 @@ int blur_impl(const cle_buf* src, const cle_buf* dst, const float sig[3], hipStr
     return separable_impl(src, dst, taps, s);
 }
 
+void blur_tuning(bool on) { t_tuning = on; }
+
+// Isotropic blurs of many images into float32 destinations, planned as in the batched
+// pipeline (one launch per group); synchronises the stream.
+static int blur_batch(const cle_buf* srcs, const cle_buf* dsts, const float* sig, int n, bool tune, hipStream_t s) {
+    if (n < 0) return fail(CLE_E_VALUE, "n_images must be >= 0");
+    if (n == 0) return CLE_OK;
+    if (!srcs || !dsts || !sig) return fail(CLE_E_VALUE, "null batch argument");
+    std::vector<BlurReq> reqs((size_t)n);
+    for (int j = 0; j < n; ++j) {
+        Geom g, gd;
+        CLE_TRY(check_buf(&srcs[j], "source", &g));
+        CLE_TRY(check_buf(&dsts[j], "destination", &gd));
+        CLE_TRY(check_same_shape(g, gd, "source", "destination"));
+        if (dsts[j].dtype != CLE_F32) return fail(CLE_E_TYPE, "destination %d: batched blurs write float32", j);
+        if (!(sig[j] >= 0.f)) return fail(CLE_E_VALUE, "sigma of image %d must be >= 0", j);
+        if (g.ndim == 1) return fail(CLE_E_SHAPE, "image %d: batched blurs take 2D and 3D images", j);
+        reqs[j] = BlurReq{srcs[j].data, srcs[j].dtype, (float*)dsts[j].data, g, sig[j]};
+    }
+    Blob blob;
+    std::vector<Launch> L;
+    CLE_TRY(plan_blur_batch(reqs, blob, L, s));
+    Scratch dev;
+    CLE_TRY(dev.alloc(blob.h.size(), s));
+    CLE_HIP(hipMemcpyAsync(dev.p, blob.h.data(), blob.h.size(), hipMemcpyHostToDevice, s));
+    t_tuning = tune;
+    int rc = CLE_OK;
+    for (auto& l : L)
+        if (rc == CLE_OK) rc = l(dev.get<char>());
+    t_tuning = false;
+    CLE_TRY(rc);
+    CLE_HIP(hipStreamSynchronize(s));  // the host tables and the scratch outlive the work
+    return CLE_OK;
+}
+
 }  // namespace cle
 
 using namespace cle;
@@ -1093,8 +1246,20 @@ int cle_set_autotune(int on) {
     if (!on) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune.clear();
+        g_btune.clear();
     }
     return CLE_OK;
+}
+
+int cle_gaussian_blur_batch(const cle_buf* srcs, const cle_buf* dsts, const float* sigmas, int n_images,
+                            void* stream) {
+    return blur_batch(srcs, dsts, sigmas, n_images, false, (hipStream_t)stream);
+}
+
+int cle_autotune_gaussian_blur_batch(const cle_buf* srcs, const cle_buf* dsts, const float* sigmas, int n_images,
+                                     void* stream) {
+    if (!autotune_on()) return blur_batch(srcs, dsts, sigmas, n_images, false, (hipStream_t)stream);
+    return blur_batch(srcs, dsts, sigmas, n_images, true, (hipStream_t)stream);
 }
 
 int cle_autotune_gaussian_blur(const cle_buf* src, const cle_buf* dst, float sigma_x, float sigma_y, float sigma_z,

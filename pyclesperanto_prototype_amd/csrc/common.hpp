@@ -382,6 +382,9 @@ struct BlurReq {
     float sigma;
 };
 int plan_blur_batch(const std::vector<BlurReq>& reqs, Blob& blob, std::vector<Launch>& out, hipStream_t s);
+// the calling thread's blurs (single and batched) time their candidate launch shapes
+// where no choice is known yet (explicit tuning calls only)
+void blur_tuning(bool on);
 // per image: mm[2 j] = min, mm[2 j + 1] = max, thr[j] = Otsu threshold of the float32
 // work image F (images flagged big run otsu_impl one by one)
 int plan_otsu_batch(const std::vector<BImg>& imgs, size_t imgs_off, size_t vstart_off, int64_t vblocks,

@@ -1882,6 +1882,17 @@ int cle_voronoi_otsu_labeling_batch(const cle_buf* srcs, const cle_buf* dsts, co
     return voronoi_otsu_batch(srcs, dsts, spot_sigmas, outline_sigmas, n_images, n_steps, (hipStream_t)stream);
 }
 
+int cle_autotune_voronoi_otsu_labeling_batch(const cle_buf* srcs, const cle_buf* dsts, const float* spot_sigmas,
+                                             const float* outline_sigmas, int n_images, void* stream) {
+    if (n_images < 0) return fail(CLE_E_VALUE, "n_images must be >= 0");
+    if (n_images == 0) return CLE_OK;
+    if (!srcs || !dsts || !spot_sigmas || !outline_sigmas) return fail(CLE_E_VALUE, "null batch argument");
+    blur_tuning(true);
+    const int rc = voronoi_otsu_batch(srcs, dsts, spot_sigmas, outline_sigmas, n_images, nullptr, (hipStream_t)stream);
+    blur_tuning(false);
+    return rc;
+}
+
 int cle_gauss_otsu_labeling(const cle_buf* src, const cle_buf* dst, float outline_sigma, void* stream) {
     Geom g, gd;
     CLE_TRY(check_buf(src, "source", &g));

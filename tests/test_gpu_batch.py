@@ -142,3 +142,48 @@ def test_batch_plan_replays_equal_eager(cle, graph_pairs, single):
         plan.close()
     with pytest.raises(ValueError):
         plan.run()
+
+
+def test_batched_blur_launch_shapes_identical(cle):
+    # the batched blur under the default launch shapes, while the autotuner times every
+    # group's shapes (tile height 32 / 16 rows, register budget, z chunking), and under
+    # the tuned shapes gives the same bits, equal to each image's single-image blur
+    import torch
+    from oracle import cle_oracle as O
+    rng = np.random.default_rng(70)
+    imgs, sig = [], []
+    for k, sh in enumerate([(70, 130), (300, 257), (20, 37, 70), (12, 100, 90), (1, 50, 60), (9, 64, 64),
+                            (64, 64), (33, 47), (5, 20, 300), (200, 200)]):
+        a = rng.integers(0, 4000, sh)
+        if k % 3 == 0:
+            imgs.append(cle.asarray(torch.from_numpy(a.astype(np.int16)).cuda().view(torch.uint16)))
+        elif k % 3 == 1:
+            imgs.append(cle.push(a.astype(np.float32)))
+        else:
+            imgs.append(cle.asarray(torch.from_numpy(a.astype(np.uint8)).cuda()))
+        sig.append(float(rng.choice([0.8, 1.5, 2.0, 3.3, 5.6])))  # 5.6: 45 taps, the per-image path
+    cle.set_autotune(False)
+    base = [cle.pull(o) for o in cle.gaussian_blur_batch(imgs, sig)]
+    cle.set_autotune(True)
+    tuning = [cle.pull(o) for o in cle.gaussian_blur_batch(imgs, sig, autotune=True)]
+    tuned = [cle.pull(o) for o in cle.gaussian_blur_batch(imgs, sig)]
+    for k, (im, s) in enumerate(zip(imgs, sig)):
+        single = cle.pull(cle.gaussian_blur(im, None, s, s, s))
+        assert np.array_equal(base[k], single), k
+        assert np.array_equal(tuning[k], single) and np.array_equal(tuned[k], single), k
+        if k < 3:
+            src = cle.pull(im).astype(np.float32)
+            want = O.gaussian_blur(src, s, s, s)
+            assert float(np.abs(base[k] - want).max()) <= 1e-4 * max(1.0, float(np.abs(want).max())), k
+
+
+def test_batch_autotuned_labels_identical(cle):
+    rng = np.random.default_rng(71)
+    imgs, ss, so = _mixed_batch(rng, cle)
+    cle.set_autotune(False)
+    want = [cle.pull(o) for o in cle.voronoi_otsu_labeling_batch(imgs, ss, so)]
+    cle.set_autotune(True)
+    got1 = [cle.pull(o) for o in cle.autotune_voronoi_otsu_labeling_batch(imgs, ss, so)]
+    got2 = [cle.pull(o) for o in cle.voronoi_otsu_labeling_batch(imgs, ss, so)]
+    for k in range(len(imgs)):
+        assert np.array_equal(got1[k], want[k]) and np.array_equal(got2[k], want[k]), k
