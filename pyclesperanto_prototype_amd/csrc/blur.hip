@@ -329,7 +329,7 @@ struct IsoLds {
 template <int C, bool Z3, typename LD, int U, int PB, int TY = ISO_TY>
 __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, const Geom& g, int zc, int ntx, int nty,
                                               const IsoW<C>& w, int64_t bid, float* __restrict__ A,
-                                              float* __restrict__ B) {
+                                              float* __restrict__ B, uint32_t* __restrict__ mm = nullptr) {
     constexpr int NT = ISO_TX * TY / 2;  // threads
     constexpr int AW = IsoLds<C, TY>::AW, AH = IsoLds<C, TY>::AH, NA = AW * AH;
     constexpr int AWP = IsoLds<C, TY>::AWP, ASZ = IsoLds<C, TY>::ASZ, BSZ = IsoLds<C, TY>::BSZ;
@@ -367,6 +367,8 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
     float r0[RL], r1[RL];
 #pragma unroll
     for (int j = 0; j < RL; ++j) { r0[j] = 0.f; r1[j] = 0.f; }
+    // minimum / maximum of this thread's outputs (folded into mm at the end when asked)
+    float omn = INFINITY, omx = -INFINITY;
     const int64_t z00 = zb - CZ;           // input slice k is z00 + k
     const int64_t N = ze + CZ - z00;       // input slices
 
@@ -453,6 +455,8 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
                                    : out + (zo * g.H + y) * g.W + x;
                 if (x + 1 < g.W) { o[0] = o0; o[1] = o1; }
                 else if (x < g.W) o[0] = o0;
+                omn = fminf(omn, fminf(x < g.W ? o0 : INFINITY, x + 1 < g.W ? o1 : INFINITY));
+                omx = fmaxf(omx, fmaxf(x < g.W ? o0 : -INFINITY, x + 1 < g.W ? o1 : -INFINITY));
             }
         }
     };
@@ -511,6 +515,26 @@ __device__ __forceinline__ void blur_iso_tile(LD in, float* __restrict__ out, co
 #pragma unroll
         for (int j = 0; j < 2 * CZ; ++j) { r0[j] = r0[j + U]; r1[j] = r1[j + U]; }
     }
+    if (mm) {  // block minimum / maximum -> two atomics (A is free after the last barrier)
+        for (int off = 32; off > 0; off >>= 1) {
+            omn = fminf(omn, __shfl_xor(omn, off));
+            omx = fmaxf(omx, __shfl_xor(omx, off));
+        }
+        constexpr int NW = NT / 64;
+        if ((threadIdx.x & 63) == 0) {
+            A[threadIdx.x >> 6] = omn;
+            A[NW + (threadIdx.x >> 6)] = omx;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int k = 1; k < NW; ++k) {
+                omn = fminf(omn, A[k]);
+                omx = fmaxf(omx, A[NW + k]);
+            }
+            atomicMin(mm, f2o(omn));
+            atomicMax(mm + 1, f2o(omx));
+        }
+    }
 }
 
 // Two register budgets of the same tile code.  k_blur_iso leaves the compiler its
@@ -526,17 +550,17 @@ template <int C> constexpr int iso_u() { return C <= 10 ? 4 : 2; }
 
 template <int C, bool Z3, typename LD>
 __global__ __launch_bounds__(ISO_THREADS) void k_blur_iso(LD in, float* __restrict__ out, Geom g, int zc, int ntx,
-                                                        int nty, const IsoW<C> w) {
+                                                        int nty, const IsoW<C> w, uint32_t* mm) {
     __shared__ __align__(16) float A[2 * IsoLds<C>::ASZ];
     __shared__ __align__(16) float B[2 * IsoLds<C>::BSZ];
-    blur_iso_tile<C, Z3, LD, iso_u<C>(), 2>(in, out, g, zc, ntx, nty, w, xcd_tile(blockIdx.x, gridDim.x), A, B);
+    blur_iso_tile<C, Z3, LD, iso_u<C>(), 2>(in, out, g, zc, ntx, nty, w, xcd_tile(blockIdx.x, gridDim.x), A, B, mm);
 }
 template <int C, bool Z3, typename LD>
 __global__ __launch_bounds__(ISO_THREADS, 8) void k_blur_iso_o8(LD in, float* __restrict__ out, Geom g, int zc,
-                                                              int ntx, int nty, const IsoW<C> w) {
+                                                              int ntx, int nty, const IsoW<C> w, uint32_t* mm) {
     __shared__ __align__(16) float A[2 * IsoLds<C>::ASZ];
     __shared__ __align__(16) float B[2 * IsoLds<C>::BSZ];
-    blur_iso_tile<C, Z3, LD, 2, 1>(in, out, g, zc, ntx, nty, w, xcd_tile(blockIdx.x, gridDim.x), A, B);
+    blur_iso_tile<C, Z3, LD, 2, 1>(in, out, g, zc, ntx, nty, w, xcd_tile(blockIdx.x, gridDim.x), A, B, mm);
 }
 
 // Batched form: one grid over the column tiles of many images that share the
@@ -708,17 +732,17 @@ static int iso_o8_default(int c, bool z3) { return z3 ? (c >= 4 && c <= 7) : (c 
 
 template <int C, bool Z3, typename LD>
 static void launch_iso_nzc(LD in, float* out, const Geom& g, const IsoW<C>& w, int ntx, int nty, int64_t nzc,
-                           int o8, hipStream_t s) {
+                           int o8, uint32_t* mm, hipStream_t s) {
     const int zc = (int)((g.D + nzc - 1) / nzc);
     nzc = (g.D + zc - 1) / zc;
     const unsigned nb = (unsigned)((int64_t)ntx * nty * nzc);
     if constexpr (iso_o8_ok<C, Z3>()) {
         if (o8) {
-            k_blur_iso_o8<C, Z3, LD><<<nb, ISO_THREADS, 0, s>>>(in, out, g, zc, ntx, nty, w);
+            k_blur_iso_o8<C, Z3, LD><<<nb, ISO_THREADS, 0, s>>>(in, out, g, zc, ntx, nty, w, mm);
             return;
         }
     }
-    k_blur_iso<C, Z3, LD><<<nb, ISO_THREADS, 0, s>>>(in, out, g, zc, ntx, nty, w);
+    k_blur_iso<C, Z3, LD><<<nb, ISO_THREADS, 0, s>>>(in, out, g, zc, ntx, nty, w, mm);
 }
 
 template <typename LD> constexpr int ld_code();
@@ -729,7 +753,7 @@ template <> constexpr int ld_code<LdU8>() { return 2; }
 static thread_local bool t_tuning = false;  // set by cle_autotune_gaussian_blur around its blur
 
 template <int C, bool Z3, typename LD>
-static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hipStream_t s) {
+static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, uint32_t* mm, hipStream_t s) {
     const int ntx = (int)((g.W + ISO_TX - 1) / ISO_TX), nty = (int)((g.H + ISO_TY - 1) / ISO_TY);
     const int64_t nxy = (int64_t)ntx * nty;
     constexpr int CZ = Z3 ? C : 0;
@@ -771,10 +795,10 @@ static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hip
                 if (hipEventCreate(&e1) == hipSuccess) {
                     float best = 1e30f;
                     for (const TuneChoice& t : cand) {
-                        launch_iso_nzc<C, Z3>(in, out, g, w, ntx, nty, t.nzc, t.o8, s);  // warm
-                        for (int rep = 0; rep < 3; ++rep) {                             // best of three
+                        launch_iso_nzc<C, Z3>(in, out, g, w, ntx, nty, t.nzc, t.o8, mm, s);  // warm
+                        for (int rep = 0; rep < 3; ++rep) {                                 // best of three
                             (void)hipEventRecord(e0, s);
-                            launch_iso_nzc<C, Z3>(in, out, g, w, ntx, nty, t.nzc, t.o8, s);
+                            launch_iso_nzc<C, Z3>(in, out, g, w, ntx, nty, t.nzc, t.o8, mm, s);
                             (void)hipEventRecord(e1, s);
                             float ms = 1e30f;
                             if (hipEventSynchronize(e1) == hipSuccess &&
@@ -793,7 +817,7 @@ static void launch_iso_c(LD in, float* out, const Geom& g, const float* wts, hip
             g_tune[key] = {nzc, o8};
         }
     }
-    launch_iso_nzc<C, Z3>(in, out, g, w, ntx, nty, nzc, o8, s);
+    launch_iso_nzc<C, Z3>(in, out, g, w, ntx, nty, nzc, o8, mm, s);
 }
 
 constexpr int ISO_MAXC = 20;
@@ -802,9 +826,9 @@ constexpr int ISO_MAXC = 20;
 static bool iso_slice_ok(const Geom& g) { return g.H * g.W < ((int64_t)1 << 30); }
 
 template <bool Z3, typename LD>
-static int launch_iso(LD in, float* out, const Geom& g, int c, const float* wts, hipStream_t s) {
+static int launch_iso(LD in, float* out, const Geom& g, int c, const float* wts, uint32_t* mm, hipStream_t s) {
     switch (c) {
-#define ISO_CASE(n) case n: launch_iso_c<n, Z3>(in, out, g, wts, s); break;
+#define ISO_CASE(n) case n: launch_iso_c<n, Z3>(in, out, g, wts, mm, s); break;
         ISO_CASE(1) ISO_CASE(2) ISO_CASE(3) ISO_CASE(4) ISO_CASE(5) ISO_CASE(6) ISO_CASE(7) ISO_CASE(8)
         ISO_CASE(9) ISO_CASE(10) ISO_CASE(11) ISO_CASE(12) ISO_CASE(13) ISO_CASE(14) ISO_CASE(15) ISO_CASE(16)
         ISO_CASE(17) ISO_CASE(18) ISO_CASE(19) ISO_CASE(20)
@@ -817,7 +841,8 @@ static int launch_iso(LD in, float* out, const Geom& g, int c, const float* wts,
 
 // Isotropic fused path: destination float32, source f32/u16/u8, equal sigma on every
 // blurred axis (x, y, and z for 3D), half-width 1..ISO_MAXC.  Sets *ran.
-static int try_iso(const cle_buf* src, const cle_buf* dst, const Taps* taps[3], int* ran, hipStream_t s) {
+static int try_iso(const cle_buf* src, const cle_buf* dst, const Taps* taps[3], int* ran, uint32_t* mm,
+                   hipStream_t s) {
     *ran = 0;
     if (dst->dtype != CLE_F32 || dst->data == src->data) return CLE_OK;
     if (src->dtype != CLE_F32 && src->dtype != CLE_U16 && src->dtype != CLE_U8) return CLE_OK;
@@ -834,13 +859,13 @@ static int try_iso(const cle_buf* src, const cle_buf* dst, const Taps* taps[3], 
     int rc;
     if (src->dtype == CLE_F32) {
         LdF32 in{(const float*)src->data};
-        rc = z3 ? launch_iso<true>(in, out, g, c, wts, s) : launch_iso<false>(in, out, g, c, wts, s);
+        rc = z3 ? launch_iso<true>(in, out, g, c, wts, mm, s) : launch_iso<false>(in, out, g, c, wts, mm, s);
     } else if (src->dtype == CLE_U16) {
         LdU16 in{(const uint16_t*)src->data};
-        rc = z3 ? launch_iso<true>(in, out, g, c, wts, s) : launch_iso<false>(in, out, g, c, wts, s);
+        rc = z3 ? launch_iso<true>(in, out, g, c, wts, mm, s) : launch_iso<false>(in, out, g, c, wts, mm, s);
     } else {
         LdU8 in{(const uint8_t*)src->data};
-        rc = z3 ? launch_iso<true>(in, out, g, c, wts, s) : launch_iso<false>(in, out, g, c, wts, s);
+        rc = z3 ? launch_iso<true>(in, out, g, c, wts, mm, s) : launch_iso<false>(in, out, g, c, wts, mm, s);
     }
     if (rc == CLE_OK) *ran = 1;
     return rc;
@@ -905,8 +930,12 @@ __global__ void k_convert(DBuf s, DBuf d, int64_t n) {
     for (; i < n; i += stride) st(d, i, ld(s, i));
 }
 
-// Runs the separable passes given by taps[axis] (c < 0 = axis skipped).
-static int separable_impl(const cle_buf* src, const cle_buf* dst, const Taps* taps[3], hipStream_t s) {
+// Runs the separable passes given by taps[axis] (c < 0 = axis skipped).  `mm` (may be
+// null): minimum / maximum words the isotropic kernel folds its output into; *folded
+// tells whether it ran.
+static int separable_impl(const cle_buf* src, const cle_buf* dst, const Taps* taps[3], hipStream_t s,
+                          uint32_t* mm = nullptr, int* folded = nullptr) {
+    if (folded) *folded = 0;
     Geom gs, gd;
     CLE_TRY(check_buf(src, "source", &gs));
     CLE_TRY(check_buf(dst, "destination", &gd));
@@ -918,8 +947,11 @@ static int separable_impl(const cle_buf* src, const cle_buf* dst, const Taps* ta
     DBuf in = dbuf(src), out = dbuf(dst);
     if (npass > 0 && gs.ndim >= 2) {
         int ran = 0;
-        CLE_TRY(try_iso(src, dst, taps, &ran, s));
-        if (ran) return CLE_OK;
+        CLE_TRY(try_iso(src, dst, taps, &ran, mm, s));
+        if (ran) {
+            if (folded) *folded = mm != nullptr;
+            return CLE_OK;
+        }
         CLE_TRY(try_fused(src, dst, taps, &ran, s));
         if (ran) return CLE_OK;
     }
@@ -1185,6 +1217,12 @@ int plan_blur_batch(const std::vector<BlurReq>& reqs, Blob& blob, std::vector<La
 }
 
 int blur_impl(const cle_buf* src, const cle_buf* dst, const float sig[3], hipStream_t s) {
+    return blur_impl_minmax(src, dst, sig, nullptr, nullptr, s);
+}
+
+int blur_impl_minmax(const cle_buf* src, const cle_buf* dst, const float sig[3], uint32_t* mm_words, int* folded,
+                     hipStream_t s) {
+    if (folded) *folded = 0;
     Taps tx, ty, tz;
     const Taps* taps[3] = {nullptr, nullptr, nullptr};
     Taps* store[3] = {&tx, &ty, &tz};
@@ -1197,7 +1235,7 @@ int blur_impl(const cle_buf* src, const cle_buf* dst, const float sig[3], hipStr
             taps[a] = store[a];
         }
     }
-    return separable_impl(src, dst, taps, s);
+    return separable_impl(src, dst, taps, s, mm_words, folded);
 }
 
 void blur_tuning(bool on) { t_tuning = on; }

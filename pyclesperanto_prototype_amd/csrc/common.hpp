@@ -311,6 +311,16 @@ __device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t
     }
 }
 
+// order-preserving map float -> uint32 (for atomicMin / atomicMax of float minima / maxima)
+__device__ __forceinline__ uint32_t f2o(float f) {
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(uint32_t o) {
+    uint32_t b = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+    return __uint_as_float(b);
+}
+
 // ----------------------------------------------------------------- batches
 // One image of a batch.  Work images of all the batch's images are packed in shared
 // buffers, image j at voxel offset `off` (a multiple of 64).  Stages that walk voxels
@@ -371,6 +381,16 @@ int minmax_impl(DBuf src, int64_t n, float* out2, hipStream_t s);
 // Otsu threshold of `src`, fully on the device: writes minmax[2] and threshold
 // (float64) to device memory.
 int otsu_impl(DBuf src, const Geom& g, float* minmax_dev, double* thr_dev, hipStream_t s);
+// the same with minmax_dev already holding src's minimum and maximum
+int otsu_after_minmax(DBuf src, const Geom& g, float* minmax_dev, double* thr_dev, hipStream_t s);
+// minimum / maximum words (order-preserving uint32, see f2o): reset, and to float[2]
+void minmax_words_init(uint32_t* words, hipStream_t s);
+void minmax_words_finish(const uint32_t* words, float* out2, hipStream_t s);
+// blur_impl that also folds the output's minimum and maximum into `mm_words` (reset
+// by the caller) when the blur runs as the isotropic one-launch kernel; *folded = 1
+// then, 0 if the caller must take them from the output
+int blur_impl_minmax(const cle_buf* src, const cle_buf* dst, const float sig[3], uint32_t* mm_words, int* folded,
+                     hipStream_t s);
 int ccl_impl(DBuf binary, const Geom& g, DBuf dst, int connectivity, int options,
              uint32_t* n_labels_dev, hipStream_t s);
 // batched stages (defined next to their single-image kernels)

@@ -19,16 +19,6 @@
 
 namespace cle {
 
-// order-preserving map float -> uint32 (for atomicMin / atomicMax)
-__device__ __forceinline__ uint32_t f2o(float f) {
-    uint32_t b = __float_as_uint(f);
-    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-}
-__device__ __forceinline__ float o2f(uint32_t o) {
-    uint32_t b = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
-    return __uint_as_float(b);
-}
-
 constexpr int RB = 256;
 
 template <typename LD>
@@ -512,6 +502,15 @@ size_t otsu_scratch_bytes(const Geom& g) {
 
 int otsu_impl(DBuf src, const Geom& g, float* minmax_dev, double* thr_dev, hipStream_t s) {
     CLE_TRY(minmax_impl(src, g.n(), minmax_dev, s));
+    return otsu_after_minmax(src, g, minmax_dev, thr_dev, s);
+}
+
+void minmax_words_init(uint32_t* words, hipStream_t s) { k_minmax_init<<<1, 1, 0, s>>>(words); }
+void minmax_words_finish(const uint32_t* words, float* out2, hipStream_t s) {
+    k_minmax_fin<<<1, 1, 0, s>>>(words, out2);
+}
+
+int otsu_after_minmax(DBuf src, const Geom& g, float* minmax_dev, double* thr_dev, hipStream_t s) {
     Scratch hist;
     CLE_TRY(hist.alloc(256 * sizeof(float), s));
     CLE_TRY(histogram_impl(src, g, 256, 0.f, 0.f, minmax_dev, DBuf{hist.p, CLE_F32}, s));

@@ -1304,9 +1304,18 @@ int voronoi_otsu_single(const cle_buf* src, const cle_buf* dst, float spot_sigma
     CLE_TRY(blur_impl(src, &bb, s1, s));
     CLE_TRY(maxima_impl(DBuf{blurred.p, CLE_F32}, DBuf{spots.p, CLE_U8}, g, s));
     // segmentation = threshold_otsu(gaussian_blur(src, outline_sigma)); binary = spots AND segmentation
+    // (the isotropic blur folds its output's minimum and maximum into mmw on the way)
     const float s2[3] = {outline_sigma, outline_sigma, outline_sigma};
-    CLE_TRY(blur_impl(src, &bb, s2, s));
-    CLE_TRY(otsu_impl(DBuf{blurred.p, CLE_F32}, g, mm, thr, s));
+    uint32_t* mmw = (uint32_t*)(aux.get<char>() + 32);
+    minmax_words_init(mmw, s);
+    int folded = 0;
+    CLE_TRY(blur_impl_minmax(src, &bb, s2, mmw, &folded, s));
+    if (folded) {
+        minmax_words_finish(mmw, mm, s);
+        CLE_TRY(otsu_after_minmax(DBuf{blurred.p, CLE_F32}, g, mm, thr, s));
+    } else {
+        CLE_TRY(otsu_impl(DBuf{blurred.p, CLE_F32}, g, mm, thr, s));
+    }
     k_seg_and<<<vg(n / 4 + 1), 256, 0, s>>>(blurred.get<float>(), thr, spots.get<uint8_t>(), seg.get<uint8_t>(), n);
     CLE_LAUNCHED();
     // masked Voronoi of the seeds inside the segmentation; the seed labels go to the
@@ -1911,8 +1920,16 @@ int cle_gauss_otsu_labeling(const cle_buf* src, const cle_buf* dst, float outlin
     bb.data = blurred.p;
     bb.dtype = CLE_F32;
     const float sg[3] = {outline_sigma, outline_sigma, outline_sigma};
-    CLE_TRY(blur_impl(src, &bb, sg, s));
-    CLE_TRY(otsu_impl(DBuf{blurred.p, CLE_F32}, g, mm, thr, s));
+    uint32_t* mmw = (uint32_t*)(aux.get<char>() + 32);
+    minmax_words_init(mmw, s);
+    int folded = 0;
+    CLE_TRY(blur_impl_minmax(src, &bb, sg, mmw, &folded, s));
+    if (folded) {
+        minmax_words_finish(mmw, mm, s);
+        CLE_TRY(otsu_after_minmax(DBuf{blurred.p, CLE_F32}, g, mm, thr, s));
+    } else {
+        CLE_TRY(otsu_impl(DBuf{blurred.p, CLE_F32}, g, mm, thr, s));
+    }
     k_threshold_u8<<<vg(n), 256, 0, s>>>(blurred.get<float>(), thr, seg.get<uint8_t>(), n);
     CLE_LAUNCHED();
     return ccl_impl(DBuf{seg.p, CLE_U8}, g, dbuf(dst), CLE_CONN_BOX, CLE_FP32_KEY_COMPAT, nullptr, s);
