@@ -1127,6 +1127,25 @@ __global__ void k_mask_out_u8_32(const float* __restrict__ A, const uint8_t* __r
         st(dst, i, mask[i] ? A[i] : 0.f);
 }
 
+// The masked result when every seed lies inside the mask (voronoi_otsu_labeling: the
+// seeds are spots AND segmentation): voxels outside the mask hold exactly -1 after the
+// growth and mask voxels hold a label >= 0 or 0, so mask(A, mask) == max(A, 0) and the
+// mask need not be read.  Four voxels per thread as k_mask_out_u8_32.
+__global__ void k_clip_out_32(const float* __restrict__ A, DBuf dst, int64_t n) {
+    const int64_t nv = n / 4;
+    const bool u = dst.dt == CLE_U32;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nv; q += (int64_t)gridDim.x * blockDim.x) {
+        const float4 a = ((const float4*)A)[q];
+        const float av[4] = {fmaxf(a.x, 0.f), fmaxf(a.y, 0.f), fmaxf(a.z, 0.f), fmaxf(a.w, 0.f)};
+        uint32_t ov[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ov[c] = u ? sat_cast<uint32_t>(av[c]) : (uint32_t)sat_cast<int32_t>(av[c]);
+        ((uint4*)dst.p)[q] = make_uint4(ov[0], ov[1], ov[2], ov[3]);
+    }
+    for (int64_t i = nv * 4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        st(dst, i, fmaxf(A[i], 0.f));
+}
+
 // seg = blurred > float32(thr); spots <- spots AND seg (in place); four voxels per
 // thread (the library's scratch buffers are 256-byte aligned), the tail one by one
 __global__ void k_seg_and(const float* blurred, const double* thr, uint8_t* spots, uint8_t* seg, int64_t n) {
@@ -1230,7 +1249,7 @@ static int grow_masked_fused(DBuf binary, const uint8_t* mask, float* A, float* 
 // CCL), grow inside the mask, mask the result into dst.  B: a float32 work buffer of
 // the image's size.
 static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, float* B, int* n_iterations,
-                               hipStream_t s) {
+                               hipStream_t s, bool seeds_in_mask = false) {
     const int64_t n = g.n();
     bool labelled = false;  // B holds the CCL labels
     // (the tile solves' stores are not part of the fixed-region bound: pairs there)
@@ -1257,7 +1276,10 @@ static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, 
         else
             CLE_TRY(grow_single(MaskedAny{mask, B}, A, B, g, n_iterations, &res, s));
     }
-    if (mask.dt == CLE_U8 && (dst.dt == CLE_U32 || dst.dt == CLE_I32) && ((uintptr_t)res & 15) == 0 &&
+    if (seeds_in_mask && (dst.dt == CLE_U32 || dst.dt == CLE_I32) && ((uintptr_t)res & 15) == 0 &&
+        ((uintptr_t)dst.p & 15) == 0)
+        k_clip_out_32<<<vg(n / 4 + 1), 256, 0, s>>>(res, dst, n);
+    else if (mask.dt == CLE_U8 && (dst.dt == CLE_U32 || dst.dt == CLE_I32) && ((uintptr_t)res & 15) == 0 &&
         ((uintptr_t)mask.p & 3) == 0 && ((uintptr_t)dst.p & 15) == 0)
         k_mask_out_u8_32<<<vg(n / 4 + 1), 256, 0, s>>>(res, (const uint8_t*)mask.p, dst, n);
     else
@@ -1321,7 +1343,7 @@ int voronoi_otsu_single(const cle_buf* src, const cle_buf* dst, float spot_sigma
     // masked Voronoi of the seeds inside the segmentation; the seed labels go to the
     // blurred buffer
     return masked_voronoi_core(DBuf{spots.p, CLE_U8}, DBuf{seg.p, CLE_U8}, dbuf(dst), g, blurred.get<float>(), nullptr,
-                               s);
+                               s, true);
 }
 
 // ----------------------------------------------------------------- batched pipeline
@@ -1402,7 +1424,7 @@ __global__ __launch_bounds__(256) void k_mask_out_batch(const BImg* __restrict__
     const int64_t n = im.g.n(), i0 = (b - vs[j]) * BV, i1 = i0 + BV < n ? i0 + BV : n;
     const DBuf d{im.dst, im.dst_dt};
     const float* A = (vor_result_buffer(iters[j]) ? P1 : P0) + im.off;
-    const uint8_t* g = G + im.off;
+    (void)G;  // the seeds lie inside the segmentation: mask(A, G) == max(A, 0) (see k_clip_out_32)
     if ((im.dst_dt == CLE_U32 || im.dst_dt == CLE_I32) && ((uintptr_t)im.dst & 15) == 0) {
         // 32-bit labels into a 16-byte aligned destination: four voxels per thread (the
         // batch buffers are aligned as in k_seg_and_batch), the image's last partial
@@ -1411,8 +1433,7 @@ __global__ __launch_bounds__(256) void k_mask_out_batch(const BImg* __restrict__
         const bool u = im.dst_dt == CLE_U32;
         for (int64_t i = i0 + 4 * threadIdx.x; i < i1; i += 4 * 256) {
             const float4 a = *(const float4*)(A + i);
-            const uchar4 m = *(const uchar4*)(g + i);
-            const float av[4] = {m.x ? a.x : 0.f, m.y ? a.y : 0.f, m.z ? a.z : 0.f, m.w ? a.w : 0.f};
+            const float av[4] = {fmaxf(a.x, 0.f), fmaxf(a.y, 0.f), fmaxf(a.z, 0.f), fmaxf(a.w, 0.f)};
             if (i + 4 <= n) {
                 uint32_t ov[4];
 #pragma unroll
@@ -1426,7 +1447,7 @@ __global__ __launch_bounds__(256) void k_mask_out_batch(const BImg* __restrict__
         }
         return;
     }
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) st(d, i, g[i] ? A[i] : 0.f);
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) st(d, i, fmaxf(A[i], 0.f));
 }
 
 // Pinned host staging of the batch tables.  Buffers are taken from a process-wide
