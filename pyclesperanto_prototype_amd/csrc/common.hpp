@@ -418,19 +418,32 @@ struct CclBatchSizes {
     int64_t bits = 0;     // key bits of all images
 };
 void ccl_batch_layout(std::vector<BImg>& imgs, CclBatchSizes* sz);
+// Batched form of the growth init: the label write of the batched CCL stores
+// v = (G - 1) + label (float32) into F and A at each image's offset, stamps
+// seed[tbase + tile] = -1 for every growth tile (16 x 8 x 8 voxels in 3D, 32 x 32 in
+// 2D) holding a positive v and clears settled[tbase + tile] (set beforehand) for every
+// tile holding a mask voxel.
+struct GrowInitBatch {
+    float* A;
+    const uint8_t* G;
+    int32_t* seed;
+    uint8_t* settled;
+};
 int plan_ccl_batch(const std::vector<BImg>& imgs, size_t imgs_off, const CclBatchSizes& sz, const uint8_t* S,
                    float* F, Blob& blob, std::vector<Launch>& out, std::vector<std::shared_ptr<void>>& keep,
-                   hipStream_t s);
+                   hipStream_t s, const GrowInitBatch* grow = nullptr);
 // Box CCL of `binary` whose write pass initialises the masked Voronoi growth instead of
 // storing labels: v = (mask - 1) + label (float32, the reference's add order) into A
-// and B, and seed[tile] = -1 for every growth tile (tx x ty x tz voxels, nx x ny tiles
-// per slice / row) holding a positive v.  The mask is uint8; A, B 16-byte and the mask
-// 4-byte aligned.
+// and B, seed[tile] = -1 for every growth tile (tx x ty x tz voxels, nx x ny tiles per
+// slice / row) holding a positive v, and settled[tile] = 0 (set beforehand) for every
+// tile holding a mask voxel.  The mask is uint8; A, B 16-byte and the mask 4-byte
+// aligned.
 struct GrowInitOut {
     float* A;
     float* B;
     const uint8_t* mask;
     int32_t* seed;
+    uint8_t* settled;
     int tx, ty, tz;
     int64_t nx, ny;
 };

@@ -496,14 +496,16 @@ struct StLabAny {
 struct StGrowInit {
     static constexpr bool kCoords = true;
     GrowInitOut o;
-    __device__ __forceinline__ void seed(int64_t x, int64_t y, int64_t z) const {
-        o.seed[((z / o.tz) * o.ny + y / o.ty) * o.nx + x / o.tx] = -1;
+    __device__ __forceinline__ int64_t tile(int64_t x, int64_t y, int64_t z) const {
+        return ((z / o.tz) * o.ny + y / o.ty) * o.nx + x / o.tx;
     }
     __device__ __forceinline__ void put(int64_t i, int64_t x, int64_t y, int64_t z, float v) const {
-        const float w = ((float)o.mask[i] - 1.f) + v;
+        const uint8_t m = o.mask[i];
+        const float w = ((float)m - 1.f) + v;
         o.A[i] = w;
         o.B[i] = w;
-        if (w > 0.f) seed(x, y, z);
+        if (w > 0.f) o.seed[tile(x, y, z)] = -1;
+        if (m) o.settled[tile(x, y, z)] = 0;  // a tile with a mask voxel is not settled
     }
     // voxels x .. x + 3 (x a multiple of 4: one growth tile, whose x extent is a
     // multiple of 4)
@@ -514,7 +516,8 @@ struct StGrowInit {
                                      ((float)m.w - 1.f) + d);
         *(float4*)(o.A + i) = w;
         *(float4*)(o.B + i) = w;
-        if (w.x > 0.f || w.y > 0.f || w.z > 0.f || w.w > 0.f) seed(x, y, z);
+        if (w.x > 0.f || w.y > 0.f || w.z > 0.f || w.w > 0.f) o.seed[tile(x, y, z)] = -1;
+        if ((m.x | m.y | m.z | m.w) != 0) o.settled[tile(x, y, z)] = 0;
     }
 };
 
@@ -1112,22 +1115,49 @@ __global__ __launch_bounds__(CT_THREADS) void k_ccl_final_batch(const BImg* __re
 }
 
 // float32 labels of the work image from the block masks, as k_bccl_write: one
-// block per row of 2x2x2 blocks of one image
-template <int BZ>
-__global__ void k_bccl_write_batch(const BImg* __restrict__ imgs, const int64_t* __restrict__ brstart, int nimg,
-                                   const uint8_t* __restrict__ M, const uint32_t* __restrict__ L,
-                                   const uint32_t* __restrict__ F, float* out) {
+// block per row of 2x2x2 blocks of one image.  GROW: the growth's initial state
+// (G - 1) + label into out and gi.A instead, with the seed tiles stamped (see
+// GrowInitBatch).
+// Rows of 2x2x2 blocks a 256-thread block of the batched write takes: several short
+// rows per block (small images), one long row otherwise.
+__host__ __device__ inline int bw_rows_per_block(int64_t np) {
+    const int64_t r = 256 / (np > 0 ? np : 1);
+    return (int)(r < 1 ? 1 : (r > 64 ? 64 : r));
+}
+
+template <int BZ, bool GROW>
+__global__ __launch_bounds__(256) void k_bccl_write_batch(const BImg* __restrict__ imgs,
+                                                         const int64_t* __restrict__ brstart, int nimg,
+                                                         const uint8_t* __restrict__ M, const uint32_t* __restrict__ L,
+                                                         const uint32_t* __restrict__ F, float* out, GrowInitBatch gi) {
     const int64_t r = blockIdx.x;
     const int j = find_seg(brstart, nimg, r);
     const BImg& im = imgs[j];
     const Geom g = im.g;
     const BGrid B{(g.D + BZ - 1) / BZ, (g.H + 1) / 2, (g.W + 1) / 2};
-    const int64_t br = r - brstart[j];
+    const int64_t np = (B.W + 1) >> 1;  // 4-voxel groups per row
+    const int rpb = bw_rows_per_block(np);
+    // this thread's row of blocks and first group
+    int64_t br, p0, pstep;
+    if (rpb > 1) {
+        if (threadIdx.x >= rpb * np) return;
+        br = (r - brstart[j]) * rpb + threadIdx.x / np;
+        p0 = threadIdx.x % np;
+        pstep = np;
+    } else {
+        br = r - brstart[j];
+        p0 = threadIdx.x;
+        pstep = blockDim.x;
+    }
+    if (br >= B.D * B.H) return;
     const int64_t bz = br / B.H, by = br - bz * B.H;
     const int64_t z0 = bz * BZ, y0 = by * 2;
     const int nz = (BZ == 2 && z0 + 1 < g.D) ? 2 : 1, ny = y0 + 1 < g.H ? 2 : 1;
-    const int64_t np = (B.W + 1) >> 1;
-    for (int64_t p = threadIdx.x; p < np; p += blockDim.x) {
+    // growth tiles: 16 x 8 x 8 (3D), 32 x 32 (2D); a thread's four voxels lie in one
+    const bool d3 = is3d(g);
+    const int gtx = d3 ? 16 : 32, gty = d3 ? 8 : 32, gtz = d3 ? 8 : 1;
+    const int64_t gnx = (g.W + gtx - 1) / gtx, gny = (g.H + gty - 1) / gty;
+    for (int64_t p = p0; p < np; p += pstep) {
         const int64_t b = im.boff + br * B.W + 2 * p, x = 4 * p;
         const uint32_t m2 = M[b] | (2 * p + 1 < B.W ? (uint32_t)M[b + 1] << 8 : 0u);
         const float f0 = (m2 & 0xFFu) ? (float)F[L[b]] : 0.f;
@@ -1135,11 +1165,43 @@ __global__ void k_bccl_write_batch(const BImg* __restrict__ imgs, const int64_t*
         for (int dz = 0; dz < nz; ++dz)
             for (int dy = 0; dy < ny; ++dy) {
                 const int sh = dz * 4 + dy * 2;
-                float* o = out + im.off + ((z0 + dz) * g.H + y0 + dy) * g.W + x;
-                o[0] = (m2 >> sh) & 1u ? f0 : 0.f;
-                if (x + 1 < g.W) o[1] = (m2 >> (sh + 1)) & 1u ? f0 : 0.f;
-                if (x + 2 < g.W) o[2] = (m2 >> (sh + 8)) & 1u ? f1 : 0.f;
-                if (x + 3 < g.W) o[3] = (m2 >> (sh + 9)) & 1u ? f1 : 0.f;
+                const int64_t i = im.off + ((z0 + dz) * g.H + y0 + dy) * g.W + x;
+                float v[4] = {(m2 >> sh) & 1u ? f0 : 0.f, (m2 >> (sh + 1)) & 1u ? f0 : 0.f,
+                              (m2 >> (sh + 8)) & 1u ? f1 : 0.f, (m2 >> (sh + 9)) & 1u ? f1 : 0.f};
+                const int nv = x + 4 <= g.W ? 4 : (int)(g.W - x);
+                // rows of a multiple of 4 voxels: 4-voxel groups are 16-byte aligned (image
+                // offsets are multiples of 64 voxels)
+                const bool vec = (g.W & 3) == 0;
+                if (GROW) {
+                    bool pos = false, msk = false;
+                    if (vec) {
+                        const uchar4 m = *(const uchar4*)(gi.G + i);
+                        // the reference's float32 add order: (mask - 1) + label
+                        v[0] = ((float)m.x - 1.f) + v[0];
+                        v[1] = ((float)m.y - 1.f) + v[1];
+                        v[2] = ((float)m.z - 1.f) + v[2];
+                        v[3] = ((float)m.w - 1.f) + v[3];
+                        *(float4*)(gi.A + i) = make_float4(v[0], v[1], v[2], v[3]);
+                        pos = v[0] > 0.f || v[1] > 0.f || v[2] > 0.f || v[3] > 0.f;
+                        msk = (m.x | m.y | m.z | m.w) != 0;
+                    } else {
+                        for (int c = 0; c < nv; ++c) {
+                            const uint8_t m = gi.G[i + c];
+                            v[c] = ((float)m - 1.f) + v[c];
+                            gi.A[i + c] = v[c];
+                            pos |= v[c] > 0.f;
+                            msk |= m != 0;
+                        }
+                    }
+                    const int64_t t = im.tbase + (((z0 + dz) / gtz) * gny + (y0 + dy) / gty) * gnx + x / gtx;
+                    if (pos) gi.seed[t] = -1;
+                    if (msk) gi.settled[t] = 0;  // a tile with a mask voxel is not settled
+                }
+                if (vec) {
+                    *(float4*)(out + i) = make_float4(v[0], v[1], v[2], v[3]);
+                } else {
+                    for (int c = 0; c < nv; ++c) out[i + c] = v[c];
+                }
             }
     }
 }
@@ -1148,7 +1210,9 @@ template <typename BT>
 static int plan_ccl_kind(const std::vector<BImg>& imgs, size_t imgs_off, bool want3d, const uint8_t* S, float* F,
                          uint8_t* M, uint32_t* L, uint32_t* K, uint32_t* Fl, RankTable* rt, Blob& blob,
                          std::vector<Launch>& pre, std::vector<Launch>& post, std::vector<std::shared_ptr<void>>& keep,
-                         hipStream_t s) {
+                         hipStream_t s, const GrowInitBatch* grow) {
+    const GrowInitBatch gi = grow ? *grow : GrowInitBatch{nullptr, nullptr, nullptr, nullptr};
+    const bool gw = grow != nullptr;
     const int nimg = (int)imgs.size();
     std::vector<int64_t> tst(nimg + 1), rst(nimg + 1);
     int64_t nt = 0, nr = 0;
@@ -1158,7 +1222,8 @@ static int plan_ccl_kind(const std::vector<BImg>& imgs, size_t imgs_off, bool wa
         if (is3d(imgs[j].g) != want3d || imgs[j].g.n() == 0) continue;
         const BGrid B = bgrid_of<BT>(imgs[j].g);
         nt += tgrid_of<BT>(B).count();
-        nr += B.D * B.H;  // rows of blocks
+        const int rpb = bw_rows_per_block((B.W + 1) >> 1);
+        nr += (B.D * B.H + rpb - 1) / rpb;  // write blocks: rpb rows of 2x2x2 blocks each
     }
     tst[nimg] = nt;
     rst[nimg] = nr;
@@ -1192,7 +1257,10 @@ static int plan_ccl_kind(const std::vector<BImg>& imgs, size_t imgs_off, bool wa
         k_ccl_final_batch<BT><<<(unsigned)nt, CT_THREADS, 0, s>>>(im, ts, nimg, L, K, roots, nroots,
                                                                  rt->bm.get<uint32_t>(), rt->prefix.get<uint32_t>(),
                                                                  Fl);
-        k_bccl_write_batch<BT::BZ><<<(unsigned)nr, 256, 0, s>>>(im, rs, nimg, M, L, Fl, F);
+        if (gw)
+            k_bccl_write_batch<BT::BZ, true><<<(unsigned)nr, 256, 0, s>>>(im, rs, nimg, M, L, Fl, F, gi);
+        else
+            k_bccl_write_batch<BT::BZ, false><<<(unsigned)nr, 256, 0, s>>>(im, rs, nimg, M, L, Fl, F, gi);
         CLE_LAUNCHED();
         return CLE_OK;
     });
@@ -1200,7 +1268,8 @@ static int plan_ccl_kind(const std::vector<BImg>& imgs, size_t imgs_off, bool wa
 }
 
 int plan_ccl_batch(const std::vector<BImg>& imgs, size_t imgs_off, const CclBatchSizes& sz, const uint8_t* S,
-                   float* F, Blob& blob, std::vector<Launch>& out, std::vector<std::shared_ptr<void>>& keep, hipStream_t s) {
+                   float* F, Blob& blob, std::vector<Launch>& out, std::vector<std::shared_ptr<void>>& keep, hipStream_t s,
+                   const GrowInitBatch* grow) {
     if (sz.nb == 0) return CLE_OK;
     auto Ms = std::make_shared<Scratch>(), Ls = std::make_shared<Scratch>(), Ks = std::make_shared<Scratch>(),
          Fs = std::make_shared<Scratch>();
@@ -1218,9 +1287,9 @@ int plan_ccl_batch(const std::vector<BImg>& imgs, size_t imgs_off, const CclBatc
     std::vector<Launch> pre, post;
     CLE_TRY(rt->init((uint64_t)sz.bits, s));
     CLE_TRY(plan_ccl_kind<BlkTile3D>(imgs, imgs_off, true, S, F, Ms->get<uint8_t>(), Ls->get<uint32_t>(),
-                                     Ks->get<uint32_t>(), Fs->get<uint32_t>(), rt, blob, pre, post, keep, s));
+                                     Ks->get<uint32_t>(), Fs->get<uint32_t>(), rt, blob, pre, post, keep, s, grow));
     CLE_TRY(plan_ccl_kind<BlkTile2D>(imgs, imgs_off, false, S, F, Ms->get<uint8_t>(), Ls->get<uint32_t>(),
-                                     Ks->get<uint32_t>(), Fs->get<uint32_t>(), rt, blob, pre, post, keep, s));
+                                     Ks->get<uint32_t>(), Fs->get<uint32_t>(), rt, blob, pre, post, keep, s, grow));
     for (auto& l : pre) out.push_back(l);
     out.push_back([=](const char*) -> int { return rt->scan(nullptr, s); });
     for (auto& l : post) out.push_back(l);

@@ -950,7 +950,9 @@ struct VorScratch {
         const int64_t cap = ntiles > 0 ? ntiles : 1;
         return (size_t)cap * 8 + 64 + (size_t)VOR_NCTR * 64 + (size_t)nimg * 16 + (size_t)cap + 4;
     }
-    int alloc(int64_t ntiles, int nimg, hipStream_t s) {
+    // settled_init 1: every tile starts settled (a fused growth init clears the tiles
+    // holding a mask voxel)
+    int alloc(int64_t ntiles, int nimg, hipStream_t s, uint32_t settled_init = 0) {
         const int64_t cap = ntiles > 0 ? ntiles : 1;
         CLE_TRY(mem.alloc(bytes(ntiles, nimg), s));
         char* p = mem.get<char>();
@@ -969,7 +971,7 @@ struct VorScratch {
         // from word); nothing settled yet
         CLE_TRY(fill_u32((uint32_t*)L.mark, 0xFFFFFFFEu, (size_t)cap * 2, s));
         CLE_TRY(fill_u32(S.word, 0u, 16 + (size_t)VOR_NCTR * 16 + (size_t)nimg * 4, s));
-        CLE_TRY(fill_u32((uint32_t*)L.settled, 0u, ((size_t)cap + 3) / 4, s));
+        CLE_TRY(fill_u32((uint32_t*)L.settled, settled_init ? 0x01010101u : 0u, ((size_t)cap + 3) / 4, s));
         return CLE_OK;
     }
 };
@@ -1185,38 +1187,6 @@ static int g_growth_method = 1;  // 0: tile solves, 1: pairs of steps
 
 // labels of `binary` grown inside `mask`, written (masked) to dst.  `B` holds
 // scratch for the CCL labels (float32, n voxels).
-// settled[t] = 1 iff growth tile t holds no mask voxel (so no zero of the initial
-// state: outside the mask it is -1); one wavefront per tile, consecutive tiles on one
-// XCD.  Tiles with a mask voxel stay 0 (the fill of VorScratch).
-__global__ __launch_bounds__(256) void k_vor_settle(const uint8_t* __restrict__ mask, Geom g, Tiles T,
-                                                   uint8_t* settled) {
-    const int64_t ntiles = T.count();
-    const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-    if (t >= ntiles) return;
-    const int lane = threadIdx.x & 63;
-    const int64_t tx = t % T.nx, ty = (t / T.nx) % T.ny, tz = t / (T.nx * T.ny);
-    const int64_t x0 = tx * T.tx, y0 = ty * T.ty, z0 = tz * T.tz;
-    bool any = false;
-    if ((g.W & 15) == 0 && ((uintptr_t)mask & 15) == 0 && x0 + T.tx <= g.W) {
-        // 16-byte pieces of the tile's rows: lane -> (row, piece)
-        const int pieces = T.tx / 16, rows = T.ty * T.tz;
-        for (int k = lane; k < rows * pieces; k += 64) {
-            const int r = k / pieces, pc = k - r * pieces;
-            const int64_t y = y0 + r % T.ty, z = z0 + r / T.ty;
-            if (y < g.H && z < g.D) {
-                const uint4 m = *(const uint4*)(mask + (z * g.H + y) * g.W + x0 + 16 * pc);
-                any |= (m.x | m.y | m.z | m.w) != 0u;
-            }
-        }
-    } else {
-        for (int k = lane; k < T.tx * T.ty * T.tz; k += 64) {
-            const int64_t x = x0 + k % T.tx, y = y0 + (k / T.tx) % T.ty, z = z0 + k / (T.tx * T.ty);
-            if (x < g.W && y < g.H && z < g.D) any |= mask[(z * g.H + y) * g.W + x] != 0;
-        }
-    }
-    if (!__ballot(any) && lane == 0) settled[t] = 1;
-}
-
 // Masked growth whose initial state comes straight from the box CCL of the seeds
 // (ccl_box_growth_init writes v = (mask - 1) + label into A and B and stamps the
 // seed tiles): no separate init pass.  Returns the buffer holding the result.
@@ -1229,14 +1199,12 @@ static int grow_masked_fused(DBuf binary, const uint8_t* mask, float* A, float* 
     im.g = g;
     CLE_TRY(vor_set_tiles(im, false));
     VorScratch V;
-    CLE_TRY(V.alloc(ntiles, 1, s));
+    CLE_TRY(V.alloc(ntiles, 1, s, 1));  // the CCL write pass clears the tiles with a mask voxel
     Scratch tab;
     CLE_TRY(tab.alloc(sizeof(BImg), s));
     k_put_img<<<1, 1, 0, s>>>(tab.get<BImg>(), im);
     CLE_LAUNCHED();
-    k_vor_settle<<<(unsigned)((ntiles + 3) / 4), 256, 0, s>>>(mask, g, T, V.L.settled);
-    CLE_LAUNCHED();
-    const GrowInitOut o{A, B, mask, V.L.mark + V.L.cap, T.tx, T.ty, T.tz, T.nx, T.ny};
+    const GrowInitOut o{A, B, mask, V.L.mark + V.L.cap, V.L.settled, T.tx, T.ty, T.tz, T.nx, T.ny};
     CLE_TRY(ccl_box_growth_init(binary, g, o, s));
     int iters = 0;
     CLE_TRY(grow_pairs(tab.get<BImg>(), 1, A, B, V.L, V.S, vor_runs_single(g), 0, &iters, s));
@@ -1570,7 +1538,7 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     double* thr = (double*)(aux.get<char>() + (size_t)n * 8);
     uint32_t* ou = (uint32_t*)(aux.get<char>() + (size_t)n * 16);
     VorScratch V;
-    if (pairs) CLE_TRY(V.alloc(ntiles, n, s));
+    if (pairs) CLE_TRY(V.alloc(ntiles, n, s, 1));  // the CCL write pass clears the tiles with a mask voxel
     const VorLists VL = V.L;
     float* Fp = F.get<float>();
     float* Ap = A.get<float>();
@@ -1635,15 +1603,15 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
         CLE_LAUNCHED();
         return CLE_OK;
     });
-    // seed labels (float32, into F), then A = F = (segmentation - 1) + labels
-    CLE_TRY(plan_ccl_batch(imgs, imgs_off, csz, Sp, Fp, blob, L, keep, s));
-    if (ntiles > 0 && pairs)
-        L.push_back([=](const char* dev) -> int {
-            k_vor_init_batch<<<(unsigned)ntiles, VB, 0, s>>>((const BImg*)(dev + imgs_off),
-                                                             (const int64_t*)(dev + ts_off), n, Gp, Ap, Fp, VL);
-            CLE_LAUNCHED();
-            return CLE_OK;
-        });
+    // seed labels; with the pairs of steps the CCL's write pass stores the growth's
+    // initial state A = F = (segmentation - 1) + label and stamps the seed tiles, and the
+    // tiles without a mask voxel are settled (no separate init pass)
+    if (pairs) {
+        const GrowInitBatch gi{Ap, Gp, VL.mark + VL.cap, VL.settled};
+        CLE_TRY(plan_ccl_batch(imgs, imgs_off, csz, Sp, Fp, blob, L, keep, s, &gi));
+    } else {
+        CLE_TRY(plan_ccl_batch(imgs, imgs_off, csz, Sp, Fp, blob, L, keep, s));
+    }
 
     // the tables go to the device with one copy
     Scratch dblob;
