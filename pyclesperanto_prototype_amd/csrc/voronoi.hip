@@ -642,11 +642,14 @@ __device__ __forceinline__ bool vor_run_begin(const BImg* __restrict__ imgs, con
     const int nx = im.vnx, ny = im.vny, nz = im.vnz;
     const int64_t tb = im.tbase;
     const int rl = im.vrun;  // tiles per run (<= VOR_RUN)
-    const int segs = (nx + rl - 1) / rl;
     const int64_t r = rs ? rg - rs[j] : rg;
-    const int64_t row = r / segs;
-    const int xs = (int)(r - row * segs) * rl;
-    const int ty = (int)(row % ny), tz = (int)(row / ny);
+    // runs in z-fastest order (then y, then x segment): the blocks an XCD runs at one
+    // time take z-adjacent runs, whose staged regions share their z halo in that XCD's L2
+    const int tz = (int)(r % nz);
+    const int64_t rest = r / nz;
+    const int ty = (int)(rest % ny);
+    const int xs = (int)(rest / ny) * rl;
+    const int64_t row = (int64_t)tz * ny + ty;
     const int nrun = nx - xs < rl ? nx - xs : rl;
     const int64_t t0 = tb + row * nx + xs;
     const int32_t* chg_prev = L.mark + (int64_t)((b - 1) & 1) * L.cap + tb;
