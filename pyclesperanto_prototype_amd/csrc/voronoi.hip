@@ -594,15 +594,35 @@ __device__ __forceinline__ int vor_pair_compute(float* __restrict__ dst, const P
 // first closes pair b - 1: images whose pair b - 1 was not "both changed" get their
 // iteration count; it also empties the change bits pair b + 1 will fill (both
 // idempotent: the 2D and the 3D launch of a pair both do it).
-constexpr int VOR_RUN = 16;
+#ifndef CLE_VOR_RUN
+#define CLE_VOR_RUN 64
+#endif
+constexpr int VOR_RUN = CLE_VOR_RUN;  // longest run (<= 64)
+#ifndef CLE_VOR_RUN_BATCH
+#define CLE_VOR_RUN_BATCH 32
+#endif
 
-// Tiles per run: VOR_RUN in a batch (images enough to fill the chip) and for a single
-// image from 4096 tiles on; one for a smaller single image, where a few blocks working
-// through runs one tile after another would leave the chip idle (its growth is
-// latency-bound).  Measured: 2000 mixed images 387 ms with VOR_RUN for all, 438 ms
-// with one tile per run below 4096 tiles.
+// Tiles per run: VOR_RUN for a single image from 4096 tiles on, CLE_VOR_RUN_BATCH in a
+// batch (images enough to fill the chip); one for a smaller single image, where a few
+// blocks working through runs one tile after another would leave the chip idle (its
+// growth is latency-bound).  A block's start (its run's stamps, the first region) is
+// not overlapped with work, so longer runs pay it less often.  Measured, growth per
+// call: 512 x 2048 x 2048 stack 47 / 27.7 / 19.6 / 19.1 ms with runs of 8 / 16 / 32 / 64,
+// 256 x 1024 x 1024 stack 3.7 / 2.4 / 2.25 ms with 16 / 32 / 64; mixed 2000-image batch
+// 180 / 175 / 177 ms with 16 / 32 / 64 (and 438 ms instead of 387 ms with one tile per
+// run below 4096 tiles).
+#ifndef CLE_VOR_MIN_RUNS
+#define CLE_VOR_MIN_RUNS 1024
+#endif
 inline int vor_run_length(const Geom& g, bool batch) {
-    return batch || make_tiles(g).count() >= 4096 ? VOR_RUN : 1;
+    if (batch) return CLE_VOR_RUN_BATCH;
+    const int64_t n = make_tiles(g).count();
+    if (n < 4096) return 1;
+    // the longest run (a power of two up to VOR_RUN) that leaves CLE_VOR_MIN_RUNS runs
+    // to fill the chip
+    int rl = VOR_RUN;
+    while (rl > 1 && n / rl < CLE_VOR_MIN_RUNS) rl >>= 1;
+    return rl;
 }
 // runs of an image of run length rl: pieces of rl tiles of every tile row
 inline int64_t vor_run_blocks(const Geom& g, int rl) {
@@ -657,10 +677,13 @@ __device__ __forceinline__ bool vor_run_begin(const BImg* __restrict__ imgs, con
     uint64_t act;
     {
         constexpr int RW = VOR_RUN + 2;  // stamps per neighbour row: x from xs - 1
+        constexpr int NWB = (9 * RW + 63) / 64;  // ballot words of the 9 rows
+        static_assert(VOR_RUN <= 64, "a run's active tiles fit one 64-bit mask");
         const int lane = threadIdx.x & 63;
-        uint64_t w[3];
+        uint64_t w[NWB + 2];
+        w[NWB] = w[NWB + 1] = 0;  // read (shifted out) past the last row
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < NWB; ++k) {
             const int idx = lane + 64 * k;  // row idx / RW (dz major, then dy), column idx % RW
             const int rr = idx / RW, cx = idx - rr * RW;
             const int qx = xs - 1 + cx, qy = ty + rr % 3 - 1, qz = tz + rr / 3 - 1;
@@ -669,18 +692,29 @@ __device__ __forceinline__ bool vor_run_begin(const BImg* __restrict__ imgs, con
                 c = chg_prev[((int64_t)qz * ny + qy) * nx + qx] == b - 1;
             w[k] = __ballot(c);
         }
-        // OR of the 9 rows: bit c = some neighbour row changed at x = xs - 1 + c
-        uint32_t m = 0;
+        // OR of the 9 rows: bit c of (m, mh) = some neighbour row changed at
+        // x = xs - 1 + c (mh: bits 64 .. RW - 1)
+        uint64_t m = 0, mh = 0;
 #pragma unroll
         for (int rr = 0; rr < 9; ++rr) {
             const int o = rr * RW, wi = o >> 6, sh = o & 63;
-            uint64_t bitsr = w[wi] >> sh;
-            if (sh + RW > 64) bitsr |= w[wi + 1] << (64 - sh);
-            m |= (uint32_t)(bitsr & ((1u << RW) - 1));
+            uint64_t lo = w[wi] >> sh, hi = 0;
+            if (sh) {
+                lo |= w[wi + 1] << (64 - sh);
+                hi = w[wi + 1] >> sh | w[wi + 2] << (64 - sh);
+            } else {
+                hi = w[wi + 1];
+            }
+            if (RW < 64) lo &= (1ull << (RW < 64 ? RW : 0)) - 1;
+            hi &= RW > 64 ? ((1ull << (RW > 64 ? RW - 64 : 0)) - 1) : 0ull;
+            m |= lo;
+            mh |= hi;
         }
         const bool settled = lane < nrun && L.settled[t0 + lane] != 0;
         const uint64_t st = __ballot(settled);
-        act = (uint64_t)((m | (m >> 1) | (m >> 2)) & ((1u << nrun) - 1)) & ~st;
+        // tile k runs if row bits k, k + 1 or k + 2 are set
+        const uint64_t run_mask = nrun >= 64 ? ~0ull : ((1ull << nrun) - 1);
+        act = ((m | (m >> 1) | (mh << 63) | (m >> 2) | (mh << 62)) & run_mask) & ~st;
     }
     R = VorRun{j, xs, ty, tz, nrun, t0, act};
     return act != 0;
