@@ -13,10 +13,11 @@
 //   gauss_otsu_labeling      _tier9/_gauss_otsu_labeling.py:6-49
 //
 // MI355X design.  One launch runs a pair of steps, box step 2b then diamond step
-// 2b + 1, on every active tile: the tile and a 2-voxel halo are staged in LDS from the
-// source buffer, the box step is computed on the tile plus a 1-voxel ring, the
-// diamond step on the tile, and only the tile is written to the destination buffer.
-// Across tiles this is exactly the reference's synchronous (Jacobi) steps.
+// 2b + 1, on every active tile: the tile and a 2-voxel halo are read from the source
+// buffer (2D: staged in LDS; 3D: one z column of the region per thread, in registers),
+// the box step is computed on the tile plus a 1-voxel ring, the diamond step on the
+// tile, and only the tile is written to the destination buffer.  Across tiles this is
+// exactly the reference's synchronous (Jacobi) steps.
 //
 // Stopping.  The reference stops after the first step that changes nothing.  A pair
 // launch reports, per image, whether its box step and its diamond step changed a
@@ -196,382 +197,124 @@ __global__ __launch_bounds__(VB) void k_vor_init(VF val, float* A, float* B, Geo
     if (vor_init_tile(val, A, B, g, T, t, L.settled + t) && threadIdx.x == 0) L.mark[L.cap + t] = -1;
 }
 
-// The staged region of a tile: the tile and a GH-voxel halo (z halo only in 3D).
-// Tile extents: 16 x 8 x 8 (3D), 32 x 32 (2D).
-template <bool D3>
-struct PairGeo {
-    static constexpr int TX = D3 ? 16 : 32, TY = D3 ? 8 : 32, TZ = D3 ? 8 : 1;
-    static constexpr int HZ = D3 ? GH : 0;
-    static constexpr int RX = TX + 2 * GH, RY = TY + 2 * GH, RZ = TZ + 2 * HZ;  // region extents
-    static constexpr int RP = RX * RY;                                          // region plane
-    static constexpr int NR = RP * RZ;
-    // registers per thread holding a staged region in flight: a z column (3D) or
-    // cells tid + k * VB (2D)
-    static constexpr int KV = D3 ? RZ : (NR + VB - 1) / VB;
-    int64_t obase;                 // offset of the region origin (may lie outside the image)
-    int64_t plane;
+// The 2D form of a pair of steps: a 32 x 32 tile and a GH-voxel halo staged in LDS.
+struct PairGeo2 {
+    static constexpr int TX = 32, TY = 32;
+    static constexpr int RX = TX + 2 * GH, RY = TY + 2 * GH;  // region extents
+    static constexpr int NR = RX * RY;
+    static constexpr int KV = (NR + VB - 1) / VB;  // region cells per thread (tid + k * VB)
+    int64_t obase;           // offset of the region origin (may lie outside the image)
     int W;
-    int xlo, xhi, ylo, yhi, zlo, zhi;  // the part inside the image, region coordinates [lo, hi)
-    bool interior;                     // the whole region lies inside the image
-    __device__ __forceinline__ PairGeo(const Geom& g, int tx, int ty, int tz) {
-        const int64_t x0 = (int64_t)tx * TX - GH, y0 = (int64_t)ty * TY - GH, z0 = (int64_t)tz * TZ - HZ;
+    int xlo, xhi, ylo, yhi;  // the part inside the image, region coordinates [lo, hi)
+    bool interior;           // the whole region lies inside the image
+    __device__ __forceinline__ PairGeo2(const Geom& g, int tx, int ty) {
+        const int64_t x0 = (int64_t)tx * TX - GH, y0 = (int64_t)ty * TY - GH;
         xlo = x0 < 0 ? (int)-x0 : 0;
         ylo = y0 < 0 ? (int)-y0 : 0;
-        zlo = z0 < 0 ? (int)-z0 : 0;
         xhi = (int)(g.W - x0 < RX ? g.W - x0 : RX);
         yhi = (int)(g.H - y0 < RY ? g.H - y0 : RY);
-        zhi = (int)(g.D - z0 < RZ ? g.D - z0 : RZ);
-        plane = g.H * g.W;
         W = (int)g.W;
-        obase = (z0 * g.H + y0) * g.W + x0;
-        interior = xlo == 0 && ylo == 0 && zlo == 0 && xhi == RX && yhi == RY && zhi == RZ;
+        obase = y0 * g.W + x0;
+        interior = xlo == 0 && ylo == 0 && xhi == RX && yhi == RY;
     }
 };
-constexpr int VOR_KV = PairGeo<true>::KV > PairGeo<false>::KV ? PairGeo<true>::KV : PairGeo<false>::KV;
 
 // issue the loads of a tile's region (voxels inside the image) into registers
-template <bool D3>
-__device__ __forceinline__ void vor_stage_load(const float* __restrict__ src, const PairGeo<D3>& R, float* v) {
-    using P = PairGeo<D3>;
+__device__ __forceinline__ void vor_stage_load2(const float* __restrict__ src, const PairGeo2& R, float* v) {
+    using P = PairGeo2;
     const int tid = threadIdx.x;
-    if (R.interior) {  // no bounds to test
-        if (D3) {
-            const float* c = src + R.obase + ((tid / P::RX) * R.W + tid % P::RX);
 #pragma unroll
-            for (int rz = 0; rz < P::KV; ++rz) v[rz] = tid < P::RP ? c[rz * R.plane] : 0.f;
-        } else {
-#pragma unroll
-            for (int k = 0; k < P::KV; ++k) {
-                const int cc = tid + k * VB;
-                v[k] = cc < P::NR ? src[R.obase + ((cc / P::RX) * R.W + cc % P::RX)] : 0.f;
-            }
-        }
-        return;
-    }
-    if (D3) {
-        const int rx = tid % P::RX, ry = tid / P::RX;
-        const bool col = tid < P::RP && rx >= R.xlo && rx < R.xhi && ry >= R.ylo && ry < R.yhi;
-        const float* c = src + R.obase + (ry * R.W + rx);
-#pragma unroll
-        for (int rz = 0; rz < P::KV; ++rz) v[rz] = (col && rz >= R.zlo && rz < R.zhi) ? c[rz * R.plane] : 0.f;
-    } else {
-#pragma unroll
-        for (int k = 0; k < P::KV; ++k) {
-            const int c = tid + k * VB;
-            const int rx = c % P::RX, ry = c / P::RX;
-            const bool in = c < P::NR && rx >= R.xlo && rx < R.xhi && ry >= R.ylo && ry < R.yhi;
-            v[k] = in ? src[R.obase + (ry * R.W + rx)] : 0.f;
-        }
+    for (int k = 0; k < P::KV; ++k) {
+        const int c = tid + k * VB;
+        const int rx = c % P::RX, ry = c / P::RX;
+        const bool in = c < P::NR && (R.interior || (rx >= R.xlo && rx < R.xhi && ry >= R.ylo && ry < R.yhi));
+        v[k] = in ? src[R.obase + (ry * R.W + rx)] : 0.f;
     }
 }
 
-template <bool D3>
-__device__ __forceinline__ void vor_stage_store(const float* v, float* lds) {
-    using P = PairGeo<D3>;
-    const int tid = threadIdx.x;
-    if (D3) {
-        if (tid < P::RP) {
+__device__ __forceinline__ void vor_stage_store2(const float* v, float* lds) {
+    using P = PairGeo2;
 #pragma unroll
-            for (int rz = 0; rz < P::KV; ++rz) lds[rz * P::RP + tid] = v[rz];
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < P::KV; ++k)
-            if (tid + k * VB < P::NR) lds[tid + k * VB] = v[k];
-    }
-}
-
-// vor_pair_compute for a tile whose region lies inside the image: the same steps with
-// no clamping, so every LDS offset is a constant.
-template <bool D3>
-__device__ __forceinline__ int vor_pair_interior(float* __restrict__ dst, const PairGeo<D3>& R, const float* lds,
-                                                 float* lds2, int* sbits) {
-    using P = PairGeo<D3>;
-    constexpr int TX = P::TX, TY = P::TY, TZ = P::TZ, RX = P::RX, RY = P::RY, RZ = P::RZ, RP = P::RP, HZ = P::HZ;
-    const int tid = threadIdx.x;
-    {
-        bool zero = false;
-#pragma unroll
-        for (int k = 0; k < (TX * TY * TZ) / VB; ++k) {
-            const int c = tid + k * VB;
-            const int rx = c % TX + GH, ry = (c / TX) % TY + GH, rz = c / (TX * TY) + HZ;
-            zero |= lds[(rz * RY + ry) * RX + rx] == 0.f;
-        }
-        if (!__syncthreads_or(zero ? 1 : 0)) {
-#pragma unroll
-            for (int k = 0; k < (TX * TY * TZ) / VB; ++k) {
-                const int c = tid + k * VB;
-                const int rx = c % TX + GH, ry = (c / TX) % TY + GH, rz = c / (TX * TY) + HZ;
-                dst[R.obase + rz * R.plane + (ry * R.W + rx)] = lds[(rz * RY + ry) * RX + rx];
-            }
-            __syncthreads();
-            return 0;
-        }
-    }
-    int bits = 0;
-    if (D3) {
-        constexpr int SX = TX + 2, SY = TY + 2;
-        if (tid < SX * SY) {
-            const int rx = tid % SX + 1, ry = tid / SX + 1;
-            const int c0 = ry * RX + rx;
-            const float* p = lds + c0;
-            bool anyzero = false;
-#pragma unroll
-            for (int rz = 1; rz < RZ - 1; ++rz) anyzero |= p[rz * RP] == 0.f;
-            if (!anyzero) {
-#pragma unroll
-                for (int rz = 1; rz < RZ - 1; ++rz) lds2[rz * RP + c0] = p[rz * RP];
-            } else {
-                auto m9 = [&](int rz) {
-                    const float* q = p + rz * RP;
-                    float m = fmaxf(fmaxf(q[-RX - 1], q[-RX]), q[-RX + 1]);
-                    m = fmaxf(m, fmaxf(fmaxf(q[-1], q[0]), q[1]));
-                    return fmaxf(m, fmaxf(fmaxf(q[RX - 1], q[RX]), q[RX + 1]));
-                };
-                const bool inner_xy = rx >= GH && rx < TX + GH && ry >= GH && ry < TY + GH;
-                float mprev = m9(0), mcur = m9(1);
-#pragma unroll
-                for (int rz = 1; rz < RZ - 1; ++rz) {
-                    const float mnext = m9(rz + 1);
-                    const float v = p[rz * RP];
-                    float out = v;
-                    if (v == 0.f) {
-                        out = fmaxf(0.f, fmaxf(fmaxf(mprev, mcur), mnext));
-                        if (inner_xy && rz >= GH && rz < TZ + GH && out != 0.f) bits |= 1;
-                    }
-                    lds2[rz * RP + c0] = out;
-                    mprev = mcur;
-                    mcur = mnext;
-                }
-            }
-        }
-        __syncthreads();
-        {
-            const int col = tid % (TX * TY), half = tid / (TX * TY);
-            const int rx = col % TX + GH, ry = col / TX + GH;
-            const int c0 = ry * RX + rx;
-            float* dcol = dst + R.obase + (ry * R.W + rx);
-#pragma unroll
-            for (int k = 0; k < TZ / 2; ++k) {
-                const int rz = GH + half * (TZ / 2) + k;
-                const float* q = lds2 + rz * RP + c0;
-                const float v = q[0];
-                float out = v;
-                if (v == 0.f) {
-                    float best = fmaxf(0.f, q[-1]);
-                    best = fmaxf(best, q[1]);
-                    best = fmaxf(best, q[-RX]);
-                    best = fmaxf(best, q[RX]);
-                    best = fmaxf(best, q[-RP]);
-                    best = fmaxf(best, q[RP]);
-                    out = best;
-                    bits |= out != 0.f ? 2 : 4;
-                }
-                dcol[rz * R.plane] = out;
-            }
-        }
-    } else {
-        constexpr int SX = TX + 2, SY = TY + 2, NS = SX * SY;
-#pragma unroll 2
-        for (int c = tid; c < NS; c += VB) {
-            const int rx = c % SX + 1, ry = c / SX + 1;
-            const int rc = ry * RX + rx;
-            const float* q = lds + rc;
-            const float v = q[0];
-            float out = v;
-            if (v == 0.f) {
-                float best = fmaxf(0.f, fmaxf(fmaxf(q[-RX - 1], q[-RX]), q[-RX + 1]));
-                best = fmaxf(best, fmaxf(q[-1], q[1]));
-                best = fmaxf(best, fmaxf(fmaxf(q[RX - 1], q[RX]), q[RX + 1]));
-                out = best;
-                const bool inner = rx >= GH && rx < TX + GH && ry >= GH && ry < TY + GH;
-                if (inner && out != 0.f) bits |= 1;
-            }
-            lds2[rc] = out;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < (TX * TY) / VB; ++k) {
-            const int c = tid + k * VB;
-            const int rx = c % TX + GH, ry = c / TX + GH;
-            const int rc = ry * RX + rx;
-            const float* q = lds2 + rc;
-            const float v = q[0];
-            float out = v;
-            if (v == 0.f) {
-                float best = fmaxf(0.f, q[-1]);
-                best = fmaxf(best, q[1]);
-                best = fmaxf(best, q[-RX]);
-                best = fmaxf(best, q[RX]);
-                out = best;
-                bits |= out != 0.f ? 2 : 4;
-            }
-            dst[R.obase + (ry * R.W + rx)] = out;
-        }
-    }
-    if (bits) atomicOr(sbits, bits);
-    __syncthreads();
-    return *sbits;
+    for (int k = 0; k < P::KV; ++k)
+        if (threadIdx.x + k * VB < P::NR) lds[threadIdx.x + k * VB] = v[k];
 }
 
 // One pair of steps (box, then diamond) on a staged tile: lds holds the region, lds2
-// receives the box step's result, the diamond step is written to dst.  Returns, in
-// every thread, bit 0: a tile voxel changed in the box step, bit 1: in the diamond
-// step, bit 2: a tile voxel is zero after the pair.  Neighbour coordinates are
-// clamped to the image, which keeps them inside the staged region.
-template <bool D3>
-__device__ __forceinline__ int vor_pair_compute(float* __restrict__ dst, const PairGeo<D3>& R, const float* lds,
-                                                float* lds2, int* sbits) {
-    if (R.interior) return vor_pair_interior<D3>(dst, R, lds, lds2, sbits);
-    using P = PairGeo<D3>;
-    constexpr int TX = P::TX, TY = P::TY, TZ = P::TZ, RX = P::RX, RZ = P::RZ, RP = P::RP;
+// receives the box step's result on the tile + 1-voxel ring, the diamond step is
+// written to dst.  Returns, in every thread, bit 0: a tile voxel changed in the box
+// step, bit 1: in the diamond step, bit 2: a tile voxel is zero after the pair.
+// Neighbour coordinates are clamped to the image, which keeps them inside the staged
+// region (interior tiles need no clamping: every LDS offset is then a constant).
+__device__ __forceinline__ int vor_pair_compute2(float* __restrict__ dst, const PairGeo2& R, const float* lds,
+                                                 float* lds2, int* sbits) {
+    using P = PairGeo2;
+    constexpr int TX = P::TX, TY = P::TY, RX = P::RX;
     const int tid = threadIdx.x;
-    const int xlo = R.xlo, xhi = R.xhi, ylo = R.ylo, yhi = R.yhi, zlo = R.zlo, zhi = R.zhi;
+    const bool inr = R.interior;
+    const int xlo = inr ? 0 : R.xlo, xhi = inr ? RX : R.xhi, ylo = inr ? 0 : R.ylo, yhi = inr ? P::RY : R.yhi;
     // a tile without a zero voxel keeps its values (non-zero voxels never change): its
     // output is its input, whatever the halo holds
     {
         bool zero = false;
 #pragma unroll
-        for (int k = 0; k < (TX * TY * TZ) / VB; ++k) {
+        for (int k = 0; k < (TX * TY) / VB; ++k) {
             const int c = tid + k * VB;
-            const int rx = c % TX + GH, ry = (c / TX) % TY + GH, rz = c / (TX * TY) + P::HZ;
-            if (rx < xhi && ry < yhi && rz < zhi) zero |= lds[(rz * P::RY + ry) * RX + rx] == 0.f;
+            const int rx = c % TX + GH, ry = c / TX + GH;
+            if (rx < xhi && ry < yhi) zero |= lds[ry * RX + rx] == 0.f;
         }
         if (!__syncthreads_or(zero ? 1 : 0)) {
 #pragma unroll
-            for (int k = 0; k < (TX * TY * TZ) / VB; ++k) {
+            for (int k = 0; k < (TX * TY) / VB; ++k) {
                 const int c = tid + k * VB;
-                const int rx = c % TX + GH, ry = (c / TX) % TY + GH, rz = c / (TX * TY) + P::HZ;
-                if (rx < xhi && ry < yhi && rz < zhi)
-                    dst[R.obase + rz * R.plane + (ry * R.W + rx)] = lds[(rz * P::RY + ry) * RX + rx];
+                const int rx = c % TX + GH, ry = c / TX + GH;
+                if (rx < xhi && ry < yhi) dst[R.obase + (ry * R.W + rx)] = lds[ry * RX + rx];
             }
             __syncthreads();  // lds is read above; the next tile's staging overwrites it
             return 0;         // no change, no zero
         }
     }
     int bits = 0;
-    if (D3) {
-        // ---- box step on the tile + 1-voxel ring: one thread per column, the 3 x 3
-        // plane maximum m(z) slid along z, box(z) = max(0, m(z-1), m(z), m(z+1))
-        constexpr int SX = TX + 2, SY = TY + 2;
-        if (tid < SX * SY) {
-            const int rx = tid % SX + 1, ry = tid / SX + 1;
-            if (rx >= xlo && rx < xhi && ry >= ylo && ry < yhi) {
-                const int c0 = ry * RX + rx;
-                const int dxm = rx > xlo ? -1 : 0, dxp = rx < xhi - 1 ? 1 : 0;
-                const int dym = ry > ylo ? -RX : 0, dyp = ry < yhi - 1 ? RX : 0;
-                const int r0 = zlo > 1 ? zlo : 1, r1 = zhi < RZ - 1 ? zhi : RZ - 1;  // box cells: [r0, r1)
-                bool anyzero = false;
-#pragma unroll
-                for (int rz = 1; rz < RZ - 1; ++rz)
-                    if (rz >= r0 && rz < r1) anyzero |= lds[rz * RP + c0] == 0.f;
-                if (!anyzero) {
-#pragma unroll
-                    for (int rz = 1; rz < RZ - 1; ++rz)
-                        if (rz >= r0 && rz < r1) lds2[rz * RP + c0] = lds[rz * RP + c0];
-                } else {
-                    auto m9 = [&](int rz) {
-                        const float* p = lds + rz * RP + c0;
-                        float m = fmaxf(fmaxf(p[dym + dxm], p[dym]), p[dym + dxp]);
-                        m = fmaxf(m, fmaxf(fmaxf(p[dxm], p[0]), p[dxp]));
-                        return fmaxf(m, fmaxf(fmaxf(p[dyp + dxm], p[dyp]), p[dyp + dxp]));
-                    };
-                    // planes r0 - 1 .. r1 (clamped to the image) feed the box cells
-                    float mprev = m9(r0 - 1 >= zlo ? r0 - 1 : r0);
-                    float mcur = m9(r0);
-#pragma unroll
-                    for (int rz = 1; rz < RZ - 1; ++rz) {
-                        if (rz < r0 || rz >= r1) continue;
-                        const float mnext = rz + 1 < zhi ? m9(rz + 1) : mcur;
-                        const float v = lds[rz * RP + c0];
-                        float out = v;
-                        if (v == 0.f) {
-                            out = fmaxf(0.f, fmaxf(fmaxf(mprev, mcur), mnext));
-                            const bool inner = rx >= GH && rx < TX + GH && ry >= GH && ry < TY + GH &&
-                                               rz >= GH && rz < TZ + GH;
-                            if (inner && out != 0.f) bits |= 1;
-                        }
-                        lds2[rz * RP + c0] = out;
-                        mprev = mcur;
-                        mcur = mnext;
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        // ---- diamond step on the tile: two threads per column (4 slices each)
-        {
-            const int col = tid % (TX * TY), half = tid / (TX * TY);
-            const int rx = col % TX + GH, ry = col / TX + GH;
-            if (rx < xhi && ry < yhi) {
-                const int c0 = ry * RX + rx;
-                const int dxm = rx > xlo ? -1 : 0, dxp = rx < xhi - 1 ? 1 : 0;
-                const int dym = ry > ylo ? -RX : 0, dyp = ry < yhi - 1 ? RX : 0;
-                float* dcol = dst + R.obase + (ry * R.W + rx);
-#pragma unroll
-                for (int k = 0; k < TZ / 2; ++k) {
-                    const int rz = GH + half * (TZ / 2) + k;
-                    if (rz >= zhi) continue;
-                    const float* p = lds2 + rz * RP + c0;
-                    const float v = p[0];
-                    float out = v;
-                    if (v == 0.f) {
-                        float best = fmaxf(0.f, p[dxm]);
-                        best = fmaxf(best, p[dxp]);
-                        best = fmaxf(best, p[dym]);
-                        best = fmaxf(best, p[dyp]);
-                        best = fmaxf(best, p[rz > zlo ? -RP : 0]);
-                        best = fmaxf(best, p[rz < zhi - 1 ? RP : 0]);
-                        out = best;
-                        if (out != 0.f) bits |= 2;
-                        else bits |= 4;
-                    }
-                    dcol[rz * R.plane] = out;
-                }
-            }
-        }
-    } else {
-        constexpr int SX = TX + 2, SY = TY + 2, NS = SX * SY;
+    // box step on the tile + 1-voxel ring
+    constexpr int SX = TX + 2, SY = TY + 2, NS = SX * SY;
 #pragma unroll 2
-        for (int c = tid; c < NS; c += VB) {
-            const int rx = c % SX + 1, ry = c / SX + 1;
-            if (rx < xlo || rx >= xhi || ry < ylo || ry >= yhi) continue;
-            const int rc = ry * RX + rx;
-            const float v = lds[rc];
-            float out = v;
-            if (v == 0.f) {
-                const int dxm = rx > xlo ? -1 : 0, dxp = rx < xhi - 1 ? 1 : 0;
-                const int dym = ry > ylo ? -RX : 0, dyp = ry < yhi - 1 ? RX : 0;
-                const float* p = lds + rc;
-                float best = fmaxf(0.f, fmaxf(fmaxf(p[dym + dxm], p[dym]), p[dym + dxp]));
-                best = fmaxf(best, fmaxf(fmaxf(p[dxm], p[0]), p[dxp]));
-                best = fmaxf(best, fmaxf(fmaxf(p[dyp + dxm], p[dyp]), p[dyp + dxp]));
-                out = best;
-                const bool inner = rx >= GH && rx < TX + GH && ry >= GH && ry < TY + GH;
-                if (inner && out != 0.f) bits |= 1;
-            }
-            lds2[rc] = out;
+    for (int c = tid; c < NS; c += VB) {
+        const int rx = c % SX + 1, ry = c / SX + 1;
+        if (rx < xlo || rx >= xhi || ry < ylo || ry >= yhi) continue;
+        const int rc = ry * RX + rx;
+        const float v = lds[rc];
+        float out = v;
+        if (v == 0.f) {
+            const int dxm = rx > xlo ? -1 : 0, dxp = rx < xhi - 1 ? 1 : 0;
+            const int dym = ry > ylo ? -RX : 0, dyp = ry < yhi - 1 ? RX : 0;
+            const float* p = lds + rc;
+            float best = fmaxf(0.f, fmaxf(fmaxf(p[dym + dxm], p[dym]), p[dym + dxp]));
+            best = fmaxf(best, fmaxf(fmaxf(p[dxm], p[0]), p[dxp]));
+            best = fmaxf(best, fmaxf(fmaxf(p[dyp + dxm], p[dyp]), p[dyp + dxp]));
+            out = best;
+            const bool inner = rx >= GH && rx < TX + GH && ry >= GH && ry < TY + GH;
+            if (inner && out != 0.f) bits |= 1;
         }
-        __syncthreads();
+        lds2[rc] = out;
+    }
+    __syncthreads();
+    // diamond step on the tile
 #pragma unroll
-        for (int k = 0; k < (TX * TY) / VB; ++k) {
-            const int c = tid + k * VB;
-            const int rx = c % TX + GH, ry = c / TX + GH;
-            if (rx >= xhi || ry >= yhi) continue;
-            const int rc = ry * RX + rx;
-            const float v = lds2[rc];
-            float out = v;
-            if (v == 0.f) {
-                float best = fmaxf(0.f, lds2[rc + (rx > xlo ? -1 : 0)]);
-                best = fmaxf(best, lds2[rc + (rx < xhi - 1 ? 1 : 0)]);
-                best = fmaxf(best, lds2[rc + (ry > ylo ? -RX : 0)]);
-                best = fmaxf(best, lds2[rc + (ry < yhi - 1 ? RX : 0)]);
-                out = best;
-                if (out != 0.f) bits |= 2;
-                else bits |= 4;
-            }
-            dst[R.obase + (ry * R.W + rx)] = out;
+    for (int k = 0; k < (TX * TY) / VB; ++k) {
+        const int c = tid + k * VB;
+        const int rx = c % TX + GH, ry = c / TX + GH;
+        if (rx >= xhi || ry >= yhi) continue;
+        const int rc = ry * RX + rx;
+        const float v = lds2[rc];
+        float out = v;
+        if (v == 0.f) {
+            float best = fmaxf(0.f, lds2[rc + (rx > xlo ? -1 : 0)]);
+            best = fmaxf(best, lds2[rc + (rx < xhi - 1 ? 1 : 0)]);
+            best = fmaxf(best, lds2[rc + (ry > ylo ? -RX : 0)]);
+            best = fmaxf(best, lds2[rc + (ry < yhi - 1 ? RX : 0)]);
+            out = best;
+            bits |= out != 0.f ? 2 : 4;
         }
+        dst[R.obase + (ry * R.W + rx)] = out;
     }
     if (bits) atomicOr(sbits, bits);
     __syncthreads();
@@ -720,23 +463,18 @@ __device__ __forceinline__ bool vor_run_begin(const BImg* __restrict__ imgs, con
     return act != 0;
 }
 
-// Tiles staged in LDS: the 2D form, and the 3D form for images with planes of
-// VOR_WIDE_PLANE voxels or more (sparse large stacks: measured on 256 x 1024 x 1024 and
-// 512 x 2048 x 2048 nuclei stacks, the register-column form below takes 5-25 % longer
-// there, while it takes 24 % less on the mixed batch's small 3D images).
-constexpr int64_t VOR_WIDE_PLANE = (int64_t)1 << 20;
-template <bool D3>
-__global__ __launch_bounds__(VB, 4) void k_vor_pair_run(const BImg* __restrict__ imgs, const int64_t* __restrict__ rs,
-                                                      int nimg, const float* __restrict__ src, float* __restrict__ dst,
-                                                      int b, VorLists L, VorState S) {
-    using P = PairGeo<D3>;
+// The 2D form: 32 x 32 tiles staged in LDS.
+__global__ __launch_bounds__(VB, 4) void k_vor_pair_run2(const BImg* __restrict__ imgs, const int64_t* __restrict__ rs,
+                                                       int nimg, const float* __restrict__ src, float* __restrict__ dst,
+                                                       int b, VorLists L, VorState S) {
+    using P = PairGeo2;
     __shared__ float lds[P::NR];
     __shared__ float lds2[P::NR];
     __shared__ int sbits[2];
     VorRun run;
     if (!vor_run_begin(imgs, rs, nimg, b, L, S, run)) return;
     uint32_t* cur = S.bits + (size_t)(b % 3) * nimg;
-    const int j = run.j, xs = run.xs, ty = run.ty, tz = run.tz;
+    const int j = run.j, xs = run.xs, ty = run.ty;
     const int64_t t0 = run.t0;
     uint64_t act = run.act;
     const BImg& im = imgs[j];
@@ -746,7 +484,7 @@ __global__ __launch_bounds__(VB, 4) void k_vor_pair_run(const BImg* __restrict__
     float v[P::KV];
     int tk = __builtin_ctzll(act);  // run index of the current tile
     act &= act - 1;
-    vor_stage_load<D3>(src, P(g, xs + tk, ty, tz), v);
+    vor_stage_load2(src, P(g, xs + tk, ty), v);
     int par = 0;
     uint32_t done = 0;
     int acc = 0;  // change bits of the block's tiles: one atomic per block
@@ -754,16 +492,16 @@ __global__ __launch_bounds__(VB, 4) void k_vor_pair_run(const BImg* __restrict__
     for (;;) {
         ++done;
         const int tx = xs + tk;
-        const P R(g, tx, ty, tz);
+        const P R(g, tx, ty);
         // sbits[par] was last read two tiles ago, before a barrier of the previous tile
         if (threadIdx.x == 0) sbits[par] = 0;
-        vor_stage_store<D3>(v, lds);
+        vor_stage_store2(v, lds);
         __syncthreads();
         const bool more = act != 0;
         const int tn = more ? __builtin_ctzll(act) : tk;
         act &= act - 1;
-        if (more) vor_stage_load<D3>(src, P(g, xs + tn, ty, tz), v);
-        const int bits = vor_pair_compute<D3>(dst, R, lds, lds2, &sbits[par]);
+        if (more) vor_stage_load2(src, P(g, xs + tn, ty), v);
+        const int bits = vor_pair_compute2(dst, R, lds, lds2, &sbits[par]);
         par ^= 1;
         if (bits & 3) {
             acc |= bits & 3;
@@ -772,7 +510,7 @@ __global__ __launch_bounds__(VB, 4) void k_vor_pair_run(const BImg* __restrict__
             L.settled[t0 + tk] = 1;
         }
         // lds is rewritten by the next tile only after every thread passed the last
-        // barrier of vor_pair_compute (its lds reads come before it)
+        // barrier of vor_pair_compute2 (its lds reads come before it)
         if (!more) break;
         tk = tn;
     }
@@ -1018,10 +756,9 @@ struct VorScratch {
 struct VorRuns {
     const int64_t* rs[2];
     int64_t blocks[2];
-    bool wide3;  // 3D launches take the LDS-staged form (planes of VOR_WIDE_PLANE+ voxels)
 };
 inline VorRuns vor_runs_single(const Geom& g) {
-    VorRuns R{{nullptr, nullptr}, {0, 0}, g.H * g.W >= VOR_WIDE_PLANE};
+    VorRuns R{{nullptr, nullptr}, {0, 0}};
     R.blocks[vor_3d(g) ? 1 : 0] = vor_run_blocks(g, vor_run_length(g, false));
     return R;
 }
@@ -1033,10 +770,8 @@ static void launch_pairs(const BImg* dimgs, int nimg, float* P0, float* P1, cons
         const float* src = (b & 1) ? P1 : P0;
         float* dst = (b & 1) ? P0 : P1;
         if (R.blocks[0] > 0)
-            k_vor_pair_run<false><<<(unsigned)R.blocks[0], VB, 0, s>>>(dimgs, R.rs[0], nimg, src, dst, b, L, S);
-        if (R.blocks[1] > 0 && R.wide3)
-            k_vor_pair_run<true><<<(unsigned)R.blocks[1], VB, 0, s>>>(dimgs, R.rs[1], nimg, src, dst, b, L, S);
-        else if (R.blocks[1] > 0)
+            k_vor_pair_run2<<<(unsigned)R.blocks[0], VB, 0, s>>>(dimgs, R.rs[0], nimg, src, dst, b, L, S);
+        if (R.blocks[1] > 0)
             k_vor_pair_run3<<<(unsigned)R.blocks[1], VB, 0, s>>>(dimgs, R.rs[1], nimg, src, dst, b, L, S);
     }
     k_vor_close<<<1, VB, 0, s>>>(nimg, b - 1, S);
@@ -1666,16 +1401,6 @@ static int voronoi_otsu_batch(const cle_buf* srcs, const cle_buf* dsts, const fl
     for (int k = 0; k < 2; ++k) {
         runs.rs[k] = (const int64_t*)(dev + rs_off[k]);
         runs.blocks[k] = nrun[k];
-    }
-    {
-        // the LDS-staged 3D form when most 3D voxels lie in images of wide planes
-        int64_t v3 = 0, vw = 0;
-        for (const BImg& im : imgs)
-            if (vor_3d(im.g)) {
-                v3 += im.g.n();
-                if (im.g.H * im.g.W >= VOR_WIDE_PLANE) vw += im.g.n();
-            }
-        runs.wide3 = 2 * vw > v3;
     }
     const int64_t* dvs = (const int64_t*)(dev + vs_off);
     if (cap) {
