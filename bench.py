@@ -175,7 +175,7 @@ def c3_section(cle, torch, np, synthetic, steps, warmup, want_crop):
     achieved = pair_bytes / pair_s / 1e9 if pair_s > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_vor_pair_run<3D> (Voronoi growth, two steps per launch)",
+                "kernel": "k_vor_pair_run3 (Voronoi growth of 3D images, two steps per launch)",
                 "launches": gc["launches"] // steps if steps else 0,
                 "tiles_per_step": gc["tiles"] // steps if steps else 0,
                 "algorithmic_bytes_per_launch": int(pair_bytes / max(1, gc["launches"])),

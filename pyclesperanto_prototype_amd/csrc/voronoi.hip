@@ -529,17 +529,19 @@ __global__ __launch_bounds__(VB, 4) void k_vor_pair_run2(const BImg* __restrict_
 // value.  Columns without a zero voxel skip the work whose result they do not need
 // (sparse masks leave most columns of an active tile without one): the box maximum of
 // a voxel is taken only where the voxel is zero, Y only next to such columns, the
-// diamond only in them.  Four barriers per tile; the next tile's columns load while
-// this one computes.
+// diamond only in them.  Four barriers per tile (the block's "tile holds a zero" vote
+// is a flag word per tile parity set before the first, not a barrier of its own); the
+// next tile's columns load while this one computes, and when it is the next tile in x
+// its left halo columns come from this tile through LDS.  Every branch on the tile's
+// z extent is uniform.  64 VGPRs: 7 blocks per CU (LDS bound).
 #ifndef CLE_VOR3_BLOCKS
-#define CLE_VOR3_BLOCKS 6  // blocks per CU the register budget allows (80 VGPRs)
+#define CLE_VOR3_BLOCKS 6  // blocks per CU the register budget must allow (<= 80 VGPRs)
 #endif
-// (the register budget matters: at 8 blocks per CU registers spill; 4 and 5 blocks
-// were measured 41 % and 21 % slower on the mixed batch)
+// (at 8 blocks per CU registers spill; 4 and 5 blocks were measured 41 % and 21 %
+// slower on the mixed batch)
 constexpr int V3_RX = 20, V3_RY = 12, V3_RZ = 12;
 constexpr int V3_COLS = V3_RX * V3_RY;  // 240 columns
 constexpr int V3_NB = V3_RZ - 2;        // box planes: region z 1 .. 10
-
 __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BImg* __restrict__ imgs,
                                                                      const int64_t* __restrict__ rs, int nimg,
                                                                      const float* __restrict__ src,
@@ -549,7 +551,9 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
     __shared__ float Cl[V3_NB * V3_COLS];  // C of every column, then the box result
     __shared__ float Yl[V3_NB * V3_COLS];
     __shared__ uint8_t Fl[V3_COLS];  // the column holds a zero among the box planes
+    __shared__ float Hl[2][2 * V3_RY * V3_RZ];  // body columns 14, 15: the next tile's left halo
     __shared__ int sbits[2];
+    __shared__ int zf[2];
     VorRun run;
     if (!vor_run_begin(imgs, rs, nimg, b, L, S, run)) return;
     uint32_t* cur = S.bits + (size_t)(b % 3) * nimg;
@@ -570,9 +574,17 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
         const int64_t z = z0 + rz;
         zoff[rz] = (z < 0 ? 0 : (z >= g.D ? g.D - 1 : z)) * plane;
     }
+    // output voxel (column offset o, region plane rz inside the image)
+    auto store = [&](int64_t o, int rz, float val) { dst[o + zoff[rz]] = val; };
     const bool ybox = col && ry >= 1 && ry <= V3_RY - 2;
     const bool ytile = col && ry >= 2 && ry < V3_RY - 2 && yr < g.H;
     const bool xbox = rx >= 1 && rx <= V3_RX - 2;
+    // When the next active tile is x + 1, its left halo columns (region x 0, 1) are this
+    // tile's body columns 14, 15 (region x 16, 17), passed through LDS instead of
+    // loaded: they lie in the 64-byte line before the next tile's body.
+    const bool hsrc = col && (rx == V3_RX - 4 || rx == V3_RX - 3);
+    const bool hdst = col && rx < 2;
+    const int hidx = ((rx & 1) * V3_RY + ry) * V3_RZ;
     auto load = [&](int tx, float* v) {
         const int64_t xr = (int64_t)tx * 16 - 2 + rx;
         const float* cp = src + yy * g.W + (xr < 0 ? 0 : (xr >= g.W ? g.W - 1 : xr));
@@ -584,6 +596,11 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
     act &= act - 1;
     float v[V3_RZ], vn[V3_RZ];
     load(run.xs + tk, v);
+    // tile planes inside the image: q = 1 .. nzin (region plane q + 1; all 8 but in the
+    // last tile plane)
+    const int nzin = (int)(g.D - z0 - 2 < 8 ? g.D - z0 - 2 : 8);
+    if (tid == 0) zf[0] = 0;
+    __syncthreads();
     int par = 0;
     uint32_t done = 0;
     int acc = 0;
@@ -603,54 +620,66 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
             c[q] = fmaxf(fmaxf(v[q], v[q + 1]), v[q + 2]);
             if (col) Cl[q * V3_COLS + tid] = c[q];
             hz |= v[q + 1] == 0.f;
-            if (q >= 1 && q <= 8) zero |= tcol && z0 + q + 1 < g.D && v[q + 1] == 0.f;
+            if (q >= 1 && q <= 8 && q <= nzin) zero |= v[q + 1] == 0.f;
         }
         if (col) Fl[tid] = hz ? 1 : 0;
+        // the tile holds a zero voxel: one flag word per tile parity, set by the waves
+        // before the barrier below and cleared for the next tile before this tile's last
+        if (__ballot(zero && tcol) != 0 && (tid & 63) == 0) zf[par] = 1;
         const bool more = act != 0;
         const int tn = more ? __builtin_ctzll(act) : tk;
         act &= act - 1;
-        if (more) load(run.xs + tn, vn);
-        const int64_t obase = yr * g.W + xr;  // output offset of the column at z = 0
-        if (!__syncthreads_or(zero ? 1 : 0)) {
+        const bool next_x = more && tn == tk + 1;
+        if (next_x && hsrc) {
+#pragma unroll
+            for (int rz = 0; rz < V3_RZ; ++rz) Hl[par][hidx + rz] = v[rz];
+        }
+        if (more && !(next_x && hdst)) load(run.xs + tn, vn);
+        const int64_t obase = yr * g.W + xr;  // output column offset
+        __syncthreads();
+        if (zf[par] == 0) {
             // no zero voxel in the tile: the output is the input
             if (tcol) {
 #pragma unroll
-                for (int rz = 2; rz < V3_RZ - 2; ++rz)
-                    if (z0 + rz < g.D) dst[obase + (z0 + rz) * plane] = v[rz];
+                for (int q = 1; q <= 8; ++q)
+                    if (q <= nzin) store(obase, q + 1, v[q + 1]);
             }
         } else {
             int bits = 0;
-            // Y: maximum of C over y - 1 .. y + 1 (box rows)
+            // Y: maximum of C over y - 1 .. y + 1 (box rows), needed where this column or
+            // an x neighbour holds a zero (rows 1 .. 10: the neighbours' flags lie in the
+            // region)
             float y[V3_NB];
-            // needed where this column or an x neighbour holds a zero (rows 1 .. 10: the
-            // neighbours' flags lie in the region)
-            const bool ny = ybox && (hz || Fl[tid - 1] || Fl[tid + 1]);
+            if (ybox && (hz || Fl[tid - 1] || Fl[tid + 1])) {
 #pragma unroll
-            for (int q = 0; q < V3_NB; ++q) {
-                y[q] = c[q];
-                if (ny) {
+                for (int q = 0; q < V3_NB; ++q) {
                     y[q] = fmaxf(fmaxf(Cl[q * V3_COLS + tid - V3_RX], c[q]), Cl[q * V3_COLS + tid + V3_RX]);
                     Yl[q * V3_COLS + tid] = y[q];
                 }
+            } else {
+#pragma unroll
+                for (int q = 0; q < V3_NB; ++q) y[q] = c[q];
             }
             __syncthreads();
             // box value of the ring + tile: maximum of Y over x - 1 .. x + 1; zero voxels
-            // take max(0, box maximum)
+            // take max(0, box maximum), the others keep their value
             float bx[V3_NB];
 #pragma unroll
-            for (int q = 0; q < V3_NB; ++q) {
-                const float vz = v[q + 1];
-                bx[q] = vz;
-                if (ybox && xbox) {
-                    if (hz) {  // non-zero voxels keep their value
+            for (int q = 0; q < V3_NB; ++q) bx[q] = v[q + 1];
+            if (ybox && xbox) {
+                if (hz) {
+#pragma unroll
+                    for (int q = 0; q < V3_NB; ++q) {
                         const float m = fmaxf(fmaxf(Yl[q * V3_COLS + tid - 1], y[q]), Yl[q * V3_COLS + tid + 1]);
-                        const bool z = vz == 0.f;
-                        bx[q] = z ? fmaxf(0.f, m) : vz;
-                        if (q >= 1 && q <= 8) bits |= (tcol && z0 + q + 1 < g.D && z && bx[q] != 0.f) ? 1 : 0;
+                        const bool z = v[q + 1] == 0.f;
+                        bx[q] = z ? fmaxf(0.f, m) : v[q + 1];
+                        if (q >= 1 && q <= 8 && q <= nzin) bits |= (z && bx[q] != 0.f) ? 1 : 0;
                     }
-                    Cl[q * V3_COLS + tid] = bx[q];
                 }
+#pragma unroll
+                for (int q = 0; q < V3_NB; ++q) Cl[q * V3_COLS + tid] = bx[q];
             }
+            if (!tcol) bits = 0;
             __syncthreads();
             // diamond on the tile: z neighbours in registers, x / y neighbours from LDS
             if (tcol && hz) {
@@ -661,23 +690,23 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
                                            fmaxf(bx[q - 1], bx[q + 1]));
                     const bool z = bx[q] == 0.f;
                     const float out = z ? fmaxf(0.f, nb) : bx[q];
-                    if (z0 + q + 1 < g.D) {
-                        dst[obase + (z0 + q + 1) * plane] = out;
+                    if (q <= nzin) {
+                        store(obase, q + 1, out);
                         bits |= z ? (out != 0.f ? 2 : 4) : 0;
                     }
                 }
             } else if (tcol) {  // no zero voxel in the column: box and diamond keep it
 #pragma unroll
                 for (int q = 1; q <= 8; ++q)
-                    if (z0 + q + 1 < g.D) dst[obase + (z0 + q + 1) * plane] = v[q + 1];
+                    if (q <= nzin) store(obase, q + 1, v[q + 1]);
             }
             if (bits) atomicOr(&sbits[par], bits);
         }
+        if (tid == 0) zf[par ^ 1] = 0;  // last read before the previous tile's last barrier
         // every thread is past its LDS reads of this tile before the next tile's C
         // writes; the block's change bits
         __syncthreads();
         const int tb = sbits[par];
-        par ^= 1;
         if (tb & 3) {
             acc |= tb & 3;
             if (tid == 0) L.mark[(int64_t)(b & 1) * L.cap + run.t0 + tk] = b;
@@ -686,8 +715,16 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
         }
         if (!more) break;
         tk = tn;
+        // the next tile's columns (its left halo from this tile when it is x + 1: Hl[par]
+        // is written again two tiles on, after the next tile's barriers)
+        if (next_x && hdst) {
 #pragma unroll
-        for (int rz = 0; rz < V3_RZ; ++rz) v[rz] = vn[rz];
+            for (int rz = 0; rz < V3_RZ; ++rz) v[rz] = Hl[par][hidx + rz];
+        } else {
+#pragma unroll
+            for (int rz = 0; rz < V3_RZ; ++rz) v[rz] = vn[rz];
+        }
+        par ^= 1;
     }
     if (tid == 0 && acc) atomicOr(&cur[run.j], (uint32_t)acc);
     vor_count_tiles(S, done);
