@@ -499,6 +499,17 @@ def statistics_table(intensity_image: Array, label_image: Array, num_labels: int
     raise RuntimeError("statistics table could not be sized")
 
 
+def _table_to_host(t) -> np.ndarray:
+    """The (columns, labels) table in host memory: one copy into page-locked memory
+    (torch's host cache reuses it across calls) instead of the chunked pageable path."""
+    if t.numel() == 0 or not t.is_cuda:
+        return t.cpu().numpy()
+    pinned = torch.empty(tuple(t.shape), dtype=t.dtype, pin_memory=True)
+    pinned.copy_(t, non_blocking=True)
+    torch.cuda.current_stream(t.device).synchronize()
+    return pinned.numpy()
+
+
 @plugin_function(output_creator=create_none, categories=['bia-bob-suggestion'])
 def statistics_of_labelled_pixels(intensity_image: Image = None, label_image: Image = None):
     """_tier9/_statistics_of_labelled_pixels.py:7-293: dict of numpy float32 arrays
@@ -513,7 +524,7 @@ def statistics_of_labelled_pixels(intensity_image: Image = None, label_image: Im
         intensity_image = label_image
     table, n = statistics_table(intensity_image, label_image)
     num_labels = n + 1
-    host = table.tensor[:, :n].cpu().numpy()
+    host = _table_to_host(table.tensor[:, :n])
     cols = {k: host[i].copy() for i, k in enumerate(_TABLE_COLUMNS)}
     out = {}
     for k in STATISTICS_KEYS:

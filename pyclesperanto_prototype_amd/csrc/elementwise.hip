@@ -115,6 +115,12 @@ __global__ __launch_bounds__(256) void k_maxima_t(LD in, DBuf d, Geom g) {
     // consecutive tiles (which share halo lines) on one XCD
     maxima_tile(in, g, xcd_tile(blockIdx.x, gridDim.x), ring, [&](int64_t i, bool m) { st(d, i, m ? 1.f : 0.f); });
 }
+// uint8 flags (the pipelines' spots)
+template <typename LD>
+__global__ __launch_bounds__(256) void k_maxima_u8(LD in, uint8_t* __restrict__ d, Geom g) {
+    __shared__ float ring[MX_LDS];
+    maxima_tile(in, g, xcd_tile(blockIdx.x, gridDim.x), ring, [&](int64_t i, bool m) { d[i] = m ? 1 : 0; });
+}
 
 struct LdAnyE {
     DBuf b;
@@ -128,7 +134,9 @@ struct LdF32E {
 int maxima_impl(DBuf src, DBuf dst, const Geom& g, hipStream_t s) {
     if (g.n() == 0) return CLE_OK;
     const unsigned grid = (unsigned)maxima_tiles(g);
-    if (src.dt == CLE_F32)
+    if (src.dt == CLE_F32 && dst.dt == CLE_U8)
+        k_maxima_u8<<<grid, 256, 0, s>>>(LdF32E{(const float*)src.p}, (uint8_t*)dst.p, g);
+    else if (src.dt == CLE_F32)
         k_maxima_t<<<grid, 256, 0, s>>>(LdF32E{(const float*)src.p}, dst, g);
     else
         k_maxima_t<<<grid, 256, 0, s>>>(LdAnyE{src}, dst, g);
