@@ -552,6 +552,7 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
     __shared__ float Yl[V3_NB * V3_COLS];
     __shared__ uint8_t Fl[V3_COLS];  // the column holds a zero among the box planes
     __shared__ float Hl[2][2 * V3_RY * V3_RZ];  // body columns 14, 15: the next tile's left halo
+    __shared__ float Rl[2][2 * V3_RY * V3_RZ];  // body columns 0, 1: the previous tile's right halo
     __shared__ int sbits[2];
     __shared__ int zf[2];
     VorRun run;
@@ -579,23 +580,47 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
     const bool ybox = col && ry >= 1 && ry <= V3_RY - 2;
     const bool ytile = col && ry >= 2 && ry < V3_RY - 2 && yr < g.H;
     const bool xbox = rx >= 1 && rx <= V3_RX - 2;
-    // When the next active tile is x + 1, its left halo columns (region x 0, 1) are this
-    // tile's body columns 14, 15 (region x 16, 17), passed through LDS instead of
-    // loaded: they lie in the 64-byte line before the next tile's body.
+    // Halo columns shared by neighbouring active tiles of the run go through LDS instead
+    // of being loaded again (they lie in the 64-byte lines of the neighbour's body): the
+    // left halo (region x 0, 1) of tile t is tile t - 1's body columns 14, 15 (region x
+    // 16, 17) when t - 1 is active, the right halo (region x 18, 19) is tile t + 1's body
+    // columns 0, 1 (region x 2, 3) when t + 1 is active.  Regions load two active tiles
+    // ahead, so both neighbours' bodies are in registers when a tile's halo is passed on.
+    const uint64_t run_act = run.act;
+    auto active = [&](int t) { return t >= 0 && t < 64 && ((run_act >> t) & 1ull) != 0; };
     const bool hsrc = col && (rx == V3_RX - 4 || rx == V3_RX - 3);
     const bool hdst = col && rx < 2;
+    const bool rsrc = col && (rx == 2 || rx == 3);
+    const bool rdst = col && rx >= V3_RX - 2;
     const int hidx = ((rx & 1) * V3_RY + ry) * V3_RZ;
-    auto load = [&](int tx, float* v) {
-        const int64_t xr = (int64_t)tx * 16 - 2 + rx;
+    const int64_t xs16 = (int64_t)run.xs * 16;  // image x of the run's first tile
+    // region columns of tile t (of the run), but the halo columns that come through LDS
+    auto load = [&](int t, float* v) {
+        if ((hdst && active(t - 1)) || (rdst && active(t + 1))) return;
+        const int64_t xr = xs16 + (int64_t)t * 16 - 2 + rx;
         const float* cp = src + yy * g.W + (xr < 0 ? 0 : (xr >= g.W ? g.W - 1 : xr));
 #pragma unroll
         for (int rz = 0; rz < V3_RZ; ++rz) v[rz] = col ? cp[zoff[rz]] : 0.f;
     };
     uint64_t act = run.act;
-    int tk = __builtin_ctzll(act);
-    act &= act - 1;
-    float v[V3_RZ], vn[V3_RZ];
-    load(run.xs + tk, v);
+    auto next_tile = [&]() {
+        if (!act) return -1;
+        const int t = __builtin_ctzll(act);
+        act &= act - 1;
+        return t;
+    };
+    int tk = next_tile();
+    int tn = next_tile();
+    int tn2 = next_tile();
+    float v[V3_RZ], vn[V3_RZ], vnn[V3_RZ];
+    {
+        // the first tile's region in full
+        const int64_t xr = xs16 + (int64_t)tk * 16 - 2 + rx;
+        const float* cp = src + yy * g.W + (xr < 0 ? 0 : (xr >= g.W ? g.W - 1 : xr));
+#pragma unroll
+        for (int rz = 0; rz < V3_RZ; ++rz) v[rz] = col ? cp[zoff[rz]] : 0.f;
+    }
+    if (tn >= 0) load(tn, vn);
     // tile planes inside the image: q = 1 .. nzin (region plane q + 1; all 8 but in the
     // last tile plane)
     const int nzin = (int)(g.D - z0 - 2 < 8 ? g.D - z0 - 2 : 8);
@@ -607,8 +632,8 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
 #pragma unroll 1
     for (;;) {
         ++done;
-        const int tx = run.xs + tk;
-        const int64_t xr = (int64_t)tx * 16 - 2 + rx;
+        if (tn2 >= 0) load(tn2, vnn);  // two tiles ahead
+        const int64_t xr = xs16 + (int64_t)tk * 16 - 2 + rx;
         // tile voxel columns inside the image
         const bool tcol = ytile && rx >= 2 && rx < V3_RX - 2 && xr < g.W;
         if (tid == 0) sbits[par] = 0;  // last read two tiles ago, before a barrier
@@ -626,15 +651,6 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
         // the tile holds a zero voxel: one flag word per tile parity, set by the waves
         // before the barrier below and cleared for the next tile before this tile's last
         if (__ballot(zero && tcol) != 0 && (tid & 63) == 0) zf[par] = 1;
-        const bool more = act != 0;
-        const int tn = more ? __builtin_ctzll(act) : tk;
-        act &= act - 1;
-        const bool next_x = more && tn == tk + 1;
-        if (next_x && hsrc) {
-#pragma unroll
-            for (int rz = 0; rz < V3_RZ; ++rz) Hl[par][hidx + rz] = v[rz];
-        }
-        if (more && !(next_x && hdst)) load(run.xs + tn, vn);
         const int64_t obase = yr * g.W + xr;  // output column offset
         __syncthreads();
         if (zf[par] == 0) {
@@ -702,6 +718,16 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
             }
             if (bits) atomicOr(&sbits[par], bits);
         }
+        // the next tile's halo columns from this tile and from the tile after it
+        const bool nl = tn >= 0 && tn == tk + 1, nr = tn >= 0 && tn2 == tn + 1;
+        if (nl && hsrc) {
+#pragma unroll
+            for (int rz = 0; rz < V3_RZ; ++rz) Hl[par][hidx + rz] = v[rz];
+        }
+        if (nr && rsrc) {
+#pragma unroll
+            for (int rz = 0; rz < V3_RZ; ++rz) Rl[par][hidx + rz] = vnn[rz];
+        }
         if (tid == 0) zf[par ^ 1] = 0;  // last read before the previous tile's last barrier
         // every thread is past its LDS reads of this tile before the next tile's C
         // writes; the block's change bits
@@ -713,17 +739,24 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
         } else if (!(tb & 4) && tid == 0) {
             L.settled[run.t0 + tk] = 1;
         }
-        if (!more) break;
-        tk = tn;
-        // the next tile's columns (its left halo from this tile when it is x + 1: Hl[par]
-        // is written again two tiles on, after the next tile's barriers)
-        if (next_x && hdst) {
+        if (tn < 0) break;
+        // on to the next tile (Hl[par] / Rl[par] are written again two tiles on, after
+        // the next tile's barriers)
+        if (nl && hdst) {
 #pragma unroll
             for (int rz = 0; rz < V3_RZ; ++rz) v[rz] = Hl[par][hidx + rz];
+        } else if (nr && rdst) {
+#pragma unroll
+            for (int rz = 0; rz < V3_RZ; ++rz) v[rz] = Rl[par][hidx + rz];
         } else {
 #pragma unroll
             for (int rz = 0; rz < V3_RZ; ++rz) v[rz] = vn[rz];
         }
+#pragma unroll
+        for (int rz = 0; rz < V3_RZ; ++rz) vn[rz] = vnn[rz];
+        tk = tn;
+        tn = tn2;
+        tn2 = next_tile();
         par ^= 1;
     }
     if (tid == 0 && acc) atomicOr(&cur[run.j], (uint32_t)acc);
