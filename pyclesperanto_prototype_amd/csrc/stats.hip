@@ -245,6 +245,143 @@ __global__ __launch_bounds__(RB_THREADS) void k_runs(LAB lab, Geom g, int64_t lc
     }
 }
 
+// The same runs with the 16-byte layout as 16 consecutive voxels per lane (x = sx0 + 16 l
+// + e, e = 0..15; four 16-byte loads per lane, 64 bytes apart across lanes): starts and
+// ends are found inside the lane and numbered with one exclusive scan of each lane's
+// counts across the wavefront, instead of a ballot and two popcounts per voxel; a label
+// is measured by one unsigned compare.  Same runs, same slots and order as k_runs.
+// (Measured: 512x2048x2048 labels 2.65 -> 2.2 ms; looping over the set bits with a
+// label picked by a select tree instead of the unrolled per-voxel stores was slower.)
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t c, int lane) {
+    uint32_t x = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    return x - c;
+}
+
+template <typename LAB>
+__global__ __launch_bounds__(RB_THREADS) void k_runs_lane(LAB lab, Geom g, int64_t lcap, KeyBits kb, uint32_t scap,
+                                                          const uint32_t* off, uint32_t* count, uint64_t* keys,
+                                                          uint32_t* ends, uint32_t* bmax) {
+    constexpr int NV = RB_SPAN / 64;  // voxels per lane
+    using V4 = typename Vec4<typename LAB::type>::V;
+    const int64_t spans = (g.W + RB_SPAN - 1) / RB_SPAN;
+    const int64_t total = g.D * g.H * spans;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    int lmax = 0;
+    uint32_t cmax = 0;
+    // measured labels 1 .. lcap - 1 as one unsigned compare of l - 1
+    const uint32_t ucap = (uint32_t)(lcap - 1 < (int64_t)0x80000000 ? (lcap - 1 > 0 ? lcap - 1 : 0) : 0x80000000);
+    typename LAB::raw_t nr[NV];  // the next span's labels, loaded while this span is processed
+    auto span_of = [&](int64_t sp, int64_t& row, int64_t& sx0, int& n) {
+        row = sp / spans;
+        sx0 = (sp - row * spans) * RB_SPAN;
+        n = (int)((sx0 + RB_SPAN < g.W ? sx0 + RB_SPAN : g.W) - sx0);
+    };
+    // lanes past the row end load voxel 0 of the span (masked below); n is a multiple of 4
+    auto fetch = [&](int64_t base, int n) {
+#pragma unroll
+        for (int k = 0; k < NV / 4; ++k) {
+            const int i0 = NV * lane + 4 * k;
+            const V4 q = *(const V4*)(lab.p + base + (i0 < n ? i0 : 0));
+            nr[4 * k + 0] = q.x; nr[4 * k + 1] = q.y; nr[4 * k + 2] = q.z; nr[4 * k + 3] = q.w;
+        }
+    };
+    if (wave0 < total) {
+        int64_t row, sx0;
+        int n;
+        span_of(wave0, row, sx0, n);
+        fetch(row * g.W + sx0, n);
+    }
+    for (int64_t sp = wave0; sp < total; sp += nwaves) {
+        int64_t row, sx0;
+        int n;
+        span_of(sp, row, sx0, n);
+        const int i0 = NV * lane;  // the lane's first voxel in the span
+        int v[NV];
+#pragma unroll
+        for (int e = 0; e < NV; ++e) v[e] = i0 + e < n ? LAB::cvt(nr[e]) : 0;
+        const uint64_t slot0 = off ? (uint64_t)off[sp] : (uint64_t)sp * scap;
+        if (sp + nwaves < total) {
+            int64_t row2, sx2;
+            int n2;
+            span_of(sp + nwaves, row2, sx2, n2);
+            fetch(row2 * g.W + sx2, n2);
+        }
+        bool any = false;
+        uint32_t S = 0, E = 0;  // bit e: voxel i0 + e starts / ends a run
+        const int left = __shfl_up(v[NV - 1], 1), right = __shfl_down(v[0], 1);
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            const int l = v[e];
+            const bool valid = (uint32_t)l - 1u < ucap;  // 0 < l < lcap; voxels past n hold 0
+            any |= valid;
+            lmax = valid && l > lmax ? l : lmax;
+            const int lf = e > 0 ? v[e - 1] : (lane > 0 ? left : 0);
+            const int rt = e < NV - 1 ? v[e + 1] : (lane < 63 ? right : 0);
+            const int i = i0 + e;
+            S |= (valid && (i == 0 || lf != l)) ? 1u << e : 0u;
+            E |= (valid && (i == n - 1 || rt != l)) ? 1u << e : 0u;
+        }
+        if (!__any(any)) {
+            if (lane == 0) count[sp] = 0;
+            continue;
+        }
+        const uint32_t lim = off ? 0xFFFFFFFFu : scap;
+        // run_key(l, y, z, x) = l << (yb + zb + xb) | ((y << zb | z) << xb) | x
+        const uint64_t yz = (((uint64_t)(row % g.H) << kb.zb) | (uint64_t)(row / g.H)) << kb.xb;
+        const int lsh = kb.yb + kb.zb + kb.xb;
+        const uint32_t cs = (uint32_t)__popc(S);
+        uint32_t ps = wave_excl_scan(cs, lane), pe = wave_excl_scan((uint32_t)__popc(E), lane);
+        const uint32_t ns = __shfl(ps + cs, 63);
+        uint64_t* kp = keys + slot0 + ps;
+        uint32_t* ep = ends + slot0 + pe;
+        const uint32_t x0 = (uint32_t)(sx0 + i0);
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            if (S & (1u << e)) {
+                if (ps < lim) *kp = ((uint64_t)(uint32_t)v[e] << lsh) | yz | (uint64_t)(x0 + e);
+                ++kp;
+                ++ps;
+            }
+            if (E & (1u << e)) {
+                if (pe < lim) *ep = x0 + e + 1;
+                ++ep;
+                ++pe;
+            }
+        }
+        cmax = ns > cmax ? ns : cmax;
+        if (lane == 0) count[sp] = ns;
+    }
+    // block maxima of (span count, label), one store per block
+    __shared__ uint32_t red[2][RB_THREADS / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        const int a = __shfl_xor(lmax, o);
+        const uint32_t c = __shfl_xor(cmax, o);
+        lmax = a > lmax ? a : lmax;
+        cmax = c > cmax ? c : cmax;
+    }
+    if (lane == 0) {
+        red[0][threadIdx.x >> 6] = cmax;
+        red[1][threadIdx.x >> 6] = (uint32_t)lmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t c = 0, l = 0;
+        for (int w = 0; w < RB_THREADS / 64; ++w) {
+            c = red[0][w] > c ? red[0][w] : c;
+            l = red[1][w] > l ? red[1][w] : l;
+        }
+        bmax[2 * blockIdx.x] = c;
+        bmax[2 * blockIdx.x + 1] = l;
+    }
+}
+
 // sum[0] = runs (off[last] + count[last]), sum[1] = largest span count, sum[2] =
 // largest measured label (one block)
 __global__ __launch_bounds__(1024) void k_runs_summary(const uint32_t* count, const uint32_t* off, int64_t spans,
@@ -613,8 +750,8 @@ static int stats_run(DBuf inten, LAB lab, const Geom& g, int64_t L, float* table
     auto runs = [&](const uint32_t* o, uint64_t* kk, uint32_t* ee) {
         if constexpr (kVecLab<LAB>) {
             if ((g.W & 3) == 0 && ((uintptr_t)lab.p & 15) == 0) {
-                k_runs<true><<<gr, RB_THREADS, 0, s>>>(lab, g, lcap, kb, scap, o, cnt.get<uint32_t>(), kk, ee,
-                                                       bmx.get<uint32_t>());
+                k_runs_lane<<<gr, RB_THREADS, 0, s>>>(lab, g, lcap, kb, scap, o, cnt.get<uint32_t>(), kk, ee,
+                                                      bmx.get<uint32_t>());
                 return;
             }
         }
