@@ -446,6 +446,9 @@ struct GrowInitOut {
     uint8_t* settled;
     int tx, ty, tz;
     int64_t nx, ny;
+    // 3D tiles whose settled flag is 2 hold no mask voxel (flagged before this pass):
+    // their state, -1 everywhere, is not written (the growth reads it as -1)
+    bool skip_empty = false;
 };
 int ccl_box_growth_init(DBuf binary, const Geom& g, const GrowInitOut& out, hipStream_t s);
 // single-image voronoi_otsu_labeling (the body of cle_voronoi_otsu_labeling)

@@ -499,7 +499,11 @@ struct StGrowInit {
     __device__ __forceinline__ int64_t tile(int64_t x, int64_t y, int64_t z) const {
         return ((z / o.tz) * o.ny + y / o.ty) * o.nx + x / o.tx;
     }
+    __device__ __forceinline__ bool empty(int64_t x, int64_t y, int64_t z) const {
+        return o.skip_empty && o.settled[((z >> 3) * o.ny + (y >> 3)) * o.nx + (x >> 4)] == 2;
+    }
     __device__ __forceinline__ void put(int64_t i, int64_t x, int64_t y, int64_t z, float v) const {
+        if (empty(x, y, z)) return;
         const uint8_t m = o.mask[i];
         const float w = ((float)m - 1.f) + v;
         o.A[i] = w;
@@ -511,6 +515,7 @@ struct StGrowInit {
     // multiple of 4)
     __device__ __forceinline__ void put4(int64_t i, int64_t x, int64_t y, int64_t z, float a, float b, float c,
                                          float d) const {
+        if (empty(x, y, z)) return;
         const uchar4 m = *(const uchar4*)(o.mask + i);
         const float4 w = make_float4(((float)m.x - 1.f) + a, ((float)m.y - 1.f) + b, ((float)m.z - 1.f) + c,
                                      ((float)m.w - 1.f) + d);

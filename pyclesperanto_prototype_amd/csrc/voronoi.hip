@@ -85,8 +85,10 @@ constexpr int VOR_MAX_IMAGES = 65535;
 struct VorLists {
     int32_t* mark;     // 2 x cap: mark[b & 1][t] == b iff tile t changed in pair b (tiles
                        // with a seed: mark[1][t] == -1, "changed" before pair 0)
-    uint8_t* settled;  // cap: the tile holds no zero and never changes again
+    uint8_t* settled;  // cap: the tile holds no zero and never changes again (1), or it
+                       // holds no mask voxel and its state is not stored (2, empty2 only)
     int64_t cap;       // tiles of all images
+    int empty2;        // 3D tiles flagged 2 are read as -1 (their buffers hold no state)
 };
 
 // Per-image state of the growth.
@@ -555,6 +557,7 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
     __shared__ float Rl[2][2 * V3_RY * V3_RZ];  // body columns 0, 1: the previous tile's right halo
     __shared__ int sbits[2];
     __shared__ int zf[2];
+    __shared__ uint64_t Em[9][2];  // empty2: the empty tiles of the 3 x 3 neighbour rows
     VorRun run;
     if (!vor_run_begin(imgs, rs, nimg, b, L, S, run)) return;
     uint32_t* cur = S.bits + (size_t)(b % 3) * nimg;
@@ -588,19 +591,81 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
     // ahead, so both neighbours' bodies are in registers when a tile's halo is passed on.
     const uint64_t run_act = run.act;
     auto active = [&](int t) { return t >= 0 && t < 64 && ((run_act >> t) & 1ull) != 0; };
+    // Tiles without a mask voxel (flag 2, empty2 only) store no state: their region
+    // voxels read as -1.  Em[(dz + 1) * 3 + dy + 1] bit c: the tile (x: xs - 1 + c) of the
+    // neighbour row (dy, dz) is empty; each thread's column lies in one neighbour row per
+    // class of region planes (0, 1 / 2 .. 9 / 10, 11: z tiles of the clamped planes).
+    const bool e2 = L.empty2 != 0;
+    int erow[3] = {4, 4, 4};
+    if (e2) {
+        if (tid < 64) {
+            constexpr int RW = VOR_RUN + 2;
+            constexpr int NWB = (9 * RW + 63) / 64;
+            const int nx = im.vnx, ny = im.vny, nz = im.vnz;
+            uint64_t w[NWB + 2];
+            w[NWB] = w[NWB + 1] = 0;
+#pragma unroll
+            for (int k = 0; k < NWB; ++k) {
+                const int idx = tid + 64 * k;
+                const int rr = idx / RW, cx = idx - rr * RW;
+                const int qx = run.xs - 1 + cx, qy = run.ty + rr % 3 - 1, qz = run.tz + rr / 3 - 1;
+                bool e = false;
+                if (idx < 9 * RW && cx < run.nrun + 2 && qx >= 0 && qx < nx && qy >= 0 && qy < ny && qz >= 0 && qz < nz)
+                    e = L.settled[im.tbase + ((int64_t)qz * ny + qy) * nx + qx] == 2;
+                w[k] = __ballot(e);
+            }
+            if (tid == 0) {
+#pragma unroll
+                for (int rr = 0; rr < 9; ++rr) {
+                    const int o = rr * RW, wi = o >> 6, sh = o & 63;
+                    uint64_t lo = w[wi] >> sh, hi;
+                    if (sh) {
+                        lo |= w[wi + 1] << (64 - sh);
+                        hi = w[wi + 1] >> sh | w[wi + 2] << (64 - sh);
+                    } else {
+                        hi = w[wi + 1];
+                    }
+                    if (RW < 64) lo &= (1ull << (RW < 64 ? RW : 0)) - 1;
+                    hi &= RW > 64 ? ((1ull << (RW > 64 ? RW - 64 : 0)) - 1) : 0ull;
+                    Em[rr][0] = lo;
+                    Em[rr][1] = hi;
+                }
+            }
+        }
+        // neighbour rows of this thread's column: its y tile and the z tiles of the
+        // clamped planes 0, 2 and 10
+        const int64_t ycl = yy >> 3;
+        const int dy = (int)(ycl - run.ty);
+        const int64_t zc0 = z0 < 0 ? 0 : z0, zc10 = z0 + 10 >= g.D ? g.D - 1 : z0 + 10;
+        erow[0] = (int)((zc0 >> 3) - run.tz + 1) * 3 + dy + 1;
+        erow[1] = 3 + dy + 1;
+        erow[2] = (int)((zc10 >> 3) - run.tz + 1) * 3 + dy + 1;
+        __syncthreads();
+    }
     const bool hsrc = col && (rx == V3_RX - 4 || rx == V3_RX - 3);
     const bool hdst = col && rx < 2;
     const bool rsrc = col && (rx == 2 || rx == 3);
     const bool rdst = col && rx >= V3_RX - 2;
     const int hidx = ((rx & 1) * V3_RY + ry) * V3_RZ;
     const int64_t xs16 = (int64_t)run.xs * 16;  // image x of the run's first tile
-    // region columns of tile t (of the run), but the halo columns that come through LDS
-    auto load = [&](int t, float* v) {
-        if ((hdst && active(t - 1)) || (rdst && active(t + 1))) return;
+    // region columns of tile t (of the run), but (unless full) the halo columns that
+    // come through LDS
+    auto load = [&](int t, float* v, bool full) {
+        if (!full && ((hdst && active(t - 1)) || (rdst && active(t + 1)))) return;
         const int64_t xr = xs16 + (int64_t)t * 16 - 2 + rx;
-        const float* cp = src + yy * g.W + (xr < 0 ? 0 : (xr >= g.W ? g.W - 1 : xr));
+        const int64_t xc = xr < 0 ? 0 : (xr >= g.W ? g.W - 1 : xr);
+        const float* cp = src + yy * g.W + xc;
+        bool em[3] = {false, false, false};
+        if (e2) {
+            const int c = (int)((xc >> 4) - (run.xs - 1));  // 0 .. nrun + 1
 #pragma unroll
-        for (int rz = 0; rz < V3_RZ; ++rz) v[rz] = col ? cp[zoff[rz]] : 0.f;
+            for (int k = 0; k < 3; ++k) em[k] = ((Em[erow[k]][c >> 6] >> (c & 63)) & 1ull) != 0;
+        }
+#pragma unroll
+        for (int rz = 0; rz < V3_RZ; ++rz) {
+            const bool e = em[rz < 2 ? 0 : (rz < V3_RZ - 2 ? 1 : 2)];
+            v[rz] = !col ? 0.f : (e ? -1.f : cp[zoff[rz]]);
+        }
     };
     uint64_t act = run.act;
     auto next_tile = [&]() {
@@ -613,14 +678,8 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
     int tn = next_tile();
     int tn2 = next_tile();
     float v[V3_RZ], vn[V3_RZ], vnn[V3_RZ];
-    {
-        // the first tile's region in full
-        const int64_t xr = xs16 + (int64_t)tk * 16 - 2 + rx;
-        const float* cp = src + yy * g.W + (xr < 0 ? 0 : (xr >= g.W ? g.W - 1 : xr));
-#pragma unroll
-        for (int rz = 0; rz < V3_RZ; ++rz) v[rz] = col ? cp[zoff[rz]] : 0.f;
-    }
-    if (tn >= 0) load(tn, vn);
+    load(tk, v, true);  // the first tile's region in full
+    if (tn >= 0) load(tn, vn, false);
     // tile planes inside the image: q = 1 .. nzin (region plane q + 1; all 8 but in the
     // last tile plane)
     const int nzin = (int)(g.D - z0 - 2 < 8 ? g.D - z0 - 2 : 8);
@@ -632,7 +691,7 @@ __global__ __launch_bounds__(VB, CLE_VOR3_BLOCKS) void k_vor_pair_run3(const BIm
 #pragma unroll 1
     for (;;) {
         ++done;
-        if (tn2 >= 0) load(tn2, vnn);  // two tiles ahead
+        if (tn2 >= 0) load(tn2, vnn, false);  // two tiles ahead
         const int64_t xr = xs16 + (int64_t)tk * 16 - 2 + rx;
         // tile voxel columns inside the image
         const bool tcol = ytile && rx >= 2 && rx < V3_RX - 2 && xr < g.W;
@@ -796,7 +855,8 @@ struct VorScratch {
         return (size_t)cap * 8 + 64 + (size_t)VOR_NCTR * 64 + (size_t)nimg * 16 + (size_t)cap + 4;
     }
     // settled_init 1: every tile starts settled (a fused growth init clears the tiles
-    // holding a mask voxel)
+    // holding a mask voxel); 2: every tile starts settled and empty (the segmentation
+    // pass clears the tiles holding a mask voxel; the others' state is never stored)
     int alloc(int64_t ntiles, int nimg, hipStream_t s, uint32_t settled_init = 0) {
         const int64_t cap = ntiles > 0 ? ntiles : 1;
         CLE_TRY(mem.alloc(bytes(ntiles, nimg), s));
@@ -812,11 +872,12 @@ struct VorScratch {
         p += (size_t)nimg * 16;
         L.settled = (uint8_t*)p;
         L.cap = cap;
+        L.empty2 = settled_init == 2 ? 1 : 0;
         // change stamps -2 (no pair); words, counters and per-image state 0 (contiguous
         // from word); nothing settled yet
         CLE_TRY(fill_u32((uint32_t*)L.mark, 0xFFFFFFFEu, (size_t)cap * 2, s));
         CLE_TRY(fill_u32(S.word, 0u, 16 + (size_t)VOR_NCTR * 16 + (size_t)nimg * 4, s));
-        CLE_TRY(fill_u32((uint32_t*)L.settled, settled_init ? 0x01010101u : 0u, ((size_t)cap + 3) / 4, s));
+        CLE_TRY(fill_u32((uint32_t*)L.settled, settled_init * 0x01010101u, ((size_t)cap + 3) / 4, s));
         return CLE_OK;
     }
 };
@@ -1011,6 +1072,60 @@ __global__ void k_seg_and(const float* blurred, const double* thr, uint8_t* spot
     }
 }
 
+// k_clip_out_32 and k_seg_and over row chunks of 1024 voxels (rows of a multiple of 4
+// voxels) for a 3D growth whose empty tiles store no state: the segmentation pass
+// clears the settled flag (2: empty) of every growth tile holding a segmentation voxel,
+// the clip writes 0 for the voxels of the tiles still flagged empty without reading A.
+struct RowChunks {
+    Geom g;
+    int64_t cpr;       // chunks per row
+    int64_t tnx, tny;  // growth tiles (16 x 8 x 8) per row / column
+    __host__ __device__ int64_t count() const { return g.D * g.H * cpr; }
+};
+__global__ __launch_bounds__(256) void k_seg_and_tiles(const float* blurred, const double* thr, uint8_t* spots,
+                                                       uint8_t* seg, RowChunks rc, uint8_t* settled) {
+    const float t = (float)thr[0];
+    const Geom& g = rc.g;
+    for (int64_t b = blockIdx.x; b < rc.count(); b += gridDim.x) {
+        const int64_t row = b / rc.cpr;
+        const int64_t x = (b - row * rc.cpr) * 1024 + 4 * threadIdx.x;
+        if (x >= g.W) continue;
+        const int64_t q = (row * g.W + x) >> 2;
+        const float4 v = ((const float4*)blurred)[q];
+        const uchar4 sp = ((const uchar4*)spots)[q];
+        const uchar4 m = make_uchar4(v.x > t ? 1 : 0, v.y > t ? 1 : 0, v.z > t ? 1 : 0, v.w > t ? 1 : 0);
+        ((uchar4*)seg)[q] = m;
+        ((uchar4*)spots)[q] = make_uchar4((sp.x != 0 && m.x) ? 1 : 0, (sp.y != 0 && m.y) ? 1 : 0,
+                                          (sp.z != 0 && m.z) ? 1 : 0, (sp.w != 0 && m.w) ? 1 : 0);
+        if (m.x | m.y | m.z | m.w) {
+            const int64_t y = row % g.H, z = row / g.H;
+            settled[((z >> 3) * rc.tny + (y >> 3)) * rc.tnx + (x >> 4)] = 0;
+        }
+    }
+}
+__global__ __launch_bounds__(256) void k_clip_out_tiles(const float* __restrict__ A, DBuf dst, RowChunks rc,
+                                                        const uint8_t* __restrict__ settled) {
+    const Geom& g = rc.g;
+    const bool u = dst.dt == CLE_U32;
+    for (int64_t b = blockIdx.x; b < rc.count(); b += gridDim.x) {
+        const int64_t row = b / rc.cpr;
+        const int64_t x = (b - row * rc.cpr) * 1024 + 4 * threadIdx.x;
+        if (x >= g.W) continue;
+        const int64_t q = (row * g.W + x) >> 2;
+        const int64_t y = row % g.H, z = row / g.H;
+        uint4 o = make_uint4(0u, 0u, 0u, 0u);
+        if (settled[((z >> 3) * rc.tny + (y >> 3)) * rc.tnx + (x >> 4)] != 2) {
+            const float4 a = ((const float4*)A)[q];
+            const float av[4] = {fmaxf(a.x, 0.f), fmaxf(a.y, 0.f), fmaxf(a.z, 0.f), fmaxf(a.w, 0.f)};
+            uint32_t ov[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) ov[c] = u ? sat_cast<uint32_t>(av[c]) : (uint32_t)sat_cast<int32_t>(av[c]);
+            o = make_uint4(ov[0], ov[1], ov[2], ov[3]);
+        }
+        ((uint4*)dst.p)[q] = o;
+    }
+}
+
 __global__ void k_threshold_u8(const float* blurred, const double* thr, uint8_t* seg, int64_t n) {
     const float t = (float)thr[0];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1033,20 +1148,23 @@ static int g_growth_method = 1;  // 0: tile solves, 1: pairs of steps
 // (ccl_box_growth_init writes v = (mask - 1) + label into A and B and stamps the
 // seed tiles): no separate init pass.  Returns the buffer holding the result.
 static int grow_masked_fused(DBuf binary, const uint8_t* mask, float* A, float* B, const Geom& g,
-                             int* n_iterations, const float** result, hipStream_t s) {
+                             int* n_iterations, const float** result, hipStream_t s, VorScratch* pre = nullptr) {
     const Tiles T = make_tiles(g);
     const int64_t ntiles = T.count();
     BImg im;
     memset(&im, 0, sizeof(im));
     im.g = g;
     CLE_TRY(vor_set_tiles(im, false));
-    VorScratch V;
-    CLE_TRY(V.alloc(ntiles, 1, s, 1));  // the CCL write pass clears the tiles with a mask voxel
+    // pre: tiles flagged empty by the segmentation pass (their state is not stored)
+    VorScratch own;
+    VorScratch& V = pre ? *pre : own;
+    if (!pre) CLE_TRY(V.alloc(ntiles, 1, s, 1));  // the CCL write pass clears the tiles with a mask voxel
     Scratch tab;
     CLE_TRY(tab.alloc(sizeof(BImg), s));
     k_put_img<<<1, 1, 0, s>>>(tab.get<BImg>(), im);
     CLE_LAUNCHED();
-    const GrowInitOut o{A, B, mask, V.L.mark + V.L.cap, V.L.settled, T.tx, T.ty, T.tz, T.nx, T.ny};
+    GrowInitOut o{A, B, mask, V.L.mark + V.L.cap, V.L.settled, T.tx, T.ty, T.tz, T.nx, T.ny};
+    o.skip_empty = pre != nullptr;
     CLE_TRY(ccl_box_growth_init(binary, g, o, s));
     int iters = 0;
     CLE_TRY(grow_pairs(tab.get<BImg>(), 1, A, B, V.L, V.S, vor_runs_single(g), 0, &iters, s));
@@ -1059,7 +1177,8 @@ static int grow_masked_fused(DBuf binary, const uint8_t* mask, float* A, float* 
 // CCL), grow inside the mask, mask the result into dst.  B: a float32 work buffer of
 // the image's size.
 static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, float* B, int* n_iterations,
-                               hipStream_t s, bool seeds_in_mask = false) {
+                               hipStream_t s, bool seeds_in_mask = false, VorScratch* pre = nullptr,
+                               const RowChunks* rc = nullptr) {
     const int64_t n = g.n();
     bool labelled = false;  // B holds the CCL labels
     // (the tile solves' stores are not part of the fixed-region bound: pairs there)
@@ -1076,8 +1195,10 @@ static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, 
     const float* res = nullptr;
     const bool u8mask = mask.dt == CLE_U8 && ((uintptr_t)mask.p & 3) == 0 && ((uintptr_t)A & 15) == 0 &&
                         ((uintptr_t)B & 15) == 0;
+    bool empty_tiles = false;  // the growth stored no state for the tiles flagged empty
     if (u8mask && !labelled) {
-        CLE_TRY(grow_masked_fused(binary, (const uint8_t*)mask.p, A, B, g, n_iterations, &res, s));
+        CLE_TRY(grow_masked_fused(binary, (const uint8_t*)mask.p, A, B, g, n_iterations, &res, s, pre));
+        empty_tiles = pre != nullptr;
     } else {
         if (!labelled) CLE_TRY(ccl_impl(binary, g, DBuf{B, CLE_F32}, CLE_CONN_BOX, CLE_FP32_KEY_COMPAT, nullptr, s));
         // a uint8 mask and 16-byte aligned buffers take the four-voxel init
@@ -1086,7 +1207,9 @@ static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, 
         else
             CLE_TRY(grow_single(MaskedAny{mask, B}, A, B, g, n_iterations, &res, s));
     }
-    if (seeds_in_mask && (dst.dt == CLE_U32 || dst.dt == CLE_I32) && ((uintptr_t)res & 15) == 0 &&
+    if (empty_tiles)
+        k_clip_out_tiles<<<vg(rc->count()), 256, 0, s>>>(res, dst, *rc, pre->L.settled);
+    else if (seeds_in_mask && (dst.dt == CLE_U32 || dst.dt == CLE_I32) && ((uintptr_t)res & 15) == 0 &&
         ((uintptr_t)dst.p & 15) == 0)
         k_clip_out_32<<<vg(n / 4 + 1), 256, 0, s>>>(res, dst, n);
     else if (mask.dt == CLE_U8 && (dst.dt == CLE_U32 || dst.dt == CLE_I32) && ((uintptr_t)res & 15) == 0 &&
@@ -1148,12 +1271,30 @@ int voronoi_otsu_single(const cle_buf* src, const cle_buf* dst, float spot_sigma
     } else {
         CLE_TRY(otsu_impl(DBuf{blurred.p, CLE_F32}, g, mm, thr, s));
     }
-    k_seg_and<<<vg(n / 4 + 1), 256, 0, s>>>(blurred.get<float>(), thr, spots.get<uint8_t>(), seg.get<uint8_t>(), n);
+    // 3D images with rows of a multiple of 4 voxels, grown by pairs of steps into 32-bit
+    // labels: the segmentation pass flags the growth tiles without a mask voxel, whose
+    // state (-1 everywhere) is then neither stored by the growth init nor read by the
+    // growth, and whose labels the final clip writes as 0 without a read
+    const bool empty_tiles = vor_3d(g) && (g.W & 3) == 0 && (g_growth_method != 0 || scratch_fixed()) &&
+                             (dst->dtype == CLE_U32 || dst->dtype == CLE_I32) && ((uintptr_t)dst->data & 15) == 0;
+    VorScratch V;
+    RowChunks rc{g, (g.W + 1023) / 1024, 0, 0};
+    if (empty_tiles) {
+        const Tiles T = make_tiles(g);
+        rc.tnx = T.nx;
+        rc.tny = T.ny;
+        CLE_TRY(V.alloc(T.count(), 1, s, 2));
+        k_seg_and_tiles<<<vg(rc.count()), 256, 0, s>>>(blurred.get<float>(), thr, spots.get<uint8_t>(),
+                                                        seg.get<uint8_t>(), rc, V.L.settled);
+    } else {
+        k_seg_and<<<vg(n / 4 + 1), 256, 0, s>>>(blurred.get<float>(), thr, spots.get<uint8_t>(), seg.get<uint8_t>(),
+                                                n);
+    }
     CLE_LAUNCHED();
     // masked Voronoi of the seeds inside the segmentation; the seed labels go to the
     // blurred buffer
     return masked_voronoi_core(DBuf{spots.p, CLE_U8}, DBuf{seg.p, CLE_U8}, dbuf(dst), g, blurred.get<float>(), nullptr,
-                               s, true);
+                               s, true, empty_tiles ? &V : nullptr, &rc);
 }
 
 // ----------------------------------------------------------------- batched pipeline
