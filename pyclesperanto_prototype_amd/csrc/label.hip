@@ -502,8 +502,8 @@ struct StGrowInit {
     __device__ __forceinline__ bool empty(int64_t x, int64_t y, int64_t z) const {
         return o.skip_empty && o.settled[((z >> 3) * o.ny + (y >> 3)) * o.nx + (x >> 4)] == 2;
     }
+    // (k_bccl_write tests empty() once for the voxels of a block pair before put / put4)
     __device__ __forceinline__ void put(int64_t i, int64_t x, int64_t y, int64_t z, float v) const {
-        if (empty(x, y, z)) return;
         const uint8_t m = o.mask[i];
         const float w = ((float)m - 1.f) + v;
         o.A[i] = w;
@@ -515,7 +515,6 @@ struct StGrowInit {
     // multiple of 4)
     __device__ __forceinline__ void put4(int64_t i, int64_t x, int64_t y, int64_t z, float a, float b, float c,
                                          float d) const {
-        if (empty(x, y, z)) return;
         const uchar4 m = *(const uchar4*)(o.mask + i);
         const float4 w = make_float4(((float)m.x - 1.f) + a, ((float)m.y - 1.f) + b, ((float)m.z - 1.f) + c,
                                      ((float)m.w - 1.f) + d);
@@ -897,6 +896,10 @@ __global__ void k_bccl_write(const uint8_t* __restrict__ M, Geom g, BGrid B, con
         const int nz = (BZ == 2 && z0 + 1 < g.D) ? 2 : 1, ny = y0 + 1 < g.H ? 2 : 1;
         for (int64_t p = threadIdx.x; p < np; p += blockDim.x) {
             const int64_t b = br * B.W + 2 * p, x = 4 * p;
+            // the pair's 4 x 2 x 2 voxels lie in one growth tile (x a multiple of 4, y0
+            // and z0 even): an empty one stores nothing
+            if constexpr (ST::kCoords)
+                if (out.empty(x, y0, z0)) continue;
             uint32_t m2;
             float f0, f1;
             if constexpr (VEC) {

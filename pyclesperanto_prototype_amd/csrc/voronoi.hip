@@ -1072,58 +1072,73 @@ __global__ void k_seg_and(const float* blurred, const double* thr, uint8_t* spot
     }
 }
 
-// k_clip_out_32 and k_seg_and over row chunks of 1024 voxels (rows of a multiple of 4
-// voxels) for a 3D growth whose empty tiles store no state: the segmentation pass
-// clears the settled flag (2: empty) of every growth tile holding a segmentation voxel,
-// the clip writes 0 for the voxels of the tiles still flagged empty without reading A.
-struct RowChunks {
+// k_clip_out_32 and k_seg_and over whole rows (of a multiple of 4 voxels) for a 3D
+// growth whose empty tiles store no state: the segmentation pass clears the settled
+// flag (2: empty) of every growth tile holding a segmentation voxel, the clip writes 0
+// for the voxels of the tiles still flagged empty without reading A.  A block takes a
+// range of consecutive rows (one division at its start; y and z then step along).
+struct RowTiles {
     Geom g;
-    int64_t cpr;       // chunks per row
     int64_t tnx, tny;  // growth tiles (16 x 8 x 8) per row / column
-    __host__ __device__ int64_t count() const { return g.D * g.H * cpr; }
+    int64_t rpb;       // rows per block
+    __host__ __device__ int64_t rows() const { return g.D * g.H; }
 };
-__global__ __launch_bounds__(256) void k_seg_and_tiles(const float* blurred, const double* thr, uint8_t* spots,
-                                                       uint8_t* seg, RowChunks rc, uint8_t* settled) {
-    const float t = (float)thr[0];
-    const Geom& g = rc.g;
-    for (int64_t b = blockIdx.x; b < rc.count(); b += gridDim.x) {
-        const int64_t row = b / rc.cpr;
-        const int64_t x = (b - row * rc.cpr) * 1024 + 4 * threadIdx.x;
-        if (x >= g.W) continue;
-        const int64_t q = (row * g.W + x) >> 2;
-        const float4 v = ((const float4*)blurred)[q];
-        const uchar4 sp = ((const uchar4*)spots)[q];
-        const uchar4 m = make_uchar4(v.x > t ? 1 : 0, v.y > t ? 1 : 0, v.z > t ? 1 : 0, v.w > t ? 1 : 0);
-        ((uchar4*)seg)[q] = m;
-        ((uchar4*)spots)[q] = make_uchar4((sp.x != 0 && m.x) ? 1 : 0, (sp.y != 0 && m.y) ? 1 : 0,
-                                          (sp.z != 0 && m.z) ? 1 : 0, (sp.w != 0 && m.w) ? 1 : 0);
-        if (m.x | m.y | m.z | m.w) {
-            const int64_t y = row % g.H, z = row / g.H;
-            settled[((z >> 3) * rc.tny + (y >> 3)) * rc.tnx + (x >> 4)] = 0;
+// the rows of this block: f(row, first tile of the row's tile row)
+template <typename F>
+__device__ __forceinline__ void for_block_rows(const RowTiles& rt, F f) {
+    const int64_t r0 = (int64_t)blockIdx.x * rt.rpb;
+    int64_t r1 = r0 + rt.rpb;
+    if (r1 > rt.rows()) r1 = rt.rows();
+    if (r0 >= r1) return;
+    int64_t z = r0 / rt.g.H, y = r0 - z * rt.g.H;
+    for (int64_t row = r0; row < r1; ++row) {
+        f(row, ((z >> 3) * rt.tny + (y >> 3)) * rt.tnx);
+        if (++y == rt.g.H) {
+            y = 0;
+            ++z;
         }
     }
 }
-__global__ __launch_bounds__(256) void k_clip_out_tiles(const float* __restrict__ A, DBuf dst, RowChunks rc,
-                                                        const uint8_t* __restrict__ settled) {
-    const Geom& g = rc.g;
-    const bool u = dst.dt == CLE_U32;
-    for (int64_t b = blockIdx.x; b < rc.count(); b += gridDim.x) {
-        const int64_t row = b / rc.cpr;
-        const int64_t x = (b - row * rc.cpr) * 1024 + 4 * threadIdx.x;
-        if (x >= g.W) continue;
-        const int64_t q = (row * g.W + x) >> 2;
-        const int64_t y = row % g.H, z = row / g.H;
-        uint4 o = make_uint4(0u, 0u, 0u, 0u);
-        if (settled[((z >> 3) * rc.tny + (y >> 3)) * rc.tnx + (x >> 4)] != 2) {
-            const float4 a = ((const float4*)A)[q];
-            const float av[4] = {fmaxf(a.x, 0.f), fmaxf(a.y, 0.f), fmaxf(a.z, 0.f), fmaxf(a.w, 0.f)};
-            uint32_t ov[4];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) ov[c] = u ? sat_cast<uint32_t>(av[c]) : (uint32_t)sat_cast<int32_t>(av[c]);
-            o = make_uint4(ov[0], ov[1], ov[2], ov[3]);
+__global__ __launch_bounds__(256) void k_seg_and_tiles(const float* blurred, const double* thr, uint8_t* spots,
+                                                       uint8_t* seg, RowTiles rt, uint8_t* settled) {
+    const float t = (float)thr[0];
+    const Geom& g = rt.g;
+    for_block_rows(rt, [&](int64_t row, int64_t tr) {
+        for (int64_t x = 4 * threadIdx.x; x < g.W; x += 4 * 256) {
+            const int64_t q = (row * g.W + x) >> 2;
+            const float4 v = ((const float4*)blurred)[q];
+            const uchar4 sp = ((const uchar4*)spots)[q];
+            const uchar4 m = make_uchar4(v.x > t ? 1 : 0, v.y > t ? 1 : 0, v.z > t ? 1 : 0, v.w > t ? 1 : 0);
+            ((uchar4*)seg)[q] = m;
+            ((uchar4*)spots)[q] = make_uchar4((sp.x != 0 && m.x) ? 1 : 0, (sp.y != 0 && m.y) ? 1 : 0,
+                                              (sp.z != 0 && m.z) ? 1 : 0, (sp.w != 0 && m.w) ? 1 : 0);
+            // the four lanes of a 16-voxel tile column: one store, when the flag is set
+            uint32_t any = (m.x | m.y | m.z | m.w) ? 1u : 0u;
+            any |= __shfl_xor(any, 1);
+            any |= __shfl_xor(any, 2);
+            if (any && (threadIdx.x & 3) == 0 && settled[tr + (x >> 4)] != 0) settled[tr + (x >> 4)] = 0;
         }
-        ((uint4*)dst.p)[q] = o;
-    }
+    });
+}
+__global__ __launch_bounds__(256) void k_clip_out_tiles(const float* __restrict__ A, DBuf dst, RowTiles rt,
+                                                        const uint8_t* __restrict__ settled) {
+    const Geom& g = rt.g;
+    const bool u = dst.dt == CLE_U32;
+    for_block_rows(rt, [&](int64_t row, int64_t tr) {
+        for (int64_t x = 4 * threadIdx.x; x < g.W; x += 4 * 256) {
+            const int64_t q = (row * g.W + x) >> 2;
+            uint4 o = make_uint4(0u, 0u, 0u, 0u);
+            if (settled[tr + (x >> 4)] != 2) {
+                const float4 a = ((const float4*)A)[q];
+                const float av[4] = {fmaxf(a.x, 0.f), fmaxf(a.y, 0.f), fmaxf(a.z, 0.f), fmaxf(a.w, 0.f)};
+                uint32_t ov[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) ov[c] = u ? sat_cast<uint32_t>(av[c]) : (uint32_t)sat_cast<int32_t>(av[c]);
+                o = make_uint4(ov[0], ov[1], ov[2], ov[3]);
+            }
+            ((uint4*)dst.p)[q] = o;
+        }
+    });
 }
 
 __global__ void k_threshold_u8(const float* blurred, const double* thr, uint8_t* seg, int64_t n) {
@@ -1178,7 +1193,7 @@ static int grow_masked_fused(DBuf binary, const uint8_t* mask, float* A, float* 
 // the image's size.
 static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, float* B, int* n_iterations,
                                hipStream_t s, bool seeds_in_mask = false, VorScratch* pre = nullptr,
-                               const RowChunks* rc = nullptr) {
+                               const RowTiles* rt = nullptr) {
     const int64_t n = g.n();
     bool labelled = false;  // B holds the CCL labels
     // (the tile solves' stores are not part of the fixed-region bound: pairs there)
@@ -1208,7 +1223,8 @@ static int masked_voronoi_core(DBuf binary, DBuf mask, DBuf dst, const Geom& g, 
             CLE_TRY(grow_single(MaskedAny{mask, B}, A, B, g, n_iterations, &res, s));
     }
     if (empty_tiles)
-        k_clip_out_tiles<<<vg(rc->count()), 256, 0, s>>>(res, dst, *rc, pre->L.settled);
+        k_clip_out_tiles<<<(unsigned)((rt->rows() + rt->rpb - 1) / rt->rpb), 256, 0, s>>>(res, dst, *rt,
+                                                                                          pre->L.settled);
     else if (seeds_in_mask && (dst.dt == CLE_U32 || dst.dt == CLE_I32) && ((uintptr_t)res & 15) == 0 &&
         ((uintptr_t)dst.p & 15) == 0)
         k_clip_out_32<<<vg(n / 4 + 1), 256, 0, s>>>(res, dst, n);
@@ -1278,14 +1294,17 @@ int voronoi_otsu_single(const cle_buf* src, const cle_buf* dst, float spot_sigma
     const bool empty_tiles = vor_3d(g) && (g.W & 3) == 0 && (g_growth_method != 0 || scratch_fixed()) &&
                              (dst->dtype == CLE_U32 || dst->dtype == CLE_I32) && ((uintptr_t)dst->data & 15) == 0;
     VorScratch V;
-    RowChunks rc{g, (g.W + 1023) / 1024, 0, 0};
+    RowTiles rt{g, 0, 0, 1};
     if (empty_tiles) {
         const Tiles T = make_tiles(g);
-        rc.tnx = T.nx;
-        rc.tny = T.ny;
+        rt.tnx = T.nx;
+        rt.tny = T.ny;
+        // about 4096 voxels per block and at most 131072 blocks
+        rt.rpb = (4096 + g.W - 1) / g.W;
+        if ((rt.rows() + rt.rpb - 1) / rt.rpb > 131072) rt.rpb = (rt.rows() + 131071) / 131072;
         CLE_TRY(V.alloc(T.count(), 1, s, 2));
-        k_seg_and_tiles<<<vg(rc.count()), 256, 0, s>>>(blurred.get<float>(), thr, spots.get<uint8_t>(),
-                                                        seg.get<uint8_t>(), rc, V.L.settled);
+        k_seg_and_tiles<<<(unsigned)((rt.rows() + rt.rpb - 1) / rt.rpb), 256, 0, s>>>(
+            blurred.get<float>(), thr, spots.get<uint8_t>(), seg.get<uint8_t>(), rt, V.L.settled);
     } else {
         k_seg_and<<<vg(n / 4 + 1), 256, 0, s>>>(blurred.get<float>(), thr, spots.get<uint8_t>(), seg.get<uint8_t>(),
                                                 n);
@@ -1294,7 +1313,7 @@ int voronoi_otsu_single(const cle_buf* src, const cle_buf* dst, float spot_sigma
     // masked Voronoi of the seeds inside the segmentation; the seed labels go to the
     // blurred buffer
     return masked_voronoi_core(DBuf{spots.p, CLE_U8}, DBuf{seg.p, CLE_U8}, dbuf(dst), g, blurred.get<float>(), nullptr,
-                               s, true, empty_tiles ? &V : nullptr, &rc);
+                               s, true, empty_tiles ? &V : nullptr, &rt);
 }
 
 // ----------------------------------------------------------------- batched pipeline
