@@ -364,15 +364,18 @@ def test_voronoi_labeling_parity(cle, shape):
     assert np.array_equal(cle.pull(cle.voronoi_labeling(b)), O.voronoi_labeling(b))
 
 
+# 3D images whose rows hold a multiple of 4 voxels take the growth that stores no state
+# for tiles without a segmentation voxel (partial 16 x 8 x 8 tiles on every axis here)
 @pytest.mark.parametrize("shape,ss,so", [((128, 128), 3.5, 1.0), ((200, 180), 2.0, 1.5), ((24, 80, 90), 2.0, 1.0),
-                                         ((9, 64, 64), 1.0, 0.5)])
+                                         ((9, 64, 64), 1.0, 0.5), ((20, 44, 100), 2.0, 1.0),
+                                         ((30, 72, 132), 2.0, 1.5)])
 def test_voronoi_otsu_parity(cle, shape, ss, so):
     img = blobs_image(shape, 15 if len(shape) == 2 else 20, 13)
     got = cle.pull(cle.voronoi_otsu_labeling(img, spot_sigma=ss, outline_sigma=so))
     assert np.array_equal(got, O.voronoi_otsu_labeling(img, ss, so))
 
 
-@pytest.mark.parametrize("shape", [(37, 53), (7, 31, 45)])
+@pytest.mark.parametrize("shape", [(37, 53), (7, 31, 45), (7, 31, 48)])
 def test_voronoi_otsu_destination_types(cle, shape):
     # uint32 / int32 destinations, aligned (four labels per store, then the n % 4 tail)
     # and offset by one element (element by element)
