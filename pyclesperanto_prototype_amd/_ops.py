@@ -501,7 +501,8 @@ def statistics_table(intensity_image: Array, label_image: Array, num_labels: int
 
 def _table_to_host(t) -> np.ndarray:
     """The (columns, labels) table in host memory: one copy into page-locked memory
-    (torch's host cache reuses it across calls) instead of the chunked pageable path."""
+    (torch's host cache reuses a block once no array of an earlier table holds it)
+    instead of the chunked pageable path."""
     if t.numel() == 0 or not t.is_cuda:
         return t.cpu().numpy()
     pinned = torch.empty(tuple(t.shape), dtype=t.dtype, pin_memory=True)
@@ -525,7 +526,9 @@ def statistics_of_labelled_pixels(intensity_image: Image = None, label_image: Im
     table, n = statistics_table(intensity_image, label_image)
     num_labels = n + 1
     host = _table_to_host(table.tensor[:, :n])
-    cols = {k: host[i].copy() for i, k in enumerate(_TABLE_COLUMNS)}
+    # each column is a contiguous row of the one host table (no per-column copy; the
+    # arrays keep the table alive)
+    cols = {k: host[i] for i, k in enumerate(_TABLE_COLUMNS)}
     out = {}
     for k in STATISTICS_KEYS:
         if k in ("label", "original_label"):
