@@ -213,6 +213,16 @@ __device__ __forceinline__ bool nz(const DBuf b, int64_t i) {
 // are double-buffered by slice parity: two barriers per slice.
 // out(i, is_max) stores voxel i's flag.
 constexpr int MX_TX = 64, MX_TY = 16, MX_ZC = 16;
+
+// IEEE maxNum of two operands that are not signalling NaNs (a quiet NaN gives the other
+// operand): one v_max_f32.  fmaxf compiles to the same instruction plus a quieting
+// v_max_f32 x, x of every operand the compiler cannot prove quiet (values loaded from
+// memory or read back from LDS); the maxima tile quiets each voxel value once instead.
+__device__ __forceinline__ float qmax(float a, float b) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
 constexpr int MX_RX = MX_TX + 2, MX_RY = MX_TY + 2, MX_RP = MX_RX * MX_RY;
 constexpr int MX_NPT = (MX_RP + 255) / 256;  // columns per thread
 constexpr int MX_LDS = 4 * MX_RP;            // floats of LDS per block: C and Y, two slices each
@@ -248,9 +258,18 @@ __device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t
 #pragma unroll
         for (int k = 0; k < MX_NPT; ++k) r[k] = live[k] ? in(zz * plane + off[k]) : qnan;
     };
+    // every value enters the maxima quieted (a signalling NaN becomes a quiet one, which
+    // the visit treats alike: a NaN never takes over)
+    auto quiet = [&](float* r) {
+#pragma unroll
+        for (int k = 0; k < MX_NPT; ++k) r[k] = __builtin_canonicalizef(r[k]);
+    };
     if (d3 && z0 > 0) load(pm, z0 - 1);
     load(p0, z0);
     if (d3 && z0 + 1 < g.D) load(pp, z0 + 1);
+    quiet(pm);
+    quiet(p0);
+    quiet(pp);
     for (int64_t z = z0; z < z1; ++z) {
         const bool below = d3 && z > 0, above = d3 && z + 1 < g.D;
         const bool pre = d3 && z + 2 < g.D && z + 1 < z1;
@@ -261,8 +280,8 @@ __device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t
 #pragma unroll
         for (int k = 0; k < MX_NPT; ++k) {
             float m = p0[k];
-            if (below) m = fmaxf(m, pm[k]);
-            if (above) m = fmaxf(m, pp[k]);
+            if (below) m = qmax(m, pm[k]);
+            if (above) m = qmax(m, pp[k]);
             cm[k] = m;
             if (tid + k * 256 < MX_RP) Cl[tid + k * 256] = m;
         }
@@ -270,7 +289,7 @@ __device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t
 #pragma unroll
         for (int k = 0; k < MX_NPT; ++k) {
             const int c = tid + k * 256, cy = c / MX_RX;
-            if (c < MX_RP && cy >= 1 && cy <= MX_TY) Yl[c] = fmaxf(fmaxf(Cl[c - MX_RX], cm[k]), Cl[c + MX_RX]);
+            if (c < MX_RP && cy >= 1 && cy <= MX_TY) Yl[c] = qmax(qmax(Cl[c - MX_RX], cm[k]), Cl[c + MX_RX]);
         }
         __syncthreads();
 #pragma unroll
@@ -283,17 +302,17 @@ __device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t
             // The visit in order (x, then y, then z offsets; start lm = v - 1, a strictly
             // larger value takes over, the flag is whether the last take-over was the
             // centre) in closed form: B / A = the largest value visited before / after
-            // the centre (fmaxf ignores NaN, which never takes over; NaN = nothing
+            // the centre (qmax ignores a NaN, which never takes over; NaN = nothing
             // visited).  Before the centre: the x - 1 columns, (x, y - 1), (x, y, z - 1);
             // after it: (x, y, z + 1), (x, y + 1), the x + 1 columns.  A before-value
             // takes over iff B > v - 1, leaving lm1 = max(v - 1, B); the centre takes over
             // iff v > lm1 (then lm2 = v); with no take-over before it, the flag stays set;
             // any after-value above lm2 clears it.  Exact for ties, infinities, NaN and
             // values where v - 1 == v.
-            float bmax = fmaxf(Yl[c - 1], Cl[c - MX_RX]);
-            float amax = fmaxf(Yl[c + 1], Cl[c + MX_RX]);
-            if (below) bmax = fmaxf(bmax, pm[k]);
-            if (d3) amax = fmaxf(amax, above ? pp[k] : v);
+            float bmax = qmax(Yl[c - 1], Cl[c - MX_RX]);
+            float amax = qmax(Yl[c + 1], Cl[c + MX_RX]);
+            if (below) bmax = qmax(bmax, pm[k]);
+            if (d3) amax = qmax(amax, above ? pp[k] : v);
             const float lm0 = v - 1.f;
             const bool took = bmax > lm0;
             const float lm1 = took ? bmax : lm0;
@@ -306,7 +325,7 @@ __device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t
         for (int k = 0; k < MX_NPT; ++k) {
             pm[k] = p0[k];
             p0[k] = pp[k];
-            pp[k] = nx[k];
+            pp[k] = __builtin_canonicalizef(nx[k]);
         }
     }
 }
