@@ -896,17 +896,21 @@ __global__ void k_bccl_write(const uint8_t* __restrict__ M, Geom g, BGrid B, con
         const int nz = (BZ == 2 && z0 + 1 < g.D) ? 2 : 1, ny = y0 + 1 < g.H ? 2 : 1;
         for (int64_t p = threadIdx.x; p < np; p += blockDim.x) {
             const int64_t b = br * B.W + 2 * p, x = 4 * p;
-            // the pair's 4 x 2 x 2 voxels lie in one growth tile (x a multiple of 4, y0
-            // and z0 even): an empty one stores nothing
-            if constexpr (ST::kCoords)
-                if (out.empty(x, y0, z0)) continue;
             uint32_t m2;
             float f0, f1;
+            uint2 l2;
+            // B.W even: the pair's mask and both roots load together (a root of an empty
+            // block is not used), then both labels together
             if constexpr (VEC) {
-                // B.W even: the pair's mask and both roots load together (a root of
-                // an empty block is not used), then both labels together
                 m2 = *(const uint16_t*)(M + b);
-                const uint2 l2 = *(const uint2*)(L + b);
+                l2 = *(const uint2*)(L + b);
+            }
+            // the pair's 4 x 2 x 2 voxels lie in one growth tile (x a multiple of 4, y0
+            // and z0 even): an empty one stores nothing (its flag is read alongside the
+            // pair's mask and roots)
+            if constexpr (ST::kCoords)
+                if (out.empty(x, y0, z0)) continue;
+            if constexpr (VEC) {
                 const uint32_t i0 = (m2 & 0xFFu) ? l2.x : 0u, i1 = (m2 >> 8) ? l2.y : 0u;
                 const uint32_t g0 = F[i0], g1 = F[i1];
                 f0 = (m2 & 0xFFu) ? (float)g0 : 0.f;
