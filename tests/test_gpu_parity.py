@@ -375,6 +375,28 @@ def test_voronoi_otsu_parity(cle, shape, ss, so):
     assert np.array_equal(got, O.voronoi_otsu_labeling(img, ss, so))
 
 
+@pytest.mark.parametrize("shape,ss,so", [((5, 40, 48), 1.0, 1.0), ((7, 24, 36), 1.5, 1.0), ((17, 33, 52), 2.0, 0.5)])
+def test_voronoi_otsu_stages_on_device_blurs(cle, shape, ss, so):
+    # one partial growth tile in z (and partial tiles in y and x): the pipeline equals
+    # the oracle's stages fed with the device's two blurs bit for bit (the full oracle
+    # can differ here by a maxima near-tie of the blur's rounding)
+    img = blobs_image(shape, 20, 13)
+    got = cle.pull(cle.voronoi_otsu_labeling(img, spot_sigma=ss, outline_sigma=so))
+    bs = cle.pull(cle.gaussian_blur(img, None, ss, ss, ss)).astype(np.float32)
+    bo = cle.pull(cle.gaussian_blur(img, None, so, so, so)).astype(np.float32)
+    want = _pipeline_from_blurs(bs, bo)[0]
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("shape", [(8, 32, 32), (12, 40, 64)])
+def test_voronoi_otsu_no_segmentation(cle, shape):
+    # a constant image: an empty segmentation, every growth tile empty, all labels 0
+    img = np.full(shape, 7, np.uint8)
+    got = cle.pull(cle.voronoi_otsu_labeling(img, spot_sigma=2.0, outline_sigma=1.0))
+    want = O.voronoi_otsu_labeling(img, 2.0, 1.0)
+    assert got.dtype == want.dtype and np.array_equal(got, want) and not got.any()
+
+
 @pytest.mark.parametrize("shape", [(37, 53), (7, 31, 45), (7, 31, 48)])
 def test_voronoi_otsu_destination_types(cle, shape):
     # uint32 / int32 destinations, aligned (four labels per store, then the n % 4 tail)
