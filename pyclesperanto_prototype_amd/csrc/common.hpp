@@ -239,24 +239,28 @@ __device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t
     const int64_t x0 = (t % ntx) * MX_TX, y0 = ((t / ntx) % nty) * MX_TY, zc = t / (ntx * nty);
     const bool d3 = g.ndim == 3;
     const int64_t z0 = d3 ? zc * MX_ZC : 0, z1 = d3 ? (z0 + MX_ZC < g.D ? z0 + MX_ZC : g.D) : 1;
-    const int64_t plane = g.H * g.W;
+    // voxel indices in 32 bits (images hold fewer than 2^32 voxels)
+    const uint32_t plane = (uint32_t)(g.H * g.W);
     const float qnan = __builtin_nanf("");
     // column k of this thread: region cell c = tid + 256 k.  Positions before the
     // image's first column / row are never visited (NaN: nothing); those past its last
     // column / row read the clamped last one, as the visit's clamped offsets do.
-    int64_t off[MX_NPT];
+    // ooff: the in-plane index of the cell's output voxel (interior cells).
+    uint32_t off[MX_NPT], ooff[MX_NPT];
     bool live[MX_NPT];
 #pragma unroll
     for (int k = 0; k < MX_NPT; ++k) {
         const int c = tid + k * 256;
         const int64_t xx = x0 - 1 + c % MX_RX, yy = y0 - 1 + c / MX_RX;
         live[k] = c < MX_RP && xx >= 0 && yy >= 0;
-        off[k] = (yy < g.H ? yy : g.H - 1) * g.W + (xx < g.W ? xx : g.W - 1);
+        off[k] = (uint32_t)((yy < g.H ? yy : g.H - 1) * g.W + (xx < g.W ? xx : g.W - 1));
+        ooff[k] = (uint32_t)(yy * g.W + xx);
     }
     float pm[MX_NPT], p0[MX_NPT], pp[MX_NPT], nx[MX_NPT];
     auto load = [&](float* r, int64_t zz) {
+        const uint32_t zb = (uint32_t)zz * plane;
 #pragma unroll
-        for (int k = 0; k < MX_NPT; ++k) r[k] = live[k] ? in(zz * plane + off[k]) : qnan;
+        for (int k = 0; k < MX_NPT; ++k) r[k] = live[k] ? in((int64_t)(zb + off[k])) : qnan;
     };
     // every value enters the maxima quieted (a signalling NaN becomes a quiet one, which
     // the visit treats alike: a NaN never takes over)
@@ -319,7 +323,7 @@ __device__ __forceinline__ void maxima_tile(const LD& in, const Geom& g, int64_t
             const bool centre = v > lm1;
             const float lm2 = centre ? v : lm1;
             const bool self = (centre || !took) && !(amax > lm2);
-            out((z * g.H + yy) * g.W + xx, self);
+            out((int64_t)((uint32_t)z * plane + ooff[k]), self);
         }
 #pragma unroll
         for (int k = 0; k < MX_NPT; ++k) {
