@@ -5,7 +5,7 @@ Headline (plain method, large): a 512 x 2048 x 2048 uint16 nuclei stack (about 1
 ellipsoidal nuclei, semi-axes 3-8 voxels, 15 % placed as touching pairs, intensity
 800-3000 on a background of 200, Poisson shot noise + Gaussian read noise sigma 20;
 seed 3) resident in HBM.  One step = voronoi_otsu_labeling(spot_sigma=2.0,
-outline_sigma=1.5) into uint32 labels, then statistics_of_labelled_pixels (the dict
+outline_sigma=1.5) into int32 labels, then statistics_of_labelled_pixels (the dict
 pulled to the host).  value = voxels / mean step time (Mvoxels/s).
 
 The JSON line also holds: the roofline of the dominant kernel (the Voronoi growth's pair
@@ -36,7 +36,7 @@ C2_SHAPE = (256, 1024, 1024)
 GROWTH_TILE_VOXELS = 1024  # a growth tile: 16 x 8 x 8 (3D), 32 x 32 (2D)
 # algorithmic bytes per input voxel (inputs read once, outputs written once, on-device
 # dtypes): voronoi_otsu_labeling 25, statistics (two passes over uint16 intensity +
-# uint32 labels) 12
+# int32 labels) 12
 C3_BYTES_PER_VOXEL = 25 + 12
 SECTIONS = ("c3", "c1", "m1", "c2", "batch", "serve", "ingest", "cpu")
 PUBLISHED_M1MAX_MS = 530.0  # benchmarks/apple_m1_voronoi_otsu_labeling.ipynb of the reference
@@ -131,7 +131,7 @@ def c3_section(cle, torch, np, synthetic, steps, warmup, want_crop):
         crop = vol_t[z0:z0 + 128, y0:y0 + 1024, x0:x0 + 1024].contiguous().view(torch.int16).cpu().numpy().view(
             np.uint16)
     vol = cle.asarray(vol_t)
-    labels = cle.create(C3_SHAPE, np.uint32)
+    labels = cle.create(C3_SHAPE, np.int32)  # float32 intermediates, int32 labels
     nvox = vol.size
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
 
@@ -191,7 +191,8 @@ def c3_section(cle, torch, np, synthetic, steps, warmup, want_crop):
         "data": f"synthetic (seeded nuclei stack, {info['nuclei']} nuclei, {n_labels} labels found)",
         "config": {"workload": "voronoi_otsu_labeling(spot_sigma=2.0, outline_sigma=1.5) + "
                                "statistics_of_labelled_pixels of a 512x2048x2048 uint16 nuclei stack "
-                               "(100k nuclei, semi-axes 3-8, 15% touching pairs, seed 3)",
+                               "(100k nuclei, semi-axes 3-8, 15% touching pairs, seed 3); float32 intermediates, "
+                               "int32 labels",
                    "voxels": nvox, "stage_ms": {"voronoi_otsu_labeling": round(vor_ms, 3),
                                                 "statistics_of_labelled_pixels": round(sta_ms, 3)},
                    "growth": {"pair_launches_per_step": roofline["launches"],
