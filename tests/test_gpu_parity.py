@@ -723,7 +723,8 @@ def test_blur_isotropic_half_widths(cle, c):
         assert_blur_close(cle.pull(cle.gaussian_blur(a.astype(np.float32), None, sig, sig, sig)), want)
 
 
-@pytest.mark.parametrize("shape", [(70, 130), (20, 37, 70), (1, 18, 66), (17, 16, 64)])
+@pytest.mark.parametrize("shape", [(70, 130), (20, 37, 70), (1, 18, 66), (17, 16, 64), (70, 128), (40, 100), (1, 4),
+                                   (20, 37, 68), (5, 30, 4), (33, 29, 132)])
 def test_maxima_special_values(cle, shape):
     # the closed-form visit rule against the oracle's literal visit order: plateaus
     # (ties), NaN (never takes over; a NaN centre stays a maximum), +-inf, and values
